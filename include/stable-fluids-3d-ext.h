@@ -1,0 +1,89 @@
+/*
+ * stable-fluids-3d-ext.h -- additive extensions of the stable-fluids C ABI.
+ *
+ * None of these has a counterpart in the reference interface (stable-fluids-3d.h); the
+ * base header and its seven entry points are unchanged. They expose what the base ABI
+ * cannot express: setting the flow state (velocity seeding, checkpoint/resume), choosing
+ * the smoother, reading residual norms and timing single stages.
+ */
+#ifndef STABLE_FLUIDS_3D_EXT_H
+#define STABLE_FLUIDS_3D_EXT_H
+
+#include "stable-fluids-3d.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum StableFluidsExtSmoother {
+    STABLE_FLUIDS_EXT_SMOOTHER_RBGS   = 0, /* red-black Gauss-Seidel, the reference smoother (default) */
+    STABLE_FLUIDS_EXT_SMOOTHER_JACOBI = 1, /* Jacobi, every cell from the previous iterate */
+} StableFluidsExtSmoother;
+
+typedef enum StableFluidsExtIterateStorage {
+    STABLE_FLUIDS_EXT_STORAGE_F32  = 0,
+    STABLE_FLUIDS_EXT_STORAGE_F16  = 1,
+    STABLE_FLUIDS_EXT_STORAGE_BF16 = 2,
+} StableFluidsExtIterateStorage;
+
+typedef enum StableFluidsExtWarmStart {
+    STABLE_FLUIDS_EXT_WARM_START_ZERO    = 0, /* pressure restarts from zero each step (reference) */
+    STABLE_FLUIDS_EXT_WARM_START_LEARNED = 1,
+} StableFluidsExtWarmStart;
+
+typedef struct StableFluidsExtOptions {
+    uint32_t struct_size;                /* sizeof(StableFluidsExtOptions) */
+    uint32_t smoother;                   /* StableFluidsExtSmoother */
+    uint32_t iterate_storage;            /* StableFluidsExtIterateStorage */
+    uint32_t defect_correction_interval; /* sweeps between FP32 defect corrections (16-bit storage); 0 = 10 */
+    uint32_t pressure_warm_start;        /* StableFluidsExtWarmStart */
+    uint32_t reserved[3];                /* must be zero */
+} StableFluidsExtOptions;
+
+/* Same as stable_fluids_create_context_cuda, with options (NULL = reference behaviour).
+ * Unsupported option values return STABLE_FLUIDS_RESULT_BACKEND_FAILURE. */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_create_context_cuda(const StableFluidsContextCreateDesc* desc, const StableFluidsExtOptions* options, void** out_context, StableFluidsScalarFieldHandle* out_scalar_field_handles, uint32_t out_scalar_field_handle_capacity, StableFluidsVectorFieldHandle* out_vector_field_handles, uint32_t out_vector_field_handle_capacity);
+
+/* Sets the flow velocity from three device arrays (NULL = zeros), asynchronously on the
+ * context stream. Needed to start from a non-zero velocity and to resume a checkpoint. */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_set_flow_velocity_cuda(void* context, const float* values_x, const float* values_y, const float* values_z);
+
+/* Sets the pressure field from a device array (NULL = zeros). The next step overwrites it
+ * unless a warm start uses it; provided for checkpoint/resume of every viewable field. */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_set_flow_pressure_cuda(void* context, const float* values);
+
+typedef struct StableFluidsExtNorms {
+    double divergence_l2;   /* sqrt(sum div^2) over all cells, FP64 accumulation */
+    double divergence_max;  /* max |div| */
+    double velocity_l2;     /* sqrt(sum |v|^2) */
+    double pressure_l2;     /* sqrt(sum p^2) */
+} StableFluidsExtNorms;
+
+/* Reduces the current divergence, velocity and pressure views on the GPU (FP64
+ * accumulation) and waits for the result. */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_field_norms_cuda(void* context, StableFluidsExtNorms* out_norms);
+
+typedef enum StableFluidsExtStage {
+    STABLE_FLUIDS_EXT_STAGE_PRESSURE_PASS  = 0, /* one smoother pass of the pressure solve */
+    STABLE_FLUIDS_EXT_STAGE_VELOCITY_PASS  = 1, /* one smoother pass of one velocity component's diffusion */
+    STABLE_FLUIDS_EXT_STAGE_ADVECT         = 2, /* velocity self-advection */
+    STABLE_FLUIDS_EXT_STAGE_PROJECT        = 3, /* projection + divergence + |v| */
+} StableFluidsExtStage;
+
+/* Enqueues `repeat` launches of one stage kernel on the context stream, outside the step
+ * graph and with the step's own buffers and shapes, for per-kernel timing. The
+ * simulation state is left undefined afterwards (re-seed before stepping again). */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_run_stage_cuda(void* context, uint32_t stage, uint32_t repeat);
+
+/* Algorithmic HBM bytes one launch of `stage` moves (each array read once and written
+ * once), as used for roofline figures. */
+STABLE_FLUIDS_API uint64_t stable_fluids_ext_stage_bytes_cuda(void* context, uint32_t stage);
+
+/* Returns the context's HIP stream (hipStream_t as void*). */
+STABLE_FLUIDS_API void* stable_fluids_ext_get_stream_cuda(void* context);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STABLE_FLUIDS_3D_EXT_H */

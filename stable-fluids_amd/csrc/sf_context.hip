@@ -1,0 +1,617 @@
+// sf_context.hip -- host runtime and C ABI of libstable-fluids.so (gfx950).
+//
+// Implements the seven entry points of include/stable-fluids-3d.h (reference
+// backend-cuda.cu:481-800) and the additive ones of include/stable-fluids-3d-ext.h.
+// The whole step is enqueued once on the context stream under hipStreamBeginCapture and
+// replayed with one hipGraphLaunch per step (reference :608-675, :721-726). The graph
+// holds kernel nodes only: buffer copies of the reference graph are folded into
+// out-of-place smoother passes and no memset node is recorded.
+//
+// Buffers (all packed float[nx*ny*nz], x fastest, so views stay plain device arrays):
+//   flow: v{x,y,z} (view), t{x,y,z} (advected velocity = diffusion rhs), a{x,y,z}
+//   (diffused velocity, projection input), p (view), p_aux, div (view), mag (view);
+//   scalar: data (view), temp (advected = rhs), source (view), aux; vector: x, y, z.
+// Smoother passes ping-pong between the output buffer and its aux partner; the pass
+// count's parity picks the first target so the last pass lands in the output buffer.
+#include <rocprofiler-sdk-roctx/roctx.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "../../include/stable-fluids-3d-ext.h"
+#include "../../include/stable-fluids-3d.h"
+#include "sf_kernels.hpp"
+
+namespace sf {
+
+namespace {
+
+struct Range {
+    explicit Range(const char* name) { roctxRangePush(name); }
+    ~Range() { roctxRangePop(); }
+    Range(const Range&)            = delete;
+    Range& operator=(const Range&) = delete;
+};
+
+struct Options {
+    SmootherKind smoother = kSmoothRBGS;
+};
+
+struct ScalarField {
+    StableFluidsScalarFieldDesc desc{};
+    float* data   = nullptr;
+    float* temp   = nullptr;
+    float* source = nullptr;
+    float* aux    = nullptr;
+};
+
+struct VectorField {
+    StableFluidsVectorFieldDesc desc{};
+    float* x = nullptr;
+    float* y = nullptr;
+    float* z = nullptr;
+};
+
+constexpr int kNormBlocks = 1024;
+
+struct Context {
+    StableFluidsSimulationConfig config{};
+    Options options{};
+    hipStream_t stream = nullptr;
+    bool owns_stream   = false;
+    Dims dims{};
+    uint64_t cells = 0;
+    size_t bytes   = 0;
+    hipGraph_t graph         = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    struct {
+        float *vx = nullptr, *vy = nullptr, *vz = nullptr;
+        float *tx = nullptr, *ty = nullptr, *tz = nullptr;
+        float *ax = nullptr, *ay = nullptr, *az = nullptr;
+        float *p = nullptr, *p_aux = nullptr, *div = nullptr, *mag = nullptr;
+    } flow;
+    std::vector<ScalarField> scalars;
+    std::vector<VectorField> vectors;
+    double* norm_partials = nullptr; // kNormBlocks * 4 partial sums, then 4 results
+    std::vector<void*> allocations;
+
+    ~Context() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void* p : allocations) (void)hipFree(p);
+        if (owns_stream && stream) (void)hipStreamDestroy(stream);
+    }
+
+    template <class T>
+    bool alloc(T*& p, size_t nbytes) {
+        void* raw = nullptr;
+        if (hipMalloc(&raw, std::max<size_t>(nbytes, 4)) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        allocations.push_back(raw);
+        p = static_cast<T*>(raw);
+        return true;
+    }
+};
+
+// ---- boundary rules for the stencil kernels (reference backend-cuda.cu:62-189) ----
+
+const StableFluidsFlowBoundaryFaceDesc& flow_face(const StableFluidsFlowBoundaryConfig& b, int k) {
+    const StableFluidsFlowBoundaryFaceDesc* faces[6] = {&b.x_minus, &b.x_plus, &b.y_minus, &b.y_plus, &b.z_minus, &b.z_plus};
+    return *faces[k];
+}
+const StableFluidsScalarBoundaryFaceDesc& scalar_face(const StableFluidsScalarBoundaryConfig& b, int k) {
+    const StableFluidsScalarBoundaryFaceDesc* faces[6] = {&b.x_minus, &b.x_plus, &b.y_minus, &b.y_plus, &b.z_minus, &b.z_plus};
+    return *faces[k];
+}
+
+StencilBC scalar_rules(const StableFluidsScalarBoundaryConfig& b) {
+    StencilBC bc{};
+    for (int k = 0; k < 6; ++k) {
+        const auto& f = scalar_face(b, k);
+        bc.face[k]    = f.type == STABLE_FLUIDS_SCALAR_BOUNDARY_FIXED_VALUE ? FaceRule{kGhostConst, f.value} : FaceRule{kGhostCopy, 0.0f};
+    }
+    for (int a = 0; a < 3; ++a)
+        if (scalar_face(b, 2 * a).type == STABLE_FLUIDS_SCALAR_BOUNDARY_PERIODIC && scalar_face(b, 2 * a + 1).type == STABLE_FLUIDS_SCALAR_BOUNDARY_PERIODIC) bc.periodic |= 1 << a;
+    return bc;
+}
+
+int flow_periodic_mask(const StableFluidsFlowBoundaryConfig& b) {
+    int m = 0;
+    for (int a = 0; a < 3; ++a)
+        if (flow_face(b, 2 * a).type == STABLE_FLUIDS_FLOW_BOUNDARY_PERIODIC && flow_face(b, 2 * a + 1).type == STABLE_FLUIDS_FLOW_BOUNDARY_PERIODIC) m |= 1 << a;
+    return m;
+}
+
+StencilBC pressure_rules(const StableFluidsFlowBoundaryConfig& b) {
+    StencilBC bc{};
+    for (int k = 0; k < 6; ++k) {
+        const auto& f = flow_face(b, k);
+        bc.face[k]    = f.type == STABLE_FLUIDS_FLOW_BOUNDARY_OUTFLOW ? FaceRule{kGhostConst, f.pressure} : FaceRule{kGhostCopy, 0.0f};
+    }
+    bc.periodic = flow_periodic_mask(b);
+    return bc;
+}
+
+FaceRule velocity_rule(const StableFluidsFlowBoundaryFaceDesc& f, const int comp, const int axis) {
+    if (f.type == STABLE_FLUIDS_FLOW_BOUNDARY_OUTFLOW) return FaceRule{kGhostCopy, 0.0f};
+    if (f.type == STABLE_FLUIDS_FLOW_BOUNDARY_FREE_SLIP_WALL && comp != axis) return FaceRule{kGhostCopy, 0.0f};
+    const float prescribed = comp == 0 ? f.velocity_x : (comp == 1 ? f.velocity_y : f.velocity_z);
+    return FaceRule{kGhostReflect, 2.0f * prescribed};
+}
+
+StencilBC velocity_rules(const StableFluidsFlowBoundaryConfig& b, const int comp) {
+    StencilBC bc{};
+    for (int k = 0; k < 6; ++k) bc.face[k] = velocity_rule(flow_face(b, k), comp, k >> 1);
+    bc.periodic = flow_periodic_mask(b);
+    return bc;
+}
+
+// face 2c+s: rule of velocity component c on its own axis c (divergence stencil)
+StencilBC divergence_rules(const StableFluidsFlowBoundaryConfig& b) {
+    StencilBC bc{};
+    for (int k = 0; k < 6; ++k) bc.face[k] = velocity_rule(flow_face(b, k), k >> 1, k >> 1);
+    bc.periodic = flow_periodic_mask(b);
+    return bc;
+}
+
+bool valid_flow_axis(const StableFluidsFlowBoundaryFaceDesc& lo, const StableFluidsFlowBoundaryFaceDesc& hi) {
+    if (lo.type > STABLE_FLUIDS_FLOW_BOUNDARY_PERIODIC || hi.type > STABLE_FLUIDS_FLOW_BOUNDARY_PERIODIC) return false;
+    return (lo.type == STABLE_FLUIDS_FLOW_BOUNDARY_PERIODIC) == (hi.type == STABLE_FLUIDS_FLOW_BOUNDARY_PERIODIC);
+}
+bool valid_scalar_axis(const StableFluidsScalarBoundaryFaceDesc& lo, const StableFluidsScalarBoundaryFaceDesc& hi) {
+    if (lo.type > STABLE_FLUIDS_SCALAR_BOUNDARY_PERIODIC || hi.type > STABLE_FLUIDS_SCALAR_BOUNDARY_PERIODIC) return false;
+    return (lo.type == STABLE_FLUIDS_SCALAR_BOUNDARY_PERIODIC) == (hi.type == STABLE_FLUIDS_SCALAR_BOUNDARY_PERIODIC);
+}
+
+// ---- step enqueue ----
+
+#define SF_TRY(expr)                                  \
+    do {                                              \
+        if ((expr) != hipSuccess) return hipErrorUnknown; \
+    } while (0)
+
+// K smoother passes from x0 (nullptr = zero guess) into `out`, ping-ponging with `aux`.
+hipError_t enqueue_solve(Context& c, SolveKind solve, const float* b, const float* x0, float* out, float* aux, int passes, float coef, float last_scale, const StencilBC& bc) {
+    const float* src = x0;
+    for (int i = 1; i <= passes; ++i) {
+        float* dst = ((passes - i) % 2 == 0) ? out : aux;
+        SF_TRY(launch_smooth(c.stream, c.options.smoother, solve, SmoothPass{src, b, dst, coef, i == passes ? last_scale : 1.0f}, c.dims, bc));
+        src = dst;
+    }
+    return hipSuccess;
+}
+
+// The step sequence of the reference graph (backend-cuda.cu:612-671).
+hipError_t enqueue_step(Context& c) {
+    const auto& cfg = c.config;
+    const auto& fb  = cfg.flow_boundary;
+    const float dt  = cfg.dt;
+    const float h   = cfg.cell_size;
+    auto& f         = c.flow;
+
+    for (auto& v : c.vectors) {
+        if (v.desc.usage != STABLE_FLUIDS_VECTOR_FIELD_FORCE) continue;
+        SF_TRY(launch_add_force(c.stream, f.vx, f.vy, f.vz, v.x, v.y, v.z, dt, c.cells));
+    }
+    SF_TRY(launch_advect_velocity(c.stream, f.tx, f.ty, f.tz, f.vx, f.vy, f.vz, dt, c.dims, h, fb));
+
+    // viscosity; alpha <= 0 keeps the forced velocity (reference :621-642)
+    const float alpha = cfg.dt * cfg.viscosity / (cfg.cell_size * cfg.cell_size);
+    const float* u[3];
+    if (alpha > 0.0f) {
+        if (cfg.diffuse_iterations <= 0) {
+            u[0] = f.tx;
+            u[1] = f.ty;
+            u[2] = f.tz;
+        } else {
+            float* t[3] = {f.tx, f.ty, f.tz};
+            float* a[3] = {f.ax, f.ay, f.az};
+            float* v[3] = {f.vx, f.vy, f.vz};
+            for (int comp = 0; comp < 3; ++comp) SF_TRY(enqueue_solve(c, kSolveDiffusion, t[comp], t[comp], a[comp], v[comp], cfg.diffuse_iterations, alpha, 1.0f, velocity_rules(fb, comp)));
+            u[0] = f.ax;
+            u[1] = f.ay;
+            u[2] = f.az;
+        }
+    } else {
+        SF_TRY(launch_copy_scaled(c.stream, f.ax, f.vx, 1.0f, c.cells));
+        SF_TRY(launch_copy_scaled(c.stream, f.ay, f.vy, 1.0f, c.cells));
+        SF_TRY(launch_copy_scaled(c.stream, f.az, f.vz, 1.0f, c.cells));
+        u[0] = f.ax;
+        u[1] = f.ay;
+        u[2] = f.az;
+    }
+
+    // projection; pressure restarts from zero (reference :643-650)
+    const StencilBC div_bc = divergence_rules(fb);
+    const StencilBC p_bc   = pressure_rules(fb);
+    SF_TRY(launch_divergence(c.stream, f.div, u[0], u[1], u[2], c.dims, h, div_bc));
+    if (cfg.pressure_iterations > 0)
+        SF_TRY(enqueue_solve(c, kSolvePoisson, f.div, nullptr, f.p, f.p_aux, cfg.pressure_iterations, cfg.cell_size * cfg.cell_size, 1.0f, p_bc));
+    else
+        SF_TRY(launch_fill(c.stream, f.p, 0.0f, c.cells));
+    SF_TRY(launch_project(c.stream, f.vx, f.vy, f.vz, f.div, f.mag, u[0], u[1], u[2], f.p, c.dims, h, p_bc, div_bc));
+
+    // scalars (reference :652-668)
+    for (auto& s : c.scalars) {
+        SF_TRY(launch_add_scaled(c.stream, s.data, s.source, dt, c.cells));
+        SF_TRY(launch_advect_scalar(c.stream, s.temp, s.data, f.vx, f.vy, f.vz, dt, c.dims, h, s.desc.boundary, fb));
+        const float alpha_s = cfg.dt * s.desc.diffusion / (cfg.cell_size * cfg.cell_size);
+        const bool dissipate = s.desc.dissipation > 0.0f;
+        const float factor  = dissipate ? 1.0f / (1.0f + cfg.dt * s.desc.dissipation) : 1.0f;
+        if (alpha_s > 0.0f) {
+            if (cfg.diffuse_iterations <= 0)
+                SF_TRY(launch_copy_scaled(c.stream, s.data, s.temp, factor, c.cells));
+            else
+                SF_TRY(enqueue_solve(c, kSolveDiffusion, s.temp, s.temp, s.data, s.aux, cfg.diffuse_iterations, alpha_s, factor, scalar_rules(s.desc.boundary)));
+        } else if (dissipate) {
+            SF_TRY(launch_copy_scaled(c.stream, s.data, s.data, factor, c.cells));
+        }
+    }
+    return hipSuccess;
+}
+
+#undef SF_TRY
+
+StableFluidsResult build_graph(Context& c) {
+    Range r("stable.create_context.graph");
+    if (hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    const hipError_t enq = enqueue_step(c);
+    hipGraph_t graph     = nullptr;
+    const hipError_t end = hipStreamEndCapture(c.stream, &graph);
+    if (enq != hipSuccess || end != hipSuccess || graph == nullptr) {
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    }
+    c.graph = graph;
+    if (hipGraphInstantiate(&c.graph_exec, c.graph, nullptr, nullptr, 0) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    return STABLE_FLUIDS_RESULT_OK;
+}
+
+StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const StableFluidsExtOptions* opt, void** out_context, StableFluidsScalarFieldHandle* out_s, uint32_t cap_s, StableFluidsVectorFieldHandle* out_v, uint32_t cap_v) {
+    Range r("stable.create_context");
+    if (out_context == nullptr || desc == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    *out_context = nullptr;
+    const auto& cfg = desc->config;
+
+    // validation happens before any HIP call (reference :516-528, :536-545)
+    Options options{};
+    if (opt != nullptr) {
+        if (opt->struct_size != sizeof(StableFluidsExtOptions)) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        if (opt->smoother > STABLE_FLUIDS_EXT_SMOOTHER_JACOBI) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        if (opt->iterate_storage != STABLE_FLUIDS_EXT_STORAGE_F32) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        if (opt->pressure_warm_start != STABLE_FLUIDS_EXT_WARM_START_ZERO) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        for (uint32_t v : opt->reserved)
+            if (v != 0) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        options.smoother = opt->smoother == STABLE_FLUIDS_EXT_SMOOTHER_JACOBI ? kSmoothJacobi : kSmoothRBGS;
+    }
+    if (cfg.nx <= 0 || cfg.ny <= 0 || cfg.nz <= 0) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    const auto& fb = cfg.flow_boundary;
+    if (!valid_flow_axis(fb.x_minus, fb.x_plus) || !valid_flow_axis(fb.y_minus, fb.y_plus) || !valid_flow_axis(fb.z_minus, fb.z_plus)) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    if (desc->scalar_field_count > 0 && desc->scalar_fields == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    if (desc->vector_field_count > 0 && desc->vector_fields == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    for (uint32_t i = 0; i < desc->scalar_field_count; ++i) {
+        const auto& b = desc->scalar_fields[i].boundary;
+        if (!valid_scalar_axis(b.x_minus, b.x_plus) || !valid_scalar_axis(b.y_minus, b.y_plus) || !valid_scalar_axis(b.z_minus, b.z_plus)) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    }
+
+    std::unique_ptr<Context> c{new (std::nothrow) Context{}};
+    if (!c) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
+    c->config  = cfg;
+    c->options = options;
+    c->dims    = Dims{cfg.nx, cfg.ny, cfg.nz};
+    c->cells   = static_cast<uint64_t>(cfg.nx) * static_cast<uint64_t>(cfg.ny) * static_cast<uint64_t>(cfg.nz);
+    c->bytes   = c->cells * sizeof(float);
+    c->stream  = static_cast<hipStream_t>(desc->stream);
+    if (c->stream == nullptr) {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        c->owns_stream = true;
+    }
+    for (uint32_t i = 0; i < desc->scalar_field_count; ++i) {
+        c->scalars.push_back(ScalarField{desc->scalar_fields[i]});
+        if (out_s != nullptr && i < cap_s) out_s[i] = i + 1u;
+    }
+    for (uint32_t i = 0; i < desc->vector_field_count; ++i) {
+        c->vectors.push_back(VectorField{desc->vector_fields[i]});
+        if (out_v != nullptr && i < cap_v) out_v[i] = i + 1u;
+    }
+
+    {
+        Range ra("stable.create_context.alloc");
+        auto& f = c->flow;
+        for (float** p : {&f.vx, &f.vy, &f.vz, &f.tx, &f.ty, &f.tz, &f.ax, &f.ay, &f.az, &f.p, &f.p_aux, &f.div, &f.mag})
+            if (!c->alloc(*p, c->bytes)) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
+        for (auto& s : c->scalars)
+            for (float** p : {&s.data, &s.temp, &s.source, &s.aux})
+                if (!c->alloc(*p, c->bytes)) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
+        for (auto& v : c->vectors)
+            for (float** p : {&v.x, &v.y, &v.z})
+                if (!c->alloc(*p, c->bytes)) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
+        if (!c->alloc(c->norm_partials, sizeof(double) * (kNormBlocks * 4 + 4))) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
+    }
+
+    {
+        // initial state (reference :583-606)
+        Range ri("stable.create_context.init");
+        auto& f = c->flow;
+        for (float* p : {f.vx, f.vy, f.vz, f.tx, f.ty, f.tz, f.ax, f.ay, f.az, f.p, f.p_aux, f.div, f.mag})
+            if (launch_fill(c->stream, p, 0.0f, c->cells) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        for (auto& s : c->scalars) {
+            if (launch_fill(c->stream, s.data, s.desc.initial_value, c->cells) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+            if (launch_fill(c->stream, s.temp, s.desc.initial_value, c->cells) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+            if (launch_fill(c->stream, s.aux, s.desc.initial_value, c->cells) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+            if (launch_fill(c->stream, s.source, 0.0f, c->cells) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        }
+        for (auto& v : c->vectors) {
+            if (launch_fill(c->stream, v.x, v.desc.initial_value_x, c->cells) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+            if (launch_fill(c->stream, v.y, v.desc.initial_value_y, c->cells) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+            if (launch_fill(c->stream, v.z, v.desc.initial_value_z, c->cells) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        }
+    }
+
+    const StableFluidsResult g = build_graph(*c);
+    if (g != STABLE_FLUIDS_RESULT_OK) return g;
+    if (hipGetLastError() != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    *out_context = c.release();
+    return STABLE_FLUIDS_RESULT_OK;
+}
+
+StableFluidsResult status(const hipError_t e) { return e == hipSuccess ? STABLE_FLUIDS_RESULT_OK : STABLE_FLUIDS_RESULT_BACKEND_FAILURE; }
+
+// Copy `values` (device, NULL = zeros) into dst on the context stream.
+hipError_t upload(Context& c, float* dst, const float* values) {
+    if (values == nullptr) return launch_fill(c.stream, dst, 0.0f, c.cells);
+    return hipMemcpyAsync(dst, values, c.bytes, hipMemcpyDeviceToDevice, c.stream);
+}
+
+// ---- norms: two deterministic passes, FP64 accumulation ----
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+template <int kCount>
+__device__ void block_reduce(double (&acc)[kCount], const bool is_max_last, double* out) {
+    __shared__ double lds[4][kCount];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kCount; ++k) acc[k] = (is_max_last && k == kCount - 1) ? wave_max(acc[k]) : wave_sum(acc[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < kCount; ++k) lds[wave][k] = acc[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < kCount; ++k) {
+            double r = lds[0][k];
+            for (int w = 1; w < 4; ++w) r = (is_max_last && k == kCount - 1) ? fmax(r, lds[w][k]) : r + lds[w][k];
+            out[k] = r;
+        }
+}
+
+// partials[block] = {sum div^2, sum |v|^2, sum p^2, max |div|}
+__global__ __launch_bounds__(256) void k_norm_partials(const float* __restrict__ div, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float* __restrict__ p, const uint64_t n, double* __restrict__ partials) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += static_cast<uint64_t>(gridDim.x) * 256) {
+        const double dv = div[i], x = vx[i], y = vy[i], z = vz[i], pv = p[i];
+        acc[0] += dv * dv;
+        acc[1] += x * x + y * y + z * z;
+        acc[2] += pv * pv;
+        acc[3] = fmax(acc[3], fabs(dv));
+    }
+    block_reduce<4>(acc, true, partials + 4 * blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_norm_final(double* __restrict__ partials, const int blocks) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < blocks; b += 256) {
+        acc[0] += partials[4 * b + 0];
+        acc[1] += partials[4 * b + 1];
+        acc[2] += partials[4 * b + 2];
+        acc[3] = fmax(acc[3], partials[4 * b + 3]);
+    }
+    block_reduce<4>(acc, true, partials + 4 * kNormBlocks);
+}
+
+} // namespace
+
+} // namespace sf
+
+using sf::Context;
+
+extern "C" {
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_create_context_cuda(const StableFluidsContextCreateDesc* desc, void** out_context, StableFluidsScalarFieldHandle* out_s, uint32_t cap_s, StableFluidsVectorFieldHandle* out_v, uint32_t cap_v) {
+    return sf::create(desc, nullptr, out_context, out_s, cap_s, out_v, cap_v);
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_create_context_cuda(const StableFluidsContextCreateDesc* desc, const StableFluidsExtOptions* options, void** out_context, StableFluidsScalarFieldHandle* out_s, uint32_t cap_s, StableFluidsVectorFieldHandle* out_v, uint32_t cap_v) {
+    return sf::create(desc, options, out_context, out_s, cap_s, out_v, cap_v);
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_destroy_context_cuda(void* context) {
+    sf::Range r("stable.destroy_context");
+    if (context == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto* c = static_cast<Context*>(context);
+    if (c->stream != nullptr) (void)hipStreamSynchronize(c->stream);
+    delete c;
+    return STABLE_FLUIDS_RESULT_OK;
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_update_scalar_field_cuda(void* context, StableFluidsScalarFieldHandle field, const float* values) {
+    sf::Range r("stable.update_scalar_field");
+    if (context == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& c = *static_cast<Context*>(context);
+    if (field == 0 || field > c.scalars.size()) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    return sf::status(sf::upload(c, c.scalars[field - 1u].data, values));
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_update_scalar_field_source_cuda(void* context, StableFluidsScalarFieldHandle field, const float* values) {
+    sf::Range r("stable.update_scalar_field_source");
+    if (context == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& c = *static_cast<Context*>(context);
+    if (field == 0 || field > c.scalars.size()) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    return sf::status(sf::upload(c, c.scalars[field - 1u].source, values));
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_update_vector_field_cuda(void* context, StableFluidsVectorFieldHandle field, const float* x, const float* y, const float* z) {
+    sf::Range r("stable.update_vector_field");
+    if (context == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& c = *static_cast<Context*>(context);
+    if (field == 0 || field > c.vectors.size()) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& v = c.vectors[field - 1u];
+    if (sf::upload(c, v.x, x) != hipSuccess || sf::upload(c, v.y, y) != hipSuccess || sf::upload(c, v.z, z) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    return STABLE_FLUIDS_RESULT_OK;
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_step_cuda(void* context) {
+    sf::Range r("stable.step");
+    if (context == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& c = *static_cast<Context*>(context);
+    return sf::status(hipGraphLaunch(c.graph_exec, c.stream));
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_get_view_cuda(void* context, const StableFluidsViewRequest* request, StableFluidsView* out_view) {
+    sf::Range r("stable.get_view");
+    if (context == nullptr || request == nullptr || out_view == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& c = *static_cast<Context*>(context);
+    StableFluidsView v{};
+    v.layout             = STABLE_FLUIDS_VIEW_LAYOUT_F32_3D;
+    v.nx                 = c.config.nx;
+    v.ny                 = c.config.ny;
+    v.nz                 = c.config.nz;
+    v.row_stride_bytes   = static_cast<uint64_t>(c.config.nx) * sizeof(float);
+    v.slice_stride_bytes = static_cast<uint64_t>(c.config.nx) * static_cast<uint64_t>(c.config.ny) * sizeof(float);
+    *out_view            = v;
+    switch (request->kind) {
+    case STABLE_FLUIDS_VIEW_SCALAR_FIELD_DATA:
+    case STABLE_FLUIDS_VIEW_SCALAR_FIELD_SOURCE: {
+        if (request->scalar_field == 0 || request->scalar_field > c.scalars.size()) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        const auto& s   = c.scalars[request->scalar_field - 1u];
+        out_view->data0 = request->kind == STABLE_FLUIDS_VIEW_SCALAR_FIELD_DATA ? s.data : s.source;
+        break;
+    }
+    case STABLE_FLUIDS_VIEW_VECTOR_FIELD: {
+        if (request->vector_field == 0 || request->vector_field > c.vectors.size()) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        const auto& vf   = c.vectors[request->vector_field - 1u];
+        out_view->layout = STABLE_FLUIDS_VIEW_LAYOUT_F32_3D_SOA3;
+        out_view->data0  = vf.x;
+        out_view->data1  = vf.y;
+        out_view->data2  = vf.z;
+        break;
+    }
+    case STABLE_FLUIDS_VIEW_FLOW_VELOCITY:
+        out_view->layout = STABLE_FLUIDS_VIEW_LAYOUT_F32_3D_SOA3;
+        out_view->data0  = c.flow.vx;
+        out_view->data1  = c.flow.vy;
+        out_view->data2  = c.flow.vz;
+        break;
+    case STABLE_FLUIDS_VIEW_FLOW_VELOCITY_MAGNITUDE: out_view->data0 = c.flow.mag; break;
+    case STABLE_FLUIDS_VIEW_FLOW_PRESSURE: out_view->data0 = c.flow.p; break;
+    case STABLE_FLUIDS_VIEW_FLOW_DIVERGENCE: out_view->data0 = c.flow.div; break;
+    default: return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    }
+    // hand-off to the consumer stream (reference :743-759)
+    if (request->consumer_stream == nullptr) return STABLE_FLUIDS_RESULT_OK;
+    auto consumer = static_cast<hipStream_t>(request->consumer_stream);
+    if (consumer == c.stream) return STABLE_FLUIDS_RESULT_OK;
+    hipEvent_t ready = nullptr;
+    if (hipEventCreateWithFlags(&ready, hipEventDisableTiming) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    const bool ok = hipEventRecord(ready, c.stream) == hipSuccess && hipStreamWaitEvent(consumer, ready, 0) == hipSuccess;
+    (void)hipEventDestroy(ready);
+    return ok ? STABLE_FLUIDS_RESULT_OK : STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_set_flow_velocity_cuda(void* context, const float* x, const float* y, const float* z) {
+    sf::Range r("stable.ext.set_flow_velocity");
+    if (context == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& c = *static_cast<Context*>(context);
+    if (sf::upload(c, c.flow.vx, x) != hipSuccess || sf::upload(c, c.flow.vy, y) != hipSuccess || sf::upload(c, c.flow.vz, z) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    return STABLE_FLUIDS_RESULT_OK;
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_set_flow_pressure_cuda(void* context, const float* values) {
+    sf::Range r("stable.ext.set_flow_pressure");
+    if (context == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& c = *static_cast<Context*>(context);
+    return sf::status(sf::upload(c, c.flow.p, values));
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_field_norms_cuda(void* context, StableFluidsExtNorms* out) {
+    sf::Range r("stable.ext.field_norms");
+    if (context == nullptr || out == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& c      = *static_cast<Context*>(context);
+    const int nb = static_cast<int>(std::min<uint64_t>(sf::kNormBlocks, (c.cells + 255) / 256));
+    sf::k_norm_partials<<<nb, 256, 0, c.stream>>>(c.flow.div, c.flow.vx, c.flow.vy, c.flow.vz, c.flow.p, c.cells, c.norm_partials);
+    sf::k_norm_final<<<1, 256, 0, c.stream>>>(c.norm_partials, nb);
+    double host[4] = {};
+    if (hipGetLastError() != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    if (hipMemcpyAsync(host, c.norm_partials + 4 * sf::kNormBlocks, sizeof(host), hipMemcpyDeviceToHost, c.stream) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    if (hipStreamSynchronize(c.stream) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    out->divergence_l2  = sqrt(host[0]);
+    out->velocity_l2    = sqrt(host[1]);
+    out->pressure_l2    = sqrt(host[2]);
+    out->divergence_max = host[3];
+    return STABLE_FLUIDS_RESULT_OK;
+}
+
+STABLE_FLUIDS_API uint64_t stable_fluids_ext_stage_bytes_cuda(void* context, uint32_t stage) {
+    if (context == nullptr) return 0;
+    const auto& c = *static_cast<Context*>(context);
+    switch (stage) {
+    case STABLE_FLUIDS_EXT_STAGE_PRESSURE_PASS:
+    case STABLE_FLUIDS_EXT_STAGE_VELOCITY_PASS: return 12ull * c.cells; // read x_old, b; write x_new
+    case STABLE_FLUIDS_EXT_STAGE_ADVECT: return 24ull * c.cells;        // read v (3); write t (3)
+    case STABLE_FLUIDS_EXT_STAGE_PROJECT: return 36ull * c.cells;       // read a (3), p; write v (3), div, |v|
+    default: return 0;
+    }
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_run_stage_cuda(void* context, uint32_t stage, uint32_t repeat) {
+    sf::Range r("stable.ext.run_stage");
+    if (context == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& c        = *static_cast<Context*>(context);
+    auto& f        = c.flow;
+    const auto& fb = c.config.flow_boundary;
+    const float h  = c.config.cell_size;
+    for (uint32_t i = 0; i < repeat; ++i) {
+        hipError_t e = hipSuccess;
+        switch (stage) {
+        case STABLE_FLUIDS_EXT_STAGE_PRESSURE_PASS: {
+            const bool odd = i & 1;
+            e = sf::launch_smooth(c.stream, c.options.smoother, sf::kSolvePoisson, sf::SmoothPass{odd ? f.p : f.p_aux, f.div, odd ? f.p_aux : f.p, h * h, 1.0f}, c.dims, sf::pressure_rules(fb));
+            break;
+        }
+        case STABLE_FLUIDS_EXT_STAGE_VELOCITY_PASS: {
+            const bool odd    = i & 1;
+            const float alpha = c.config.dt * c.config.viscosity / (h * h);
+            e = sf::launch_smooth(c.stream, c.options.smoother, sf::kSolveDiffusion, sf::SmoothPass{odd ? f.ax : f.vx, f.tx, odd ? f.vx : f.ax, alpha, 1.0f}, c.dims, sf::velocity_rules(fb, 0));
+            break;
+        }
+        case STABLE_FLUIDS_EXT_STAGE_ADVECT: e = sf::launch_advect_velocity(c.stream, f.tx, f.ty, f.tz, f.vx, f.vy, f.vz, c.config.dt, c.dims, h, fb); break;
+        case STABLE_FLUIDS_EXT_STAGE_PROJECT: e = sf::launch_project(c.stream, f.vx, f.vy, f.vz, f.div, f.mag, f.ax, f.ay, f.az, f.p, c.dims, h, sf::pressure_rules(fb), sf::divergence_rules(fb)); break;
+        default: return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        }
+        if (e != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    }
+    return STABLE_FLUIDS_RESULT_OK;
+}
+
+STABLE_FLUIDS_API void* stable_fluids_ext_get_stream_cuda(void* context) {
+    if (context == nullptr) return nullptr;
+    return static_cast<Context*>(context)->stream;
+}
+
+} // extern "C"

@@ -1,0 +1,343 @@
+// sf_kernels.hip -- gfx950 stage kernels of the stable-fluids 3D step.
+//
+// Arithmetic follows the reference kernels (backend-cuda.cu:302-434) expression by
+// expression (same operand order, IEEE division, floorf/fmodf/sqrtf), so the FP32 results
+// track the reference within rounding. The kernel set is fused differently:
+//  * one RK2 trace per cell serves all three velocity components (reference traces 3x);
+//  * a red-black Gauss-Seidel iteration is one out-of-place pass (red from the old
+//    iterate, black from the red values), so no memcpy node and no in-place race;
+//  * projection, divergence of the projected field and |v| run in one pass.
+#include "sf_kernels.hpp"
+
+namespace sf {
+
+namespace {
+
+constexpr int kElemBlock = 256;
+constexpr int kTileX     = 64; // one wavefront per row of a 3D tile
+constexpr int kTileY     = 4;
+
+__device__ __forceinline__ uint64_t global_tid() { return static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; }
+
+// ---------------- elementwise ----------------
+
+__global__ __launch_bounds__(kElemBlock) void k_fill(float* __restrict__ p, const float value, const uint64_t n) {
+    const uint64_t i = global_tid() * 4;
+    if (i + 3 < n) {
+        *reinterpret_cast<float4*>(p + i) = make_float4(value, value, value, value);
+    } else {
+        for (uint64_t k = i; k < n; ++k) p[k] = value;
+    }
+}
+
+// dst = src * factor (reference dissipate_kernel backend-cuda.cu:377-384; factor 1 is a copy)
+__global__ __launch_bounds__(kElemBlock) void k_copy_scaled(float* __restrict__ dst, const float* __restrict__ src, const float factor, const uint64_t n) {
+    const uint64_t i = global_tid() * 4;
+    if (i + 3 < n) {
+        const float4 a = *reinterpret_cast<const float4*>(src + i);
+        *reinterpret_cast<float4*>(dst + i) = make_float4(a.x * factor, a.y * factor, a.z * factor, a.w * factor);
+    } else {
+        for (uint64_t k = i; k < n; ++k) dst[k] = src[k] * factor;
+    }
+}
+
+// dst += dt * src (reference add_field_source_kernel backend-cuda.cu:321-328)
+__global__ __launch_bounds__(kElemBlock) void k_add_scaled(float* __restrict__ dst, const float* __restrict__ src, const float dt, const uint64_t n) {
+    const uint64_t i = global_tid() * 4;
+    if (i + 3 < n) {
+        float4 a       = *reinterpret_cast<float4*>(dst + i);
+        const float4 f = *reinterpret_cast<const float4*>(src + i);
+        a.x += dt * f.x;
+        a.y += dt * f.y;
+        a.z += dt * f.z;
+        a.w += dt * f.w;
+        *reinterpret_cast<float4*>(dst + i) = a;
+    } else {
+        for (uint64_t k = i; k < n; ++k) dst[k] += dt * src[k];
+    }
+}
+
+// v += dt * f for all three components (reference add_force_kernel backend-cuda.cu:310-319)
+__global__ __launch_bounds__(kElemBlock) void k_add_force(float* __restrict__ vx, float* __restrict__ vy, float* __restrict__ vz, const float* __restrict__ fx, const float* __restrict__ fy, const float* __restrict__ fz, const float dt, const uint64_t n) {
+    const uint64_t i = global_tid() * 4;
+    float* v[3]       = {vx, vy, vz};
+    const float* f[3] = {fx, fy, fz};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (i + 3 < n) {
+            float4 a        = *reinterpret_cast<float4*>(v[c] + i);
+            const float4 fv = *reinterpret_cast<const float4*>(f[c] + i);
+            a.x += dt * fv.x;
+            a.y += dt * fv.y;
+            a.z += dt * fv.z;
+            a.w += dt * fv.w;
+            *reinterpret_cast<float4*>(v[c] + i) = a;
+        } else {
+            for (uint64_t k = i; k < n; ++k) v[c][k] += dt * f[c][k];
+        }
+    }
+}
+
+// ---------------- advection ----------------
+
+// One midpoint trace per cell, three component samples (reference
+// advect_velocity_component_kernel backend-cuda.cu:330-340, launched per component).
+__global__ __launch_bounds__(kTileX* kTileY) void k_advect_velocity(float* __restrict__ tx, float* __restrict__ ty, float* __restrict__ tz, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float dt, const Dims d, const float h, const StableFluidsFlowBoundaryConfig bc) {
+    const int x = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
+    const int z = static_cast<int>(blockIdx.z);
+    if (x >= d.nx || y >= d.ny) return;
+    const float px = (static_cast<float>(x) + 0.5f) * h;
+    const float py = (static_cast<float>(y) + 0.5f) * h;
+    const float pz = (static_cast<float>(z) + 0.5f) * h;
+    float qx, qy, qz;
+    trace_rk2(vx, vy, vz, px, py, pz, dt, d, h, bc, qx, qy, qz);
+    const uint64_t i = lin(x, y, z, d);
+    tx[i]            = sample_velocity(vx, 0, qx, qy, qz, d, h, bc);
+    ty[i]            = sample_velocity(vy, 1, qx, qy, qz, d, h, bc);
+    tz[i]            = sample_velocity(vz, 2, qx, qy, qz, d, h, bc);
+}
+
+// reference advect_scalar_component_kernel backend-cuda.cu:342-352
+__global__ __launch_bounds__(kTileX* kTileY) void k_advect_scalar(float* __restrict__ out, const float* __restrict__ src, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float dt, const Dims d, const float h, const StableFluidsScalarBoundaryConfig sbc, const StableFluidsFlowBoundaryConfig fbc) {
+    const int x = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
+    const int z = static_cast<int>(blockIdx.z);
+    if (x >= d.nx || y >= d.ny) return;
+    const float px = (static_cast<float>(x) + 0.5f) * h;
+    const float py = (static_cast<float>(y) + 0.5f) * h;
+    const float pz = (static_cast<float>(z) + 0.5f) * h;
+    float qx, qy, qz;
+    trace_rk2(vx, vy, vz, px, py, pz, dt, d, h, fbc, qx, qy, qz);
+    out[lin(x, y, z, d)] = sample_scalar(src, qx, qy, qz, d, h, sbc);
+}
+
+// ---------------- smoothers ----------------
+
+template <int kSolve>
+__device__ __forceinline__ float relax(const float bv, const float sum, const float coef) {
+    if constexpr (kSolve == kSolvePoisson)
+        return (sum - coef * bv) / 6.0f; // reference backend-cuda.cu:406
+    else
+        return (bv + coef * sum) / (1.0f + 6.0f * coef); // reference backend-cuda.cu:363,374
+}
+
+template <bool kZero>
+__device__ __forceinline__ float old_value(const float* __restrict__ x_old, const uint64_t i) {
+    if constexpr (kZero)
+        return 0.0f;
+    else
+        return x_old[i];
+}
+
+// Relaxed value of cell (x,y,z) from the old iterate (Jacobi update, or the red half-sweep).
+template <int kSolve, bool kZero>
+__device__ __forceinline__ float relax_from_old(const float* __restrict__ x_old, const float* __restrict__ b, const int x, const int y, const int z, const Dims d, const StencilBC& bc, const float coef) {
+    const uint64_t i   = lin(x, y, z, d);
+    const float centre = old_value<kZero>(x_old, i);
+    const int n[3]     = {d.nx, d.ny, d.nz};
+    float sum          = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int axis = k >> 1;
+        const int dir  = (k & 1) ? 1 : -1;
+        int q[3]       = {x, y, z};
+        float v;
+        if (step_axis(q[axis], dir, n[axis], (bc.periodic >> axis) & 1))
+            v = old_value<kZero>(x_old, lin(q[0], q[1], q[2], d));
+        else
+            v = apply_ghost(bc.face[k], centre);
+        sum = k == 0 ? v : sum + v;
+    }
+    return relax<kSolve>(b[i], sum, coef);
+}
+
+// One smoother pass over every cell. RBGS: red cells from the old iterate; black cells
+// from the red values of this pass (recomputed for each neighbour) and, through wall
+// ghosts or an odd periodic wrap, the old values. Jacobi: every cell from the old iterate.
+template <int kSmoother, int kSolve, bool kZero>
+__global__ __launch_bounds__(kTileX* kTileY) void k_smooth(const float* __restrict__ x_old, const float* __restrict__ b, float* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc) {
+    const int x = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
+    const int z = static_cast<int>(blockIdx.z);
+    if (x >= d.nx || y >= d.ny) return;
+    float out;
+    if (kSmoother == kSmoothJacobi || ((x + y + z) & 1) == 0) {
+        out = relax_from_old<kSolve, kZero>(x_old, b, x, y, z, d, bc, coef);
+    } else {
+        const uint64_t i   = lin(x, y, z, d);
+        const float centre = old_value<kZero>(x_old, i);
+        const int n[3]     = {d.nx, d.ny, d.nz};
+        float sum          = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int axis = k >> 1;
+            const int dir  = (k & 1) ? 1 : -1;
+            int q[3]       = {x, y, z};
+            float v;
+            if (step_axis(q[axis], dir, n[axis], (bc.periodic >> axis) & 1)) {
+                if (((q[0] + q[1] + q[2]) & 1) == 0)
+                    v = relax_from_old<kSolve, kZero>(x_old, b, q[0], q[1], q[2], d, bc, coef);
+                else
+                    v = old_value<kZero>(x_old, lin(q[0], q[1], q[2], d));
+            } else {
+                v = apply_ghost(bc.face[k], centre);
+            }
+            sum = k == 0 ? v : sum + v;
+        }
+        out = relax<kSolve>(b[i], sum, coef);
+    }
+    x_new[lin(x, y, z, d)] = out * out_scale;
+}
+
+// ---------------- divergence / projection ----------------
+
+// Component `axis` of the field at the neighbour of (x,y,z) along `axis` (dir = +-1),
+// where `value_at(q)` gives the in-grid value and `centre` the value at (x,y,z).
+template <class F>
+__device__ __forceinline__ float along_axis(const int axis, const int dir, const int x, const int y, const int z, const Dims d, const StencilBC& bc, const float centre, F value_at) {
+    const int n[3] = {d.nx, d.ny, d.nz};
+    int q[3]       = {x, y, z};
+    if (step_axis(q[axis], dir, n[axis], (bc.periodic >> axis) & 1)) return value_at(q[0], q[1], q[2]);
+    return apply_ghost(bc.face[2 * axis + (dir > 0 ? 1 : 0)], centre);
+}
+
+// reference compute_divergence_kernel backend-cuda.cu:386-396
+__global__ __launch_bounds__(kTileX* kTileY) void k_divergence(float* __restrict__ div, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const Dims d, const float h, const StencilBC dbc) {
+    const int x = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
+    const int z = static_cast<int>(blockIdx.z);
+    if (x >= d.nx || y >= d.ny) return;
+    const uint64_t i    = lin(x, y, z, d);
+    const float inv_2h  = 0.5f / h;
+    const float* v[3]   = {vx, vy, vz};
+    float acc           = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float* f     = v[c];
+        const float centre = f[i];
+        auto at            = [&](int a, int b2, int c2) { return f[lin(a, b2, c2, d)]; };
+        const float plus   = along_axis(c, 1, x, y, z, d, dbc, centre, at);
+        const float minus  = along_axis(c, -1, x, y, z, d, dbc, centre, at);
+        const float dd     = (plus - minus) * inv_2h;
+        acc                = c == 0 ? dd : acc + dd;
+    }
+    div[i] = acc;
+}
+
+// Gradient component `c` of p at (x,y,z) (reference backend-cuda.cu:416-418).
+__device__ __forceinline__ float grad_p(const float* __restrict__ p, const int c, const int x, const int y, const int z, const Dims d, const StencilBC& pbc, const float inv_2h) {
+    const float centre = p[lin(x, y, z, d)];
+    auto at            = [&](int a, int b2, int c2) { return p[lin(a, b2, c2, d)]; };
+    const float plus   = along_axis(c, 1, x, y, z, d, pbc, centre, at);
+    const float minus  = along_axis(c, -1, x, y, z, d, pbc, centre, at);
+    return (plus - minus) * inv_2h;
+}
+
+// reference project_velocity_kernel :409-422, then compute_divergence_kernel :386-396 and
+// velocity_magnitude_kernel :424-434 on the projected field (the scalar stage between
+// them does not touch the velocity). Out of place: neighbours are re-projected locally.
+__global__ __launch_bounds__(kTileX* kTileY) void k_project(float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz, float* __restrict__ div, float* __restrict__ mag, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float* __restrict__ p, const Dims d, const float h, const StencilBC pbc, const StencilBC dbc) {
+    const int x = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int y = static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
+    const int z = static_cast<int>(blockIdx.z);
+    if (x >= d.nx || y >= d.ny) return;
+    const uint64_t i   = lin(x, y, z, d);
+    const float inv_2h = 0.5f / h;
+    const float* v[3]  = {vx, vy, vz};
+    float proj[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) proj[c] = v[c][i] - grad_p(p, c, x, y, z, d, pbc, inv_2h);
+    float acc = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float* f    = v[c];
+        auto at           = [&](int a, int b2, int c2) { return f[lin(a, b2, c2, d)] - grad_p(p, c, a, b2, c2, d, pbc, inv_2h); };
+        const float plus  = along_axis(c, 1, x, y, z, d, dbc, proj[c], at);
+        const float minus = along_axis(c, -1, x, y, z, d, dbc, proj[c], at);
+        const float dd    = (plus - minus) * inv_2h;
+        acc               = c == 0 ? dd : acc + dd;
+    }
+    ox[i]  = proj[0];
+    oy[i]  = proj[1];
+    oz[i]  = proj[2];
+    div[i] = acc;
+    mag[i] = sqrtf(proj[0] * proj[0] + proj[1] * proj[1] + proj[2] * proj[2]);
+}
+
+inline unsigned elem_blocks(const uint64_t n) { return static_cast<unsigned>((n + 4 * kElemBlock - 1) / (4 * kElemBlock)); }
+inline dim3 tile_grid(const Dims d) { return dim3(static_cast<unsigned>((d.nx + kTileX - 1) / kTileX), static_cast<unsigned>((d.ny + kTileY - 1) / kTileY), static_cast<unsigned>(d.nz)); }
+inline dim3 tile_block() { return dim3(kTileX, kTileY, 1); }
+
+} // namespace
+
+hipError_t launch_fill(hipStream_t s, float* p, const float value, const uint64_t n) {
+    if (n == 0) return hipSuccess;
+    k_fill<<<elem_blocks(n), kElemBlock, 0, s>>>(p, value, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_scaled(hipStream_t s, float* dst, const float* src, const float factor, const uint64_t n) {
+    if (n == 0) return hipSuccess;
+    k_copy_scaled<<<elem_blocks(n), kElemBlock, 0, s>>>(dst, src, factor, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_scaled(hipStream_t s, float* dst, const float* src, const float dt, const uint64_t n) {
+    if (n == 0) return hipSuccess;
+    k_add_scaled<<<elem_blocks(n), kElemBlock, 0, s>>>(dst, src, dt, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_add_force(hipStream_t s, float* vx, float* vy, float* vz, const float* fx, const float* fy, const float* fz, const float dt, const uint64_t n) {
+    if (n == 0) return hipSuccess;
+    k_add_force<<<elem_blocks(n), kElemBlock, 0, s>>>(vx, vy, vz, fx, fy, fz, dt, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_advect_velocity(hipStream_t s, float* tx, float* ty, float* tz, const float* vx, const float* vy, const float* vz, const float dt, const Dims d, const float h, const StableFluidsFlowBoundaryConfig& bc) {
+    k_advect_velocity<<<tile_grid(d), tile_block(), 0, s>>>(tx, ty, tz, vx, vy, vz, dt, d, h, bc);
+    return hipGetLastError();
+}
+
+hipError_t launch_advect_scalar(hipStream_t s, float* out, const float* src, const float* vx, const float* vy, const float* vz, const float dt, const Dims d, const float h, const StableFluidsScalarBoundaryConfig& sbc, const StableFluidsFlowBoundaryConfig& fbc) {
+    k_advect_scalar<<<tile_grid(d), tile_block(), 0, s>>>(out, src, vx, vy, vz, dt, d, h, sbc, fbc);
+    return hipGetLastError();
+}
+
+hipError_t launch_smooth(hipStream_t s, const SmootherKind smoother, const SolveKind solve, const SmoothPass& pass, const Dims d, const StencilBC& bc) {
+    const dim3 g = tile_grid(d), b = tile_block();
+    const bool zero = pass.x_old == nullptr;
+#define SF_SMOOTH(SM, SO, Z) k_smooth<SM, SO, Z><<<g, b, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc)
+    if (smoother == kSmoothRBGS) {
+        if (solve == kSolvePoisson) {
+            if (zero) SF_SMOOTH(kSmoothRBGS, kSolvePoisson, true);
+            else SF_SMOOTH(kSmoothRBGS, kSolvePoisson, false);
+        } else {
+            if (zero) SF_SMOOTH(kSmoothRBGS, kSolveDiffusion, true);
+            else SF_SMOOTH(kSmoothRBGS, kSolveDiffusion, false);
+        }
+    } else {
+        if (solve == kSolvePoisson) {
+            if (zero) SF_SMOOTH(kSmoothJacobi, kSolvePoisson, true);
+            else SF_SMOOTH(kSmoothJacobi, kSolvePoisson, false);
+        } else {
+            if (zero) SF_SMOOTH(kSmoothJacobi, kSolveDiffusion, true);
+            else SF_SMOOTH(kSmoothJacobi, kSolveDiffusion, false);
+        }
+    }
+#undef SF_SMOOTH
+    return hipGetLastError();
+}
+
+hipError_t launch_divergence(hipStream_t s, float* div, const float* vx, const float* vy, const float* vz, const Dims d, const float h, const StencilBC& div_bc) {
+    k_divergence<<<tile_grid(d), tile_block(), 0, s>>>(div, vx, vy, vz, d, h, div_bc);
+    return hipGetLastError();
+}
+
+hipError_t launch_project(hipStream_t s, float* ox, float* oy, float* oz, float* div, float* mag, const float* vx, const float* vy, const float* vz, const float* p, const Dims d, const float h, const StencilBC& p_bc, const StencilBC& div_bc) {
+    k_project<<<tile_grid(d), tile_block(), 0, s>>>(ox, oy, oz, div, mag, vx, vy, vz, p, d, h, p_bc, div_bc);
+    return hipGetLastError();
+}
+
+} // namespace sf

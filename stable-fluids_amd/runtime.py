@@ -1,0 +1,203 @@
+"""Host-side mirror of the stable-fluids C ABI over libstable-fluids.so.
+
+`Context` wraps the seven entry points of include/stable-fluids-3d.h (reference
+stable-fluids-3d.h:112-154) with the same names, argument meaning and error behaviour:
+every call returns the library's StableFluidsResult and `check=True` turns a non-zero
+result into `StableFluidsError`. Device buffers come from hip.DeviceBuffer. There is no
+CPU fallback: loading fails loudly when the HIP library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi, hip
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libstable-fluids.so")
+_lib = None
+
+ENTRY_POINTS = (
+    "stable_fluids_create_context_cuda", "stable_fluids_destroy_context_cuda",
+    "stable_fluids_update_scalar_field_cuda", "stable_fluids_update_scalar_field_source_cuda",
+    "stable_fluids_update_vector_field_cuda", "stable_fluids_step_cuda", "stable_fluids_get_view_cuda",
+)
+EXT_ENTRY_POINTS = (
+    "stable_fluids_ext_create_context_cuda", "stable_fluids_ext_set_flow_velocity_cuda",
+    "stable_fluids_ext_set_flow_pressure_cuda", "stable_fluids_ext_field_norms_cuda",
+    "stable_fluids_ext_run_stage_cuda", "stable_fluids_ext_stage_bytes_cuda", "stable_fluids_ext_get_stream_cuda",
+)
+
+
+class StableFluidsError(RuntimeError):
+    def __init__(self, what, code):
+        super().__init__(f"{what} returned {code}")
+        self.code = code
+
+
+class ExtNorms(C.Structure):
+    _fields_ = [("divergence_l2", C.c_double), ("divergence_max", C.c_double),
+                ("velocity_l2", C.c_double), ("pressure_l2", C.c_double)]
+
+
+def load_library(path=LIB_PATH):
+    """Loads libstable-fluids.so and declares every exported signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(path)
+    vp, u32, fp = C.c_void_p, C.c_uint32, C.POINTER(C.c_float)
+    lib.stable_fluids_create_context_cuda.argtypes = [C.POINTER(abi.ContextCreateDesc), C.POINTER(vp), C.POINTER(u32), u32, C.POINTER(u32), u32]
+    lib.stable_fluids_ext_create_context_cuda.argtypes = [C.POINTER(abi.ContextCreateDesc), C.POINTER(abi.ExtOptions), C.POINTER(vp), C.POINTER(u32), u32, C.POINTER(u32), u32]
+    lib.stable_fluids_destroy_context_cuda.argtypes = [vp]
+    lib.stable_fluids_update_scalar_field_cuda.argtypes = [vp, u32, vp]
+    lib.stable_fluids_update_scalar_field_source_cuda.argtypes = [vp, u32, vp]
+    lib.stable_fluids_update_vector_field_cuda.argtypes = [vp, u32, vp, vp, vp]
+    lib.stable_fluids_step_cuda.argtypes = [vp]
+    lib.stable_fluids_get_view_cuda.argtypes = [vp, C.POINTER(abi.ViewRequest), C.POINTER(abi.View)]
+    lib.stable_fluids_ext_set_flow_velocity_cuda.argtypes = [vp, vp, vp, vp]
+    lib.stable_fluids_ext_set_flow_pressure_cuda.argtypes = [vp, vp]
+    lib.stable_fluids_ext_field_norms_cuda.argtypes = [vp, C.POINTER(ExtNorms)]
+    lib.stable_fluids_ext_run_stage_cuda.argtypes = [vp, u32, u32]
+    lib.stable_fluids_ext_stage_bytes_cuda.argtypes = [vp, u32]
+    lib.stable_fluids_ext_stage_bytes_cuda.restype = C.c_uint64
+    lib.stable_fluids_ext_get_stream_cuda.argtypes = [vp]
+    lib.stable_fluids_ext_get_stream_cuda.restype = vp
+    for name in ENTRY_POINTS + EXT_ENTRY_POINTS:
+        if name not in ("stable_fluids_ext_stage_bytes_cuda", "stable_fluids_ext_get_stream_cuda"):
+            getattr(lib, name).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def ext_options(smoother=abi.SMOOTHER_RBGS, iterate_storage=abi.STORAGE_F32, defect_correction_interval=0,
+                pressure_warm_start=abi.WARM_START_ZERO):
+    o = abi.ExtOptions()
+    o.struct_size = C.sizeof(abi.ExtOptions)
+    o.smoother = smoother
+    o.iterate_storage = iterate_storage
+    o.defect_correction_interval = defect_correction_interval
+    o.pressure_warm_start = pressure_warm_start
+    return o
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, hip.DeviceBuffer):
+        return x.value
+    return int(x)
+
+
+class Context:
+    """One simulation context (reference ContextStorage, backend-cuda.cu:11-56)."""
+
+    def __init__(self, desc, options=None, stream=None):
+        lib = load_library()
+        self.lib = lib
+        self.desc = desc
+        if stream is not None:
+            desc.desc.stream = stream.handle if isinstance(stream, hip.Stream) else stream
+        self.scalar_handles = (C.c_uint32 * max(1, desc.n_scalars))()
+        self.vector_handles = (C.c_uint32 * max(1, desc.n_vectors))()
+        handle = C.c_void_p()
+        if options is None:
+            r = lib.stable_fluids_create_context_cuda(C.byref(desc.desc), C.byref(handle), self.scalar_handles, desc.n_scalars, self.vector_handles, desc.n_vectors)
+        else:
+            r = lib.stable_fluids_ext_create_context_cuda(C.byref(desc.desc), C.byref(options), C.byref(handle), self.scalar_handles, desc.n_scalars, self.vector_handles, desc.n_vectors)
+        if r != abi.RESULT_OK:
+            raise StableFluidsError("stable_fluids_create_context_cuda", r)
+        self.handle = handle
+        self.shape = desc.shape
+        self.cells = desc.cells
+        self._staging = []
+
+    # ---- reference entry points ----
+    def destroy(self):
+        if self.handle:
+            r = self.lib.stable_fluids_destroy_context_cuda(self.handle)
+            self.handle = C.c_void_p()
+            self._staging.clear()
+            return r
+        return abi.RESULT_OK
+
+    def update_scalar_field(self, handle, values):
+        return self._call("stable_fluids_update_scalar_field_cuda", handle, _ptr(values))
+
+    def update_scalar_field_source(self, handle, values):
+        return self._call("stable_fluids_update_scalar_field_source_cuda", handle, _ptr(values))
+
+    def update_vector_field(self, handle, x, y, z):
+        return self._call("stable_fluids_update_vector_field_cuda", handle, _ptr(x), _ptr(y), _ptr(z))
+
+    def step(self, steps=1):
+        for _ in range(steps):
+            r = self.lib.stable_fluids_step_cuda(self.handle)
+            if r != abi.RESULT_OK:
+                raise StableFluidsError("stable_fluids_step_cuda", r)
+
+    def get_view(self, kind, scalar_field=0, vector_field=0, consumer_stream=None):
+        req = abi.ViewRequest(kind, scalar_field, vector_field, consumer_stream)
+        view = abi.View()
+        r = self.lib.stable_fluids_get_view_cuda(self.handle, C.byref(req), C.byref(view))
+        if r != abi.RESULT_OK:
+            raise StableFluidsError("stable_fluids_get_view_cuda", r)
+        return view
+
+    # ---- extensions ----
+    def set_flow_velocity(self, x, y, z):
+        return self._call("stable_fluids_ext_set_flow_velocity_cuda", _ptr(x), _ptr(y), _ptr(z))
+
+    def set_flow_pressure(self, p):
+        return self._call("stable_fluids_ext_set_flow_pressure_cuda", _ptr(p))
+
+    def norms(self):
+        out = ExtNorms()
+        r = self.lib.stable_fluids_ext_field_norms_cuda(self.handle, C.byref(out))
+        if r != abi.RESULT_OK:
+            raise StableFluidsError("stable_fluids_ext_field_norms_cuda", r)
+        return {"divergence_l2": out.divergence_l2, "divergence_max": out.divergence_max,
+                "velocity_l2": out.velocity_l2, "pressure_l2": out.pressure_l2}
+
+    def run_stage(self, stage, repeat):
+        return self._call("stable_fluids_ext_run_stage_cuda", stage, repeat)
+
+    def stage_bytes(self, stage):
+        return int(self.lib.stable_fluids_ext_stage_bytes_cuda(self.handle, stage))
+
+    @property
+    def stream(self):
+        return self.lib.stable_fluids_ext_get_stream_cuda(self.handle)
+
+    # ---- host conveniences (upload from numpy, read views back) ----
+    def upload(self, array):
+        """Copies a host array into a new device buffer kept alive with the context."""
+        buf = hip.DeviceBuffer.from_array(np.ascontiguousarray(array, dtype=np.float32).ravel())
+        self._staging.append(buf)
+        return buf
+
+    def synchronize(self):
+        hip.check(hip.lib().hipStreamSynchronize(C.c_void_p(self.stream)), "hipStreamSynchronize")
+
+    def read(self, kind, handle=0):
+        """Returns the view's field(s) as numpy arrays shaped (nz, ny, nx)."""
+        self.synchronize()
+        view = self.get_view(kind, handle if kind in (abi.VIEW_SCALAR_DATA, abi.VIEW_SCALAR_SOURCE) else 0,
+                             handle if kind == abi.VIEW_VECTOR else 0)
+        ptrs = [view.data0] + ([view.data1, view.data2] if view.layout == abi.LAYOUT_F32_3D_SOA3 else [])
+        out = [hip.to_host(p, self.cells).reshape(self.shape) for p in ptrs]
+        return out[0] if len(out) == 1 else out
+
+    def _call(self, name, *args):
+        r = getattr(self.lib, name)(self.handle, *args)
+        if r != abi.RESULT_OK:
+            raise StableFluidsError(name, r)
+        return r
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()

@@ -1,0 +1,141 @@
+"""GPU step parity against the host-loop oracle, through the C ABI.
+
+Two bars:
+  * bit-exact against the FP32 restatement (same operations, same rounding): the HIP
+    kernels compute every value with the reference's expression order and no FMA
+    contraction, so every field must match to the last bit, at any step count;
+  * norm-wise relative error max|gpu - o64| / max|o64| <= 1e-5 per field against the
+    FP64 oracle, on horizons where FP32 rounding has not been amplified. The 64^3 plume
+    at dt=0.1, force 5 is chaotic (the FP32 and FP64 oracles themselves drift apart
+    ~10x every 2-3 steps, 40% by step 20), so its FP64 bar is checked for 3 steps and
+    for 100 steps on the same scene with force 0.5, which is well conditioned.
+"""
+import zlib
+
+import numpy as np
+import pytest
+
+from conftest import sf, sf_oracle
+from paritylib import Inputs, normwise, report, run_gpu, run_oracle
+
+abi = sf.abi
+scenes = sf.scenes
+pytestmark = pytest.mark.gpu
+TOL64 = 1e-5
+
+
+def plume_seeder(scene):
+    return (lambda ctx: scenes.seed_context(ctx, scene)), (lambda sim: scenes.seed_oracle(sim, scene))
+
+
+def assert_bitexact(gpu, ref, tag):
+    errs = normwise(gpu, ref)
+    report(tag, errs)
+    bad = {k: v for k, v in errs.items() if v != 0.0}
+    assert not bad, f"{tag}: not bit-exact: {bad}"
+
+
+def assert_fp64(gpu, ref, tag, tol=TOL64):
+    errs = normwise(gpu, ref)
+    report(tag, errs)
+    bad = {k: v for k, v in errs.items() if v > tol}
+    assert not bad, f"{tag}: above {tol}: {bad}"
+
+
+def test_plume64_100_steps_bitexact_fp32_oracle():
+    """The 64^3 plume (dt 0.1, nu=kappa=1e-4, D=20, P=40, free-slip walls), 100 steps."""
+    scene = scenes.plume_scene(64)
+    g_seed, o_seed = plume_seeder(scene)
+    gpu = run_gpu(scene.desc, g_seed, 100)
+    ref32 = run_oracle(scene.desc, o_seed, 100, 32)
+    assert_bitexact(gpu, ref32, "plume64 100 steps vs fp32 oracle")
+    assert np.max(gpu["scalar1"]) > 1.0  # the plume is there
+
+
+def test_plume64_3_steps_fp64():
+    scene = scenes.plume_scene(64)
+    g_seed, o_seed = plume_seeder(scene)
+    assert_fp64(run_gpu(scene.desc, g_seed, 3), run_oracle(scene.desc, o_seed, 3, 64), "plume64 3 steps vs fp64")
+
+
+def test_plume64_gentle_100_steps_fp64():
+    """Same scene and solver settings with force 0.5: FP32 stays within 1e-5 of FP64."""
+    scene = scenes.plume_scene(64)
+    src, fx, fy, fz = scenes.plume_fields(scene.desc.shape, (32, 6, 32), 4, 100.0, 0.5)
+    scene.force = (fx, fy, fz)
+    g_seed, o_seed = plume_seeder(scene)
+    assert_fp64(run_gpu(scene.desc, g_seed, 100), run_oracle(scene.desc, o_seed, 100, 64), "plume64 force0.5 100 steps vs fp64")
+
+
+def mixed_desc(n=(12, 10, 8), d_iters=6, p_iters=8, viscosity=2e-3, diffusion=1e-3, dissipation=0.0):
+    nx, ny, nz = n
+    wall_v = [(0, 0, 0), (0, 0, 0), (0.3, 0.0, 0.1), (0, 0, 0), (0.05, -0.2, 0.0), (0.1, 0.0, 0.0)]
+    flow = abi.flow_boundary([abi.FLOW_PERIODIC, abi.FLOW_PERIODIC, abi.FLOW_NO_SLIP, abi.FLOW_OUTFLOW,
+                              abi.FLOW_FREE_SLIP, abi.FLOW_NO_SLIP], velocity=wall_v, pressure=[0, 0, 0, 0.5, 0, 0])
+    s1 = abi.scalar_boundary([abi.SCALAR_PERIODIC, abi.SCALAR_PERIODIC, abi.SCALAR_FIXED, abi.SCALAR_FIXED,
+                              abi.SCALAR_ZERO_FLUX, abi.SCALAR_FIXED], values=[0, 0, 1.0, 0.0, 0, -0.5])
+    s2 = abi.scalar_boundary([abi.SCALAR_FIXED, abi.SCALAR_ZERO_FLUX, abi.SCALAR_PERIODIC, abi.SCALAR_PERIODIC,
+                              abi.SCALAR_ZERO_FLUX, abi.SCALAR_ZERO_FLUX], values=[2.0, 0, 0, 0, 0, 0])
+    scalars = [abi.scalar_desc(diffusion, dissipation, 0.25, s1), abi.scalar_desc(diffusion * 3, 0.7, -1.0, s2)]
+    vectors = [abi.force_desc(0.1, 0.0, -0.2), abi.force_desc(0.0, 0.3, 0.0)]
+    return abi.Desc(nx, ny, nz, 1.0 / 16, 0.05, viscosity, d_iters, p_iters, flow, scalars=scalars, vectors=vectors)
+
+
+EDGE_CASES = {
+    "mixed_bc": dict(),
+    "odd_sizes_walls": dict(n=(7, 9, 5)),
+    "odd_periodic_x": dict(n=(7, 6, 5)),
+    "slab_nz1": dict(n=(16, 12, 1)),
+    "line_nx1": dict(n=(1, 9, 6)),
+    "wide_nx": dict(n=(130, 3, 4)),
+    "nu0_kappa0_quirk": dict(viscosity=0.0, diffusion=0.0),
+    "dissipation": dict(dissipation=0.35),
+    "no_iterations": dict(d_iters=0, p_iters=0),
+    "odd_iterations": dict(d_iters=5, p_iters=7),
+}
+
+
+@pytest.mark.parametrize("case", sorted(EDGE_CASES))
+def test_edge_cases_bitexact_and_fp64(case):
+    desc = mixed_desc(**EDGE_CASES[case])
+    inputs = Inputs(desc, seed=zlib.crc32(case.encode()) % 1000, velocity_scale=0.5)
+    steps = 6
+    gpu = run_gpu(desc, inputs.seed_gpu, steps)
+    assert_bitexact(gpu, run_oracle(desc, inputs.seed_oracle, steps, 32), f"{case} vs fp32")
+    assert_fp64(gpu, run_oracle(desc, inputs.seed_oracle, steps, 64), f"{case} vs fp64", tol=1e-4)
+
+
+def test_all_periodic_large_cfl_bitexact():
+    """Backtraces that wrap several times around periodic axes (fmodf path)."""
+    flow = abi.flow_boundary([abi.FLOW_PERIODIC] * 6)
+    sb = abi.scalar_boundary([abi.SCALAR_PERIODIC] * 6)
+    desc = abi.Desc(16, 8, 12, 1.0 / 16, 0.5, 1e-3, 4, 6, flow, scalars=[abi.scalar_desc(1e-3, 0.0, 0.0, sb)], vectors=[abi.force_desc()])
+    inputs = Inputs(desc, seed=5, velocity_scale=6.0)
+    gpu = run_gpu(desc, inputs.seed_gpu, 4)
+    assert_bitexact(gpu, run_oracle(desc, inputs.seed_oracle, 4, 32), "periodic large cfl vs fp32")
+
+
+def test_jacobi_smoother_bitexact():
+    desc = mixed_desc(d_iters=7, p_iters=9)
+    inputs = Inputs(desc, seed=11, velocity_scale=0.5)
+    opts = sf.ext_options(smoother=abi.SMOOTHER_JACOBI)
+    gpu = run_gpu(desc, inputs.seed_gpu, 5, options=opts)
+    assert_bitexact(gpu, run_oracle(desc, inputs.seed_oracle, 5, 32, smoother=sf_oracle.JACOBI), "jacobi vs fp32")
+    assert_fp64(gpu, run_oracle(desc, inputs.seed_oracle, 5, 64, smoother=sf_oracle.JACOBI), "jacobi vs fp64", tol=1e-4)
+
+
+def test_plume256_noise_one_step_bitexact():
+    """The 256^3 throughput workload (seeded divergent initial velocity, D=40, P=80), one step."""
+    scene = scenes.plume_scene(256, diffuse_iterations=40, pressure_iterations=80, noise_seed=0)
+    g_seed, o_seed = plume_seeder(scene)
+    gpu = run_gpu(scene.desc, g_seed, 1)
+    assert_bitexact(gpu, run_oracle(scene.desc, o_seed, 1, 32), "plume256 1 step vs fp32")
+
+
+def test_determinism_two_runs_bitwise():
+    scene = scenes.plume_scene(64)
+    g_seed, _ = plume_seeder(scene)
+    a = run_gpu(scene.desc, g_seed, 20)
+    b = run_gpu(scene.desc, g_seed, 20)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
