@@ -4,8 +4,7 @@
 // expression (same operand order, IEEE division, floorf/fmodf/sqrtf), so the FP32 results
 // track the reference within rounding. The kernel set is fused differently:
 //  * one RK2 trace per cell serves all three velocity components (reference traces 3x);
-//  * a red-black Gauss-Seidel iteration is one out-of-place pass (red from the old
-//    iterate, black from the red values), so no memcpy node and no in-place race;
+//  * smoother passes live in sf_smooth.hip (LDS z-marching tiles);
 //  * projection, divergence of the projected field and |v| run in one pass.
 #include "sf_kernels.hpp"
 
@@ -110,84 +109,6 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_advect_scalar(float* __restr
     float qx, qy, qz;
     trace_rk2(vx, vy, vz, px, py, pz, dt, d, h, fbc, qx, qy, qz);
     out[lin(x, y, z, d)] = sample_scalar(src, qx, qy, qz, d, h, sbc);
-}
-
-// ---------------- smoothers ----------------
-
-template <int kSolve>
-__device__ __forceinline__ float relax(const float bv, const float sum, const float coef) {
-    if constexpr (kSolve == kSolvePoisson)
-        return (sum - coef * bv) / 6.0f; // reference backend-cuda.cu:406
-    else
-        return (bv + coef * sum) / (1.0f + 6.0f * coef); // reference backend-cuda.cu:363,374
-}
-
-template <bool kZero>
-__device__ __forceinline__ float old_value(const float* __restrict__ x_old, const uint64_t i) {
-    if constexpr (kZero)
-        return 0.0f;
-    else
-        return x_old[i];
-}
-
-// Relaxed value of cell (x,y,z) from the old iterate (Jacobi update, or the red half-sweep).
-template <int kSolve, bool kZero>
-__device__ __forceinline__ float relax_from_old(const float* __restrict__ x_old, const float* __restrict__ b, const int x, const int y, const int z, const Dims d, const StencilBC& bc, const float coef) {
-    const uint64_t i   = lin(x, y, z, d);
-    const float centre = old_value<kZero>(x_old, i);
-    const int n[3]     = {d.nx, d.ny, d.nz};
-    float sum          = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        const int axis = k >> 1;
-        const int dir  = (k & 1) ? 1 : -1;
-        int q[3]       = {x, y, z};
-        float v;
-        if (step_axis(q[axis], dir, n[axis], (bc.periodic >> axis) & 1))
-            v = old_value<kZero>(x_old, lin(q[0], q[1], q[2], d));
-        else
-            v = apply_ghost(bc.face[k], centre);
-        sum = k == 0 ? v : sum + v;
-    }
-    return relax<kSolve>(b[i], sum, coef);
-}
-
-// One smoother pass over every cell. RBGS: red cells from the old iterate; black cells
-// from the red values of this pass (recomputed for each neighbour) and, through wall
-// ghosts or an odd periodic wrap, the old values. Jacobi: every cell from the old iterate.
-template <int kSmoother, int kSolve, bool kZero>
-__global__ __launch_bounds__(kTileX* kTileY) void k_smooth(const float* __restrict__ x_old, const float* __restrict__ b, float* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc) {
-    const int x = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    const int y = static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
-    const int z = static_cast<int>(blockIdx.z);
-    if (x >= d.nx || y >= d.ny) return;
-    float out;
-    if (kSmoother == kSmoothJacobi || ((x + y + z) & 1) == 0) {
-        out = relax_from_old<kSolve, kZero>(x_old, b, x, y, z, d, bc, coef);
-    } else {
-        const uint64_t i   = lin(x, y, z, d);
-        const float centre = old_value<kZero>(x_old, i);
-        const int n[3]     = {d.nx, d.ny, d.nz};
-        float sum          = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const int axis = k >> 1;
-            const int dir  = (k & 1) ? 1 : -1;
-            int q[3]       = {x, y, z};
-            float v;
-            if (step_axis(q[axis], dir, n[axis], (bc.periodic >> axis) & 1)) {
-                if (((q[0] + q[1] + q[2]) & 1) == 0)
-                    v = relax_from_old<kSolve, kZero>(x_old, b, q[0], q[1], q[2], d, bc, coef);
-                else
-                    v = old_value<kZero>(x_old, lin(q[0], q[1], q[2], d));
-            } else {
-                v = apply_ghost(bc.face[k], centre);
-            }
-            sum = k == 0 ? v : sum + v;
-        }
-        out = relax<kSolve>(b[i], sum, coef);
-    }
-    x_new[lin(x, y, z, d)] = out * out_scale;
 }
 
 // ---------------- divergence / projection ----------------
@@ -302,31 +223,6 @@ hipError_t launch_advect_velocity(hipStream_t s, float* tx, float* ty, float* tz
 
 hipError_t launch_advect_scalar(hipStream_t s, float* out, const float* src, const float* vx, const float* vy, const float* vz, const float dt, const Dims d, const float h, const StableFluidsScalarBoundaryConfig& sbc, const StableFluidsFlowBoundaryConfig& fbc) {
     k_advect_scalar<<<tile_grid(d), tile_block(), 0, s>>>(out, src, vx, vy, vz, dt, d, h, sbc, fbc);
-    return hipGetLastError();
-}
-
-hipError_t launch_smooth(hipStream_t s, const SmootherKind smoother, const SolveKind solve, const SmoothPass& pass, const Dims d, const StencilBC& bc) {
-    const dim3 g = tile_grid(d), b = tile_block();
-    const bool zero = pass.x_old == nullptr;
-#define SF_SMOOTH(SM, SO, Z) k_smooth<SM, SO, Z><<<g, b, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc)
-    if (smoother == kSmoothRBGS) {
-        if (solve == kSolvePoisson) {
-            if (zero) SF_SMOOTH(kSmoothRBGS, kSolvePoisson, true);
-            else SF_SMOOTH(kSmoothRBGS, kSolvePoisson, false);
-        } else {
-            if (zero) SF_SMOOTH(kSmoothRBGS, kSolveDiffusion, true);
-            else SF_SMOOTH(kSmoothRBGS, kSolveDiffusion, false);
-        }
-    } else {
-        if (solve == kSolvePoisson) {
-            if (zero) SF_SMOOTH(kSmoothJacobi, kSolvePoisson, true);
-            else SF_SMOOTH(kSmoothJacobi, kSolvePoisson, false);
-        } else {
-            if (zero) SF_SMOOTH(kSmoothJacobi, kSolveDiffusion, true);
-            else SF_SMOOTH(kSmoothJacobi, kSolveDiffusion, false);
-        }
-    }
-#undef SF_SMOOTH
     return hipGetLastError();
 }
 
