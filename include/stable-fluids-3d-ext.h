@@ -68,6 +68,7 @@ typedef enum StableFluidsExtStage {
     STABLE_FLUIDS_EXT_STAGE_VELOCITY_PASS  = 1, /* one smoother pass of one velocity component's diffusion */
     STABLE_FLUIDS_EXT_STAGE_ADVECT         = 2, /* velocity self-advection */
     STABLE_FLUIDS_EXT_STAGE_PROJECT        = 3, /* projection + divergence + |v| */
+    STABLE_FLUIDS_EXT_STAGE_JACOBI_PASS    = 4, /* one Jacobi pass of the pressure solve, whatever the smoother option */
 } StableFluidsExtStage;
 
 /* Enqueues `repeat` launches of one stage kernel on the context stream, outside the step

@@ -572,7 +572,8 @@ STABLE_FLUIDS_API uint64_t stable_fluids_ext_stage_bytes_cuda(void* context, uin
     const auto& c = *static_cast<Context*>(context);
     switch (stage) {
     case STABLE_FLUIDS_EXT_STAGE_PRESSURE_PASS:
-    case STABLE_FLUIDS_EXT_STAGE_VELOCITY_PASS: return 12ull * c.cells; // read x_old, b; write x_new
+    case STABLE_FLUIDS_EXT_STAGE_VELOCITY_PASS:
+    case STABLE_FLUIDS_EXT_STAGE_JACOBI_PASS: return 12ull * c.cells; // read x_old, b; write x_new
     case STABLE_FLUIDS_EXT_STAGE_ADVECT: return 24ull * c.cells;        // read v (3); write t (3)
     case STABLE_FLUIDS_EXT_STAGE_PROJECT: return 36ull * c.cells;       // read a (3), p; write v (3), div, |v|
     default: return 0;
@@ -598,6 +599,11 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_run_stage_cuda(void* cont
             const bool odd    = i & 1;
             const float alpha = c.config.dt * c.config.viscosity / (h * h);
             e = sf::launch_smooth(c.stream, c.options.smoother, sf::kSolveDiffusion, sf::SmoothPass{odd ? f.ax : f.vx, f.tx, odd ? f.vx : f.ax, alpha, 1.0f}, c.dims, sf::velocity_rules(fb, 0));
+            break;
+        }
+        case STABLE_FLUIDS_EXT_STAGE_JACOBI_PASS: {
+            const bool odd = i & 1;
+            e = sf::launch_smooth(c.stream, sf::kSmoothJacobi, sf::kSolvePoisson, sf::SmoothPass{odd ? f.p : f.p_aux, f.div, odd ? f.p_aux : f.p, h * h, 1.0f}, c.dims, sf::pressure_rules(fb));
             break;
         }
         case STABLE_FLUIDS_EXT_STAGE_ADVECT: e = sf::launch_advect_velocity(c.stream, f.tx, f.ty, f.tz, f.vx, f.vy, f.vz, c.config.dt, c.dims, h, fb); break;

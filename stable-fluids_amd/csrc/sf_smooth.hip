@@ -5,38 +5,48 @@
 //   Poisson:   x = (sum6 - c * b) / 6             (reference backend-cuda.cu:398-407)
 // read from x_old (or zeros) and b, written to x_new: 12 B/cell of compulsory traffic.
 //
-// Work decomposition: a 256-thread workgroup owns a kTX x kTY column of cells in x-y and
-// a chunk of zc planes, and marches upward in z. For RBGS, plane k of the half-updated
-// iterate H (red cells relaxed from the old iterate, black cells old) is computed on the
-// tile plus a 1-cell halo from an LDS ring of old planes carrying a 2-cell halo; the
-// final plane k-1 (black cells relaxed from H) is then computed on the tile. Red values
-// on the halo are recomputed redundantly, so tiles never exchange data within a pass.
-// Old and b planes for the next step are fetched into registers before the current
-// plane is computed, hiding HBM latency behind the LDS work.
+// Work decomposition. A workgroup owns a 128 x 16 column of cells in x-y and a chunk of
+// zc planes and marches upward in z. Each lane works on x-pairs (cells 2i, 2i+1 of a
+// row): in a checkerboard a pair holds one red and one black cell, so with 64-bit LDS
+// accesses every lane relaxes exactly one cell per stage instead of masking half a
+// wavefront.
+//  RBGS: plane k of the half-updated iterate H (red relaxed from the old iterate, black
+//  kept) is computed on the tile plus a one-pair halo from an LDS ring of three old
+//  planes with a two-pair halo; then the black cells of plane k-1 are relaxed from the
+//  LDS ring of H planes. Halo red values are recomputed, so tiles exchange nothing.
+//  Jacobi: every cell of plane k from the old ring (one-pair halo).
+// The next old plane is fetched into registers before the current plane is computed.
 //
-// Boundary handling: positions outside the grid on a periodic axis are loaded from the
-// wrapped cell (so LDS neighbours are always the right values); on other axes the
-// neighbour is replaced by the face's ghost rule applied to the centre value, decided
-// per position from its global coordinates. Red/black parity uses wrapped coordinates.
+// Boundaries. A neighbour outside the grid on a non-periodic axis is always adjacent to
+// the cell being relaxed, and the reference's ghost value for it is a function of that
+// cell's current value (copy, 2*v_face - value, or a constant; backend-cuda.cu:62-189).
+// The halo position is therefore filled, when its plane is loaded, with the rule applied
+// to the clamped (adjacent) value; the stencil itself reads LDS unconditionally. Out-of-
+// grid planes below/above a non-periodic z range hold the z-face rule of the edge plane.
+// Periodic axes load the wrapped cell. Red/black parity uses wrapped coordinates, and an
+// odd periodic axis, where a pair can hold two cells of one colour, takes a second relax.
 #include "sf_kernels.hpp"
 
 namespace sf {
 
 namespace {
 
-constexpr int kTX      = 64;
-constexpr int kTY      = 16;
-constexpr int kThreads = 256;
-constexpr int kOX      = kTX + 4; // old iterate region (halo 2)
+constexpr int kThreads = 512;
+constexpr int kPX      = 64;       // output pairs per row (128 cells)
+constexpr int kTY      = 16;       // output rows
+constexpr int kTX      = 2 * kPX;  // output cells per row
+constexpr int kOPX     = kPX + 4;  // RBGS old region: pairs x0-4 .. x0+131, rows y0-2 .. y0+17
 constexpr int kOY      = kTY + 4;
-constexpr int kHX      = kTX + 2; // half-updated iterate and b region (halo 1)
+constexpr int kHPX     = kPX + 2;  // H region (and Jacobi old region): pairs x0-2 .. x0+129, rows y0-1 .. y0+16
 constexpr int kHY      = kTY + 2;
-constexpr int kOldN    = kOX * kOY;
-constexpr int kHN      = kHX * kHY;
-constexpr int kOldPer  = (kOldN + kThreads - 1) / kThreads;
-constexpr int kHPer    = (kHN + kThreads - 1) / kThreads;
-constexpr int kOutPer  = kTX * kTY / kThreads;
-static_assert(kTX == 64 && kTX * kTY % kThreads == 0, "one wavefront per output row");
+constexpr int kOldF    = 2 * kOPX * kOY; // floats
+constexpr int kHP      = kHPX * kHY;     // pairs
+constexpr int kJOldF   = 2 * kHPX * kHY; // floats
+constexpr int kOldPer  = (kOldF + kThreads - 1) / kThreads;
+constexpr int kHPer    = (kHP + kThreads - 1) / kThreads;
+constexpr int kJOldPer = (kJOldF + kThreads - 1) / kThreads;
+constexpr int kOutPer  = kPX * kTY / kThreads;
+static_assert(kPX * kTY % kThreads == 0, "whole output rows per lane group");
 
 struct TileGeom {
     int tiles_x, tiles_y, chunks, zc;
@@ -55,6 +65,21 @@ __device__ __forceinline__ float relax_value(const float bv, const float sum, co
         return (bv + coef * sum) / (1.0f + 6.0f * coef);
 }
 
+// Relax element j of the pair C, whose row neighbours are the pairs L and R, from the
+// six neighbours in the reference's order x-, x+, y-, y+, z-, z+.
+template <int kSolve>
+__device__ __forceinline__ float relax_elem(const int j, const float2 L, const float2 C, const float2 R, const float2 YM, const float2 YP, const float2 ZM, const float2 ZP, const float bv, const float coef) {
+    const float xm = j ? C.x : L.y;
+    const float xp = j ? R.x : C.y;
+    float sum      = xm;
+    sum            = sum + xp;
+    sum            = sum + (j ? YM.y : YM.x);
+    sum            = sum + (j ? YP.y : YP.x);
+    sum            = sum + (j ? ZM.y : ZM.x);
+    sum            = sum + (j ? ZP.y : ZP.x);
+    return relax_value<kSolve>(bv, sum, coef);
+}
+
 // Workgroup -> (tile x, tile y, z chunk). Workgroups are dispatched round-robin over the
 // 8 XCDs; renumbering gives each XCD (and its L2) a contiguous run of chunks and tiles,
 // so z-halo planes and x-y halo columns are mostly shared within one L2.
@@ -70,11 +95,40 @@ __device__ __forceinline__ void tile_of(const TileGeom g, int& tx, int& ty, int&
 
 __device__ __forceinline__ int mod3(const int k) { return ((k % 3) + 3) % 3; }
 
+// Ghost code of a halo position: 0 inside (or periodic), else 1 + face index.
+__device__ __forceinline__ int ghost_code(const int gx, const int gy, const Dims d, const bool px, const bool py) {
+    if (!px && gx < 0) return 1;
+    if (!px && gx >= d.nx) return 2;
+    if (!py && gy < 0) return 3;
+    if (!py && gy >= d.ny) return 4;
+    return 0;
+}
+
+// Loads plane k of a row-major region (`per` floats per lane) into registers, applying
+// the ghost rules: in-grid planes take the x/y face rule of ghost positions, out-of-grid
+// planes (non-periodic z) take the z face rule of the adjacent edge plane.
+template <int kPer, bool kZero>
+__device__ __forceinline__ void fetch_region(float (&r)[kPer], const float* __restrict__ src, const int (&off)[kPer], const int (&gc)[kPer], const int k, const Dims d, const bool pz, const uint64_t plane, const StencilBC& bc) {
+    const bool below = !pz && k < 0, above = !pz && k >= d.nz;
+    const uint64_t base = static_cast<uint64_t>(fix_coord(k, d.nz, pz)) * plane;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        float v = 0.0f;
+        if (!kZero && off[i] >= 0) v = src[base + static_cast<uint64_t>(off[i])];
+        if (below)
+            v = apply_ghost(bc.face[4], v);
+        else if (above)
+            v = apply_ghost(bc.face[5], v);
+        else if (gc[i])
+            v = apply_ghost(bc.face[gc[i] - 1], v);
+        r[i] = v;
+    }
+}
+
 template <int kSolve, bool kZero>
 __global__ __launch_bounds__(kThreads) void k_rbgs_pass(const float* __restrict__ x_old, const float* __restrict__ b, float* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g) {
-    __shared__ float s_old[3][kOldN];
-    __shared__ float s_h[3][kHN];
-    __shared__ float s_b[3][kHN];
+    __shared__ float2 s_old[3][kOPX * kOY];
+    __shared__ float2 s_h[3][kHPX * kHY];
 
     int tx, ty, cz;
     tile_of(g, tx, ty, cz);
@@ -85,179 +139,175 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_pass(const float* __restrict_
     const bool px  = bc.periodic & 1, py = (bc.periodic >> 1) & 1, pz = (bc.periodic >> 2) & 1;
     const uint64_t plane = static_cast<uint64_t>(d.nx) * static_cast<uint64_t>(d.ny);
 
-    // in-plane global offsets of this thread's load positions
-    int old_off[kOldPer];
+    // old region load slots (floats, row-major 2*kOPX wide)
+    int o_off[kOldPer], o_gc[kOldPer];
 #pragma unroll
     for (int i = 0; i < kOldPer; ++i) {
         const int e = tid + i * kThreads;
-        old_off[i]  = -1;
-        if (e < kOldN) {
-            const int gx = fix_coord(x0 - 2 + e % kOX, d.nx, px);
-            const int gy = fix_coord(y0 - 2 + e / kOX, d.ny, py);
-            old_off[i]   = gy * d.nx + gx;
+        o_off[i]    = -1;
+        o_gc[i]     = 0;
+        if (e < kOldF) {
+            const int gx = x0 - 4 + e % (2 * kOPX), gy = y0 - 2 + e / (2 * kOPX);
+            o_off[i]     = fix_coord(gy, d.ny, py) * d.nx + fix_coord(gx, d.nx, px);
+            o_gc[i]      = ghost_code(gx, gy, d, px, py);
         }
     }
-    // H-stage positions: b offset, validity, parity of (x+y), ghost faces in x/y
-    int h_off[kHPer];
-    int h_info[kHPer]; // bit 0..3: ghost x-,x+,y-,y+; bit 4: parity(x+y); bit 5: valid
+    // H pair slots: b offsets of both elements, parity and reality bits
+    int h_off[kHPer], h_offb[kHPer], h_info[kHPer]; // info: bit0 real a, bit1 real b, bit2 parity a, bit3 parity b, bit4 slot used
 #pragma unroll
     for (int i = 0; i < kHPer; ++i) {
         const int e = tid + i * kThreads;
-        h_off[i]    = -1;
+        h_off[i]    = 0;
+        h_offb[i]   = 0;
         h_info[i]   = 0;
-        if (e < kHN) {
-            const int gx     = x0 - 1 + e % kHX;
-            const int gy     = y0 - 1 + e / kHX;
-            const bool valid = (px || (gx >= 0 && gx < d.nx)) && (py || (gy >= 0 && gy < d.ny));
-            const int wx     = fix_coord(gx, d.nx, px);
-            const int wy     = fix_coord(gy, d.ny, py);
-            h_off[i]         = wy * d.nx + wx;
-            int info         = 0;
-            if (!px && gx - 1 < 0) info |= 1;
-            if (!px && gx + 1 >= d.nx) info |= 2;
-            if (!py && gy - 1 < 0) info |= 4;
-            if (!py && gy + 1 >= d.ny) info |= 8;
-            info |= ((wx + wy) & 1) << 4;
-            if (valid) info |= 32;
-            h_info[i] = info;
+        if (e < kHP) {
+            const int xa = x0 - 2 + 2 * (e % kHPX), y = y0 - 1 + e / kHPX;
+            const bool ry  = py || (y >= 0 && y < d.ny);
+            const bool ra  = ry && (px || (xa >= 0 && xa < d.nx));
+            const bool rb  = ry && (px || (xa + 1 >= 0 && xa + 1 < d.nx));
+            const int wy   = fix_coord(y, d.ny, py);
+            const int wa   = fix_coord(xa, d.nx, px), wb = fix_coord(xa + 1, d.nx, px);
+            h_off[i]       = wy * d.nx + wa;
+            h_offb[i]      = wy * d.nx + wb;
+            h_info[i]      = (ra ? 1 : 0) | (rb ? 2 : 0) | (((wa + wy) & 1) << 2) | (((wb + wy) & 1) << 3) | 16;
         }
     }
-    // output positions: one row of 64 per wavefront, rows tid/64 + 4*i
-    const int fx = tid & (kTX - 1);
-    int f_info[kOutPer]; // ghost bits, parity(x+y) in bit 4, in-grid in bit 5
+    // output pair slots: one row of 64 pairs per wavefront
+    const int fp = tid & (kPX - 1);
+    int f_info[kOutPer]; // bit0 in-grid a, bit1 in-grid b, bit2 parity(xa+y)
 #pragma unroll
     for (int i = 0; i < kOutPer; ++i) {
-        const int fy = (tid >> 6) + 4 * i;
-        const int gx = x0 + fx, gy = y0 + fy;
-        int info     = 0;
-        if (!px && gx - 1 < 0) info |= 1;
-        if (!px && gx + 1 >= d.nx) info |= 2;
-        if (!py && gy - 1 < 0) info |= 4;
-        if (!py && gy + 1 >= d.ny) info |= 8;
-        info |= ((gx + gy) & 1) << 4;
-        if (gx < d.nx && gy < d.ny) info |= 32;
-        f_info[i] = info;
+        const int fy = (tid >> 6) + (kThreads / 64) * i;
+        const int xa = x0 + 2 * fp, y = y0 + fy;
+        f_info[i]    = (xa < d.nx && y < d.ny ? 1 : 0) | (xa + 1 < d.nx && y < d.ny ? 2 : 0) | (((xa + y) & 1) << 2);
     }
 
-    auto plane_valid = [&](const int k) { return pz || (k >= 0 && k < d.nz); };
-    auto plane_base  = [&](const int k) { return static_cast<uint64_t>(fix_coord(k, d.nz, pz)) * plane; };
+    auto real_plane = [&](const int k) { return pz || (k >= 0 && k < d.nz); };
 
-    float r_old[kOldPer], r_b[kHPer];
-    auto fetch = [&](const int k_old, const int k_b) {
-        if (!kZero && plane_valid(k_old)) {
-            const uint64_t base = plane_base(k_old);
+    // b of the red element of each H slot at plane k, and of the black element of each
+    // output slot at plane k (loaded one iteration ahead)
+    float r_old[kOldPer], r_bh[kHPer], r_bf[kOutPer];
+    auto fetch_bh = [&](const int k) {
+        if (!real_plane(k)) return;
+        const int wz        = fix_coord(k, d.nz, pz);
+        const uint64_t base = static_cast<uint64_t>(wz) * plane;
 #pragma unroll
-            for (int i = 0; i < kOldPer; ++i) r_old[i] = old_off[i] >= 0 ? x_old[base + old_off[i]] : 0.0f;
-        } else {
-#pragma unroll
-            for (int i = 0; i < kOldPer; ++i) r_old[i] = 0.0f;
+        for (int i = 0; i < kHPer; ++i) {
+            const int info   = h_info[i];
+            const bool red_a = (((info >> 2) + wz) & 1) == 0;
+            r_bh[i]          = (info & 16) ? b[base + static_cast<uint64_t>(red_a ? h_off[i] : h_offb[i])] : 0.0f;
         }
-        if (plane_valid(k_b)) {
-            const uint64_t base = plane_base(k_b);
+    };
+    auto fetch_bf = [&](const int k) {
+        if (k < 0 || k >= d.nz) return;
+        const uint64_t base = static_cast<uint64_t>(k) * plane;
 #pragma unroll
-            for (int i = 0; i < kHPer; ++i) r_b[i] = h_off[i] >= 0 ? b[base + h_off[i]] : 0.0f;
+        for (int i = 0; i < kOutPer; ++i) {
+            const int fy   = (tid >> 6) + (kThreads / 64) * i;
+            const int info = f_info[i];
+            const int j    = (((info >> 2) + k) & 1) ? 0 : 1; // black element
+            if (info & (1 << j)) r_bf[i] = b[base + static_cast<uint64_t>((y0 + fy) * d.nx + x0 + 2 * fp + j)];
         }
     };
     auto stash_old = [&](const int k) {
-        float* s = s_old[mod3(k)];
+        float* s = reinterpret_cast<float*>(s_old[mod3(k)]);
 #pragma unroll
         for (int i = 0; i < kOldPer; ++i)
-            if (old_off[i] >= 0) s[tid + i * kThreads] = r_old[i];
-    };
-    auto stash_b = [&](const int k) {
-        float* s = s_b[mod3(k)];
-#pragma unroll
-        for (int i = 0; i < kHPer; ++i)
-            if (h_off[i] >= 0) s[tid + i * kThreads] = r_b[i];
+            if (tid + i * kThreads < kOldF) s[tid + i * kThreads] = r_old[i];
     };
 
-    // prologue: old planes z0-2 and z0-1 in LDS, old z0 and b z0-1 in registers
-    fetch(z0 - 2, z0 - 2);
+    // prologue: old planes z0-2, z0-1 in LDS; old z0, b for H(z0-1) in registers
+    fetch_region<kOldPer, kZero>(r_old, x_old, o_off, o_gc, z0 - 2, d, pz, plane, bc);
     stash_old(z0 - 2);
-    fetch(z0 - 1, z0 - 1);
+    fetch_region<kOldPer, kZero>(r_old, x_old, o_off, o_gc, z0 - 1, d, pz, plane, bc);
     stash_old(z0 - 1);
-    fetch(z0, z0 - 1);
+    fetch_region<kOldPer, kZero>(r_old, x_old, o_off, o_gc, z0, d, pz, plane, bc);
+    fetch_bh(z0 - 1);
 
     for (int k = z0 - 1; k <= z1; ++k) {
-        stash_old(k + 1); // registers hold old(k+1) and b(k)
-        stash_b(k);
-        if (k + 1 <= z1) fetch(k + 2, k + 1);
+        stash_old(k + 1);
+        float bh[kHPer], bf[kOutPer];
+#pragma unroll
+        for (int i = 0; i < kHPer; ++i) bh[i] = r_bh[i];
+#pragma unroll
+        for (int i = 0; i < kOutPer; ++i) bf[i] = r_bf[i];
+        if (k + 1 <= z1) {
+            fetch_region<kOldPer, kZero>(r_old, x_old, o_off, o_gc, k + 2, d, pz, plane, bc);
+            fetch_bh(k + 1);
+        }
+        if (k >= z0 - 1 && k < z1) fetch_bf(k); // black b of plane k, used by final(k) next iteration
         __syncthreads();
 
-        // H(k) on tile + 1 halo
-        if (plane_valid(k)) {
-            const float* so = s_old[mod3(k)];
-            const float* sm = s_old[mod3(k - 1)];
-            const float* sp = s_old[mod3(k + 1)];
-            const float* sb = s_b[mod3(k)];
-            float* sh       = s_h[mod3(k)];
-            const int wz    = fix_coord(k, d.nz, pz);
-            const bool zlo  = !pz && k - 1 < 0;
-            const bool zhi  = !pz && k + 1 >= d.nz;
+        // H(k) on the tile plus a one-pair halo
+        {
+            const float2* so = s_old[mod3(k)];
+            const float2* sm = s_old[mod3(k - 1)];
+            const float2* sp = s_old[mod3(k + 1)];
+            float2* sh       = s_h[mod3(k)];
+            const bool real  = real_plane(k);
+            const int wz     = fix_coord(k, d.nz, pz);
 #pragma unroll
             for (int i = 0; i < kHPer; ++i) {
-                const int e    = tid + i * kThreads;
+                const int e = tid + i * kThreads;
+                if (e >= kHP) continue;
+                const int c    = (e / kHPX + 1) * kOPX + (e % kHPX + 1);
+                const float2 C = so[c];
+                float2 out     = C;
                 const int info = h_info[i];
-                if (e >= kHN || !(info & 32)) continue;
-                const int c        = (e / kHX + 1) * kOX + (e % kHX + 1);
-                const float centre = so[c];
-                float out          = centre;
-                if ((((info >> 4) + wz) & 1) == 0) {
-                    float sum = (info & 1) ? apply_ghost(bc.face[0], centre) : so[c - 1];
-                    sum       = sum + ((info & 2) ? apply_ghost(bc.face[1], centre) : so[c + 1]);
-                    sum       = sum + ((info & 4) ? apply_ghost(bc.face[2], centre) : so[c - kOX]);
-                    sum       = sum + ((info & 8) ? apply_ghost(bc.face[3], centre) : so[c + kOX]);
-                    sum       = sum + (zlo ? apply_ghost(bc.face[4], centre) : sm[c]);
-                    sum       = sum + (zhi ? apply_ghost(bc.face[5], centre) : sp[c]);
-                    out       = relax_value<kSolve>(sb[e], sum, coef);
+                if (real) {
+                    const bool red_a = (info & 1) && ((((info >> 2) + wz) & 1) == 0);
+                    const bool red_b = (info & 2) && ((((info >> 3) + wz) & 1) == 0);
+                    if (red_a || red_b) {
+                        const float2 L = so[c - 1], R = so[c + 1], YM = so[c - kOPX], YP = so[c + kOPX];
+                        const float2 ZM = sm[c], ZP = sp[c];
+                        const int j     = red_a ? 0 : 1;
+                        const float v   = relax_elem<kSolve>(j, L, C, R, YM, YP, ZM, ZP, bh[i], coef);
+                        if (j == 0) out.x = v;
+                        else out.y = v;
+                        if (red_a && red_b) { // odd periodic wrap: both cells red
+                            const float bb = b[static_cast<uint64_t>(wz) * plane + static_cast<uint64_t>(h_offb[i])];
+                            out.y          = relax_elem<kSolve>(1, L, C, R, YM, YP, ZM, ZP, bb, coef);
+                        }
+                    }
                 }
                 sh[e] = out;
             }
         }
         __syncthreads();
 
-        // final plane k-1 on the tile
+        // final plane k-1 on the tile: black cells from H
         const int kf = k - 1;
         if (kf >= z0 && kf < z1) {
-            const float* hm = s_h[mod3(kf - 1)];
-            const float* h0 = s_h[mod3(kf)];
-            const float* hp = s_h[mod3(kf + 1)];
-            const float* sb = s_b[mod3(kf)];
-            const bool zlo  = !pz && kf - 1 < 0;
-            const bool zhi  = !pz && kf + 1 >= d.nz;
+            const float2* hm = s_h[mod3(kf - 1)];
+            const float2* h0 = s_h[mod3(kf)];
+            const float2* hp = s_h[mod3(kf + 1)];
             const uint64_t base = static_cast<uint64_t>(kf) * plane;
 #pragma unroll
             for (int i = 0; i < kOutPer; ++i) {
                 const int info = f_info[i];
-                if (!(info & 32)) continue;
-                const int fy       = (tid >> 6) + 4 * i;
-                const int c        = (fy + 1) * kHX + (fx + 1);
-                const float centre = h0[c];
-                float out          = centre;
-                if ((((info >> 4) + kf) & 1) == 1) {
-                    float sum = (info & 1) ? apply_ghost(bc.face[0], centre) : h0[c - 1];
-                    sum       = sum + ((info & 2) ? apply_ghost(bc.face[1], centre) : h0[c + 1]);
-                    sum       = sum + ((info & 4) ? apply_ghost(bc.face[2], centre) : h0[c - kHX]);
-                    sum       = sum + ((info & 8) ? apply_ghost(bc.face[3], centre) : h0[c + kHX]);
-                    sum       = sum + (zlo ? apply_ghost(bc.face[4], centre) : hm[c]);
-                    sum       = sum + (zhi ? apply_ghost(bc.face[5], centre) : hp[c]);
-                    out       = relax_value<kSolve>(sb[c], sum, coef);
+                if (!(info & 1)) continue;
+                const int fy   = (tid >> 6) + (kThreads / 64) * i;
+                const int c    = (fy + 1) * kHPX + (fp + 1);
+                const float2 C = h0[c];
+                float2 out     = C;
+                const int j    = (((info >> 2) + kf) & 1) ? 0 : 1;
+                if (info & (1 << j)) {
+                    const float v = relax_elem<kSolve>(j, h0[c - 1], C, h0[c + 1], h0[c - kHPX], h0[c + kHPX], hm[c], hp[c], bf[i], coef);
+                    if (j == 0) out.x = v;
+                    else out.y = v;
                 }
-                x_new[base + static_cast<uint64_t>(y0 + fy) * d.nx + (x0 + fx)] = out * out_scale;
+                float* dst = x_new + base + static_cast<uint64_t>((y0 + fy) * d.nx + x0 + 2 * fp);
+                dst[0]     = out.x * out_scale;
+                if (info & 2) dst[1] = out.y * out_scale;
             }
         }
         __syncthreads();
     }
 }
 
-// Jacobi: every cell from the old iterate; old planes carry a 1-cell halo, b is read
-// at the centre only.
-constexpr int kJOldN   = kHX * kHY;
-constexpr int kJOldPer = (kJOldN + kThreads - 1) / kThreads;
-
 template <int kSolve, bool kZero>
 __global__ __launch_bounds__(kThreads) void k_jacobi_pass(const float* __restrict__ x_old, const float* __restrict__ b, float* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g) {
-    __shared__ float s_old[3][kJOldN];
+    __shared__ float2 s_old[3][kHPX * kHY];
 
     int tx, ty, cz;
     tile_of(g, tx, ty, cz);
@@ -268,97 +318,76 @@ __global__ __launch_bounds__(kThreads) void k_jacobi_pass(const float* __restric
     const bool px  = bc.periodic & 1, py = (bc.periodic >> 1) & 1, pz = (bc.periodic >> 2) & 1;
     const uint64_t plane = static_cast<uint64_t>(d.nx) * static_cast<uint64_t>(d.ny);
 
-    int old_off[kJOldPer];
+    int o_off[kJOldPer], o_gc[kJOldPer];
 #pragma unroll
     for (int i = 0; i < kJOldPer; ++i) {
         const int e = tid + i * kThreads;
-        old_off[i]  = -1;
-        if (e < kJOldN) {
-            const int gx = fix_coord(x0 - 1 + e % kHX, d.nx, px);
-            const int gy = fix_coord(y0 - 1 + e / kHX, d.ny, py);
-            old_off[i]   = gy * d.nx + gx;
+        o_off[i]    = -1;
+        o_gc[i]     = 0;
+        if (e < kJOldF) {
+            const int gx = x0 - 2 + e % (2 * kHPX), gy = y0 - 1 + e / (2 * kHPX);
+            o_off[i]     = fix_coord(gy, d.ny, py) * d.nx + fix_coord(gx, d.nx, px);
+            o_gc[i]      = ghost_code(gx, gy, d, px, py);
         }
     }
-    const int fx = tid & (kTX - 1);
+    const int fp = tid & (kPX - 1);
     int f_info[kOutPer];
-    int f_off[kOutPer];
 #pragma unroll
     for (int i = 0; i < kOutPer; ++i) {
-        const int fy = (tid >> 6) + 4 * i;
-        const int gx = x0 + fx, gy = y0 + fy;
-        int info     = 0;
-        if (!px && gx - 1 < 0) info |= 1;
-        if (!px && gx + 1 >= d.nx) info |= 2;
-        if (!py && gy - 1 < 0) info |= 4;
-        if (!py && gy + 1 >= d.ny) info |= 8;
-        if (gx < d.nx && gy < d.ny) info |= 32;
-        f_info[i] = info;
-        f_off[i]  = (info & 32) ? gy * d.nx + gx : 0;
+        const int fy = (tid >> 6) + (kThreads / 64) * i;
+        const int xa = x0 + 2 * fp, y = y0 + fy;
+        f_info[i]    = (xa < d.nx && y < d.ny ? 1 : 0) | (xa + 1 < d.nx && y < d.ny ? 2 : 0);
     }
-    auto plane_valid = [&](const int k) { return pz || (k >= 0 && k < d.nz); };
-    auto plane_base  = [&](const int k) { return static_cast<uint64_t>(fix_coord(k, d.nz, pz)) * plane; };
-
-    float r_old[kJOldPer], r_b[kOutPer];
-    auto fetch_old = [&](const int k) {
-        if (!kZero && plane_valid(k)) {
-            const uint64_t base = plane_base(k);
-#pragma unroll
-            for (int i = 0; i < kJOldPer; ++i) r_old[i] = old_off[i] >= 0 ? x_old[base + old_off[i]] : 0.0f;
-        } else {
-#pragma unroll
-            for (int i = 0; i < kJOldPer; ++i) r_old[i] = 0.0f;
-        }
-    };
+    float r_old[kJOldPer];
+    float2 r_b[kOutPer];
     auto fetch_b = [&](const int k) {
-        if (k >= 0 && k < d.nz) {
-            const uint64_t base = static_cast<uint64_t>(k) * plane;
+        if (k < 0 || k >= d.nz) return;
+        const uint64_t base = static_cast<uint64_t>(k) * plane;
 #pragma unroll
-            for (int i = 0; i < kOutPer; ++i) r_b[i] = b[base + f_off[i]];
+        for (int i = 0; i < kOutPer; ++i) {
+            const int fy  = (tid >> 6) + (kThreads / 64) * i;
+            const float* p = b + base + static_cast<uint64_t>((y0 + fy) * d.nx + x0 + 2 * fp);
+            if (f_info[i] & 1) r_b[i].x = p[0];
+            if (f_info[i] & 2) r_b[i].y = p[1];
         }
     };
     auto stash_old = [&](const int k) {
-        float* s = s_old[mod3(k)];
+        float* s = reinterpret_cast<float*>(s_old[mod3(k)]);
 #pragma unroll
         for (int i = 0; i < kJOldPer; ++i)
-            if (old_off[i] >= 0) s[tid + i * kThreads] = r_old[i];
+            if (tid + i * kThreads < kJOldF) s[tid + i * kThreads] = r_old[i];
     };
 
-    fetch_old(z0 - 1);
+    fetch_region<kJOldPer, kZero>(r_old, x_old, o_off, o_gc, z0 - 1, d, pz, plane, bc);
     stash_old(z0 - 1);
-    fetch_old(z0);
+    fetch_region<kJOldPer, kZero>(r_old, x_old, o_off, o_gc, z0, d, pz, plane, bc);
     stash_old(z0);
-    fetch_old(z0 + 1);
+    fetch_region<kJOldPer, kZero>(r_old, x_old, o_off, o_gc, z0 + 1, d, pz, plane, bc);
     fetch_b(z0);
     for (int k = z0; k < z1; ++k) {
         stash_old(k + 1);
-        float bv[kOutPer];
+        float2 bv[kOutPer];
 #pragma unroll
         for (int i = 0; i < kOutPer; ++i) bv[i] = r_b[i];
         if (k + 1 < z1) {
-            fetch_old(k + 2);
+            fetch_region<kJOldPer, kZero>(r_old, x_old, o_off, o_gc, k + 2, d, pz, plane, bc);
             fetch_b(k + 1);
         }
         __syncthreads();
-        const float* so = s_old[mod3(k)];
-        const float* sm = s_old[mod3(k - 1)];
-        const float* sp = s_old[mod3(k + 1)];
-        const bool zlo  = !pz && k - 1 < 0;
-        const bool zhi  = !pz && k + 1 >= d.nz;
+        const float2* so = s_old[mod3(k)];
+        const float2* sm = s_old[mod3(k - 1)];
+        const float2* sp = s_old[mod3(k + 1)];
         const uint64_t base = static_cast<uint64_t>(k) * plane;
 #pragma unroll
         for (int i = 0; i < kOutPer; ++i) {
             const int info = f_info[i];
-            if (!(info & 32)) continue;
-            const int fy       = (tid >> 6) + 4 * i;
-            const int c        = (fy + 1) * kHX + (fx + 1);
-            const float centre = so[c];
-            float sum          = (info & 1) ? apply_ghost(bc.face[0], centre) : so[c - 1];
-            sum                = sum + ((info & 2) ? apply_ghost(bc.face[1], centre) : so[c + 1]);
-            sum                = sum + ((info & 4) ? apply_ghost(bc.face[2], centre) : so[c - kHX]);
-            sum                = sum + ((info & 8) ? apply_ghost(bc.face[3], centre) : so[c + kHX]);
-            sum                = sum + (zlo ? apply_ghost(bc.face[4], centre) : sm[c]);
-            sum                = sum + (zhi ? apply_ghost(bc.face[5], centre) : sp[c]);
-            x_new[base + f_off[i]] = relax_value<kSolve>(bv[i], sum, coef) * out_scale;
+            if (!(info & 1)) continue;
+            const int fy   = (tid >> 6) + (kThreads / 64) * i;
+            const int c    = (fy + 1) * kHPX + (fp + 1);
+            const float2 L = so[c - 1], C = so[c], R = so[c + 1], YM = so[c - kHPX], YP = so[c + kHPX], ZM = sm[c], ZP = sp[c];
+            float* dst     = x_new + base + static_cast<uint64_t>((y0 + fy) * d.nx + x0 + 2 * fp);
+            dst[0]         = relax_elem<kSolve>(0, L, C, R, YM, YP, ZM, ZP, bv[i].x, coef) * out_scale;
+            if (info & 2) dst[1] = relax_elem<kSolve>(1, L, C, R, YM, YP, ZM, ZP, bv[i].y, coef) * out_scale;
         }
         __syncthreads();
     }
@@ -366,11 +395,11 @@ __global__ __launch_bounds__(kThreads) void k_jacobi_pass(const float* __restric
 
 TileGeom plan_tiles(const Dims d) {
     TileGeom g{};
-    g.tiles_x   = (d.nx + kTX - 1) / kTX;
-    g.tiles_y   = (d.ny + kTY - 1) / kTY;
+    g.tiles_x        = (d.nx + kTX - 1) / kTX;
+    g.tiles_y        = (d.ny + kTY - 1) / kTY;
     const long tiles = static_cast<long>(g.tiles_x) * g.tiles_y;
-    // about 1024 workgroups (4 per CU), at least 4 planes per chunk
-    long zc = (static_cast<long>(d.nz) * tiles + 1023) / 1024;
+    // about 512 workgroups (two 512-thread groups per CU), at least 4 planes per chunk
+    long zc = (static_cast<long>(d.nz) * tiles + 511) / 512;
     if (zc < 4) zc = 4;
     if (zc > 64) zc = 64;
     if (zc > d.nz) zc = d.nz;

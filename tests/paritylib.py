@@ -38,13 +38,15 @@ def normwise(a, b):
 class Inputs:
     """Seeded host inputs for every field the ABI can set."""
 
-    def __init__(self, desc, seed, velocity_scale=0.0, source_scale=1.0, force_scale=1.0):
+    def __init__(self, desc, seed, velocity_scale=0.0, source_scale=1.0, force_scale=1.0, smooth=False):
+        """White noise per cell, or (smooth=True) the same noise after a 5^3 box filter."""
         rng = np.random.default_rng(seed)
         shape = desc.shape
-        self.sources = [(rng.standard_normal(shape) * source_scale).astype(np.float32) for _ in range(desc.n_scalars)]
-        self.densities = [(rng.random(shape)).astype(np.float32) for _ in range(desc.n_scalars)]
-        self.forces = [tuple((rng.standard_normal(shape) * force_scale).astype(np.float32) for _ in range(3)) for _ in range(desc.n_vectors)]
-        self.velocity = tuple((rng.standard_normal(shape) * velocity_scale).astype(np.float32) for _ in range(3)) if velocity_scale else None
+        f = (lambda a: sf.scenes.box_filter5(a) * 3.0) if smooth else (lambda a: a)
+        self.sources = [f(rng.standard_normal(shape) * source_scale).astype(np.float32) for _ in range(desc.n_scalars)]
+        self.densities = [f(rng.random(shape)).astype(np.float32) for _ in range(desc.n_scalars)]
+        self.forces = [tuple(f(rng.standard_normal(shape) * force_scale).astype(np.float32) for _ in range(3)) for _ in range(desc.n_vectors)]
+        self.velocity = tuple(f(rng.standard_normal(shape) * velocity_scale).astype(np.float32) for _ in range(3)) if velocity_scale else None
 
     def seed_gpu(self, ctx):
         for i, (s, d) in enumerate(zip(self.sources, self.densities)):
