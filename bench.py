@@ -97,7 +97,7 @@ def main():
         return ev1.elapsed_ms(ev0) / repeat
 
     stages = {}
-    for name, stage in (("velocity_pass", 1), ("pressure_pass", 0), ("jacobi_pass", 4), ("advect", 2), ("project", 3)):
+    for name, stage in (("velocity_pass", 1), ("pressure_pass", 0), ("jacobi_pass", 4), ("advect", 2), ("project", 3), ("copy", 5)):
         ms = time_stage(stage, args.stage_repeat)
         by = ctx.stage_bytes(stage)
         stages[name] = {"us": round(ms * 1e3, 3), "bytes": by, "gbs": round(by / (ms * 1e-3) / 1e9, 1)}

@@ -69,6 +69,7 @@ typedef enum StableFluidsExtStage {
     STABLE_FLUIDS_EXT_STAGE_ADVECT         = 2, /* velocity self-advection */
     STABLE_FLUIDS_EXT_STAGE_PROJECT        = 3, /* projection + divergence + |v| */
     STABLE_FLUIDS_EXT_STAGE_JACOBI_PASS    = 4, /* one Jacobi pass of the pressure solve, whatever the smoother option */
+    STABLE_FLUIDS_EXT_STAGE_COPY           = 5, /* plain copy of one field (8 B/cell), to calibrate byte counters */
 } StableFluidsExtStage;
 
 /* Enqueues `repeat` launches of one stage kernel on the context stream, outside the step

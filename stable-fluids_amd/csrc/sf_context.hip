@@ -576,6 +576,7 @@ STABLE_FLUIDS_API uint64_t stable_fluids_ext_stage_bytes_cuda(void* context, uin
     case STABLE_FLUIDS_EXT_STAGE_JACOBI_PASS: return 12ull * c.cells; // read x_old, b; write x_new
     case STABLE_FLUIDS_EXT_STAGE_ADVECT: return 24ull * c.cells;        // read v (3); write t (3)
     case STABLE_FLUIDS_EXT_STAGE_PROJECT: return 36ull * c.cells;       // read a (3), p; write v (3), div, |v|
+    case STABLE_FLUIDS_EXT_STAGE_COPY: return 8ull * c.cells;           // read one field, write one
     default: return 0;
     }
 }
@@ -606,6 +607,7 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_run_stage_cuda(void* cont
             e = sf::launch_smooth(c.stream, sf::kSmoothJacobi, sf::kSolvePoisson, sf::SmoothPass{odd ? f.p : f.p_aux, f.div, odd ? f.p_aux : f.p, h * h, 1.0f}, c.dims, sf::pressure_rules(fb));
             break;
         }
+        case STABLE_FLUIDS_EXT_STAGE_COPY: e = sf::launch_copy_scaled(c.stream, f.ax, f.vx, 1.0f, c.cells); break;
         case STABLE_FLUIDS_EXT_STAGE_ADVECT: e = sf::launch_advect_velocity(c.stream, f.tx, f.ty, f.tz, f.vx, f.vy, f.vz, c.config.dt, c.dims, h, fb); break;
         case STABLE_FLUIDS_EXT_STAGE_PROJECT: e = sf::launch_project(c.stream, f.vx, f.vy, f.vz, f.div, f.mag, f.ax, f.ay, f.az, f.p, c.dims, h, sf::pressure_rules(fb), sf::divergence_rules(fb)); break;
         default: return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;

@@ -119,7 +119,7 @@ def test_norms_match_host_reduction():
 def test_stage_timing_hooks():
     d = small_desc(32, 16, 8)
     with sf.Context(d) as ctx:
-        for stage, per_cell in ((0, 12), (1, 12), (2, 24), (3, 36), (4, 12)):
+        for stage, per_cell in ((0, 12), (1, 12), (2, 24), (3, 36), (4, 12), (5, 8)):
             assert ctx.stage_bytes(stage) == per_cell * d.cells
             ctx.run_stage(stage, 3)
         ctx.synchronize()
