@@ -393,6 +393,226 @@ __global__ __launch_bounds__(kThreads) void k_jacobi_pass(const float* __restric
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// RBGS fast path (no periodic axis of odd length). LDS planes are split into even-x and
+// odd-x arrays, so the colour of an array alternates by row: the black stage of a row is
+// wave-uniform (one row per wavefront) and every stencil read is a contiguous 32-bit LDS
+// access. H planes hold only the red value of each pair. With even periodic lengths the
+// unwrapped parity equals the wrapped one.
+// ---------------------------------------------------------------------------------------
+constexpr int kW        = kPX + 4;     // pairs per LDS row, x from xs = x0 - 4
+constexpr int kR        = kTY + 4;     // rows, y from ys = y0 - 2
+constexpr int kArr      = kR * kW;     // floats per even/odd array of a plane
+constexpr int kOldPlane = 2 * kArr;
+constexpr int kHPlane   = kArr;
+constexpr int kSlots3   = (kR * kW + kThreads - 1) / kThreads;        // old pair slots
+constexpr int kHVals    = (kTY + 2) * (kPX + 2);                      // red values per H plane
+constexpr int kHSlots3  = (kHVals + kThreads - 1) / kThreads;
+
+template <int kSolve, bool kZero>
+__global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const float* __restrict__ x_old, const float* __restrict__ b, float* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g) {
+    __shared__ float s_old[3][kOldPlane];
+    __shared__ float s_h[3][kHPlane];
+
+    int tx, ty, cz;
+    tile_of(g, tx, ty, cz);
+    const int tid  = static_cast<int>(threadIdx.x);
+    const int x0   = tx * kTX, y0 = ty * kTY;
+    const int xs   = x0 - 4, ys = y0 - 2;
+    const int z0   = cz * g.zc;
+    const int z1   = min(z0 + g.zc, d.nz);
+    const bool px  = bc.periodic & 1, py = (bc.periodic >> 1) & 1, pz = (bc.periodic >> 2) & 1;
+    const uint64_t plane = static_cast<uint64_t>(d.nx) * static_cast<uint64_t>(d.ny);
+
+    // old pair slots: offsets and ghost codes of the even (e=0) and odd (e=1) cell
+    uint32_t o_off[kSlots3][2];
+    int o_gc[kSlots3][2];
+    bool any_gc = false;
+#pragma unroll
+    for (int j = 0; j < kSlots3; ++j) {
+        const int p = tid + j * kThreads;
+        const int r = p / kW, i = p % kW;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int gx = xs + 2 * i + e, gy = ys + r;
+            o_off[j][e]  = p < kR * kW ? static_cast<uint32_t>(fix_coord(gy, d.ny, py) * d.nx + fix_coord(gx, d.nx, px)) : 0u;
+            o_gc[j][e]   = p < kR * kW ? ghost_code(gx, gy, d, px, py) : 0;
+            any_gc       = any_gc || o_gc[j][e] != 0;
+        }
+    }
+    // H slots: row r in [1, kTY+3), pair i in [1, kPX+3); per plane parity q the red
+    // cell is the even one when (ys + r + q) is even
+    int h_ca[kHSlots3][2], h_cx[kHSlots3][2], h_st[kHSlots3];
+    uint32_t h_ob[kHSlots3][2];
+    int h_real[kHSlots3]; // bit q: the red cell at parity q is a real cell
+#pragma unroll
+    for (int j = 0; j < kHSlots3; ++j) {
+        const int p = tid + j * kThreads;
+        const int r = 1 + p / (kPX + 2), i = 1 + p % (kPX + 2);
+        const int y = ys + r;
+        h_st[j]     = p < kHVals ? r * kW + i : -1;
+        h_real[j]   = 0;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int a   = (y + q) & 1; // 0: red in the even array
+            h_ca[j][q]    = a * kArr + r * kW + i;
+            h_cx[j][q]    = (1 - a) * kArr + r * kW + (a == 0 ? i - 1 : i);
+            const int gx  = xs + 2 * i + a;
+            const bool rl = (py || (y >= 0 && y < d.ny)) && (px || (gx >= 0 && gx < d.nx));
+            h_ob[j][q]    = static_cast<uint32_t>(fix_coord(y, d.ny, py) * d.nx + fix_coord(gx, d.nx, px));
+            if (p < kHVals && rl) h_real[j] |= 1 << q;
+        }
+    }
+    // output slots: wavefront w handles row w (+8), lane = pair
+    const int lane = tid & 63;
+    const int wrow = tid >> 6;
+
+    auto real_plane = [&](const int k) { return pz || (k >= 0 && k < d.nz); };
+
+    float r_old[kSlots3][2], r_bh[kHSlots3], r_bf[kOutPer];
+    auto fetch_old = [&](const int k) {
+        const bool below = !pz && k < 0, above = !pz && k >= d.nz;
+        const float* src = x_old + static_cast<uint64_t>(fix_coord(k, d.nz, pz)) * plane;
+#pragma unroll
+        for (int j = 0; j < kSlots3; ++j)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) r_old[j][e] = kZero ? 0.0f : src[o_off[j][e]];
+        if (below || above) {
+            const FaceRule f = bc.face[below ? 4 : 5];
+#pragma unroll
+            for (int j = 0; j < kSlots3; ++j)
+#pragma unroll
+                for (int e = 0; e < 2; ++e) r_old[j][e] = apply_ghost(f, r_old[j][e]);
+        } else if (any_gc) {
+#pragma unroll
+            for (int j = 0; j < kSlots3; ++j)
+#pragma unroll
+                for (int e = 0; e < 2; ++e)
+                    if (o_gc[j][e]) r_old[j][e] = apply_ghost(bc.face[o_gc[j][e] - 1], r_old[j][e]);
+        }
+    };
+    auto fetch_bh = [&](const int k) {
+        if (!real_plane(k)) return;
+        const int q      = k & 1; // equals the wrapped parity: periodic lengths are even here
+        const float* src = b + static_cast<uint64_t>(fix_coord(k, d.nz, pz)) * plane;
+#pragma unroll
+        for (int j = 0; j < kHSlots3; ++j) r_bh[j] = (h_real[j] >> q) & 1 ? src[q ? h_ob[j][1] : h_ob[j][0]] : 0.0f;
+    };
+    auto fetch_bf = [&](const int k) {
+        if (k < 0 || k >= d.nz) return;
+        const float* src = b + static_cast<uint64_t>(k) * plane;
+#pragma unroll
+        for (int j = 0; j < kOutPer; ++j) {
+            const int y = y0 + wrow + (kThreads / 64) * j;
+            const int e = ((y + k) & 1) == 0 ? 1 : 0; // black cell: odd when the even one is red
+            const int x = x0 + 2 * lane + e;
+            if (y < d.ny && x < d.nx) r_bf[j] = src[static_cast<uint32_t>(y * d.nx + x)];
+        }
+    };
+    auto stash_old = [&](const int k) {
+        float* s = s_old[mod3(k)];
+#pragma unroll
+        for (int j = 0; j < kSlots3; ++j) {
+            const int p = tid + j * kThreads;
+            if (p < kR * kW) {
+                const int r = p / kW, i = p % kW;
+                s[r * kW + i]        = r_old[j][0];
+                s[kArr + r * kW + i] = r_old[j][1];
+            }
+        }
+    };
+
+    fetch_old(z0 - 2);
+    stash_old(z0 - 2);
+    fetch_old(z0 - 1);
+    stash_old(z0 - 1);
+    fetch_old(z0);
+    fetch_bh(z0 - 1);
+
+    for (int k = z0 - 1; k <= z1; ++k) {
+        stash_old(k + 1);
+        float bh[kHSlots3], bf[kOutPer];
+#pragma unroll
+        for (int j = 0; j < kHSlots3; ++j) bh[j] = r_bh[j];
+#pragma unroll
+        for (int j = 0; j < kOutPer; ++j) bf[j] = r_bf[j];
+        if (k + 1 <= z1) {
+            fetch_old(k + 2);
+            fetch_bh(k + 1);
+        }
+        if (k >= z0 && k < z1) fetch_bf(k);
+        __syncthreads();
+
+        // H(k): red value of every pair on the tile plus a one-cell halo
+        {
+            const float* so = s_old[mod3(k)];
+            const float* sm = s_old[mod3(k - 1)];
+            const float* sp = s_old[mod3(k + 1)];
+            float* sh       = s_h[mod3(k)];
+            const bool real = real_plane(k);
+            const int q     = k & 1; // also for the out-of-grid planes k = -1, nz
+#pragma unroll
+            for (int j = 0; j < kHSlots3; ++j) {
+                if (h_st[j] < 0) continue;
+                const int ca = q ? h_ca[j][1] : h_ca[j][0];
+                float v      = so[ca];
+                if (real && ((h_real[j] >> q) & 1)) {
+                    const int cx = q ? h_cx[j][1] : h_cx[j][0];
+                    float sum    = so[cx];
+                    sum          = sum + so[cx + 1];
+                    sum          = sum + so[ca - kW];
+                    sum          = sum + so[ca + kW];
+                    sum          = sum + sm[ca];
+                    sum          = sum + sp[ca];
+                    v            = relax_value<kSolve>(bh[j], sum, coef);
+                }
+                sh[h_st[j]] = v;
+            }
+        }
+        __syncthreads();
+
+        // final plane k-1: black cells relaxed from the red values, red cells copied
+        const int kf = k - 1;
+        if (kf >= z0 && kf < z1) {
+            const float* hm = s_h[mod3(kf - 1)];
+            const float* h0 = s_h[mod3(kf)];
+            const float* hp = s_h[mod3(kf + 1)];
+            float* dst_plane = x_new + static_cast<uint64_t>(kf) * plane;
+#pragma unroll
+            for (int j = 0; j < kOutPer; ++j) {
+                const int rr = wrow + (kThreads / 64) * j;
+                const int y  = y0 + rr;
+                if (y >= d.ny) continue;
+                const int c       = (rr + 2) * kW + (lane + 2);
+                const bool even_red = __builtin_amdgcn_readfirstlane((y + kf) & 1) == 0;
+                const float red   = h0[c];
+                float sum;
+                if (even_red) { // black is the odd cell: x- = red[i], x+ = red[i+1]
+                    sum = h0[c];
+                    sum = sum + h0[c + 1];
+                } else { // black is the even cell: x- = red[i-1], x+ = red[i]
+                    sum = h0[c - 1];
+                    sum = sum + h0[c];
+                }
+                sum               = sum + h0[c - kW];
+                sum               = sum + h0[c + kW];
+                sum               = sum + hm[c];
+                sum               = sum + hp[c];
+                const float black = relax_value<kSolve>(bf[j], sum, coef);
+                const int x       = x0 + 2 * lane;
+                float* dst        = dst_plane + static_cast<uint32_t>(y * d.nx + x);
+                if (x < d.nx) dst[0] = (even_red ? red : black) * out_scale;
+                if (x + 1 < d.nx) dst[1] = (even_red ? black : red) * out_scale;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+bool has_odd_periodic(const Dims d, const StencilBC& bc) {
+    return ((bc.periodic & 1) && (d.nx & 1)) || ((bc.periodic & 2) && (d.ny & 1)) || ((bc.periodic & 4) && (d.nz & 1));
+}
+
 TileGeom plan_tiles(const Dims d) {
     TileGeom g{};
     g.tiles_x        = (d.nx + kTX - 1) / kTX;
@@ -415,7 +635,15 @@ hipError_t launch_smooth(hipStream_t s, const SmootherKind smoother, const Solve
     const unsigned blocks = static_cast<unsigned>(g.tiles_x * g.tiles_y * g.chunks);
     const bool zero       = pass.x_old == nullptr;
 #define SF_PASS(KERNEL, SO, Z) KERNEL<SO, Z><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g)
-    if (smoother == kSmoothRBGS) {
+    if (smoother == kSmoothRBGS && !has_odd_periodic(d, bc)) {
+        if (solve == kSolvePoisson) {
+            if (zero) SF_PASS(k_rbgs_split_pass, kSolvePoisson, true);
+            else SF_PASS(k_rbgs_split_pass, kSolvePoisson, false);
+        } else {
+            if (zero) SF_PASS(k_rbgs_split_pass, kSolveDiffusion, true);
+            else SF_PASS(k_rbgs_split_pass, kSolveDiffusion, false);
+        }
+    } else if (smoother == kSmoothRBGS) {
         if (solve == kSolvePoisson) {
             if (zero) SF_PASS(k_rbgs_pass, kSolvePoisson, true);
             else SF_PASS(k_rbgs_pass, kSolvePoisson, false);
