@@ -95,6 +95,20 @@ __device__ __forceinline__ void tile_of(const TileGeom g, int& tx, int& ty, int&
 
 __device__ __forceinline__ int mod3(const int k) { return ((k % 3) + 3) % 3; }
 
+// FaceRule of face f (0..5) by a register select chain: indexing the kernel-argument
+// struct with a runtime value would make the compiler copy it to scratch memory.
+__device__ __forceinline__ FaceRule face_rule(const StencilBC& bc, const int f) {
+    int mode    = bc.face[0].mode;
+    float value = bc.face[0].value;
+#pragma unroll
+    for (int k = 1; k < 6; ++k)
+        if (f == k) {
+            mode  = bc.face[k].mode;
+            value = bc.face[k].value;
+        }
+    return FaceRule{mode, value};
+}
+
 // Ghost code of a halo position: 0 inside (or periodic), else 1 + face index.
 __device__ __forceinline__ int ghost_code(const int gx, const int gy, const Dims d, const bool px, const bool py) {
     if (!px && gx < 0) return 1;
@@ -120,7 +134,7 @@ __device__ __forceinline__ void fetch_region(float (&r)[kPer], const float* __re
         else if (above)
             v = apply_ghost(bc.face[5], v);
         else if (gc[i])
-            v = apply_ghost(bc.face[gc[i] - 1], v);
+            v = apply_ghost(face_rule(bc, gc[i] - 1), v);
         r[i] = v;
     }
 }
@@ -297,8 +311,12 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_pass(const float* __restrict_
                     else out.y = v;
                 }
                 float* dst = x_new + base + static_cast<uint64_t>((y0 + fy) * d.nx + x0 + 2 * fp);
-                dst[0]     = out.x * out_scale;
-                if (info & 2) dst[1] = out.y * out_scale;
+                if ((d.nx & 1) == 0) {
+                    *reinterpret_cast<float2*>(dst) = make_float2(out.x * out_scale, out.y * out_scale);
+                } else {
+                    dst[0] = out.x * out_scale;
+                    if (info & 2) dst[1] = out.y * out_scale;
+                }
             }
         }
         __syncthreads();
@@ -386,8 +404,14 @@ __global__ __launch_bounds__(kThreads) void k_jacobi_pass(const float* __restric
             const int c    = (fy + 1) * kHPX + (fp + 1);
             const float2 L = so[c - 1], C = so[c], R = so[c + 1], YM = so[c - kHPX], YP = so[c + kHPX], ZM = sm[c], ZP = sp[c];
             float* dst     = x_new + base + static_cast<uint64_t>((y0 + fy) * d.nx + x0 + 2 * fp);
-            dst[0]         = relax_elem<kSolve>(0, L, C, R, YM, YP, ZM, ZP, bv[i].x, coef) * out_scale;
-            if (info & 2) dst[1] = relax_elem<kSolve>(1, L, C, R, YM, YP, ZM, ZP, bv[i].y, coef) * out_scale;
+            const float v0 = relax_elem<kSolve>(0, L, C, R, YM, YP, ZM, ZP, bv[i].x, coef) * out_scale;
+            if ((d.nx & 1) == 0) {
+                const float v1 = relax_elem<kSolve>(1, L, C, R, YM, YP, ZM, ZP, bv[i].y, coef) * out_scale;
+                *reinterpret_cast<float2*>(dst) = make_float2(v0, v1);
+            } else {
+                dst[0] = v0;
+                if (info & 2) dst[1] = relax_elem<kSolve>(1, L, C, R, YM, YP, ZM, ZP, bv[i].y, coef) * out_scale;
+            }
         }
         __syncthreads();
     }
@@ -405,7 +429,6 @@ constexpr int kR        = kTY + 4;     // rows, y from ys = y0 - 2
 constexpr int kArr      = kR * kW;     // floats per even/odd array of a plane
 constexpr int kOldPlane = 2 * kArr;
 constexpr int kHPlane   = kArr;
-constexpr int kSlots3   = (kR * kW + kThreads - 1) / kThreads;        // old pair slots
 constexpr int kHVals    = (kTY + 2) * (kPX + 2);                      // red values per H plane
 constexpr int kHSlots3  = (kHVals + kThreads - 1) / kThreads;
 
@@ -424,26 +447,28 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const float* __res
     const bool px  = bc.periodic & 1, py = (bc.periodic >> 1) & 1, pz = (bc.periodic >> 2) & 1;
     const uint64_t plane = static_cast<uint64_t>(d.nx) * static_cast<uint64_t>(d.ny);
 
-    // old pair slots: offsets and ghost codes of the even (e=0) and odd (e=1) cell
-    uint32_t o_off[kSlots3][2];
-    int o_gc[kSlots3][2];
+    // old-plane load slots: contiguous floats of the (2*kW)-wide region rows, stored to
+    // the even or odd LDS array by column parity
+    uint32_t o_off[kOldPer];
+    int o_gc[kOldPer], o_lds[kOldPer];
     bool any_gc = false;
 #pragma unroll
-    for (int j = 0; j < kSlots3; ++j) {
-        const int p = tid + j * kThreads;
-        const int r = p / kW, i = p % kW;
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-            const int gx = xs + 2 * i + e, gy = ys + r;
-            o_off[j][e]  = p < kR * kW ? static_cast<uint32_t>(fix_coord(gy, d.ny, py) * d.nx + fix_coord(gx, d.nx, px)) : 0u;
-            o_gc[j][e]   = p < kR * kW ? ghost_code(gx, gy, d, px, py) : 0;
-            any_gc       = any_gc || o_gc[j][e] != 0;
-        }
+    for (int j = 0; j < kOldPer; ++j) {
+        const int e = tid + j * kThreads;
+        const int r = e / (2 * kW), c = e % (2 * kW);
+        const int gx = xs + c, gy = ys + r;
+        const bool used = e < kOldPlane;
+        o_off[j] = used ? static_cast<uint32_t>(fix_coord(gy, d.ny, py) * d.nx + fix_coord(gx, d.nx, px)) : 0u;
+        o_gc[j]  = used ? ghost_code(gx, gy, d, px, py) : 0;
+        o_lds[j] = used ? (c & 1) * kArr + r * kW + (c >> 1) : -1;
+        any_gc   = any_gc || o_gc[j] != 0;
     }
     // H slots: row r in [1, kTY+3), pair i in [1, kPX+3); per plane parity q the red
     // cell is the even one when (ys + r + q) is even
-    int h_ca[kHSlots3][2], h_cx[kHSlots3][2], h_st[kHSlots3];
-    uint32_t h_ob[kHSlots3][2];
+    // (one array per parity: a runtime select between two elements of one array would be
+    // turned into a dynamically indexed load from scratch memory)
+    int h_ca0[kHSlots3], h_ca1[kHSlots3], h_cx0[kHSlots3], h_cx1[kHSlots3], h_st[kHSlots3];
+    uint32_t h_ob0[kHSlots3], h_ob1[kHSlots3];
     int h_real[kHSlots3]; // bit q: the red cell at parity q is a real cell
 #pragma unroll
     for (int j = 0; j < kHSlots3; ++j) {
@@ -454,41 +479,45 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const float* __res
         h_real[j]   = 0;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            const int a   = (y + q) & 1; // 0: red in the even array
-            h_ca[j][q]    = a * kArr + r * kW + i;
-            h_cx[j][q]    = (1 - a) * kArr + r * kW + (a == 0 ? i - 1 : i);
-            const int gx  = xs + 2 * i + a;
-            const bool rl = (py || (y >= 0 && y < d.ny)) && (px || (gx >= 0 && gx < d.nx));
-            h_ob[j][q]    = static_cast<uint32_t>(fix_coord(y, d.ny, py) * d.nx + fix_coord(gx, d.nx, px));
+            const int a       = (y + q) & 1; // 0: red in the even array
+            const int ca      = a * kArr + r * kW + i;
+            const int cx      = (1 - a) * kArr + r * kW + (a == 0 ? i - 1 : i);
+            const int gx      = xs + 2 * i + a;
+            const bool rl     = (py || (y >= 0 && y < d.ny)) && (px || (gx >= 0 && gx < d.nx));
+            const uint32_t ob = static_cast<uint32_t>(fix_coord(y, d.ny, py) * d.nx + fix_coord(gx, d.nx, px));
+            if (q == 0) {
+                h_ca0[j] = ca;
+                h_cx0[j] = cx;
+                h_ob0[j] = ob;
+            } else {
+                h_ca1[j] = ca;
+                h_cx1[j] = cx;
+                h_ob1[j] = ob;
+            }
             if (p < kHVals && rl) h_real[j] |= 1 << q;
         }
     }
     // output slots: wavefront w handles row w (+8), lane = pair
-    const int lane = tid & 63;
-    const int wrow = tid >> 6;
+    const int lane     = tid & 63;
+    const int wrow     = tid >> 6;
+    const bool even_nx = (d.nx & 1) == 0;
 
     auto real_plane = [&](const int k) { return pz || (k >= 0 && k < d.nz); };
 
-    float r_old[kSlots3][2], r_bh[kHSlots3], r_bf[kOutPer];
+    float r_old[kOldPer], r_bh[kHSlots3], r_bf[kOutPer];
     auto fetch_old = [&](const int k) {
         const bool below = !pz && k < 0, above = !pz && k >= d.nz;
         const float* src = x_old + static_cast<uint64_t>(fix_coord(k, d.nz, pz)) * plane;
 #pragma unroll
-        for (int j = 0; j < kSlots3; ++j)
-#pragma unroll
-            for (int e = 0; e < 2; ++e) r_old[j][e] = kZero ? 0.0f : src[o_off[j][e]];
+        for (int j = 0; j < kOldPer; ++j) r_old[j] = kZero ? 0.0f : src[o_off[j]];
         if (below || above) {
-            const FaceRule f = bc.face[below ? 4 : 5];
+            const FaceRule f = face_rule(bc, below ? 4 : 5);
 #pragma unroll
-            for (int j = 0; j < kSlots3; ++j)
-#pragma unroll
-                for (int e = 0; e < 2; ++e) r_old[j][e] = apply_ghost(f, r_old[j][e]);
+            for (int j = 0; j < kOldPer; ++j) r_old[j] = apply_ghost(f, r_old[j]);
         } else if (any_gc) {
 #pragma unroll
-            for (int j = 0; j < kSlots3; ++j)
-#pragma unroll
-                for (int e = 0; e < 2; ++e)
-                    if (o_gc[j][e]) r_old[j][e] = apply_ghost(bc.face[o_gc[j][e] - 1], r_old[j][e]);
+            for (int j = 0; j < kOldPer; ++j)
+                if (o_gc[j]) r_old[j] = apply_ghost(face_rule(bc, o_gc[j] - 1), r_old[j]);
         }
     };
     auto fetch_bh = [&](const int k) {
@@ -496,7 +525,7 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const float* __res
         const int q      = k & 1; // equals the wrapped parity: periodic lengths are even here
         const float* src = b + static_cast<uint64_t>(fix_coord(k, d.nz, pz)) * plane;
 #pragma unroll
-        for (int j = 0; j < kHSlots3; ++j) r_bh[j] = (h_real[j] >> q) & 1 ? src[q ? h_ob[j][1] : h_ob[j][0]] : 0.0f;
+        for (int j = 0; j < kHSlots3; ++j) r_bh[j] = (h_real[j] >> q) & 1 ? src[q ? h_ob1[j] : h_ob0[j]] : 0.0f;
     };
     auto fetch_bf = [&](const int k) {
         if (k < 0 || k >= d.nz) return;
@@ -512,14 +541,8 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const float* __res
     auto stash_old = [&](const int k) {
         float* s = s_old[mod3(k)];
 #pragma unroll
-        for (int j = 0; j < kSlots3; ++j) {
-            const int p = tid + j * kThreads;
-            if (p < kR * kW) {
-                const int r = p / kW, i = p % kW;
-                s[r * kW + i]        = r_old[j][0];
-                s[kArr + r * kW + i] = r_old[j][1];
-            }
-        }
+        for (int j = 0; j < kOldPer; ++j)
+            if (o_lds[j] >= 0) s[o_lds[j]] = r_old[j];
     };
 
     fetch_old(z0 - 2);
@@ -554,10 +577,10 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const float* __res
 #pragma unroll
             for (int j = 0; j < kHSlots3; ++j) {
                 if (h_st[j] < 0) continue;
-                const int ca = q ? h_ca[j][1] : h_ca[j][0];
+                const int ca = q ? h_ca1[j] : h_ca0[j];
                 float v      = so[ca];
                 if (real && ((h_real[j] >> q) & 1)) {
-                    const int cx = q ? h_cx[j][1] : h_cx[j][0];
+                    const int cx = q ? h_cx1[j] : h_cx0[j];
                     float sum    = so[cx];
                     sum          = sum + so[cx + 1];
                     sum          = sum + so[ca - kW];
@@ -601,8 +624,14 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const float* __res
                 const float black = relax_value<kSolve>(bf[j], sum, coef);
                 const int x       = x0 + 2 * lane;
                 float* dst        = dst_plane + static_cast<uint32_t>(y * d.nx + x);
-                if (x < d.nx) dst[0] = (even_red ? red : black) * out_scale;
-                if (x + 1 < d.nx) dst[1] = (even_red ? black : red) * out_scale;
+                const float ve    = (even_red ? red : black) * out_scale;
+                const float vo    = (even_red ? black : red) * out_scale;
+                if (even_nx) { // 8-byte aligned pair, one vector store
+                    if (x < d.nx) *reinterpret_cast<float2*>(dst) = make_float2(ve, vo);
+                } else {
+                    if (x < d.nx) dst[0] = ve;
+                    if (x + 1 < d.nx) dst[1] = vo;
+                }
             }
         }
         __syncthreads();
