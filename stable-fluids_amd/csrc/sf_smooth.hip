@@ -25,6 +25,8 @@
 // grid planes below/above a non-periodic z range hold the z-face rule of the edge plane.
 // Periodic axes load the wrapped cell. Red/black parity uses wrapped coordinates, and an
 // odd periodic axis, where a pair can hold two cells of one colour, takes a second relax.
+#include <type_traits>
+
 #include "sf_kernels.hpp"
 
 namespace sf {
@@ -432,6 +434,12 @@ constexpr int kHPlane   = kArr;
 constexpr int kHVals    = (kTY + 2) * (kPX + 2);                      // red values per H plane
 constexpr int kHSlots3  = (kHVals + kThreads - 1) / kThreads;
 
+// Periodic plane index for k in [-2, n + 1] without a division (n >= 2); n = 1 maps to 0.
+__device__ __forceinline__ int wrap_near(const int k, const int n) {
+    if (n < 2) return 0;
+    return k < 0 ? k + n : (k >= n ? k - n : k);
+}
+
 template <int kSolve, bool kZero>
 __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const float* __restrict__ x_old, const float* __restrict__ b, float* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g) {
     __shared__ float s_old[3][kOldPlane];
@@ -447,36 +455,36 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const float* __res
     const bool px  = bc.periodic & 1, py = (bc.periodic >> 1) & 1, pz = (bc.periodic >> 2) & 1;
     const uint64_t plane = static_cast<uint64_t>(d.nx) * static_cast<uint64_t>(d.ny);
 
-    // old-plane load slots: contiguous floats of the (2*kW)-wide region rows, stored to
-    // the even or odd LDS array by column parity
+    // Old-plane load slots: contiguous floats of the (2*kW)-wide region rows, stored to the
+    // even or odd LDS array by column parity. Lanes past the region repeat its last slot
+    // (same address, same value), so no lane needs a branch.
     uint32_t o_off[kOldPer];
     int o_gc[kOldPer], o_lds[kOldPer];
     bool any_gc = false;
 #pragma unroll
     for (int j = 0; j < kOldPer; ++j) {
-        const int e = tid + j * kThreads;
-        const int r = e / (2 * kW), c = e % (2 * kW);
+        const int e  = min(tid + j * kThreads, kOldPlane - 1);
+        const int r  = e / (2 * kW), c = e % (2 * kW);
         const int gx = xs + c, gy = ys + r;
-        const bool used = e < kOldPlane;
-        o_off[j] = used ? static_cast<uint32_t>(fix_coord(gy, d.ny, py) * d.nx + fix_coord(gx, d.nx, px)) : 0u;
-        o_gc[j]  = used ? ghost_code(gx, gy, d, px, py) : 0;
-        o_lds[j] = used ? (c & 1) * kArr + r * kW + (c >> 1) : -1;
-        any_gc   = any_gc || o_gc[j] != 0;
+        o_off[j]     = static_cast<uint32_t>(fix_coord(gy, d.ny, py) * d.nx + fix_coord(gx, d.nx, px));
+        o_gc[j]      = ghost_code(gx, gy, d, px, py);
+        o_lds[j]     = (c & 1) * kArr + r * kW + (c >> 1);
+        any_gc       = any_gc || o_gc[j] != 0;
     }
-    // H slots: row r in [1, kTY+3), pair i in [1, kPX+3); per plane parity q the red
-    // cell is the even one when (ys + r + q) is even
-    // (one array per parity: a runtime select between two elements of one array would be
-    // turned into a dynamically indexed load from scratch memory)
+    const bool wave_gc = __builtin_amdgcn_readfirstlane(__any(any_gc)) != 0;
+
+    // H slots: row r in [1, kTY+3), pair i in [1, kPX+3), clamped like the load slots. At
+    // plane parity q the red cell is the even one when (ys + r + q) is even. One array per
+    // parity: a runtime select between two elements of one array would become a scratch load.
     int h_ca0[kHSlots3], h_ca1[kHSlots3], h_cx0[kHSlots3], h_cx1[kHSlots3], h_st[kHSlots3];
     uint32_t h_ob0[kHSlots3], h_ob1[kHSlots3];
-    int h_real[kHSlots3]; // bit q: the red cell at parity q is a real cell
+    bool h_rl0[kHSlots3], h_rl1[kHSlots3]; // the red cell at parity q is a real cell
 #pragma unroll
     for (int j = 0; j < kHSlots3; ++j) {
-        const int p = tid + j * kThreads;
+        const int p = min(tid + j * kThreads, kHVals - 1);
         const int r = 1 + p / (kPX + 2), i = 1 + p % (kPX + 2);
         const int y = ys + r;
-        h_st[j]     = p < kHVals ? r * kW + i : -1;
-        h_real[j]   = 0;
+        h_st[j]     = r * kW + i;
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const int a       = (y + q) & 1; // 0: red in the even array
@@ -489,71 +497,73 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const float* __res
                 h_ca0[j] = ca;
                 h_cx0[j] = cx;
                 h_ob0[j] = ob;
+                h_rl0[j] = rl;
             } else {
                 h_ca1[j] = ca;
                 h_cx1[j] = cx;
                 h_ob1[j] = ob;
+                h_rl1[j] = rl;
             }
-            if (p < kHVals && rl) h_real[j] |= 1 << q;
         }
     }
-    // output slots: wavefront w handles row w (+8), lane = pair
+    // output: wavefront w handles rows w and w+8, lane = pair; row validity is wave-uniform
     const int lane     = tid & 63;
-    const int wrow     = tid >> 6;
+    const int wrow     = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const bool full_x  = x0 + kTX <= d.nx && (d.nx & 1) == 0;
     const bool even_nx = (d.nx & 1) == 0;
 
+    auto plane_of = [&](const int k) { return pz ? wrap_near(k, d.nz) : (k < 0 ? 0 : (k >= d.nz ? d.nz - 1 : k)); };
     auto real_plane = [&](const int k) { return pz || (k >= 0 && k < d.nz); };
 
     float r_old[kOldPer], r_bh[kHSlots3], r_bf[kOutPer];
     auto fetch_old = [&](const int k) {
-        const bool below = !pz && k < 0, above = !pz && k >= d.nz;
-        const float* src = x_old + static_cast<uint64_t>(fix_coord(k, d.nz, pz)) * plane;
+        const float* src = x_old + static_cast<uint64_t>(plane_of(k)) * plane;
 #pragma unroll
         for (int j = 0; j < kOldPer; ++j) r_old[j] = kZero ? 0.0f : src[o_off[j]];
-        if (below || above) {
-            const FaceRule f = face_rule(bc, below ? 4 : 5);
+        if (!pz && (k < 0 || k >= d.nz)) {
+            const FaceRule f = k < 0 ? FaceRule{bc.face[4].mode, bc.face[4].value} : FaceRule{bc.face[5].mode, bc.face[5].value};
 #pragma unroll
             for (int j = 0; j < kOldPer; ++j) r_old[j] = apply_ghost(f, r_old[j]);
-        } else if (any_gc) {
+        } else if (wave_gc) {
 #pragma unroll
-            for (int j = 0; j < kOldPer; ++j)
-                if (o_gc[j]) r_old[j] = apply_ghost(face_rule(bc, o_gc[j] - 1), r_old[j]);
+            for (int j = 0; j < kOldPer; ++j) {
+                const float gv = apply_ghost(face_rule(bc, o_gc[j] - 1), r_old[j]);
+                r_old[j]       = o_gc[j] ? gv : r_old[j];
+            }
         }
     };
-    auto fetch_bh = [&](const int k) {
-        if (!real_plane(k)) return;
-        const int q      = k & 1; // equals the wrapped parity: periodic lengths are even here
-        const float* src = b + static_cast<uint64_t>(fix_coord(k, d.nz, pz)) * plane;
+    auto fetch_bh = [&](const int k, const bool odd) {
+        const float* src = b + static_cast<uint64_t>(plane_of(k)) * plane;
 #pragma unroll
-        for (int j = 0; j < kHSlots3; ++j) r_bh[j] = (h_real[j] >> q) & 1 ? src[q ? h_ob1[j] : h_ob0[j]] : 0.0f;
+        for (int j = 0; j < kHSlots3; ++j) r_bh[j] = src[odd ? h_ob1[j] : h_ob0[j]];
     };
     auto fetch_bf = [&](const int k) {
-        if (k < 0 || k >= d.nz) return;
         const float* src = b + static_cast<uint64_t>(k) * plane;
 #pragma unroll
         for (int j = 0; j < kOutPer; ++j) {
-            const int y = y0 + wrow + (kThreads / 64) * j;
+            const int y = min(y0 + wrow + (kThreads / 64) * j, d.ny - 1);
             const int e = ((y + k) & 1) == 0 ? 1 : 0; // black cell: odd when the even one is red
-            const int x = x0 + 2 * lane + e;
-            if (y < d.ny && x < d.nx) r_bf[j] = src[static_cast<uint32_t>(y * d.nx + x)];
+            const int x = min(x0 + 2 * lane + e, d.nx - 1);
+            r_bf[j]     = src[static_cast<uint32_t>(y * d.nx + x)];
         }
     };
-    auto stash_old = [&](const int k) {
-        float* s = s_old[mod3(k)];
+    auto stash_old = [&](float* s) {
 #pragma unroll
-        for (int j = 0; j < kOldPer; ++j)
-            if (o_lds[j] >= 0) s[o_lds[j]] = r_old[j];
+        for (int j = 0; j < kOldPer; ++j) s[o_lds[j]] = r_old[j];
     };
 
-    fetch_old(z0 - 2);
-    stash_old(z0 - 2);
-    fetch_old(z0 - 1);
-    stash_old(z0 - 1);
-    fetch_old(z0);
-    fetch_bh(z0 - 1);
+    // LDS rings, rotated each plane: old planes k-1, k, k+1 and H planes k-2, k-1, k
+    float* om = s_old[0];
+    float* o0 = s_old[1];
+    float* op = s_old[2];
+    float* hm2 = s_h[0];
+    float* hm1 = s_h[1];
+    float* h0  = s_h[2];
 
-    for (int k = z0 - 1; k <= z1; ++k) {
-        stash_old(k + 1);
+    // one plane of the pipeline; Q is the parity of k (compile time)
+    auto body = [&](const int k, auto qc) {
+        constexpr int Q = decltype(qc)::value;
+        stash_old(op); // old(k+1), fetched one plane earlier
         float bh[kHSlots3], bf[kOutPer];
 #pragma unroll
         for (int j = 0; j < kHSlots3; ++j) bh[j] = r_bh[j];
@@ -561,72 +571,65 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const float* __res
         for (int j = 0; j < kOutPer; ++j) bf[j] = r_bf[j];
         if (k + 1 <= z1) {
             fetch_old(k + 2);
-            fetch_bh(k + 1);
+            if (real_plane(k + 1)) fetch_bh(k + 1, Q == 0);
         }
         if (k >= z0 && k < z1) fetch_bf(k);
         __syncthreads();
 
         // H(k): red value of every pair on the tile plus a one-cell halo
-        {
-            const float* so = s_old[mod3(k)];
-            const float* sm = s_old[mod3(k - 1)];
-            const float* sp = s_old[mod3(k + 1)];
-            float* sh       = s_h[mod3(k)];
-            const bool real = real_plane(k);
-            const int q     = k & 1; // also for the out-of-grid planes k = -1, nz
+        if (real_plane(k)) {
 #pragma unroll
             for (int j = 0; j < kHSlots3; ++j) {
-                if (h_st[j] < 0) continue;
-                const int ca = q ? h_ca1[j] : h_ca0[j];
-                float v      = so[ca];
-                if (real && ((h_real[j] >> q) & 1)) {
-                    const int cx = q ? h_cx1[j] : h_cx0[j];
-                    float sum    = so[cx];
-                    sum          = sum + so[cx + 1];
-                    sum          = sum + so[ca - kW];
-                    sum          = sum + so[ca + kW];
-                    sum          = sum + sm[ca];
-                    sum          = sum + sp[ca];
-                    v            = relax_value<kSolve>(bh[j], sum, coef);
-                }
-                sh[h_st[j]] = v;
+                const int ca   = Q ? h_ca1[j] : h_ca0[j];
+                const int cx   = Q ? h_cx1[j] : h_cx0[j];
+                const bool rl  = Q ? h_rl1[j] : h_rl0[j];
+                float sum      = o0[cx];
+                sum            = sum + o0[cx + 1];
+                sum            = sum + o0[ca - kW];
+                sum            = sum + o0[ca + kW];
+                sum            = sum + om[ca];
+                sum            = sum + op[ca];
+                const float v  = relax_value<kSolve>(bh[j], sum, coef);
+                h0[h_st[j]]    = rl ? v : o0[ca];
             }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kHSlots3; ++j) h0[h_st[j]] = o0[Q ? h_ca1[j] : h_ca0[j]];
         }
         __syncthreads();
 
         // final plane k-1: black cells relaxed from the red values, red cells copied
         const int kf = k - 1;
         if (kf >= z0 && kf < z1) {
-            const float* hm = s_h[mod3(kf - 1)];
-            const float* h0 = s_h[mod3(kf)];
-            const float* hp = s_h[mod3(kf + 1)];
             float* dst_plane = x_new + static_cast<uint64_t>(kf) * plane;
 #pragma unroll
             for (int j = 0; j < kOutPer; ++j) {
                 const int rr = wrow + (kThreads / 64) * j;
                 const int y  = y0 + rr;
-                if (y >= d.ny) continue;
-                const int c       = (rr + 2) * kW + (lane + 2);
-                const bool even_red = __builtin_amdgcn_readfirstlane((y + kf) & 1) == 0;
-                const float red   = h0[c];
+                if (y >= d.ny) continue; // wave-uniform
+                const int c         = (rr + 2) * kW + (lane + 2);
+                const bool even_red = ((y + (1 - Q)) & 1) == 0; // parity of kf is 1 - Q
+                const float red     = hm1[c];
                 float sum;
                 if (even_red) { // black is the odd cell: x- = red[i], x+ = red[i+1]
-                    sum = h0[c];
-                    sum = sum + h0[c + 1];
+                    sum = hm1[c];
+                    sum = sum + hm1[c + 1];
                 } else { // black is the even cell: x- = red[i-1], x+ = red[i]
-                    sum = h0[c - 1];
-                    sum = sum + h0[c];
+                    sum = hm1[c - 1];
+                    sum = sum + hm1[c];
                 }
-                sum               = sum + h0[c - kW];
-                sum               = sum + h0[c + kW];
-                sum               = sum + hm[c];
-                sum               = sum + hp[c];
+                sum               = sum + hm1[c - kW];
+                sum               = sum + hm1[c + kW];
+                sum               = sum + hm2[c];
+                sum               = sum + h0[c];
                 const float black = relax_value<kSolve>(bf[j], sum, coef);
-                const int x       = x0 + 2 * lane;
-                float* dst        = dst_plane + static_cast<uint32_t>(y * d.nx + x);
                 const float ve    = (even_red ? red : black) * out_scale;
                 const float vo    = (even_red ? black : red) * out_scale;
-                if (even_nx) { // 8-byte aligned pair, one vector store
+                const int x       = x0 + 2 * lane;
+                float* dst        = dst_plane + static_cast<uint32_t>(y * d.nx + x);
+                if (full_x) {
+                    *reinterpret_cast<float2*>(dst) = make_float2(ve, vo);
+                } else if (even_nx) {
                     if (x < d.nx) *reinterpret_cast<float2*>(dst) = make_float2(ve, vo);
                 } else {
                     if (x < d.nx) dst[0] = ve;
@@ -635,7 +638,36 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const float* __res
             }
         }
         __syncthreads();
+        float* t = om;
+        om       = o0;
+        o0       = op;
+        op       = t;
+        t        = hm2;
+        hm2      = hm1;
+        hm1      = h0;
+        h0       = t;
+    };
+
+    // prologue: old planes z0-2, z0-1 in LDS, old z0 and b for H(z0-1) in registers
+    fetch_old(z0 - 2);
+    stash_old(om);
+    fetch_old(z0 - 1);
+    stash_old(o0);
+    fetch_old(z0);
+    if (real_plane(z0 - 1)) fetch_bh(z0 - 1, ((z0 - 1) & 1) != 0);
+
+    using Even = std::integral_constant<int, 0>;
+    using Odd  = std::integral_constant<int, 1>;
+    int k = z0 - 1;
+    if (k & 1) {
+        body(k, Odd{});
+        ++k;
     }
+    for (; k + 1 <= z1; k += 2) {
+        body(k, Even{});
+        body(k + 1, Odd{});
+    }
+    if (k <= z1) body(k, Even{});
 }
 
 bool has_odd_periodic(const Dims d, const StencilBC& bc) {
