@@ -170,6 +170,104 @@ __device__ inline float sample_scalar(const float* __restrict__ f, float x, floa
     return trilinear(x, y, z, h, [&](int i, int j, int k) { return tap_scalar(f, i, j, k, d, bc); });
 }
 
+// A trilinear sample position after the periodic wrap: lower corner, weights, and (when
+// all 8 taps lie inside the grid) the 32-bit linear index of the lower corner. Positions
+// are located once and shared by every field sampled there.
+struct TriPos {
+    int x0, y0, z0;
+    float tx, ty, tz;
+    bool interior;
+    uint32_t base;
+};
+
+__device__ __forceinline__ TriPos locate(const float x, const float y, const float z, const Dims d, const float h) {
+    TriPos p;
+    const float gx = x / h - 0.5f;
+    const float gy = y / h - 0.5f;
+    const float gz = z / h - 0.5f;
+    p.x0           = static_cast<int>(floorf(gx));
+    p.y0           = static_cast<int>(floorf(gy));
+    p.z0           = static_cast<int>(floorf(gz));
+    p.tx           = gx - static_cast<float>(p.x0);
+    p.ty           = gy - static_cast<float>(p.y0);
+    p.tz           = gz - static_cast<float>(p.z0);
+    p.interior     = p.x0 >= 0 && p.x0 + 1 < d.nx && p.y0 >= 0 && p.y0 + 1 < d.ny && p.z0 >= 0 && p.z0 + 1 < d.nz;
+    p.base         = p.interior ? (static_cast<uint32_t>(p.z0) * static_cast<uint32_t>(d.ny) + static_cast<uint32_t>(p.y0)) * static_cast<uint32_t>(d.nx) + static_cast<uint32_t>(p.x0) : 0u;
+    return p;
+}
+
+// Same value as trilinear() at the located position: interior taps come from the base
+// index and constant strides, other taps from `tap` (the boundary rules).
+template <class Tap>
+__device__ __forceinline__ float sample_at(const float* __restrict__ f, const TriPos& p, const Dims d, Tap tap) {
+    float c000, c100, c010, c110, c001, c101, c011, c111;
+    if (p.interior) {
+        const uint32_t nx = static_cast<uint32_t>(d.nx), pl = nx * static_cast<uint32_t>(d.ny);
+        const float* q    = f + p.base;
+        c000              = q[0];
+        c100              = q[1];
+        c010              = q[nx];
+        c110              = q[nx + 1];
+        c001              = q[pl];
+        c101              = q[pl + 1];
+        c011              = q[pl + nx];
+        c111              = q[pl + nx + 1];
+    } else {
+        c000 = tap(p.x0, p.y0, p.z0);
+        c100 = tap(p.x0 + 1, p.y0, p.z0);
+        c010 = tap(p.x0, p.y0 + 1, p.z0);
+        c110 = tap(p.x0 + 1, p.y0 + 1, p.z0);
+        c001 = tap(p.x0, p.y0, p.z0 + 1);
+        c101 = tap(p.x0 + 1, p.y0, p.z0 + 1);
+        c011 = tap(p.x0, p.y0 + 1, p.z0 + 1);
+        c111 = tap(p.x0 + 1, p.y0 + 1, p.z0 + 1);
+    }
+    const float c00 = c000 + (c100 - c000) * p.tx;
+    const float c10 = c010 + (c110 - c010) * p.tx;
+    const float c01 = c001 + (c101 - c001) * p.tx;
+    const float c11 = c011 + (c111 - c011) * p.tx;
+    const float c0  = c00 + (c10 - c00) * p.ty;
+    const float c1  = c01 + (c11 - c01) * p.ty;
+    return c0 + (c1 - c0) * p.tz;
+}
+
+// Velocity at world position (x,y,z) for all three components (reference sample_velocity,
+// backend-cuda.cu:291-293): one wrap and one located position serve the three fields.
+__device__ __forceinline__ void sample_velocity3(const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, float x, float y, float z, const Dims d, const float h, const StableFluidsFlowBoundaryConfig& bc, float& u, float& v, float& w) {
+    if (flow_periodic(bc, 0)) x = wrap_position(x, d.nx, h);
+    if (flow_periodic(bc, 1)) y = wrap_position(y, d.ny, h);
+    if (flow_periodic(bc, 2)) z = wrap_position(z, d.nz, h);
+    const TriPos p = locate(x, y, z, d, h);
+    u              = sample_at(vx, p, d, [&](int i, int j, int k) { return tap_velocity(vx, 0, i, j, k, d, bc); });
+    v              = sample_at(vy, p, d, [&](int i, int j, int k) { return tap_velocity(vy, 1, i, j, k, d, bc); });
+    w              = sample_at(vz, p, d, [&](int i, int j, int k) { return tap_velocity(vz, 2, i, j, k, d, bc); });
+}
+
+// Midpoint backtrace from the centre of cell (cx,cy,cz), world position p (reference
+// trace_particle_rk2, backend-cuda.cu:295-300). When the centre maps to grid coordinates
+// exactly (h a power of two), every trilinear weight of the first sample is zero and the
+// sample is the cell value itself, which is loaded directly.
+__device__ __forceinline__ void trace_rk2_cell(const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const int cx, const int cy, const int cz, const float px, const float py, const float pz, const float dt, const Dims d, const float h, const StableFluidsFlowBoundaryConfig& bc, float& ox, float& oy, float& oz) {
+    float u0, v0, w0;
+    const bool exact = px / h - 0.5f == static_cast<float>(cx) && py / h - 0.5f == static_cast<float>(cy) && pz / h - 0.5f == static_cast<float>(cz);
+    if (exact) {
+        const uint64_t i = lin(cx, cy, cz, d);
+        u0               = vx[i];
+        v0               = vy[i];
+        w0               = vz[i];
+    } else {
+        sample_velocity3(vx, vy, vz, px, py, pz, d, h, bc, u0, v0, w0);
+    }
+    const float mx = px - 0.5f * dt * u0;
+    const float my = py - 0.5f * dt * v0;
+    const float mz = pz - 0.5f * dt * w0;
+    float u1, v1, w1;
+    sample_velocity3(vx, vy, vz, mx, my, mz, d, h, bc, u1, v1, w1);
+    ox = px - dt * u1;
+    oy = py - dt * v1;
+    oz = pz - dt * w1;
+}
+
 // Midpoint backtrace from world position p (reference backend-cuda.cu:291-300).
 __device__ inline void trace_rk2(const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float px, const float py, const float pz, const float dt, const Dims d, const float h, const StableFluidsFlowBoundaryConfig& bc, float& ox, float& oy, float& oz) {
     const float u0 = sample_velocity(vx, 0, px, py, pz, d, h, bc);

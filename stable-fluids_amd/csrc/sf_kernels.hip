@@ -90,11 +90,13 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_advect_velocity(float* __res
     const float py = (static_cast<float>(y) + 0.5f) * h;
     const float pz = (static_cast<float>(z) + 0.5f) * h;
     float qx, qy, qz;
-    trace_rk2(vx, vy, vz, px, py, pz, dt, d, h, bc, qx, qy, qz);
+    trace_rk2_cell(vx, vy, vz, x, y, z, px, py, pz, dt, d, h, bc, qx, qy, qz);
     const uint64_t i = lin(x, y, z, d);
-    tx[i]            = sample_velocity(vx, 0, qx, qy, qz, d, h, bc);
-    ty[i]            = sample_velocity(vy, 1, qx, qy, qz, d, h, bc);
-    tz[i]            = sample_velocity(vz, 2, qx, qy, qz, d, h, bc);
+    float u, v, w;
+    sample_velocity3(vx, vy, vz, qx, qy, qz, d, h, bc, u, v, w);
+    tx[i] = u;
+    ty[i] = v;
+    tz[i] = w;
 }
 
 // reference advect_scalar_component_kernel backend-cuda.cu:342-352
@@ -107,8 +109,12 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_advect_scalar(float* __restr
     const float py = (static_cast<float>(y) + 0.5f) * h;
     const float pz = (static_cast<float>(z) + 0.5f) * h;
     float qx, qy, qz;
-    trace_rk2(vx, vy, vz, px, py, pz, dt, d, h, fbc, qx, qy, qz);
-    out[lin(x, y, z, d)] = sample_scalar(src, qx, qy, qz, d, h, sbc);
+    trace_rk2_cell(vx, vy, vz, x, y, z, px, py, pz, dt, d, h, fbc, qx, qy, qz);
+    if (scalar_periodic(sbc, 0)) qx = wrap_position(qx, d.nx, h);
+    if (scalar_periodic(sbc, 1)) qy = wrap_position(qy, d.ny, h);
+    if (scalar_periodic(sbc, 2)) qz = wrap_position(qz, d.nz, h);
+    const TriPos p       = locate(qx, qy, qz, d, h);
+    out[lin(x, y, z, d)] = sample_at(src, p, d, [&](int i, int j, int k) { return tap_scalar(src, i, j, k, d, sbc); });
 }
 
 // ---------------- divergence / projection ----------------
