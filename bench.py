@@ -104,6 +104,14 @@ def main():
     dom = stages["velocity_pass"]
     ctx.destroy()
 
+    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE counts of this workload (copy-kernel calibrated)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_plume256_kernels.json")
+    if n == 256 and args.smoother == "rbgs" and os.path.exists(tfile):
+        rec = json.load(open(tfile))["kernels"].get("k_rbgs_split_pass<0, false>")
+        traffic = rec["traffic_bytes"] if rec else None
+
     cpu = None
     if not args.no_cpu_baseline:
         oracle = entry.load_oracle()
@@ -138,7 +146,8 @@ def main():
                    "dt": 0.1, "viscosity": 1e-4, "diffusion": 1e-4},
         "roofline": {"bound": "hbm", "kernel": "velocity diffusion smoother pass", "achieved": dom["gbs"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": None, "bytes_per_launch": dom["bytes"], "avg_us": dom["us"]},
+                     "traffic": traffic, "bytes_per_launch": dom["bytes"], "avg_us": dom["us"],
+                     "traffic_source": "profiles/traffic_plume256_kernels.json" if traffic else None},
         "jacobi_hbm_gbs": stages["jacobi_pass"]["gbs"],
         "stages": stages,
         "final_divergence_l2": norms["divergence_l2"],
