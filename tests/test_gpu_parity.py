@@ -145,3 +145,50 @@ def test_determinism_two_runs_bitwise():
     b = run_gpu(scene.desc, g_seed, 20)
     for k in a:
         assert np.array_equal(a[k], b[k]), k
+
+
+def test_reference_benchmark_scene_bitexact():
+    """The reference benchmark's scene (benchmark.cu:72-155): periodic x/z, no-slip floor,
+    outflow top, dissipation 0.35, swirling ellipsoidal plume, h = 0.01 (not a power of
+    two, so no trilinear weight is exact), at 32x40x32."""
+    scene = scenes.reference_benchmark_scene(32, 40, 32, diffuse_iterations=6, pressure_iterations=12)
+    g_seed, o_seed = plume_seeder(scene)
+    assert_bitexact(run_gpu(scene.desc, g_seed, 12), run_oracle(scene.desc, o_seed, 12, 32), "benchmark scene 12 steps vs fp32")
+    assert_fp64(run_gpu(scene.desc, g_seed, 2), run_oracle(scene.desc, o_seed, 2, 64), "benchmark scene 2 steps vs fp64")
+
+
+def test_checkpoint_resume_bitwise():
+    """Saving velocity and density after 10 steps and restoring them into a fresh context
+    continues the run bit for bit (pressure restarts from zero every step)."""
+    scene = scenes.plume_scene(48, diffuse_iterations=8, pressure_iterations=16)
+    g_seed, _ = plume_seeder(scene)
+    straight = run_gpu(scene.desc, g_seed, 20)
+    with sf.Context(scene.desc) as a:
+        g_seed(a)
+        a.step(10)
+        vel = a.read(abi.VIEW_FLOW_VELOCITY)
+        rho = a.read(abi.VIEW_SCALAR_FIELD_DATA, 1)
+    with sf.Context(scene.desc) as b:
+        g_seed(b)
+        b.set_flow_velocity(*(b.upload(c) for c in vel))
+        b.update_scalar_field(1, b.upload(rho))
+        b.step(10)
+        resumed = {"scalar1": b.read(abi.VIEW_SCALAR_FIELD_DATA, 1).astype(np.float64)}
+        for name, arr in zip(("vx", "vy", "vz"), b.read(abi.VIEW_FLOW_VELOCITY)):
+            resumed[name] = arr.astype(np.float64)
+    for k, v in resumed.items():
+        assert np.array_equal(v, straight[k]), k
+
+
+def test_reference_benchmark_driver_output():
+    """stable-fluids-benchmark keeps the reference driver's flags and output lines."""
+    import os
+    import re
+    import subprocess
+    exe = os.path.join(os.path.dirname(sf.runtime.LIB_PATH), "stable-fluids-benchmark")
+    out = subprocess.run([exe, "--nx", "32", "--ny", "40", "--nz", "32", "--warmup", "2", "--steps", "4"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.strip().splitlines()
+    assert lines[0].startswith("benchmark grid=32x40x32 warmup=2 steps=4 ")
+    m = re.match(r"timing total_ms=([\d.]+) avg_step_ms=([\d.]+) mlups=([\d.]+) total_density=([-\d.]+) peak_density=([-\d.]+)", lines[1])
+    assert m and float(m.group(3)) > 0 and float(m.group(5)) > 0
