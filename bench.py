@@ -10,7 +10,14 @@ bandwidth against the 8 TB/s peak (timed live with HIP events on the context str
 the host-loop baseline (FP64 oracle on this host's cores, a bounded sample).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid N] [--smoother rbgs|jacobi]
+                  [--storage f32|f16|bf16|f32-refined] [--interval M]
                   [--no-cpu-baseline] [--stage-repeat R]
+
+Precision study (--precision): the plume at 512^3 (centre (256,48,256), radius 32,
+D=20, P=40, zero initial velocity), 50 steps with FP32, FP16 and BF16 iterate storage
+(refinement cycles with an FP32 defect every M=10 passes). Prints one JSON line with the
+per-field max and L2 error of each 16-bit run against the FP32-storage run, the final
+divergence L2 (FP64 reduction on the GPU) and ms/step of each.
 """
 import argparse
 import json
@@ -24,6 +31,7 @@ sys.path.insert(0, ROOT)
 import __graft_entry__ as entry  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+STORAGE = {"f32": 0, "f16": 1, "bf16": 2, "f32-refined": 3}
 METRIC = "steps/s at 256³; Jacobi HBM GB/s; learned-vs-plain speedup at equal div L2"
 
 
@@ -49,7 +57,10 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--smoother", choices=("rbgs", "jacobi"), default="rbgs")
     ap.add_argument("--stage-repeat", type=int, default=40)
+    ap.add_argument("--storage", choices=tuple(STORAGE), default="f32")
+    ap.add_argument("--interval", type=int, default=10, help="passes per refinement cycle (16-bit storage)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", action="store_true", help="run the 512^3 storage precision study instead")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the stable-fluids step here runs one simulation per GPU; "
@@ -57,11 +68,14 @@ def main():
         sys.exit(2)
 
     pkg = entry.load_package()
+    if args.precision:
+        precision_study(pkg, args)
+        return
     abi, hip, scenes = pkg.abi, pkg.hip, pkg.scenes
     n = args.grid
     scene = scenes.plume_scene(n, diffuse_iterations=40, pressure_iterations=80, noise_seed=0)
     smoother = abi.SMOOTHER_JACOBI if args.smoother == "jacobi" else abi.SMOOTHER_RBGS
-    opts = pkg.ext_options(smoother=smoother)
+    opts = pkg.ext_options(smoother=smoother, iterate_storage=STORAGE[args.storage], defect_correction_interval=args.interval)
     ctx = pkg.Context(scene.desc, options=opts)
     scenes.seed_context(ctx, scene)
     stream = ctx.stream
@@ -143,6 +157,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"plume {n}^3, seeded smoothed-noise initial velocity, D=40 P=80 {args.smoother}, hipGraph step",
                    "grid": [n, n, n], "diffuse_iterations": 40, "pressure_iterations": 80, "smoother": args.smoother,
+                   "iterate_storage": args.storage, "refinement_interval": args.interval if args.storage != "f32" else None,
                    "dt": 0.1, "viscosity": 1e-4, "diffusion": 1e-4},
         "roofline": {"bound": "hbm", "kernel": "velocity diffusion smoother pass", "achieved": dom["gbs"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4),
@@ -154,6 +169,58 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def precision_study(pkg, args):
+    """Per-field error of 16-bit iterate storage against FP32 storage after K steps."""
+    import numpy as np
+    abi, hip, scenes = pkg.abi, pkg.hip, pkg.scenes
+    n = args.grid if args.grid != 256 else 512
+    steps = args.steps if args.steps != 200 else 50
+    scene = scenes.plume_scene(n, diffuse_iterations=20, pressure_iterations=40)
+    smoother = abi.SMOOTHER_JACOBI if args.smoother == "jacobi" else abi.SMOOTHER_RBGS
+    kinds = {"pressure": abi.VIEW_PRESSURE, "divergence": abi.VIEW_DIVERGENCE, "speed": abi.VIEW_VELOCITY_MAGNITUDE}
+
+    def run(storage):
+        opts = pkg.ext_options(smoother=smoother, iterate_storage=STORAGE[storage], defect_correction_interval=args.interval)
+        ctx = pkg.Context(scene.desc, options=opts)
+        try:
+            scenes.seed_context(ctx, scene)
+            ctx.step(1)
+            hip.synchronize()
+            # restart from the seeded state so every run takes exactly `steps` steps
+            ctx.set_flow_velocity(None, None, None)
+            ctx.update_scalar_field(ctx.scalar_handles[0], None)
+            hip.synchronize()
+            t = time.perf_counter()
+            ctx.step(steps)
+            hip.synchronize()
+            ms = 1e3 * (time.perf_counter() - t) / steps
+            fields = {k: ctx.read(v) for k, v in kinds.items()}
+            fields.update(zip(("vx", "vy", "vz"), ctx.read(abi.VIEW_VELOCITY)))
+            fields["density"] = ctx.read(abi.VIEW_SCALAR_DATA, 1)
+            return fields, ctx.norms(), ms
+        finally:
+            ctx.destroy()
+
+    ref, ref_norms, ref_ms = run("f32")
+    runs = {"f32": {"ms_per_step": round(ref_ms, 3), "final_divergence_l2": ref_norms["divergence_l2"]}}
+    for storage in ("f16", "bf16"):
+        got, norms, ms = run(storage)
+        errs = {}
+        for k, r in ref.items():
+            r64, g64 = r.astype(np.float64), got[k].astype(np.float64)
+            scale, l2 = float(np.max(np.abs(r64))), float(np.sqrt(np.sum(r64 * r64)))
+            d = g64 - r64
+            errs[k] = {"max_rel": float(np.max(np.abs(d))) / max(scale, 1e-300),
+                       "l2_rel": float(np.sqrt(np.sum(d * d))) / max(l2, 1e-300)}
+            del r64, g64, d
+        runs[storage] = {"ms_per_step": round(ms, 3), "final_divergence_l2": norms["divergence_l2"], "error_vs_f32_storage": errs}
+        del got
+    print(json.dumps({"study": "iterate storage precision", "grid": [n, n, n], "steps": steps, "smoother": args.smoother,
+                      "refinement_interval": args.interval, "diffuse_iterations": 20, "pressure_iterations": 40,
+                      "workload": f"plume {n}^3, centre ({n // 2},{3 * n // 32},{n // 2}) radius {n // 16}, zero initial velocity",
+                      "runs": runs}), flush=True)
 
 
 if __name__ == "__main__":

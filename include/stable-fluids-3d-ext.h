@@ -20,10 +20,18 @@ typedef enum StableFluidsExtSmoother {
     STABLE_FLUIDS_EXT_SMOOTHER_JACOBI = 1, /* Jacobi, every cell from the previous iterate */
 } StableFluidsExtSmoother;
 
+/* Storage of the smoother iterate. F32 is the reference path. The other values run
+ * every diffusion and pressure solve of K passes as ceil(K/m) refinement cycles: an FP32
+ * defect pass r = rhs - A x, then up to m smoother passes on A e = r from e = 0 with the
+ * correction e stored in the named format (scaled per cycle by a power of two >= max|r|),
+ * then x += e. In exact arithmetic the cycles equal the K plain passes. F32_REFINED runs
+ * the same cycles with an FP32 correction and exists to check them. Not available with
+ * the RBGS smoother on a periodic axis of odd extent. */
 typedef enum StableFluidsExtIterateStorage {
-    STABLE_FLUIDS_EXT_STORAGE_F32  = 0,
-    STABLE_FLUIDS_EXT_STORAGE_F16  = 1,
-    STABLE_FLUIDS_EXT_STORAGE_BF16 = 2,
+    STABLE_FLUIDS_EXT_STORAGE_F32         = 0,
+    STABLE_FLUIDS_EXT_STORAGE_F16         = 1,
+    STABLE_FLUIDS_EXT_STORAGE_BF16        = 2,
+    STABLE_FLUIDS_EXT_STORAGE_F32_REFINED = 3,
 } StableFluidsExtIterateStorage;
 
 typedef enum StableFluidsExtWarmStart {
@@ -35,7 +43,7 @@ typedef struct StableFluidsExtOptions {
     uint32_t struct_size;                /* sizeof(StableFluidsExtOptions) */
     uint32_t smoother;                   /* StableFluidsExtSmoother */
     uint32_t iterate_storage;            /* StableFluidsExtIterateStorage */
-    uint32_t defect_correction_interval; /* sweeps between FP32 defect corrections (16-bit storage); 0 = 10 */
+    uint32_t defect_correction_interval; /* m, passes per refinement cycle; 0 = 10, at most 4096 */
     uint32_t pressure_warm_start;        /* StableFluidsExtWarmStart */
     uint32_t reserved[3];                /* must be zero */
 } StableFluidsExtOptions;

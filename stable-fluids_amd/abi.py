@@ -46,6 +46,7 @@ SMOOTHER_JACOBI = 1
 STORAGE_F32 = 0
 STORAGE_F16 = 1
 STORAGE_BF16 = 2
+STORAGE_F32_REFINED = 3
 WARM_START_ZERO = 0
 WARM_START_LEARNED = 1
 

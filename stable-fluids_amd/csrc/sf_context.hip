@@ -38,6 +38,9 @@ struct Range {
 
 struct Options {
     SmootherKind smoother = kSmoothRBGS;
+    bool refine           = false;     // refinement cycles (sf_mixed.hip) instead of plain passes
+    StorageKind storage   = kStoreF32; // correction iterate storage of the cycles
+    int interval          = 10;        // smoother passes per refinement cycle
 };
 
 struct ScalarField {
@@ -76,6 +79,13 @@ struct Context {
     std::vector<ScalarField> scalars;
     std::vector<VectorField> vectors;
     double* norm_partials = nullptr; // kNormBlocks * 4 partial sums, then 4 results
+    struct {
+        float* r     = nullptr; // FP32 defect
+        void* e[2]   = {nullptr, nullptr}; // correction ping-pong (storage format)
+        float* slots = nullptr; // max|r| per (solve, cycle), cleared at step start
+        int cycles   = 0;       // slots per solve
+        int count    = 0;
+    } mixed;
     std::vector<void*> allocations;
 
     ~Context() {
@@ -175,8 +185,44 @@ bool valid_scalar_axis(const StableFluidsScalarBoundaryFaceDesc& lo, const Stabl
         if ((expr) != hipSuccess) return hipErrorUnknown; \
     } while (0)
 
+StencilBC homogeneous(StencilBC bc) {
+    for (auto& f : bc.face) f.value = 0.0f;
+    return bc;
+}
+
+// K smoother passes as refinement cycles with a 16-bit correction iterate (sf_mixed.hip):
+// per cycle one FP32 defect pass, then up to `interval` correction passes, and a final
+// pass writing (x + s*e) * last_scale into `out`. x ping-pongs between aux and out.
+hipError_t enqueue_solve_mixed(Context& c, SolveKind solve, const float* b, const float* x0, float* out, float* aux, int passes, float coef, float last_scale, const StencilBC& bc, int slot_base) {
+    auto& m                = c.mixed;
+    const StencilBC bc0    = homogeneous(bc);
+    const float corr_coef  = solve == kSolvePoisson ? -1.0f : coef;
+    const int cycles       = (passes + c.options.interval - 1) / c.options.interval;
+    const float* x         = x0;
+    const void* e          = nullptr;
+    const float* e_slot    = nullptr;
+    for (int cyc = 0; cyc < cycles; ++cyc) {
+        float* slot  = m.slots + slot_base + cyc;
+        float* x_out = nullptr;
+        if (e != nullptr) x_out = (x == aux) ? out : aux;
+        SF_TRY(launch_defect(c.stream, solve, c.options.storage, x, e, e_slot, b, coef, x_out, m.r, slot, c.dims, bc));
+        if (x_out != nullptr) x = x_out;
+        const int sweeps = std::min(c.options.interval, passes - cyc * c.options.interval);
+        const void* src  = nullptr;
+        for (int i = 0; i < sweeps; ++i) {
+            void* dst = m.e[i & 1];
+            SF_TRY(launch_correction(c.stream, c.options.smoother, solve, c.options.storage, CorrectionPass{src, m.r, dst, corr_coef, slot}, c.dims, bc0));
+            src = dst;
+        }
+        e      = src;
+        e_slot = slot;
+    }
+    return launch_finalize(c.stream, c.options.storage, out, x, e, e_slot, last_scale, c.cells);
+}
+
 // K smoother passes from x0 (nullptr = zero guess) into `out`, ping-ponging with `aux`.
-hipError_t enqueue_solve(Context& c, SolveKind solve, const float* b, const float* x0, float* out, float* aux, int passes, float coef, float last_scale, const StencilBC& bc) {
+hipError_t enqueue_solve(Context& c, SolveKind solve, const float* b, const float* x0, float* out, float* aux, int passes, float coef, float last_scale, const StencilBC& bc, int slot_base) {
+    if (c.options.refine) return enqueue_solve_mixed(c, solve, b, x0, out, aux, passes, coef, last_scale, bc, slot_base);
     const float* src = x0;
     for (int i = 1; i <= passes; ++i) {
         float* dst = ((passes - i) % 2 == 0) ? out : aux;
@@ -194,6 +240,7 @@ hipError_t enqueue_step(Context& c) {
     const float h   = cfg.cell_size;
     auto& f         = c.flow;
 
+    if (c.mixed.count > 0) SF_TRY(launch_fill(c.stream, c.mixed.slots, 0.0f, static_cast<uint64_t>(c.mixed.count)));
     for (auto& v : c.vectors) {
         if (v.desc.usage != STABLE_FLUIDS_VECTOR_FIELD_FORCE) continue;
         SF_TRY(launch_add_force(c.stream, f.vx, f.vy, f.vz, v.x, v.y, v.z, dt, c.cells));
@@ -212,7 +259,7 @@ hipError_t enqueue_step(Context& c) {
             float* t[3] = {f.tx, f.ty, f.tz};
             float* a[3] = {f.ax, f.ay, f.az};
             float* v[3] = {f.vx, f.vy, f.vz};
-            for (int comp = 0; comp < 3; ++comp) SF_TRY(enqueue_solve(c, kSolveDiffusion, t[comp], t[comp], a[comp], v[comp], cfg.diffuse_iterations, alpha, 1.0f, velocity_rules(fb, comp)));
+            for (int comp = 0; comp < 3; ++comp) SF_TRY(enqueue_solve(c, kSolveDiffusion, t[comp], t[comp], a[comp], v[comp], cfg.diffuse_iterations, alpha, 1.0f, velocity_rules(fb, comp), comp * c.mixed.cycles));
             u[0] = f.ax;
             u[1] = f.ay;
             u[2] = f.az;
@@ -231,13 +278,14 @@ hipError_t enqueue_step(Context& c) {
     const StencilBC p_bc   = pressure_rules(fb);
     SF_TRY(launch_divergence(c.stream, f.div, u[0], u[1], u[2], c.dims, h, div_bc));
     if (cfg.pressure_iterations > 0)
-        SF_TRY(enqueue_solve(c, kSolvePoisson, f.div, nullptr, f.p, f.p_aux, cfg.pressure_iterations, cfg.cell_size * cfg.cell_size, 1.0f, p_bc));
+        SF_TRY(enqueue_solve(c, kSolvePoisson, f.div, nullptr, f.p, f.p_aux, cfg.pressure_iterations, cfg.cell_size * cfg.cell_size, 1.0f, p_bc, 3 * c.mixed.cycles));
     else
         SF_TRY(launch_fill(c.stream, f.p, 0.0f, c.cells));
     SF_TRY(launch_project(c.stream, f.vx, f.vy, f.vz, f.div, f.mag, u[0], u[1], u[2], f.p, c.dims, h, p_bc, div_bc));
 
     // scalars (reference :652-668)
-    for (auto& s : c.scalars) {
+    for (size_t si = 0; si < c.scalars.size(); ++si) {
+        auto& s = c.scalars[si];
         SF_TRY(launch_add_scaled(c.stream, s.data, s.source, dt, c.cells));
         SF_TRY(launch_advect_scalar(c.stream, s.temp, s.data, f.vx, f.vy, f.vz, dt, c.dims, h, s.desc.boundary, fb));
         const float alpha_s = cfg.dt * s.desc.diffusion / (cfg.cell_size * cfg.cell_size);
@@ -247,7 +295,7 @@ hipError_t enqueue_step(Context& c) {
             if (cfg.diffuse_iterations <= 0)
                 SF_TRY(launch_copy_scaled(c.stream, s.data, s.temp, factor, c.cells));
             else
-                SF_TRY(enqueue_solve(c, kSolveDiffusion, s.temp, s.temp, s.data, s.aux, cfg.diffuse_iterations, alpha_s, factor, scalar_rules(s.desc.boundary)));
+                SF_TRY(enqueue_solve(c, kSolveDiffusion, s.temp, s.temp, s.data, s.aux, cfg.diffuse_iterations, alpha_s, factor, scalar_rules(s.desc.boundary), static_cast<int>(4 + si) * c.mixed.cycles));
         } else if (dissipate) {
             SF_TRY(launch_copy_scaled(c.stream, s.data, s.data, factor, c.cells));
         }
@@ -284,11 +332,15 @@ StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const Stabl
     if (opt != nullptr) {
         if (opt->struct_size != sizeof(StableFluidsExtOptions)) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         if (opt->smoother > STABLE_FLUIDS_EXT_SMOOTHER_JACOBI) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
-        if (opt->iterate_storage != STABLE_FLUIDS_EXT_STORAGE_F32) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        if (opt->iterate_storage > STABLE_FLUIDS_EXT_STORAGE_F32_REFINED) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        if (opt->defect_correction_interval > 4096) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         if (opt->pressure_warm_start != STABLE_FLUIDS_EXT_WARM_START_ZERO) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         for (uint32_t v : opt->reserved)
             if (v != 0) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         options.smoother = opt->smoother == STABLE_FLUIDS_EXT_SMOOTHER_JACOBI ? kSmoothJacobi : kSmoothRBGS;
+        options.refine   = opt->iterate_storage != STABLE_FLUIDS_EXT_STORAGE_F32;
+        options.storage  = opt->iterate_storage == STABLE_FLUIDS_EXT_STORAGE_F16 ? kStoreF16 : (opt->iterate_storage == STABLE_FLUIDS_EXT_STORAGE_BF16 ? kStoreBF16 : kStoreF32);
+        if (opt->defect_correction_interval > 0) options.interval = static_cast<int>(opt->defect_correction_interval);
     }
     if (cfg.nx <= 0 || cfg.ny <= 0 || cfg.nz <= 0) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     const auto& fb = cfg.flow_boundary;
@@ -298,6 +350,16 @@ StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const Stabl
     for (uint32_t i = 0; i < desc->scalar_field_count; ++i) {
         const auto& b = desc->scalar_fields[i].boundary;
         if (!valid_scalar_axis(b.x_minus, b.x_plus) || !valid_scalar_axis(b.y_minus, b.y_plus) || !valid_scalar_axis(b.z_minus, b.z_plus)) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    }
+
+    // the narrow-storage RBGS correction runs on the split smoother, which needs even
+    // extents on periodic axes
+    if (options.refine && options.smoother == kSmoothRBGS) {
+        const Dims d{cfg.nx, cfg.ny, cfg.nz};
+        auto odd_periodic = [&](const int mask) { return ((mask & 1) && (d.nx & 1)) || ((mask & 2) && (d.ny & 1)) || ((mask & 4) && (d.nz & 1)); };
+        if (odd_periodic(flow_periodic_mask(fb))) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        for (uint32_t i = 0; i < desc->scalar_field_count; ++i)
+            if (odd_periodic(scalar_rules(desc->scalar_fields[i].boundary).periodic)) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     }
 
     std::unique_ptr<Context> c{new (std::nothrow) Context{}};
@@ -333,6 +395,14 @@ StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const Stabl
             for (float** p : {&v.x, &v.y, &v.z})
                 if (!c->alloc(*p, c->bytes)) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
         if (!c->alloc(c->norm_partials, sizeof(double) * (kNormBlocks * 4 + 4))) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
+        if (c->options.refine) {
+            auto& m          = c->mixed;
+            const size_t eb  = c->cells * (c->options.storage == kStoreF32 ? 4 : 2);
+            const int passes = std::max(std::max(cfg.diffuse_iterations, cfg.pressure_iterations), 1);
+            m.cycles         = (passes + c->options.interval - 1) / c->options.interval;
+            m.count          = (4 + static_cast<int>(c->scalars.size())) * m.cycles;
+            if (!c->alloc(m.r, c->bytes) || !c->alloc(m.e[0], eb) || !c->alloc(m.e[1], eb) || !c->alloc(m.slots, sizeof(float) * static_cast<size_t>(m.count))) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
+        }
     }
 
     {

@@ -29,6 +29,24 @@ struct SmoothPass {
     float out_scale;
 };
 
+// Element storage of a correction iterate (arithmetic is FP32 in every case).
+enum StorageKind : int {
+    kStoreF32  = 0,
+    kStoreF16  = 1,
+    kStoreBF16 = 2,
+};
+
+// One smoother pass of a correction solve A e = r with homogeneous boundary rules: reads
+// e_old (zeros when nullptr) in `storage` and r in FP32, writes e_new in `storage`.
+// The iterate is held scaled by 1/s, s the power of two at least *scale_slot (max|r|).
+struct CorrectionPass {
+    const void* e_old;
+    const float* r;
+    void* e_new;
+    float coef; // diffusion: dt*diffusivity/h^2; Poisson: -1 (6e - sum = r)
+    const float* scale_slot;
+};
+
 hipError_t launch_fill(hipStream_t s, float* p, float value, uint64_t n);
 hipError_t launch_copy_scaled(hipStream_t s, float* dst, const float* src, float factor, uint64_t n);
 hipError_t launch_add_scaled(hipStream_t s, float* dst, const float* src, float dt, uint64_t n);
@@ -36,6 +54,14 @@ hipError_t launch_add_force(hipStream_t s, float* vx, float* vy, float* vz, cons
 hipError_t launch_advect_velocity(hipStream_t s, float* tx, float* ty, float* tz, const float* vx, const float* vy, const float* vz, float dt, Dims d, float h, const StableFluidsFlowBoundaryConfig& bc);
 hipError_t launch_advect_scalar(hipStream_t s, float* out, const float* src, const float* vx, const float* vy, const float* vz, float dt, Dims d, float h, const StableFluidsScalarBoundaryConfig& sbc, const StableFluidsFlowBoundaryConfig& fbc);
 hipError_t launch_smooth(hipStream_t s, SmootherKind smoother, SolveKind solve, const SmoothPass& pass, Dims d, const StencilBC& bc);
+hipError_t launch_correction(hipStream_t s, SmootherKind smoother, SolveKind solve, StorageKind storage, const CorrectionPass& pass, Dims d, const StencilBC& homogeneous_bc);
+// Defect of a refinement cycle: x' = x_in + s_e*e (x_in nullptr = zeros, e nullptr = no
+// correction yet, s_e from e_slot), r = rhs - A x' with the full boundary rules
+// (diffusion: rhs = b; Poisson: rhs = -coef*b), max|r| atomically into *r_slot (zeroed
+// beforehand). x' is written to x_out when e is given.
+hipError_t launch_defect(hipStream_t s, SolveKind solve, StorageKind storage, const float* x_in, const void* e, const float* e_slot, const float* b, float coef, float* x_out, float* r, float* r_slot, Dims d, const StencilBC& bc);
+// out = (x + s_e*e) * out_scale, element-wise (out may equal x).
+hipError_t launch_finalize(hipStream_t s, StorageKind storage, float* out, const float* x, const void* e, const float* e_slot, float out_scale, uint64_t n);
 // divergence of (vx,vy,vz); bc.face[2c+side] is the rule of component c on axis c.
 hipError_t launch_divergence(hipStream_t s, float* div, const float* vx, const float* vy, const float* vz, Dims d, float h, const StencilBC& div_bc);
 // v_out = v_in - grad p; div and |v| of the projected field, all in one pass.

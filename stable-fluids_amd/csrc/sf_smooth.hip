@@ -28,6 +28,7 @@
 #include <type_traits>
 
 #include "sf_kernels.hpp"
+#include "sf_storage.hpp"
 
 namespace sf {
 
@@ -133,6 +134,27 @@ __device__ __forceinline__ FaceRule face_rule(const StencilBC& bc, const int f) 
     return FaceRule{mode, value};
 }
 
+// Reciprocal of the power-of-two scale of a narrow-storage correction solve (the slot
+// holds max|r| of its right-hand side); 1 for FP32 storage.
+template <class St>
+__device__ __forceinline__ float rhs_scale(const float* __restrict__ slot) {
+    if constexpr (St::kNarrow) return slot ? 1.0f / correction_scale<St>(slot) : 1.0f;
+    else return 1.0f;
+}
+
+// Stores the aligned pair (a, b) at dst[0], dst[1] with one vector store.
+template <class St>
+__device__ __forceinline__ void store_vec2(typename St::T* dst, const float a, const float b) {
+    if constexpr (St::kNarrow) {
+        struct alignas(2 * sizeof(typename St::T)) P {
+            typename St::T x, y;
+        };
+        *reinterpret_cast<P*>(dst) = P{St::store(a), St::store(b)};
+    } else {
+        *reinterpret_cast<float2*>(dst) = make_float2(a, b);
+    }
+}
+
 // Ghost code of a halo position: 0 inside (or periodic), else 1 + face index.
 __device__ __forceinline__ int ghost_code(const int gx, const int gy, const Dims d, const bool px, const bool py) {
     if (!px && gx < 0) return 1;
@@ -145,14 +167,14 @@ __device__ __forceinline__ int ghost_code(const int gx, const int gy, const Dims
 // Loads plane k of a row-major region (`per` floats per lane) into registers, applying
 // the ghost rules: in-grid planes take the x/y face rule of ghost positions, out-of-grid
 // planes (non-periodic z) take the z face rule of the adjacent edge plane.
-template <int kPer, bool kZero>
-__device__ __forceinline__ void fetch_region(float (&r)[kPer], const float* __restrict__ src, const int (&off)[kPer], const int (&gc)[kPer], const int k, const Dims d, const bool pz, const uint64_t plane, const StencilBC& bc) {
+template <int kPer, bool kZero, class St = StoreF32>
+__device__ __forceinline__ void fetch_region(float (&r)[kPer], const typename St::T* __restrict__ src, const int (&off)[kPer], const int (&gc)[kPer], const int k, const Dims d, const bool pz, const uint64_t plane, const StencilBC& bc) {
     const bool below = !pz && k < 0, above = !pz && k >= d.nz;
     const uint64_t base = static_cast<uint64_t>(fix_coord(k, d.nz, pz)) * plane;
 #pragma unroll
     for (int i = 0; i < kPer; ++i) {
         float v = 0.0f;
-        if (!kZero && off[i] >= 0) v = src[base + static_cast<uint64_t>(off[i])];
+        if (!kZero && off[i] >= 0) v = St::load(src[base + static_cast<uint64_t>(off[i])]);
         if (below)
             v = apply_ghost(bc.face[4], v);
         else if (above)
@@ -347,9 +369,10 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_pass(const float* __restrict_
     }
 }
 
-template <int kSolve, bool kZero>
-__global__ __launch_bounds__(kThreads) void k_jacobi_pass(const float* __restrict__ x_old, const float* __restrict__ b, float* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g) {
+template <int kSolve, bool kZero, class St>
+__global__ __launch_bounds__(kThreads) void k_jacobi_pass(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g, const float* __restrict__ scale_slot) {
     __shared__ float2 s_old[3][kHPX * kHY];
+    const float inv_s = rhs_scale<St>(scale_slot);
 
     int tx, ty, cz;
     tile_of(g, tx, ty, cz);
@@ -389,8 +412,8 @@ __global__ __launch_bounds__(kThreads) void k_jacobi_pass(const float* __restric
         for (int i = 0; i < kOutPer; ++i) {
             const int fy  = (tid >> 6) + (kThreads / 64) * i;
             const float* p = b + base + static_cast<uint64_t>((y0 + fy) * d.nx + x0 + 2 * fp);
-            if (f_info[i] & 1) r_b[i].x = p[0];
-            if (f_info[i] & 2) r_b[i].y = p[1];
+            if (f_info[i] & 1) r_b[i].x = p[0] * inv_s;
+            if (f_info[i] & 2) r_b[i].y = p[1] * inv_s;
         }
     };
     auto stash_old = [&](const int k) {
@@ -400,11 +423,11 @@ __global__ __launch_bounds__(kThreads) void k_jacobi_pass(const float* __restric
             if (tid + i * kThreads < kJOldF) s[tid + i * kThreads] = r_old[i];
     };
 
-    fetch_region<kJOldPer, kZero>(r_old, x_old, o_off, o_gc, z0 - 1, d, pz, plane, bc);
+    fetch_region<kJOldPer, kZero, St>(r_old, x_old, o_off, o_gc, z0 - 1, d, pz, plane, bc);
     stash_old(z0 - 1);
-    fetch_region<kJOldPer, kZero>(r_old, x_old, o_off, o_gc, z0, d, pz, plane, bc);
+    fetch_region<kJOldPer, kZero, St>(r_old, x_old, o_off, o_gc, z0, d, pz, plane, bc);
     stash_old(z0);
-    fetch_region<kJOldPer, kZero>(r_old, x_old, o_off, o_gc, z0 + 1, d, pz, plane, bc);
+    fetch_region<kJOldPer, kZero, St>(r_old, x_old, o_off, o_gc, z0 + 1, d, pz, plane, bc);
     fetch_b(z0);
     for (int k = z0; k < z1; ++k) {
         stash_old(k + 1);
@@ -412,7 +435,7 @@ __global__ __launch_bounds__(kThreads) void k_jacobi_pass(const float* __restric
 #pragma unroll
         for (int i = 0; i < kOutPer; ++i) bv[i] = r_b[i];
         if (k + 1 < z1) {
-            fetch_region<kJOldPer, kZero>(r_old, x_old, o_off, o_gc, k + 2, d, pz, plane, bc);
+            fetch_region<kJOldPer, kZero, St>(r_old, x_old, o_off, o_gc, k + 2, d, pz, plane, bc);
             fetch_b(k + 1);
         }
         __syncthreads();
@@ -427,14 +450,14 @@ __global__ __launch_bounds__(kThreads) void k_jacobi_pass(const float* __restric
             const int fy   = (tid >> 6) + (kThreads / 64) * i;
             const int c    = (fy + 1) * kHPX + (fp + 1);
             const float2 L = so[c - 1], C = so[c], R = so[c + 1], YM = so[c - kHPX], YP = so[c + kHPX], ZM = sm[c], ZP = sp[c];
-            float* dst     = x_new + base + static_cast<uint64_t>((y0 + fy) * d.nx + x0 + 2 * fp);
+            auto* dst      = x_new + base + static_cast<uint64_t>((y0 + fy) * d.nx + x0 + 2 * fp);
             const float v0 = relax_elem<kSolve>(0, L, C, R, YM, YP, ZM, ZP, bv[i].x, coef) * out_scale;
             if ((d.nx & 1) == 0) {
                 const float v1 = relax_elem<kSolve>(1, L, C, R, YM, YP, ZM, ZP, bv[i].y, coef) * out_scale;
-                *reinterpret_cast<float2*>(dst) = make_float2(v0, v1);
+                store_vec2<St>(dst, v0, v1);
             } else {
-                dst[0] = v0;
-                if (info & 2) dst[1] = relax_elem<kSolve>(1, L, C, R, YM, YP, ZM, ZP, bv[i].y, coef) * out_scale;
+                dst[0] = St::store(v0);
+                if (info & 2) dst[1] = St::store(relax_elem<kSolve>(1, L, C, R, YM, YP, ZM, ZP, bv[i].y, coef) * out_scale);
             }
         }
         __syncthreads();
@@ -462,10 +485,11 @@ __device__ __forceinline__ int wrap_near(const int k, const int n) {
     return k < 0 ? k + n : (k >= n ? k - n : k);
 }
 
-template <int kSolve, bool kZero>
-__global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const float* __restrict__ x_old, const float* __restrict__ b, float* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g) {
+template <int kSolve, bool kZero, class St>
+__global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g, const float* __restrict__ scale_slot) {
     __shared__ float s_old[3][kOldPlane];
     __shared__ float s_h[3][kHPlane];
+    const float inv_s = rhs_scale<St>(scale_slot);
 
     int tx, ty, cz;
     tile_of(g, tx, ty, cz);
@@ -539,9 +563,9 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const float* __res
 
     float r_old[kOldPer], r_bh[kHSlots3], r_bf[kOutPer];
     auto fetch_old = [&](const int k) {
-        const float* src = x_old + static_cast<uint64_t>(plane_of(k)) * plane;
+        const typename St::T* src = x_old + static_cast<uint64_t>(plane_of(k)) * plane;
 #pragma unroll
-        for (int j = 0; j < kOldPer; ++j) r_old[j] = kZero ? 0.0f : src[o_off[j]];
+        for (int j = 0; j < kOldPer; ++j) r_old[j] = kZero ? 0.0f : St::load(src[o_off[j]]);
         if (!pz && (k < 0 || k >= d.nz)) {
             const FaceRule f = k < 0 ? FaceRule{bc.face[4].mode, bc.face[4].value} : FaceRule{bc.face[5].mode, bc.face[5].value};
 #pragma unroll
@@ -557,7 +581,10 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const float* __res
     auto fetch_bh = [&](const int k, const bool odd) {
         const float* src = b + static_cast<uint64_t>(plane_of(k)) * plane;
 #pragma unroll
-        for (int j = 0; j < kHSlots3; ++j) r_bh[j] = src[odd ? h_ob1[j] : h_ob0[j]];
+        for (int j = 0; j < kHSlots3; ++j) {
+            r_bh[j] = src[odd ? h_ob1[j] : h_ob0[j]];
+            if constexpr (St::kNarrow) r_bh[j] *= inv_s;
+        }
     };
     auto fetch_bf = [&](const int k) {
         const float* src = b + static_cast<uint64_t>(k) * plane;
@@ -567,6 +594,7 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const float* __res
             const int e = ((y + k) & 1) == 0 ? 1 : 0; // black cell: odd when the even one is red
             const int x = min(x0 + 2 * lane + e, d.nx - 1);
             r_bf[j]     = src[static_cast<uint32_t>(y * d.nx + x)];
+            if constexpr (St::kNarrow) r_bf[j] *= inv_s;
         }
     };
     auto stash_old = [&](float* s) {
@@ -623,7 +651,7 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const float* __res
         // final plane k-1: black cells relaxed from the red values, red cells copied
         const int kf = k - 1;
         if (kf >= z0 && kf < z1) {
-            float* dst_plane = x_new + static_cast<uint64_t>(kf) * plane;
+            auto* dst_plane = x_new + static_cast<uint64_t>(kf) * plane;
 #pragma unroll
             for (int j = 0; j < kOutPer; ++j) {
                 const int rr = wrow + (kThreads / 64) * j;
@@ -648,14 +676,14 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const float* __res
                 const float ve    = (even_red ? red : black) * out_scale;
                 const float vo    = (even_red ? black : red) * out_scale;
                 const int x       = x0 + 2 * lane;
-                float* dst        = dst_plane + static_cast<uint32_t>(y * d.nx + x);
+                auto* dst         = dst_plane + static_cast<uint32_t>(y * d.nx + x);
                 if (full_x) {
-                    *reinterpret_cast<float2*>(dst) = make_float2(ve, vo);
+                    store_vec2<St>(dst, ve, vo);
                 } else if (even_nx) {
-                    if (x < d.nx) *reinterpret_cast<float2*>(dst) = make_float2(ve, vo);
+                    if (x < d.nx) store_vec2<St>(dst, ve, vo);
                 } else {
-                    if (x < d.nx) dst[0] = ve;
-                    if (x + 1 < d.nx) dst[1] = vo;
+                    if (x < d.nx) dst[0] = St::store(ve);
+                    if (x + 1 < d.nx) dst[1] = St::store(vo);
                 }
             }
         }
@@ -717,7 +745,8 @@ hipError_t launch_smooth(hipStream_t s, const SmootherKind smoother, const Solve
     const TileGeom g      = plan_tiles(d);
     const unsigned blocks = static_cast<unsigned>(g.tiles_x * g.tiles_y * g.chunks);
     const bool zero       = pass.x_old == nullptr;
-#define SF_PASS(KERNEL, SO, Z) KERNEL<SO, Z><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g)
+#define SF_PASS_GENERAL(SO, Z) k_rbgs_pass<SO, Z><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g)
+#define SF_PASS(KERNEL, SO, Z) KERNEL<SO, Z, StoreF32><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g, nullptr)
     if (smoother == kSmoothRBGS && !has_odd_periodic(d, bc)) {
         if (solve == kSolvePoisson) {
             if (zero) SF_PASS(k_rbgs_split_pass, kSolvePoisson, true);
@@ -728,11 +757,11 @@ hipError_t launch_smooth(hipStream_t s, const SmootherKind smoother, const Solve
         }
     } else if (smoother == kSmoothRBGS) {
         if (solve == kSolvePoisson) {
-            if (zero) SF_PASS(k_rbgs_pass, kSolvePoisson, true);
-            else SF_PASS(k_rbgs_pass, kSolvePoisson, false);
+            if (zero) SF_PASS_GENERAL(kSolvePoisson, true);
+            else SF_PASS_GENERAL(kSolvePoisson, false);
         } else {
-            if (zero) SF_PASS(k_rbgs_pass, kSolveDiffusion, true);
-            else SF_PASS(k_rbgs_pass, kSolveDiffusion, false);
+            if (zero) SF_PASS_GENERAL(kSolveDiffusion, true);
+            else SF_PASS_GENERAL(kSolveDiffusion, false);
         }
     } else {
         if (solve == kSolvePoisson) {
@@ -744,7 +773,51 @@ hipError_t launch_smooth(hipStream_t s, const SmootherKind smoother, const Solve
         }
     }
 #undef SF_PASS
+#undef SF_PASS_GENERAL
     return hipGetLastError();
+}
+
+namespace {
+
+template <class St>
+hipError_t correction_pass(hipStream_t s, const SmootherKind smoother, const SolveKind solve, const CorrectionPass& pass, const Dims d, const StencilBC& bc) {
+    const TileGeom g      = plan_tiles(d);
+    const unsigned blocks = static_cast<unsigned>(g.tiles_x * g.tiles_y * g.chunks);
+    const auto* e_old     = static_cast<const typename St::T*>(pass.e_old);
+    auto* e_new           = static_cast<typename St::T*>(pass.e_new);
+#define SF_CORR(KERNEL, SO, Z) KERNEL<SO, Z, St><<<blocks, kThreads, 0, s>>>(e_old, pass.r, e_new, pass.coef, 1.0f, d, bc, g, pass.scale_slot)
+    const bool zero = e_old == nullptr;
+    if (smoother == kSmoothRBGS) {
+        if (solve == kSolvePoisson) {
+            if (zero) SF_CORR(k_rbgs_split_pass, kSolvePoisson, true);
+            else SF_CORR(k_rbgs_split_pass, kSolvePoisson, false);
+        } else {
+            if (zero) SF_CORR(k_rbgs_split_pass, kSolveDiffusion, true);
+            else SF_CORR(k_rbgs_split_pass, kSolveDiffusion, false);
+        }
+    } else {
+        if (solve == kSolvePoisson) {
+            if (zero) SF_CORR(k_jacobi_pass, kSolvePoisson, true);
+            else SF_CORR(k_jacobi_pass, kSolvePoisson, false);
+        } else {
+            if (zero) SF_CORR(k_jacobi_pass, kSolveDiffusion, true);
+            else SF_CORR(k_jacobi_pass, kSolveDiffusion, false);
+        }
+    }
+#undef SF_CORR
+    return hipGetLastError();
+}
+
+} // namespace
+
+hipError_t launch_correction(hipStream_t s, const SmootherKind smoother, const SolveKind solve, const StorageKind storage, const CorrectionPass& pass, const Dims d, const StencilBC& homogeneous_bc) {
+    // the split RBGS kernel covers every grid except an odd periodic axis
+    if (smoother == kSmoothRBGS && has_odd_periodic(d, homogeneous_bc)) return hipErrorInvalidValue;
+    switch (storage) {
+    case kStoreF16: return correction_pass<StoreF16>(s, smoother, solve, pass, d, homogeneous_bc);
+    case kStoreBF16: return correction_pass<StoreBF16>(s, smoother, solve, pass, d, homogeneous_bc);
+    default: return correction_pass<StoreF32>(s, smoother, solve, pass, d, homogeneous_bc);
+    }
 }
 
 } // namespace sf

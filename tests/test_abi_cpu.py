@@ -130,9 +130,14 @@ def test_bad_ext_options_rejected(abi):
     lib = sf.load_library()
     d = _desc(abi)
     h = C.c_void_p()
-    for kw in ({"smoother": 5}, {"pressure_warm_start": 9}, {"iterate_storage": 7}):
+    for kw in ({"smoother": 5}, {"pressure_warm_start": 9}, {"iterate_storage": 7}, {"defect_correction_interval": 4097}):
         o = sf.ext_options(**kw)
         assert lib.stable_fluids_ext_create_context_cuda(C.byref(d.desc), C.byref(o), C.byref(h), None, 0, None, 0) == abi.RESULT_BACKEND_FAILURE
+    # refinement cycles with the RBGS smoother need even extents on periodic axes
+    odd = _desc(abi, flow_types=[abi.FLOW_PERIODIC] * 2 + [abi.FLOW_FREE_SLIP] * 4, n=(7, 8, 8))
+    for storage in (abi.STORAGE_F16, abi.STORAGE_BF16, abi.STORAGE_F32_REFINED):
+        o = sf.ext_options(iterate_storage=storage)
+        assert lib.stable_fluids_ext_create_context_cuda(C.byref(odd.desc), C.byref(o), C.byref(h), None, 0, None, 0) == abi.RESULT_BACKEND_FAILURE
     o = sf.ext_options()
     o.struct_size = 12
     assert lib.stable_fluids_ext_create_context_cuda(C.byref(d.desc), C.byref(o), C.byref(h), None, 0, None, 0) == abi.RESULT_BACKEND_FAILURE

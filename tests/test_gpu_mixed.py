@@ -1,0 +1,136 @@
+"""Refinement cycles with 16-bit correction storage (iterate_storage F16/BF16/F32_REFINED).
+
+Every diffusion and pressure solve of K passes runs as ceil(K/m) cycles: FP32 defect
+r = rhs - A x with the full boundary rules, m passes on A e = r from e = 0 with
+homogeneous rules and e stored in the narrow format, x += e. The smoother passes are
+affine, so in exact arithmetic the cycles give exactly the K plain passes. The checks:
+  * F32_REFINED (same cycles, FP32 correction) meets the plain path's FP64 bar (1e-5
+    per field) for m = 1, 3, 10 and one cycle: a wrong defect sign, a boundary value
+    left in the correction rules or a lost cycle all miss it by orders of magnitude;
+  * F16 / BF16 per-field error against the FP64 oracle on the 64^3 plume (3 steps of
+    the chaotic scene, 50 steps of the well-conditioned one), with stated bars;
+  * with enough passes to converge, the cycles reach the FP32 fixed point whatever the
+    correction storage: within 1e-5 of the converged FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from conftest import sf, sf_oracle
+from paritylib import Inputs, normwise, report, run_gpu, run_oracle
+
+abi = sf.abi
+scenes = sf.scenes
+pytestmark = pytest.mark.gpu
+
+# per-field bars of the narrow formats against FP64 on the 64^3 plume after 3 steps
+# (measured on MI355X: FP16 7.0e-5, BF16 6.7e-4 worst field)
+NARROW_TOL = {abi.STORAGE_F16: 2e-4, abi.STORAGE_BF16: 2e-3}
+
+
+def small_desc(n=(16, 12, 8), d_iters=6, p_iters=8):
+    """Periodic x (even), no-slip moving floor, outflow top (pressure 0.5), free-slip z-,
+    no-slip z+; two scalars with fixed / zero-flux / periodic faces; one force field."""
+    nx, ny, nz = n
+    wall_v = [(0, 0, 0), (0, 0, 0), (0.3, 0.0, 0.1), (0, 0, 0), (0.05, -0.2, 0.0), (0.1, 0.0, 0.0)]
+    flow = abi.flow_boundary([abi.FLOW_PERIODIC, abi.FLOW_PERIODIC, abi.FLOW_NO_SLIP, abi.FLOW_OUTFLOW,
+                              abi.FLOW_FREE_SLIP, abi.FLOW_NO_SLIP], velocity=wall_v, pressure=[0, 0, 0, 0.5, 0, 0])
+    s1 = abi.scalar_boundary([abi.SCALAR_PERIODIC, abi.SCALAR_PERIODIC, abi.SCALAR_FIXED, abi.SCALAR_FIXED,
+                              abi.SCALAR_ZERO_FLUX, abi.SCALAR_FIXED], values=[0, 0, 1.0, 0.0, 0, -0.5])
+    s2 = abi.scalar_boundary([abi.SCALAR_FIXED, abi.SCALAR_ZERO_FLUX, abi.SCALAR_PERIODIC, abi.SCALAR_PERIODIC,
+                              abi.SCALAR_ZERO_FLUX, abi.SCALAR_ZERO_FLUX], values=[2.0, 0, 0, 0, 0, 0])
+    scalars = [abi.scalar_desc(1e-3, 0.0, 0.25, s1), abi.scalar_desc(3e-3, 0.7, -1.0, s2)]
+    return abi.Desc(nx, ny, nz, 1.0 / 16, 0.05, 2e-3, d_iters, p_iters, flow, scalars=scalars, vectors=[abi.force_desc(0.1, 0.0, -0.2)])
+
+
+def assert_below(errs, tol, tag):
+    report(tag, errs)
+    bad = {k: v for k, v in errs.items() if v > tol}
+    assert not bad, f"{tag}: above {tol}: {bad}"
+
+
+def oracle_smoother(smoother):
+    return sf_oracle.JACOBI if smoother == abi.SMOOTHER_JACOBI else sf_oracle.RBGS
+
+
+@pytest.mark.parametrize("smoother", [abi.SMOOTHER_RBGS, abi.SMOOTHER_JACOBI])
+@pytest.mark.parametrize("interval", [1, 3, 10, 4096])
+def test_refined_f32_meets_fp64_bar(smoother, interval):
+    desc = small_desc(d_iters=7, p_iters=12)
+    inputs = Inputs(desc, seed=21, velocity_scale=0.5, smooth=True)
+    opts = sf.ext_options(smoother=smoother, iterate_storage=abi.STORAGE_F32_REFINED, defect_correction_interval=interval)
+    gpu = run_gpu(desc, inputs.seed_gpu, 1, options=opts)
+    ref = run_oracle(desc, inputs.seed_oracle, 1, 64, smoother=oracle_smoother(smoother))
+    assert_below(normwise(gpu, ref), 1e-5, f"f32-refined smoother={smoother} m={interval} vs fp64")
+
+
+def test_refined_f32_plume64_matches_plain():
+    """Same cycles on the plume (all walls, singular pressure system): 3 steps."""
+    scene = scenes.plume_scene(64)
+    seed = lambda ctx: scenes.seed_context(ctx, scene)
+    plain = run_gpu(scene.desc, seed, 3)
+    refined = run_gpu(scene.desc, seed, 3, options=sf.ext_options(iterate_storage=abi.STORAGE_F32_REFINED))
+    assert_below(normwise(refined, plain), 1e-5, "plume64 f32-refined vs plain")
+
+
+@pytest.mark.parametrize("storage", [abi.STORAGE_F16, abi.STORAGE_BF16])
+def test_narrow_storage_error_vs_fp64(storage):
+    scene = scenes.plume_scene(64)
+    ref = run_oracle(scene.desc, lambda sim: scenes.seed_oracle(sim, scene), 3, 64)
+    opts = sf.ext_options(iterate_storage=storage)
+    gpu = run_gpu(scene.desc, lambda ctx: scenes.seed_context(ctx, scene), 3, options=opts)
+    assert_below(normwise(gpu, ref), NARROW_TOL[storage], f"plume64 storage={storage} 3 steps vs fp64")
+
+
+def l2_rel(a, b):
+    return {k: float(np.sqrt(np.sum((a[k] - b[k]) ** 2)) / max(np.sqrt(np.sum(b[k] ** 2)), 1e-300)) for k in b}
+
+
+def test_storage_error_gentle_plume64_50_steps_vs_fp64():
+    """The well-conditioned plume (force 0.5, where FP32 itself stays within 1e-5 of FP64
+    for 100 steps): 50 steps of each storage, per-field max and L2 error against FP64."""
+    scene = scenes.plume_scene(64)
+    src, fx, fy, fz = scenes.plume_fields(scene.desc.shape, (32, 6, 32), 4, 100.0, 0.5)
+    scene.force = (fx, fy, fz)
+    ref = run_oracle(scene.desc, lambda sim: scenes.seed_oracle(sim, scene), 50, 64)
+    bars = {abi.STORAGE_F32: 1e-5, abi.STORAGE_F16: 2e-4, abi.STORAGE_BF16: 2e-3}
+    for storage, tol in bars.items():
+        gpu = run_gpu(scene.desc, lambda ctx: scenes.seed_context(ctx, scene), 50, options=sf.ext_options(iterate_storage=storage))
+        report(f"gentle plume64 50 steps storage={storage} L2 vs fp64", l2_rel(gpu, ref))
+        assert_below(normwise(gpu, ref), tol, f"gentle plume64 50 steps storage={storage} max vs fp64")
+
+
+@pytest.mark.parametrize("storage", [abi.STORAGE_F16, abi.STORAGE_BF16])
+@pytest.mark.parametrize("smoother", [abi.SMOOTHER_RBGS, abi.SMOOTHER_JACOBI])
+def test_refinement_converges_to_fp32_fixed_point(storage, smoother):
+    """4000 pressure and 100 diffusion passes converge every system of this 8x6x4 grid
+    (slowest pressure mode ~0.99 per Jacobi pass; the outflow face makes the system
+    regular): the 16-bit correction then does not limit the answer."""
+    desc = small_desc(n=(8, 6, 4), d_iters=100, p_iters=4000)
+    inputs = Inputs(desc, seed=4, velocity_scale=0.5, smooth=True)
+    ref = run_oracle(desc, inputs.seed_oracle, 1, 64, smoother=oracle_smoother(smoother))
+    plain = run_gpu(desc, inputs.seed_gpu, 1, options=sf.ext_options(smoother=smoother))
+    report(f"converged plain fp32 smoother={smoother} vs fp64", normwise(plain, ref))
+    gpu = run_gpu(desc, inputs.seed_gpu, 1, options=sf.ext_options(smoother=smoother, iterate_storage=storage))
+    assert_below(normwise(gpu, ref), 1e-5, f"converged storage={storage} smoother={smoother} vs fp64")
+
+
+def test_jacobi_narrow_storage_odd_periodic():
+    """Jacobi corrections cover odd periodic extents (the RBGS cycles reject them)."""
+    desc = small_desc(n=(7, 9, 5))
+    inputs = Inputs(desc, seed=9, velocity_scale=0.5, smooth=True)
+    opts = sf.ext_options(smoother=abi.SMOOTHER_JACOBI, iterate_storage=abi.STORAGE_F32_REFINED, defect_correction_interval=3)
+    gpu = run_gpu(desc, inputs.seed_gpu, 1, options=opts)
+    ref = run_oracle(desc, inputs.seed_oracle, 1, 64, smoother=sf_oracle.JACOBI)
+    assert_below(normwise(gpu, ref), 1e-5, "jacobi odd periodic f32-refined vs fp64")
+    with pytest.raises(sf.StableFluidsError):
+        sf.Context(desc, options=sf.ext_options(iterate_storage=abi.STORAGE_F16))
+
+
+def test_narrow_storage_repeatable():
+    scene = scenes.plume_scene(32)
+    seed = lambda ctx: scenes.seed_context(ctx, scene)
+    opts = sf.ext_options(iterate_storage=abi.STORAGE_BF16)
+    a = run_gpu(scene.desc, seed, 5, options=opts)
+    b = run_gpu(scene.desc, seed, 5, options=opts)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
