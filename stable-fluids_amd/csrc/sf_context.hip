@@ -82,7 +82,8 @@ struct Context {
     struct {
         float* r     = nullptr; // FP32 defect
         void* e[2]   = {nullptr, nullptr}; // correction ping-pong (storage format)
-        float* slots = nullptr; // max|r| per (solve, cycle), cleared at step start
+        float* partials = nullptr; // per-workgroup max|r| of a defect pass
+        float* slots = nullptr; // max|r| per (solve, cycle)
         int cycles   = 0;       // slots per solve
         int count    = 0;
     } mixed;
@@ -205,7 +206,7 @@ hipError_t enqueue_solve_mixed(Context& c, SolveKind solve, const float* b, cons
         float* slot  = m.slots + slot_base + cyc;
         float* x_out = nullptr;
         if (e != nullptr) x_out = (x == aux) ? out : aux;
-        SF_TRY(launch_defect(c.stream, solve, c.options.storage, x, e, e_slot, b, coef, x_out, m.r, slot, c.dims, bc));
+        SF_TRY(launch_defect(c.stream, solve, c.options.storage, x, e, e_slot, b, coef, x_out, m.r, m.partials, slot, c.dims, bc));
         if (x_out != nullptr) x = x_out;
         const int sweeps = std::min(c.options.interval, passes - cyc * c.options.interval);
         const void* src  = nullptr;
@@ -240,7 +241,6 @@ hipError_t enqueue_step(Context& c) {
     const float h   = cfg.cell_size;
     auto& f         = c.flow;
 
-    if (c.mixed.count > 0) SF_TRY(launch_fill(c.stream, c.mixed.slots, 0.0f, static_cast<uint64_t>(c.mixed.count)));
     for (auto& v : c.vectors) {
         if (v.desc.usage != STABLE_FLUIDS_VECTOR_FIELD_FORCE) continue;
         SF_TRY(launch_add_force(c.stream, f.vx, f.vy, f.vz, v.x, v.y, v.z, dt, c.cells));
@@ -401,7 +401,7 @@ StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const Stabl
             const int passes = std::max(std::max(cfg.diffuse_iterations, cfg.pressure_iterations), 1);
             m.cycles         = (passes + c->options.interval - 1) / c->options.interval;
             m.count          = (4 + static_cast<int>(c->scalars.size())) * m.cycles;
-            if (!c->alloc(m.r, c->bytes) || !c->alloc(m.e[0], eb) || !c->alloc(m.e[1], eb) || !c->alloc(m.slots, sizeof(float) * static_cast<size_t>(m.count))) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
+            if (!c->alloc(m.r, c->bytes) || !c->alloc(m.e[0], eb) || !c->alloc(m.e[1], eb) || !c->alloc(m.slots, sizeof(float) * static_cast<size_t>(m.count)) || !c->alloc(m.partials, sizeof(float) * defect_partials(c->dims))) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
         }
     }
 

@@ -57,9 +57,10 @@ hipError_t launch_smooth(hipStream_t s, SmootherKind smoother, SolveKind solve, 
 hipError_t launch_correction(hipStream_t s, SmootherKind smoother, SolveKind solve, StorageKind storage, const CorrectionPass& pass, Dims d, const StencilBC& homogeneous_bc);
 // Defect of a refinement cycle: x' = x_in + s_e*e (x_in nullptr = zeros, e nullptr = no
 // correction yet, s_e from e_slot), r = rhs - A x' with the full boundary rules
-// (diffusion: rhs = b; Poisson: rhs = -coef*b), max|r| atomically into *r_slot (zeroed
-// beforehand). x' is written to x_out when e is given.
-hipError_t launch_defect(hipStream_t s, SolveKind solve, StorageKind storage, const float* x_in, const void* e, const float* e_slot, const float* b, float coef, float* x_out, float* r, float* r_slot, Dims d, const StencilBC& bc);
+// (diffusion: rhs = b; Poisson: rhs = -coef*b), max|r| into *r_slot through `partials`
+// (defect_partials(d) floats). x' is written to x_out when e is given.
+hipError_t launch_defect(hipStream_t s, SolveKind solve, StorageKind storage, const float* x_in, const void* e, const float* e_slot, const float* b, float coef, float* x_out, float* r, float* partials, float* r_slot, Dims d, const StencilBC& bc);
+uint64_t defect_partials(Dims d);
 // out = (x + s_e*e) * out_scale, element-wise (out may equal x).
 hipError_t launch_finalize(hipStream_t s, StorageKind storage, float* out, const float* x, const void* e, const float* e_slot, float out_scale, uint64_t n);
 // divergence of (vx,vy,vz); bc.face[2c+side] is the rule of component c on axis c.

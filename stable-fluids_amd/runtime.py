@@ -27,6 +27,12 @@ EXT_ENTRY_POINTS = (
     "stable_fluids_ext_set_flow_pressure_cuda", "stable_fluids_ext_field_norms_cuda",
     "stable_fluids_ext_run_stage_cuda", "stable_fluids_ext_stage_bytes_cuda", "stable_fluids_ext_get_stream_cuda",
 )
+UNET_ENTRY_POINTS = (
+    "stable_fluids_unet_param_count", "stable_fluids_unet_create_cuda", "stable_fluids_unet_destroy_cuda",
+    "stable_fluids_unet_set_params_cuda", "stable_fluids_unet_get_params_cuda", "stable_fluids_unet_get_grads_cuda",
+    "stable_fluids_unet_forward_cuda", "stable_fluids_unet_backward_cuda", "stable_fluids_unet_adam_step_cuda",
+    "stable_fluids_unet_conv3_cuda", "stable_fluids_unet_conv3_wgrad_cuda",
+)
 
 
 class StableFluidsError(RuntimeError):
@@ -65,8 +71,20 @@ def load_library(path=LIB_PATH):
     lib.stable_fluids_ext_stage_bytes_cuda.restype = C.c_uint64
     lib.stable_fluids_ext_get_stream_cuda.argtypes = [vp]
     lib.stable_fluids_ext_get_stream_cuda.restype = vp
-    for name in ENTRY_POINTS + EXT_ENTRY_POINTS:
-        if name not in ("stable_fluids_ext_stage_bytes_cuda", "stable_fluids_ext_get_stream_cuda"):
+    i32, u64, f32 = C.c_int32, C.c_uint64, C.c_float
+    lib.stable_fluids_unet_param_count.argtypes = []
+    lib.stable_fluids_unet_param_count.restype = u64
+    lib.stable_fluids_unet_create_cuda.argtypes = [i32, i32, i32, vp, C.POINTER(vp)]
+    lib.stable_fluids_unet_destroy_cuda.argtypes = [vp]
+    for name in ("set_params", "get_params", "get_grads"):
+        getattr(lib, f"stable_fluids_unet_{name}_cuda").argtypes = [vp, vp, u64]
+    lib.stable_fluids_unet_forward_cuda.argtypes = [vp, vp, vp]
+    lib.stable_fluids_unet_backward_cuda.argtypes = [vp, vp, vp, vp]
+    lib.stable_fluids_unet_adam_step_cuda.argtypes = [vp, f32, f32, f32, f32]
+    lib.stable_fluids_unet_conv3_cuda.argtypes = [vp, i32, i32, vp, vp, vp, vp, i32, i32, i32, i32]
+    lib.stable_fluids_unet_conv3_wgrad_cuda.argtypes = [vp, i32, i32, vp, vp, vp, vp, i32, i32, i32]
+    for name in ENTRY_POINTS + EXT_ENTRY_POINTS + UNET_ENTRY_POINTS:
+        if name not in ("stable_fluids_ext_stage_bytes_cuda", "stable_fluids_ext_get_stream_cuda", "stable_fluids_unet_param_count"):
             getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib

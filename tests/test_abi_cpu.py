@@ -10,7 +10,7 @@ import pytest
 
 from conftest import ROOT, sf
 
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("stable-fluids-3d.h", "stable-fluids-3d-ext.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("stable-fluids-3d.h", "stable-fluids-3d-ext.h", "stable-fluids-unet.h")]
 
 
 def declared_entry_points():
