@@ -36,6 +36,13 @@ typedef enum StableFluidsExtIterateStorage {
 
 typedef enum StableFluidsExtWarmStart {
     STABLE_FLUIDS_EXT_WARM_START_ZERO    = 0, /* pressure restarts from zero each step (reference) */
+    /* The first pressure iterate is the warm-start network's prediction (see
+     * stable-fluids-unet.h) from the half-resolution divergence, followed by
+     * warm_start_iterations smoother passes. When the guess residual ||b - A p0|| exceeds
+     * 10x the zero-guess residual ||b||, the guess is dropped and the step runs the plain
+     * pressure_iterations passes from zero (bit-identical to the reference path).
+     * Needs nx, ny, nz multiples of 8, FP32 iterate storage and a model loaded with
+     * stable_fluids_ext_set_pressure_model_cuda (until then the network is all zeros). */
     STABLE_FLUIDS_EXT_WARM_START_LEARNED = 1,
 } StableFluidsExtWarmStart;
 
@@ -45,7 +52,9 @@ typedef struct StableFluidsExtOptions {
     uint32_t iterate_storage;            /* StableFluidsExtIterateStorage */
     uint32_t defect_correction_interval; /* m, passes per refinement cycle; 0 = 10, at most 4096 */
     uint32_t pressure_warm_start;        /* StableFluidsExtWarmStart */
-    uint32_t reserved[3];                /* must be zero */
+    uint32_t warm_start_iterations;      /* passes after a learned guess (<= pressure_iterations;
+                                            raised by one when pressure_iterations minus it is odd) */
+    uint32_t reserved[2];                /* must be zero */
 } StableFluidsExtOptions;
 
 /* Same as stable_fluids_create_context_cuda, with options (NULL = reference behaviour).
@@ -91,6 +100,29 @@ STABLE_FLUIDS_API uint64_t stable_fluids_ext_stage_bytes_cuda(void* context, uin
 
 /* Returns the context's HIP stream (hipStream_t as void*). */
 STABLE_FLUIDS_API void* stable_fluids_ext_get_stream_cuda(void* context);
+
+/* Device array [nz][ny][nx] FP32 of the last step's pre-projection divergence (the
+ * pressure right-hand side; the DIVERGENCE view holds the post-projection field). Valid
+ * until the context is destroyed; NULL for a NULL context. */
+STABLE_FLUIDS_API const float* stable_fluids_ext_get_pressure_rhs_cuda(void* context);
+
+/* Loads the warm-start network parameters (host FP32, stable-fluids-unet.h layout, count =
+ * stable_fluids_unet_param_count()) into a context created with
+ * STABLE_FLUIDS_EXT_WARM_START_LEARNED. Synchronous; takes effect from the next step. */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_set_pressure_model_cuda(void* context, const float* params, uint64_t count);
+
+/* Warm-start outcome of the last step: out[0] = ||b - A p0|| (guess), out[1] = ||b||
+ * (zero guess), out[2] = 1 when the step fell back to the plain passes. Synchronises the
+ * context stream. */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_warm_start_stats_cuda(void* context, double* out3);
+
+/* Training pair of the warm-start network from the last step's pressure system: the
+ * pre-projection divergence div and the exact solution p* of A p = -h^2 div (pressure
+ * boundary rules; FP64 conjugate gradients to ||r|| <= rel_tol ||b||, at most
+ * max_iterations, started from the previous call's solution; zero-mean when the system
+ * has no outflow face). Writes input[v][2] = (div / sigma, wall mask) and target[v] =
+ * p* / (h^2 sigma), sigma = RMS(div), device FP32. Synchronous. */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_training_pair_cuda(void* context, float* input, float* target, double rel_tol, uint32_t max_iterations, double* out_rel_residual, uint32_t* out_iterations);
 
 #ifdef __cplusplus
 }

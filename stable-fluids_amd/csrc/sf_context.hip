@@ -23,7 +23,10 @@
 
 #include "../../include/stable-fluids-3d-ext.h"
 #include "../../include/stable-fluids-3d.h"
+#include "../../include/stable-fluids-unet.h"
 #include "sf_kernels.hpp"
+#include "sf_unet.hpp"
+#include "sf_warm.hpp"
 
 namespace sf {
 
@@ -41,6 +44,8 @@ struct Options {
     bool refine           = false;     // refinement cycles (sf_mixed.hip) instead of plain passes
     StorageKind storage   = kStoreF32; // correction iterate storage of the cycles
     int interval          = 10;        // smoother passes per refinement cycle
+    bool learned          = false;     // learned pressure warm start (sf_warm.hip)
+    int warm_iterations   = 0;         // passes after an accepted guess
 };
 
 struct ScalarField {
@@ -75,6 +80,7 @@ struct Context {
         float *tx = nullptr, *ty = nullptr, *tz = nullptr;
         float *ax = nullptr, *ay = nullptr, *az = nullptr;
         float *p = nullptr, *p_aux = nullptr, *div = nullptr, *mag = nullptr;
+        float* rhs = nullptr; // pre-projection divergence (pressure right-hand side)
     } flow;
     std::vector<ScalarField> scalars;
     std::vector<VectorField> vectors;
@@ -87,9 +93,21 @@ struct Context {
         int cycles   = 0;       // slots per solve
         int count    = 0;
     } mixed;
+    struct {
+        unet::Net* net   = nullptr;
+        float* in2       = nullptr; // half-resolution (div, wall mask)
+        float* scal      = nullptr; // sigma, 1/sigma
+        double* partials = nullptr;
+        double* stats    = nullptr; // guess residual, zero residual, fallback
+        int* skip        = nullptr; // 1: skip the fallback passes
+    } warm;
+    struct {
+        double *x = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *partials = nullptr, *scal = nullptr;
+    } cg;
     std::vector<void*> allocations;
 
     ~Context() {
+        if (warm.net) unet::net_destroy(warm.net);
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
         for (void* p : allocations) (void)hipFree(p);
@@ -233,6 +251,35 @@ hipError_t enqueue_solve(Context& c, SolveKind solve, const float* b, const floa
     return hipSuccess;
 }
 
+// Learned warm start in place of the zero-initialised pressure solve (reference
+// backend-cuda.cu:643-650): network guess from the half-resolution divergence, residual
+// test, then P passes of which the first C = P - K are skipped when the guess is
+// accepted. C is even, so the skipped passes leave the guess in the buffer the first
+// unskipped pass reads; on fallback the guess is zeroed and all P passes run, exactly the
+// plain solve (its first pass reads zeros instead of assuming them).
+hipError_t enqueue_learned_pressure(Context& c, const StencilBC& p_bc) {
+    auto& f        = c.flow;
+    auto& w        = c.warm;
+    const int P    = c.config.pressure_iterations;
+    const int K    = c.options.warm_iterations;
+    const int C    = P - K;
+    const float h  = c.config.cell_size;
+    const Dims cd  = unet::net_dims(*w.net);
+    const uint64_t vc = static_cast<uint64_t>(cd.nx) * cd.ny * cd.nz;
+    float* buf[2]  = {(P % 2 == 0) ? f.p : f.p_aux, (P % 2 == 0) ? f.p_aux : f.p}; // buf[P % 2] = p
+    SF_TRY(warm::launch_restrict_input(c.stream, f.rhs, w.in2, w.partials, w.scal, cd, p_bc));
+    SF_TRY(unet::launch_pack_input(c.stream, w.in2, w.scal + 1, 1.0f, unet::net_input(*w.net), vc));
+    SF_TRY(unet::net_forward_packed(*w.net, false));
+    SF_TRY(warm::launch_prolong(c.stream, unet::net_output(*w.net), w.scal, 4.0f * h * h, buf[0], cd));
+    SF_TRY(warm::launch_guess_test(c.stream, buf[0], f.rhs, h * h, 10.0, w.partials, w.stats, w.skip, c.dims, p_bc));
+    for (int i = 1; i <= P; ++i) {
+        SmoothPass pass{buf[(i - 1) & 1], f.rhs, buf[i & 1], h * h, 1.0f};
+        if (i <= C) pass.skip = w.skip;
+        SF_TRY(launch_smooth(c.stream, c.options.smoother, kSolvePoisson, pass, c.dims, p_bc));
+    }
+    return hipSuccess;
+}
+
 // The step sequence of the reference graph (backend-cuda.cu:612-671).
 hipError_t enqueue_step(Context& c) {
     const auto& cfg = c.config;
@@ -276,9 +323,11 @@ hipError_t enqueue_step(Context& c) {
     // projection; pressure restarts from zero (reference :643-650)
     const StencilBC div_bc = divergence_rules(fb);
     const StencilBC p_bc   = pressure_rules(fb);
-    SF_TRY(launch_divergence(c.stream, f.div, u[0], u[1], u[2], c.dims, h, div_bc));
-    if (cfg.pressure_iterations > 0)
-        SF_TRY(enqueue_solve(c, kSolvePoisson, f.div, nullptr, f.p, f.p_aux, cfg.pressure_iterations, cfg.cell_size * cfg.cell_size, 1.0f, p_bc, 3 * c.mixed.cycles));
+    SF_TRY(launch_divergence(c.stream, f.rhs, u[0], u[1], u[2], c.dims, h, div_bc));
+    if (cfg.pressure_iterations > 0 && c.options.learned)
+        SF_TRY(enqueue_learned_pressure(c, p_bc));
+    else if (cfg.pressure_iterations > 0)
+        SF_TRY(enqueue_solve(c, kSolvePoisson, f.rhs, nullptr, f.p, f.p_aux, cfg.pressure_iterations, cfg.cell_size * cfg.cell_size, 1.0f, p_bc, 3 * c.mixed.cycles));
     else
         SF_TRY(launch_fill(c.stream, f.p, 0.0f, c.cells));
     SF_TRY(launch_project(c.stream, f.vx, f.vy, f.vz, f.div, f.mag, u[0], u[1], u[2], f.p, c.dims, h, p_bc, div_bc));
@@ -334,13 +383,21 @@ StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const Stabl
         if (opt->smoother > STABLE_FLUIDS_EXT_SMOOTHER_JACOBI) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         if (opt->iterate_storage > STABLE_FLUIDS_EXT_STORAGE_F32_REFINED) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         if (opt->defect_correction_interval > 4096) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
-        if (opt->pressure_warm_start != STABLE_FLUIDS_EXT_WARM_START_ZERO) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        if (opt->pressure_warm_start > STABLE_FLUIDS_EXT_WARM_START_LEARNED) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         for (uint32_t v : opt->reserved)
             if (v != 0) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         options.smoother = opt->smoother == STABLE_FLUIDS_EXT_SMOOTHER_JACOBI ? kSmoothJacobi : kSmoothRBGS;
         options.refine   = opt->iterate_storage != STABLE_FLUIDS_EXT_STORAGE_F32;
         options.storage  = opt->iterate_storage == STABLE_FLUIDS_EXT_STORAGE_F16 ? kStoreF16 : (opt->iterate_storage == STABLE_FLUIDS_EXT_STORAGE_BF16 ? kStoreBF16 : kStoreF32);
         if (opt->defect_correction_interval > 0) options.interval = static_cast<int>(opt->defect_correction_interval);
+        options.learned = opt->pressure_warm_start == STABLE_FLUIDS_EXT_WARM_START_LEARNED;
+        if (options.learned) {
+            const int P = cfg.pressure_iterations;
+            int k       = static_cast<int>(std::min<uint32_t>(opt->warm_start_iterations, 1u << 20));
+            if (P <= 0 || k > P || options.refine || cfg.nx % 8 || cfg.ny % 8 || cfg.nz % 8) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+            if ((P - k) % 2) ++k;
+            options.warm_iterations = k;
+        }
     }
     if (cfg.nx <= 0 || cfg.ny <= 0 || cfg.nz <= 0) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     const auto& fb = cfg.flow_boundary;
@@ -386,8 +443,13 @@ StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const Stabl
     {
         Range ra("stable.create_context.alloc");
         auto& f = c->flow;
-        for (float** p : {&f.vx, &f.vy, &f.vz, &f.tx, &f.ty, &f.tz, &f.ax, &f.ay, &f.az, &f.p, &f.p_aux, &f.div, &f.mag})
+        for (float** p : {&f.vx, &f.vy, &f.vz, &f.tx, &f.ty, &f.tz, &f.ax, &f.ay, &f.az, &f.p, &f.p_aux, &f.div, &f.mag, &f.rhs})
             if (!c->alloc(*p, c->bytes)) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
+        if (c->options.learned) {
+            auto& w = c->warm;
+            if (unet::net_create(cfg.nx / 2, cfg.ny / 2, cfg.nz / 2, c->stream, &w.net) != hipSuccess) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
+            if (!c->alloc(w.in2, c->bytes / 4) || !c->alloc(w.scal, 16) || !c->alloc(w.partials, sizeof(double) * 2 * warm::kPartialBlocks) || !c->alloc(w.stats, 4 * sizeof(double)) || !c->alloc(w.skip, 16)) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
+        }
         for (auto& s : c->scalars)
             for (float** p : {&s.data, &s.temp, &s.source, &s.aux})
                 if (!c->alloc(*p, c->bytes)) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
@@ -409,8 +471,9 @@ StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const Stabl
         // initial state (reference :583-606)
         Range ri("stable.create_context.init");
         auto& f = c->flow;
-        for (float* p : {f.vx, f.vy, f.vz, f.tx, f.ty, f.tz, f.ax, f.ay, f.az, f.p, f.p_aux, f.div, f.mag})
+        for (float* p : {f.vx, f.vy, f.vz, f.tx, f.ty, f.tz, f.ax, f.ay, f.az, f.p, f.p_aux, f.div, f.mag, f.rhs})
             if (launch_fill(c->stream, p, 0.0f, c->cells) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        if (c->options.learned && launch_fill(c->stream, reinterpret_cast<float*>(c->warm.stats), 0.0f, 8) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         for (auto& s : c->scalars) {
             if (launch_fill(c->stream, s.data, s.desc.initial_value, c->cells) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
             if (launch_fill(c->stream, s.temp, s.desc.initial_value, c->cells) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
@@ -649,6 +712,48 @@ STABLE_FLUIDS_API uint64_t stable_fluids_ext_stage_bytes_cuda(void* context, uin
     case STABLE_FLUIDS_EXT_STAGE_COPY: return 8ull * c.cells;           // read one field, write one
     default: return 0;
     }
+}
+
+STABLE_FLUIDS_API const float* stable_fluids_ext_get_pressure_rhs_cuda(void* context) {
+    return context == nullptr ? nullptr : static_cast<Context*>(context)->flow.rhs;
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_set_pressure_model_cuda(void* context, const float* params, uint64_t count) {
+    sf::Range r("stable.ext.set_pressure_model");
+    if (context == nullptr || params == nullptr || count != stable_fluids_unet_param_count()) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& c = *static_cast<Context*>(context);
+    if (c.warm.net == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    return sf::status(sf::unet::net_set_params(*c.warm.net, params));
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_warm_start_stats_cuda(void* context, double* out3) {
+    if (context == nullptr || out3 == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& c = *static_cast<Context*>(context);
+    if (c.warm.stats == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    if (hipStreamSynchronize(c.stream) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    return sf::status(hipMemcpy(out3, c.warm.stats, 3 * sizeof(double), hipMemcpyDeviceToHost));
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_training_pair_cuda(void* context, float* input, float* target, double rel_tol, uint32_t max_iterations, double* out_rel_residual, uint32_t* out_iterations) {
+    sf::Range r("stable.ext.training_pair");
+    if (context == nullptr || input == nullptr || target == nullptr || !(rel_tol > 0.0)) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& c  = *static_cast<Context*>(context);
+    auto& g  = c.cg;
+    if (g.x == nullptr) {
+        const size_t db = c.cells * sizeof(double);
+        if (!c.alloc(g.x, db) || !c.alloc(g.r, db) || !c.alloc(g.p, db) || !c.alloc(g.ap, db) || !c.alloc(g.partials, sizeof(double) * 2 * sf::warm::kPartialBlocks) || !c.alloc(g.scal, 8 * sizeof(double))) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
+        if (hipMemsetAsync(g.x, 0, db, c.stream) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    }
+    const float h2       = c.config.cell_size * c.config.cell_size;
+    const sf::StencilBC bc = sf::pressure_rules(c.config.flow_boundary);
+    sf::warm::CgResult res{};
+    const int max_it = static_cast<int>(std::min<uint32_t>(max_iterations, 1u << 30));
+    if (sf::warm::cg_solve(c.stream, c.flow.rhs, h2, g.x, g.r, g.p, g.ap, g.partials, g.scal, c.dims, bc, rel_tol, max_it, &res) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    if (sf::warm::launch_pair(c.stream, c.flow.rhs, g.x, h2, input, target, g.partials, g.scal, c.dims, bc) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    if (hipStreamSynchronize(c.stream) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    if (out_rel_residual) *out_rel_residual = res.rel_residual;
+    if (out_iterations) *out_iterations = static_cast<uint32_t>(res.iterations);
+    return STABLE_FLUIDS_RESULT_OK;
 }
 
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_run_stage_cuda(void* context, uint32_t stage, uint32_t repeat) {

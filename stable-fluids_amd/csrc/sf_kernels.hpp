@@ -21,12 +21,14 @@ enum SolveKind : int {
 
 // One smoother pass: reads x_old (or zeros when x_old == nullptr) and b, writes x_new.
 // out_scale multiplies every written value (dissipation fused into the last pass).
+// When skip is given and *skip != 0 at run time, the pass writes nothing.
 struct SmoothPass {
     const float* x_old;
     const float* b;
     float* x_new;
     float coef;
     float out_scale;
+    const int* skip = nullptr;
 };
 
 // Element storage of a correction iterate (arithmetic is FP32 in every case).

@@ -186,9 +186,10 @@ __device__ __forceinline__ void fetch_region(float (&r)[kPer], const typename St
 }
 
 template <int kSolve, bool kZero>
-__global__ __launch_bounds__(kThreads) void k_rbgs_pass(const float* __restrict__ x_old, const float* __restrict__ b, float* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g) {
+__global__ __launch_bounds__(kThreads) void k_rbgs_pass(const float* __restrict__ x_old, const float* __restrict__ b, float* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g, const int* __restrict__ skip) {
     __shared__ float2 s_old[3][kOPX * kOY];
     __shared__ float2 s_h[3][kHPX * kHY];
+    if (skip != nullptr && *skip != 0) return;
 
     int tx, ty, cz;
     tile_of(g, tx, ty, cz);
@@ -370,8 +371,9 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_pass(const float* __restrict_
 }
 
 template <int kSolve, bool kZero, class St>
-__global__ __launch_bounds__(kThreads) void k_jacobi_pass(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g, const float* __restrict__ scale_slot) {
+__global__ __launch_bounds__(kThreads) void k_jacobi_pass(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g, const float* __restrict__ scale_slot, const int* __restrict__ skip) {
     __shared__ float2 s_old[3][kHPX * kHY];
+    if (skip != nullptr && *skip != 0) return;
     const float inv_s = rhs_scale<St>(scale_slot);
 
     int tx, ty, cz;
@@ -486,9 +488,10 @@ __device__ __forceinline__ int wrap_near(const int k, const int n) {
 }
 
 template <int kSolve, bool kZero, class St>
-__global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g, const float* __restrict__ scale_slot) {
+__global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g, const float* __restrict__ scale_slot, const int* __restrict__ skip) {
     __shared__ float s_old[3][kOldPlane];
     __shared__ float s_h[3][kHPlane];
+    if (skip != nullptr && *skip != 0) return;
     const float inv_s = rhs_scale<St>(scale_slot);
 
     int tx, ty, cz;
@@ -745,8 +748,8 @@ hipError_t launch_smooth(hipStream_t s, const SmootherKind smoother, const Solve
     const TileGeom g      = plan_tiles(d);
     const unsigned blocks = static_cast<unsigned>(g.tiles_x * g.tiles_y * g.chunks);
     const bool zero       = pass.x_old == nullptr;
-#define SF_PASS_GENERAL(SO, Z) k_rbgs_pass<SO, Z><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g)
-#define SF_PASS(KERNEL, SO, Z) KERNEL<SO, Z, StoreF32><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g, nullptr)
+#define SF_PASS_GENERAL(SO, Z) k_rbgs_pass<SO, Z><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g, pass.skip)
+#define SF_PASS(KERNEL, SO, Z) KERNEL<SO, Z, StoreF32><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g, nullptr, pass.skip)
     if (smoother == kSmoothRBGS && !has_odd_periodic(d, bc)) {
         if (solve == kSolvePoisson) {
             if (zero) SF_PASS(k_rbgs_split_pass, kSolvePoisson, true);
@@ -785,7 +788,7 @@ hipError_t correction_pass(hipStream_t s, const SmootherKind smoother, const Sol
     const unsigned blocks = static_cast<unsigned>(g.tiles_x * g.tiles_y * g.chunks);
     const auto* e_old     = static_cast<const typename St::T*>(pass.e_old);
     auto* e_new           = static_cast<typename St::T*>(pass.e_new);
-#define SF_CORR(KERNEL, SO, Z) KERNEL<SO, Z, St><<<blocks, kThreads, 0, s>>>(e_old, pass.r, e_new, pass.coef, 1.0f, d, bc, g, pass.scale_slot)
+#define SF_CORR(KERNEL, SO, Z) KERNEL<SO, Z, St><<<blocks, kThreads, 0, s>>>(e_old, pass.r, e_new, pass.coef, 1.0f, d, bc, g, pass.scale_slot, nullptr)
     const bool zero = e_old == nullptr;
     if (smoother == kSmoothRBGS) {
         if (solve == kSolvePoisson) {

@@ -1,0 +1,389 @@
+// sf_warm.hip -- learned pressure warm start and the FP64 CG solve of its training pairs.
+//
+// No counterpart in the reference (its projection restarts pressure from zero,
+// backend-cuda.cu:644). Stage kernels of the warm start, in graph order:
+//   restrict_input: half-resolution divergence (mean of 8 cells) and wall mask, sigma;
+//   (network forward, sf_unet.hip)
+//   prolong:        trilinear guess on the full grid, scaled by (2h)^2 sigma;
+//   guess_test:     ||b - A p0|| against ||b|| (FP64), fallback flag, skip flag.
+// All reductions are deterministic two-pass sums (per-block partials, then one block).
+#include "sf_warm.hpp"
+
+namespace sf::warm {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sum of v over the block (256 threads) into *out (thread 0).
+__device__ __forceinline__ void block_sum_to(double v, double* out) {
+    __shared__ double s[kThreads / 64];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = s[0] + s[1] + s[2] + s[3];
+    __syncthreads();
+}
+
+__device__ __forceinline__ void decode(const uint64_t i, const Dims d, int& x, int& y, int& z) {
+    x = static_cast<int>(i % static_cast<uint64_t>(d.nx));
+    const uint64_t r = i / static_cast<uint64_t>(d.nx);
+    y = static_cast<int>(r % static_cast<uint64_t>(d.ny));
+    z = static_cast<int>(r / static_cast<uint64_t>(d.ny));
+}
+
+// 1 when the cell touches a non-periodic face whose pressure rule is a wall (copy).
+__device__ __forceinline__ float wall_mask(const int x, const int y, const int z, const Dims d, const StencilBC& bc) {
+    const int c[3] = {x, y, z}, n[3] = {d.nx, d.ny, d.nz};
+    for (int f = 0; f < 6; ++f) {
+        const int a = f >> 1;
+        if ((bc.periodic >> a) & 1) continue;
+        const bool at = (f & 1) ? c[a] == n[a] - 1 : c[a] == 0;
+        if (at && bc.face[f].mode == kGhostCopy) return 1.0f;
+    }
+    return 0.0f;
+}
+
+// sum of the 6 ghosted neighbours of field f (T = float or double) at (x, y, z);
+// `homogeneous` drops the outflow pressure (ghost value 0)
+template <class T>
+__device__ __forceinline__ T neighbour_sum(const T* __restrict__ f, const int x, const int y, const int z, const Dims d, const StencilBC& bc, const T centre, const bool homogeneous) {
+    const int n[3] = {d.nx, d.ny, d.nz};
+    T sum          = 0;
+#pragma unroll
+    for (int fc = 0; fc < 6; ++fc) {
+        const int a = fc >> 1;
+        int q[3]    = {x, y, z};
+        if (step_axis(q[a], (fc & 1) ? 1 : -1, n[a], (bc.periodic >> a) & 1)) {
+            sum += f[lin(q[0], q[1], q[2], d)];
+        } else {
+            const FaceRule r = bc.face[fc];
+            sum += r.mode == kGhostCopy ? centre : (homogeneous ? T(0) : T(r.value));
+        }
+    }
+    return sum;
+}
+
+// constant part of the ghosts: sum of the outflow pressures around the cell
+__device__ __forceinline__ double ghost_constant(const int x, const int y, const int z, const Dims d, const StencilBC& bc) {
+    const int c[3] = {x, y, z}, n[3] = {d.nx, d.ny, d.nz};
+    double s = 0.0;
+    for (int f = 0; f < 6; ++f) {
+        const int a = f >> 1;
+        if ((bc.periodic >> a) & 1) continue;
+        const bool at = (f & 1) ? c[a] == n[a] - 1 : c[a] == 0;
+        if (at && bc.face[f].mode != kGhostCopy) s += bc.face[f].value;
+    }
+    return s;
+}
+
+__host__ __device__ inline uint64_t cells_of(const Dims d) { return static_cast<uint64_t>(d.nx) * d.ny * d.nz; }
+
+#define GRID_STRIDE(i, n) for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x; i < (n); i += static_cast<uint64_t>(gridDim.x) * kThreads)
+
+__global__ __launch_bounds__(kThreads) void k_restrict_input(const float* __restrict__ div, float* __restrict__ in2, double* __restrict__ partials, const Dims cd, const StencilBC bc) {
+    const Dims fd{2 * cd.nx, 2 * cd.ny, 2 * cd.nz};
+    double acc = 0.0;
+    GRID_STRIDE(v, cells_of(cd)) {
+        int x, y, z;
+        decode(v, cd, x, y, z);
+        float s = 0.0f;
+        for (int k = 0; k < 8; ++k) s += div[lin(2 * x + (k & 1), 2 * y + ((k >> 1) & 1), 2 * z + (k >> 2), fd)];
+        s *= 0.125f;
+        in2[2 * v]     = s;
+        in2[2 * v + 1] = wall_mask(x, y, z, cd, bc);
+        acc += static_cast<double>(s) * s;
+    }
+    block_sum_to(acc, partials + blockIdx.x);
+}
+
+__global__ __launch_bounds__(kThreads) void k_sigma(const double* __restrict__ partials, const int n_partials, const double n, float* __restrict__ scal) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < n_partials; b += kThreads) acc += partials[b];
+    __shared__ double total;
+    block_sum_to(acc, &total);
+    if (threadIdx.x == 0) {
+        const double sigma = sqrt(total / n);
+        scal[0]            = static_cast<float>(sigma);
+        scal[1]            = sigma > 0.0 ? static_cast<float>(1.0 / sigma) : 0.0f;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_prolong(const float* __restrict__ q, const float* __restrict__ scal, const float scale, float* __restrict__ p0, const Dims cd) {
+    const Dims fd{2 * cd.nx, 2 * cd.ny, 2 * cd.nz};
+    const float amp = scale * scal[0];
+    GRID_STRIDE(i, cells_of(fd)) {
+        int x, y, z;
+        decode(i, fd, x, y, z);
+        const int c[3] = {x, y, z}, n[3] = {cd.nx, cd.ny, cd.nz};
+        int i0[3], i1[3];
+        float t[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float cc = fminf(fmaxf(0.5f * c[a] - 0.25f, 0.0f), static_cast<float>(n[a] - 1));
+            i0[a]          = static_cast<int>(cc);
+            i1[a]          = min(i0[a] + 1, n[a] - 1);
+            t[a]           = cc - i0[a];
+        }
+        float v = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int xx = (k & 1) ? i1[0] : i0[0], yy = (k & 2) ? i1[1] : i0[1], zz = (k & 4) ? i1[2] : i0[2];
+            const float w = ((k & 1) ? t[0] : 1.0f - t[0]) * ((k & 2) ? t[1] : 1.0f - t[1]) * ((k & 4) ? t[2] : 1.0f - t[2]);
+            v += w * q[lin(xx, yy, zz, cd)];
+        }
+        p0[i] = amp * v;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_guess_partials(const float* __restrict__ p0, const float* __restrict__ div, const float h2, double* __restrict__ partials, const Dims d, const StencilBC bc) {
+    double acc_g = 0.0, acc_z = 0.0;
+    GRID_STRIDE(i, cells_of(d)) {
+        int x, y, z;
+        decode(i, d, x, y, z);
+        const double b  = -static_cast<double>(h2) * div[i] + ghost_constant(x, y, z, d, bc);
+        const double pc = p0[i];
+        const double ap = 6.0 * pc - static_cast<double>(neighbour_sum<float>(p0, x, y, z, d, bc, p0[i], true));
+        acc_g += (b - ap) * (b - ap);
+        acc_z += b * b;
+    }
+    block_sum_to(acc_g, partials + 2 * blockIdx.x);
+    block_sum_to(acc_z, partials + 2 * blockIdx.x + 1);
+}
+
+__global__ __launch_bounds__(kThreads) void k_guess_decide(const double* __restrict__ partials, const int n_partials, const double ratio, double* __restrict__ stats, int* __restrict__ skip) {
+    double g = 0.0, zz = 0.0;
+    for (int b = threadIdx.x; b < n_partials; b += kThreads) {
+        g += partials[2 * b];
+        zz += partials[2 * b + 1];
+    }
+    __shared__ double tg, tz;
+    block_sum_to(g, &tg);
+    block_sum_to(zz, &tz);
+    if (threadIdx.x == 0) {
+        const double rg = sqrt(tg), rz = sqrt(tz);
+        const bool fallback = !(rg <= ratio * rz); // NaN guesses fall back too
+        stats[0]            = rg;
+        stats[1]            = rz;
+        stats[2]            = fallback ? 1.0 : 0.0;
+        *skip               = fallback ? 0 : 1;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_zero_on_fallback(float* __restrict__ p0, const double* __restrict__ stats, const uint64_t n) {
+    if (stats[2] == 0.0) return;
+    GRID_STRIDE(i, n) p0[i] = 0.0f;
+}
+
+// ---- conjugate gradients (FP64) ----
+
+// r = b (= -h2 div + outflow constants); partials: sum b, sum b^2
+__global__ __launch_bounds__(kThreads) void k_cg_rhs(const float* __restrict__ div, const float h2, double* __restrict__ r, double* __restrict__ partials, const Dims d, const StencilBC bc) {
+    double s = 0.0, s2 = 0.0;
+    GRID_STRIDE(i, cells_of(d)) {
+        int x, y, z;
+        decode(i, d, x, y, z);
+        const double b = -static_cast<double>(h2) * div[i] + ghost_constant(x, y, z, d, bc);
+        r[i]           = b;
+        s += b;
+        s2 += b * b;
+    }
+    block_sum_to(s, partials + 2 * blockIdx.x);
+    block_sum_to(s2, partials + 2 * blockIdx.x + 1);
+}
+
+// scal[slot] = sum of partials[stride * b + offset]
+__global__ __launch_bounds__(kThreads) void k_reduce(const double* __restrict__ partials, const int n_partials, const int stride, const int offset, double* __restrict__ scal, const int slot) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < n_partials; b += kThreads) acc += partials[stride * b + offset];
+    __shared__ double t;
+    block_sum_to(acc, &t);
+    if (threadIdx.x == 0) scal[slot] = t;
+}
+
+// v -= scal[slot] / n
+__global__ __launch_bounds__(kThreads) void k_sub_mean(double* __restrict__ v, const double* __restrict__ scal, const int slot, const uint64_t n) {
+    const double m = scal[slot] / static_cast<double>(n);
+    GRID_STRIDE(i, n) v[i] -= m;
+}
+
+// r -= A x (homogeneous A); then p = r; partial r.r
+__global__ __launch_bounds__(kThreads) void k_cg_init(const double* __restrict__ x, double* __restrict__ r, double* __restrict__ p, double* __restrict__ partials, const Dims d, const StencilBC bc) {
+    double acc = 0.0;
+    GRID_STRIDE(i, cells_of(d)) {
+        int xx, y, z;
+        decode(i, d, xx, y, z);
+        const double ax = 6.0 * x[i] - neighbour_sum<double>(x, xx, y, z, d, bc, x[i], true);
+        const double ri = r[i] - ax;
+        r[i]            = ri;
+        p[i]            = ri;
+        acc += ri * ri;
+    }
+    block_sum_to(acc, partials + blockIdx.x);
+}
+
+// ap = A p; partial p.ap
+__global__ __launch_bounds__(kThreads) void k_cg_apply(const double* __restrict__ p, double* __restrict__ ap, double* __restrict__ partials, const Dims d, const StencilBC bc) {
+    double acc = 0.0;
+    GRID_STRIDE(i, cells_of(d)) {
+        int x, y, z;
+        decode(i, d, x, y, z);
+        const double v = 6.0 * p[i] - neighbour_sum<double>(p, x, y, z, d, bc, p[i], true);
+        ap[i]          = v;
+        acc += p[i] * v;
+    }
+    block_sum_to(acc, partials + blockIdx.x);
+}
+
+// alpha = rr / pAp; x += alpha p; r -= alpha ap; partial r.r
+__global__ __launch_bounds__(kThreads) void k_cg_update(double* __restrict__ x, double* __restrict__ r, const double* __restrict__ p, const double* __restrict__ ap, const double* __restrict__ scal, double* __restrict__ partials, const uint64_t n) {
+    const double pap   = scal[1];
+    const double alpha = pap > 0.0 ? scal[0] / pap : 0.0;
+    double acc         = 0.0;
+    GRID_STRIDE(i, n) {
+        x[i] += alpha * p[i];
+        const double ri = r[i] - alpha * ap[i];
+        r[i]            = ri;
+        acc += ri * ri;
+    }
+    block_sum_to(acc, partials + blockIdx.x);
+}
+
+// beta = rr_new / rr; p = r + beta p; rr = rr_new (one block does the bookkeeping)
+__global__ __launch_bounds__(kThreads) void k_cg_direction(double* __restrict__ p, const double* __restrict__ r, double* __restrict__ scal, const uint64_t n) {
+    const double rr = scal[0], rr_new = scal[2];
+    const double beta = rr > 0.0 ? rr_new / rr : 0.0;
+    GRID_STRIDE(i, n) p[i] = r[i] + beta * p[i];
+}
+__global__ void k_cg_shift(double* __restrict__ scal) { scal[0] = scal[2]; }
+
+// partials[2b] = sum v, partials[2b + 1] = sum v^2
+__global__ __launch_bounds__(kThreads) void k_moments(const double* __restrict__ v, double* __restrict__ partials, const uint64_t n) {
+    double s = 0.0, s2 = 0.0;
+    GRID_STRIDE(i, n) {
+        s += v[i];
+        s2 += v[i] * v[i];
+    }
+    block_sum_to(s, partials + 2 * blockIdx.x);
+    block_sum_to(s2, partials + 2 * blockIdx.x + 1);
+}
+
+// partial sums of div^2 (for sigma)
+__global__ __launch_bounds__(kThreads) void k_sum_sq(const float* __restrict__ div, double* __restrict__ partials, const uint64_t n) {
+    double acc = 0.0;
+    GRID_STRIDE(i, n) acc += static_cast<double>(div[i]) * div[i];
+    block_sum_to(acc, partials + blockIdx.x);
+}
+
+__global__ __launch_bounds__(kThreads) void k_pair(const float* __restrict__ div, const double* __restrict__ x, const float h2, float* __restrict__ in2, float* __restrict__ target, const double* __restrict__ scal, const Dims d, const StencilBC bc) {
+    const double sigma = sqrt(scal[5] / static_cast<double>(cells_of(d)));
+    const double inv   = sigma > 0.0 ? 1.0 / sigma : 0.0;
+    GRID_STRIDE(i, cells_of(d)) {
+        int xx, y, z;
+        decode(i, d, xx, y, z);
+        in2[2 * i]     = static_cast<float>(div[i] * inv);
+        in2[2 * i + 1] = wall_mask(xx, y, z, d, bc);
+        target[i]      = static_cast<float>(x[i] * inv / h2);
+    }
+}
+
+#undef GRID_STRIDE
+
+unsigned blocks_for(const uint64_t n) {
+    const uint64_t b = (n + kThreads - 1) / kThreads;
+    return static_cast<unsigned>(b < kPartialBlocks ? (b ? b : 1) : kPartialBlocks);
+}
+
+bool singular(const Dims, const StencilBC& bc) {
+    for (int f = 0; f < 6; ++f)
+        if (!((bc.periodic >> (f >> 1)) & 1) && bc.face[f].mode != kGhostCopy) return false;
+    return true;
+}
+
+} // namespace
+
+hipError_t launch_restrict_input(hipStream_t s, const float* div, float* in2, double* partials, float* scal, const Dims cd, const StencilBC& p_bc) {
+    const unsigned b = blocks_for(cells_of(cd));
+    k_restrict_input<<<b, kThreads, 0, s>>>(div, in2, partials, cd, p_bc);
+    k_sigma<<<1, kThreads, 0, s>>>(partials, static_cast<int>(b), static_cast<double>(cells_of(cd)), scal);
+    return hipGetLastError();
+}
+
+hipError_t launch_prolong(hipStream_t s, const float* q, const float* scal, const float scale, float* p0, const Dims cd) {
+    const Dims fd{2 * cd.nx, 2 * cd.ny, 2 * cd.nz};
+    k_prolong<<<blocks_for(cells_of(fd)), kThreads, 0, s>>>(q, scal, scale, p0, cd);
+    return hipGetLastError();
+}
+
+hipError_t launch_guess_test(hipStream_t s, float* p0, const float* div, const float h2, const double ratio, double* partials, double* stats, int* skip, const Dims d, const StencilBC& p_bc) {
+    const unsigned b = blocks_for(cells_of(d));
+    k_guess_partials<<<b, kThreads, 0, s>>>(p0, div, h2, partials, d, p_bc);
+    k_guess_decide<<<1, kThreads, 0, s>>>(partials, static_cast<int>(b), ratio, stats, skip);
+    k_zero_on_fallback<<<b, kThreads, 0, s>>>(p0, stats, cells_of(d));
+    return hipGetLastError();
+}
+
+hipError_t cg_solve(hipStream_t s, const float* div, const float h2, double* x, double* r, double* p, double* ap, double* partials, double* scal, const Dims d, const StencilBC& p_bc, const double tol, const int max_iter, CgResult* out) {
+    constexpr int kCheck = 10;
+    const uint64_t n     = cells_of(d);
+    const unsigned b     = blocks_for(n);
+    const int nb         = static_cast<int>(b);
+    const bool sing      = singular(d, p_bc);
+    // r = b, projected to zero mean when the system is singular; ||b||^2 -> scal[3]
+    k_cg_rhs<<<b, kThreads, 0, s>>>(div, h2, r, partials, d, p_bc);
+    if (sing) {
+        k_reduce<<<1, kThreads, 0, s>>>(partials, nb, 2, 0, scal, 4);
+        k_sub_mean<<<b, kThreads, 0, s>>>(r, scal, 4, n);
+        k_moments<<<b, kThreads, 0, s>>>(r, partials, n);
+    }
+    k_reduce<<<1, kThreads, 0, s>>>(partials, nb, 2, 1, scal, 3);
+    // r = b - A x, p = r, rr -> scal[0]
+    k_cg_init<<<b, kThreads, 0, s>>>(x, r, p, partials, d, p_bc);
+    k_reduce<<<1, kThreads, 0, s>>>(partials, nb, 1, 0, scal, 0);
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+
+    double host[4] = {};
+    auto fetch = [&]() -> bool {
+        if (hipMemcpyAsync(host, scal, sizeof(host), hipMemcpyDeviceToHost, s) != hipSuccess) return false;
+        return hipStreamSynchronize(s) == hipSuccess;
+    };
+    int it = 0;
+    for (;; ++it) {
+        if (it % kCheck == 0 || it == max_iter) {
+            if (!fetch()) return hipErrorUnknown;
+            if (!(host[0] > tol * tol * host[3]) || it >= max_iter) break;
+        }
+        k_cg_apply<<<b, kThreads, 0, s>>>(p, ap, partials, d, p_bc);
+        k_reduce<<<1, kThreads, 0, s>>>(partials, nb, 1, 0, scal, 1);
+        k_cg_update<<<b, kThreads, 0, s>>>(x, r, p, ap, scal, partials, n);
+        k_reduce<<<1, kThreads, 0, s>>>(partials, nb, 1, 0, scal, 2);
+        k_cg_direction<<<b, kThreads, 0, s>>>(p, r, scal, n);
+        k_cg_shift<<<1, 1, 0, s>>>(scal);
+        if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    }
+    if (sing) {
+        k_moments<<<b, kThreads, 0, s>>>(x, partials, n);
+        k_reduce<<<1, kThreads, 0, s>>>(partials, nb, 2, 0, scal, 4);
+        k_sub_mean<<<b, kThreads, 0, s>>>(x, scal, 4, n);
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) return hipErrorUnknown;
+    out->iterations   = it;
+    out->rel_residual = host[3] > 0.0 ? sqrt(host[0] / host[3]) : 0.0;
+    return hipSuccess;
+}
+
+hipError_t launch_pair(hipStream_t s, const float* div, const double* x, const float h2, float* in2, float* target, double* partials, double* scal, const Dims d, const StencilBC& p_bc) {
+    const unsigned b = blocks_for(cells_of(d));
+    k_sum_sq<<<b, kThreads, 0, s>>>(div, partials, cells_of(d));
+    k_reduce<<<1, kThreads, 0, s>>>(partials, static_cast<int>(b), 1, 0, scal, 5);
+    k_pair<<<b, kThreads, 0, s>>>(div, x, h2, in2, target, scal, d, p_bc);
+    return hipGetLastError();
+}
+
+} // namespace sf::warm

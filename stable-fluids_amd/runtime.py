@@ -26,6 +26,8 @@ EXT_ENTRY_POINTS = (
     "stable_fluids_ext_create_context_cuda", "stable_fluids_ext_set_flow_velocity_cuda",
     "stable_fluids_ext_set_flow_pressure_cuda", "stable_fluids_ext_field_norms_cuda",
     "stable_fluids_ext_run_stage_cuda", "stable_fluids_ext_stage_bytes_cuda", "stable_fluids_ext_get_stream_cuda",
+    "stable_fluids_ext_set_pressure_model_cuda", "stable_fluids_ext_warm_start_stats_cuda", "stable_fluids_ext_training_pair_cuda",
+    "stable_fluids_ext_get_pressure_rhs_cuda",
 )
 UNET_ENTRY_POINTS = (
     "stable_fluids_unet_param_count", "stable_fluids_unet_create_cuda", "stable_fluids_unet_destroy_cuda",
@@ -72,6 +74,11 @@ def load_library(path=LIB_PATH):
     lib.stable_fluids_ext_get_stream_cuda.argtypes = [vp]
     lib.stable_fluids_ext_get_stream_cuda.restype = vp
     i32, u64, f32 = C.c_int32, C.c_uint64, C.c_float
+    lib.stable_fluids_ext_set_pressure_model_cuda.argtypes = [vp, vp, u64]
+    lib.stable_fluids_ext_get_pressure_rhs_cuda.argtypes = [vp]
+    lib.stable_fluids_ext_get_pressure_rhs_cuda.restype = vp
+    lib.stable_fluids_ext_warm_start_stats_cuda.argtypes = [vp, C.POINTER(C.c_double)]
+    lib.stable_fluids_ext_training_pair_cuda.argtypes = [vp, vp, vp, C.c_double, u32, C.POINTER(C.c_double), C.POINTER(u32)]
     lib.stable_fluids_unet_param_count.argtypes = []
     lib.stable_fluids_unet_param_count.restype = u64
     lib.stable_fluids_unet_create_cuda.argtypes = [i32, i32, i32, vp, C.POINTER(vp)]
@@ -84,20 +91,22 @@ def load_library(path=LIB_PATH):
     lib.stable_fluids_unet_conv3_cuda.argtypes = [vp, i32, i32, vp, vp, vp, vp, i32, i32, i32, i32]
     lib.stable_fluids_unet_conv3_wgrad_cuda.argtypes = [vp, i32, i32, vp, vp, vp, vp, i32, i32, i32]
     for name in ENTRY_POINTS + EXT_ENTRY_POINTS + UNET_ENTRY_POINTS:
-        if name not in ("stable_fluids_ext_stage_bytes_cuda", "stable_fluids_ext_get_stream_cuda", "stable_fluids_unet_param_count"):
+        if name not in ("stable_fluids_ext_stage_bytes_cuda", "stable_fluids_ext_get_stream_cuda", "stable_fluids_unet_param_count",
+                        "stable_fluids_ext_get_pressure_rhs_cuda"):
             getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
 
 
 def ext_options(smoother=abi.SMOOTHER_RBGS, iterate_storage=abi.STORAGE_F32, defect_correction_interval=0,
-                pressure_warm_start=abi.WARM_START_ZERO):
+                pressure_warm_start=abi.WARM_START_ZERO, warm_start_iterations=0):
     o = abi.ExtOptions()
     o.struct_size = C.sizeof(abi.ExtOptions)
     o.smoother = smoother
     o.iterate_storage = iterate_storage
     o.defect_correction_interval = defect_correction_interval
     o.pressure_warm_start = pressure_warm_start
+    o.warm_start_iterations = warm_start_iterations
     return o
 
 
@@ -181,6 +190,27 @@ class Context:
 
     def run_stage(self, stage, repeat):
         return self._call("stable_fluids_ext_run_stage_cuda", stage, repeat)
+
+    def set_pressure_model(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        return self._call("stable_fluids_ext_set_pressure_model_cuda", p.ctypes.data_as(C.c_void_p), p.size)
+
+    def warm_start_stats(self):
+        out = (C.c_double * 3)()
+        self._call("stable_fluids_ext_warm_start_stats_cuda", out)
+        return {"guess_residual": out[0], "zero_residual": out[1], "fallback": bool(out[2])}
+
+    def pressure_rhs(self):
+        """Host copy of the last step's pre-projection divergence, [nz][ny][nx] float32."""
+        hip.synchronize()
+        ptr = self.lib.stable_fluids_ext_get_pressure_rhs_cuda(self.handle)
+        return hip.to_host(ptr, self.desc.cells).reshape(self.desc.shape)
+
+    def training_pair(self, inp, target, rel_tol=1e-6, max_iterations=20000):
+        """Fills device buffers inp [v][2] and target [v]; returns (relative residual, iterations)."""
+        res, its = C.c_double(), C.c_uint32()
+        self._call("stable_fluids_ext_training_pair_cuda", _ptr(inp), _ptr(target), rel_tol, max_iterations, C.byref(res), C.byref(its))
+        return res.value, its.value
 
     def stage_bytes(self, stage):
         return int(self.lib.stable_fluids_ext_stage_bytes_cuda(self.handle, stage))

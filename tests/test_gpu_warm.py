@@ -1,0 +1,132 @@
+"""Learned pressure warm start through the C ABI (pressure_warm_start = LEARNED).
+
+* the training pair is the exact pressure solve: ||b - A p*|| <= 1e-5 ||b|| checked with
+  the pinned FP64 residual (oracle/warm_oracle.py), on mixed faces and on all walls
+  (singular system, zero-mean solution);
+* fallback (guess residual > 10x the zero-guess residual) gives the plain step bit for
+  bit: P passes from zero;
+* an all-zero network (guess 0, residual equal to the zero guess, accepted) gives the
+  plain step with K passes bit for bit;
+* an accepted guess equals the FP64 restatement of restrict -> normalise -> network ->
+  prolong (BF16 network emulated) within 1e-2, and the residual statistics match.
+"""
+import numpy as np
+import pytest
+
+from conftest import sf
+from oracle import unet_oracle as uo, warm_oracle as wo
+from paritylib import Inputs, gpu_fields, run_gpu
+
+abi = sf.abi
+hip = sf.hip
+scenes = sf.scenes
+pytestmark = pytest.mark.gpu
+
+
+def flow_desc(n=(16, 16, 8), walls_only=False, p_iters=12, d_iters=4):
+    nx, ny, nz = n
+    if walls_only:
+        flow = abi.flow_boundary([abi.FLOW_FREE_SLIP] * 6)
+    else:
+        flow = abi.flow_boundary([abi.FLOW_PERIODIC, abi.FLOW_PERIODIC, abi.FLOW_NO_SLIP, abi.FLOW_OUTFLOW,
+                                  abi.FLOW_FREE_SLIP, abi.FLOW_NO_SLIP], pressure=[0, 0, 0, 0.0, 0, 0])
+    return abi.Desc(nx, ny, nz, 1.0 / 16, 0.05, 1e-3, d_iters, p_iters, flow,
+                    scalars=[abi.scalar_desc(1e-3)], vectors=[abi.force_desc()])
+
+
+@pytest.mark.parametrize("walls_only", [False, True])
+def test_training_pair_is_exact_solve(walls_only):
+    desc = flow_desc(walls_only=walls_only)
+    inputs = Inputs(desc, seed=3, velocity_scale=1.0, smooth=True)
+    with sf.Context(desc) as ctx:
+        inputs.seed_gpu(ctx)
+        ctx.step(1)
+        rhs = ctx.pressure_rhs().astype(np.float64)
+        inp = hip.DeviceBuffer(desc.cells * 8)
+        tgt = hip.DeviceBuffer(desc.cells * 4)
+        rel, its = ctx.training_pair(inp, tgt, rel_tol=1e-6, max_iterations=5000)
+        x2 = hip.to_host(inp.value, 2 * desc.cells).reshape(desc.shape + (2,))
+        t = hip.to_host(tgt.value, desc.cells).reshape(desc.shape).astype(np.float64)
+        # a second call starts from the first solution and converges at once
+        rel2, its2 = ctx.training_pair(inp, tgt, rel_tol=1e-6, max_iterations=5000)
+    print(f"walls_only={walls_only}: cg iterations {its} rel residual {rel:.2e}; warm restart {its2}", flush=True)
+    assert rel <= 1e-6 and its > 0 and its2 <= 10
+    rules = wo.pressure_rules(desc.desc.config.flow_boundary)
+    sigma = np.sqrt(np.mean(rhs * rhs))
+    assert np.max(np.abs(x2[..., 0] - rhs / sigma)) <= 1e-6 * np.max(np.abs(rhs / sigma))
+    assert np.array_equal(x2[..., 1], wo.wall_mask(desc.shape, rules))
+    h2 = (1.0 / 16) ** 2
+    p = t * h2 * sigma
+    b = wo.pressure_residual(np.zeros_like(p), rhs, h2, rules)
+    r = wo.pressure_residual(p, rhs, h2, rules)
+    if walls_only:
+        b -= b.mean()
+        r -= r.mean()
+        assert abs(p.mean()) <= 1e-6 * np.max(np.abs(p))
+    err = np.sqrt(np.sum(r * r)) / np.sqrt(np.sum(b * b))
+    print(f"residual of the FP32 target {err:.2e}", flush=True)
+    assert err <= 1e-5
+
+
+def plume(n=32, p_iters=12):
+    return scenes.plume_scene(n, diffuse_iterations=4, pressure_iterations=p_iters, noise_seed=1)
+
+
+def learned_opts(k):
+    return sf.ext_options(pressure_warm_start=abi.WARM_START_LEARNED, warm_start_iterations=k)
+
+
+def run_learned(scene, steps, params, k):
+    with sf.Context(scene.desc, options=learned_opts(k)) as ctx:
+        ctx.set_pressure_model(params)
+        scenes.seed_context(ctx, scene)
+        ctx.step(steps)
+        stats = ctx.warm_start_stats()
+        fields = gpu_fields(ctx, scene.desc)
+        rhs = ctx.pressure_rhs()
+    return fields, stats, rhs
+
+
+def assert_same(a, b, tag):
+    for k in b:
+        assert np.array_equal(a[k], b[k]), f"{tag}: {k}"
+
+
+def test_fallback_is_the_plain_step_bitwise():
+    scene = plume(p_iters=12)
+    wild = uo.join_params(uo.init_params(3, scale=300.0)).astype(np.float32)
+    got, stats, _ = run_learned(scene, 4, wild, k=4)
+    print("fallback stats", stats, flush=True)
+    assert stats["fallback"]
+    plain = run_gpu(scene.desc, lambda ctx: scenes.seed_context(ctx, scene), 4)
+    assert_same(got, plain, "fallback vs plain P")
+
+
+def test_zero_network_is_the_plain_k_pass_step_bitwise():
+    scene = plume(p_iters=12)
+    zero = np.zeros(uo.param_count(), np.float32)
+    got, stats, _ = run_learned(scene, 4, zero, k=4)
+    assert not stats["fallback"] and stats["guess_residual"] == stats["zero_residual"]
+    short = plume(p_iters=4)
+    plain = run_gpu(short.desc, lambda ctx: scenes.seed_context(ctx, short), 4)
+    assert_same(got, plain, "zero net vs plain K")
+
+
+def test_accepted_guess_matches_oracle():
+    """K = 0 and every pass skipped: the pressure view is the guess itself."""
+    scene = plume(p_iters=12)
+    layers = uo.init_params(4)
+    w, b = layers[-1]
+    layers[-1] = (w * 0.05, b)
+    params = uo.join_params(layers).astype(np.float32)
+    got, stats, rhs = run_learned(scene, 1, params, k=0)
+    assert not stats["fallback"]
+    h = float(np.float32(1.0 / 32))
+    rules = wo.pressure_rules(scene.desc.desc.config.flow_boundary)
+    p_bf = [(uo.bf16_round(w), np.asarray(b, np.float32).astype(np.float64)) for w, b in uo.split_params(params.astype(np.float64))]
+    p0, sigma, _ = wo.guess(rhs.astype(np.float64), p_bf, h, rules, uo.bf16_round)
+    err = np.max(np.abs(got["pressure"] - p0)) / np.max(np.abs(p0))
+    g, z, fb = wo.guess_test(got["pressure"], rhs.astype(np.float64), h * h, rules)
+    print(f"guess err={err:.2e} sigma={sigma:.3e} stats={stats} oracle g={g:.6e} z={z:.6e}", flush=True)
+    assert err <= 1e-2
+    assert abs(stats["guess_residual"] - g) <= 1e-5 * g and abs(stats["zero_residual"] - z) <= 1e-6 * z and not fb
