@@ -6,8 +6,11 @@ Workload (default): 256^3 cells, h = 1/256, dt = 0.1, nu = kappa = 1e-4, 40 diff
 noise smoothed by a 5^3 box filter. One step = one hipGraph launch of the whole step.
 
 Printed JSON line: steps/s (value), Gcell-updates/s, the dominant kernel's achieved HBM
-bandwidth against the 8 TB/s peak (timed live with HIP events on the context stream) and
-the host-loop baseline (FP64 oracle on this host's cores, a bounded sample).
+bandwidth against the 8 TB/s peak (timed live with HIP events on the context stream),
+the Jacobi pass bandwidth, the learned-vs-plain speedup at equal post-projection
+divergence L2 (warm-start network + K passes against 80 plain passes, from common
+states of the 256^3 plume; needs the trained parameters under stable-fluids_amd/weights)
+and the host-loop baseline (FP64 oracle on this host's cores, a bounded sample).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid N] [--smoother rbgs|jacobi]
                   [--storage f32|f16|bf16|f32-refined] [--interval M]
@@ -61,7 +64,10 @@ def main():
     ap.add_argument("--storage", choices=tuple(STORAGE), default="f32")
     ap.add_argument("--interval", type=int, default=10, help="passes per refinement cycle (16-bit storage)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain projection comparison")
     ap.add_argument("--precision", action="store_true", help="run the 512^3 storage precision study instead")
+    ap.add_argument("--batch", action="store_true", help="run the 64-simulation batched study instead")
+    ap.add_argument("--tuning-cache", default=None, help="autotuner cache file for --batch")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the stable-fluids step here runs one simulation per GPU; "
@@ -71,6 +77,9 @@ def main():
     pkg = entry.load_package()
     if args.precision:
         precision_study(pkg, args)
+        return
+    if args.batch:
+        batch_study(pkg, args)
         return
     abi, hip, scenes = pkg.abi, pkg.hip, pkg.scenes
     n = args.grid
@@ -127,6 +136,26 @@ def main():
         rec = json.load(open(tfile))["kernels"].get("k_rbgs_split_pass<0, false>")
         traffic = rec["traffic_bytes"] if rec else None
 
+    # learned warm start vs plain pressure passes at equal post-projection divergence L2
+    learned = None
+    if not args.no_learned and n == 256 and args.storage == "f32":
+        ev = pkg.warm_eval.evaluate(n=256)
+        if ev is not None:
+            learned = {
+                "workload": "plume 256^3, centre (128,24,128) radius 16, zero initial velocity, D=20; one step from "
+                            "checkpoints at steps 10/20/30; network on the 32^3 restriction",
+                "plain_passes": ev["plain_passes"],
+                "k_equal_rbgs": ev["k_equal_rbgs"], "k_equal_jacobi": ev["k_equal_jacobi"],
+                "speedup_vs_rbgs": ev["speedup_vs_rbgs"], "speedup_vs_jacobi": ev["speedup_vs_jacobi"],
+                "pressure_stage_speedup_vs_rbgs": ev.get("pressure_stage_speedup_vs_rbgs"),
+                "pressure_stage_speedup_vs_jacobi": ev.get("pressure_stage_speedup_vs_jacobi"),
+                "pressure_stage_ms": ev["pressure_stage_ms"],
+                "plain_rbgs_div_l2": ev["plain_rbgs_div_l2"], "plain_jacobi_div_l2": ev["plain_jacobi_div_l2"],
+                "learned_div_l2_by_k": {str(c["k"]): c["div_l2"] for c in ev["learned_curve"]},
+                "fallbacks": sum(sum(c["fallback"]) for c in ev["learned_curve"]),
+                "steps_per_s": {k: v for k, v in ev.items() if k.endswith("steps_per_s")},
+            }
+
     cpu = None
     if not args.no_cpu_baseline:
         oracle = entry.load_oracle()
@@ -165,6 +194,8 @@ def main():
                      "traffic": traffic, "bytes_per_launch": dom["bytes"], "avg_us": dom["us"],
                      "traffic_source": "profiles/traffic_plume256_kernels.json" if traffic else None},
         "jacobi_hbm_gbs": stages["jacobi_pass"]["gbs"],
+        "learned_vs_plain_speedup": learned["speedup_vs_rbgs"] if learned else None,
+        "learned_vs_plain": learned,
         "stages": stages,
         "final_divergence_l2": norms["divergence_l2"],
         "cpu_baseline": cpu,
@@ -224,6 +255,76 @@ def precision_study(pkg, args):
                       "refinement_interval": args.interval, "diffuse_iterations": 20, "pressure_iterations": 40,
                       "workload": f"plume {n}^3, centre ({n // 2},{3 * n // 32},{n // 2}) radius {n // 16}, zero initial velocity",
                       "runs": runs}), flush=True)
+
+
+def batch_scenes(pkg, count=64, seed=0, d_iters=40, p_iters=80):
+    """64 plume simulations: sizes drawn from {32^3, 48^3, 64^3, 96^3, 128^3, 128x64x64},
+    random centre, radius U[2, 8] cells, upward force U[1, 10]; free-slip box, h = 1/max(n),
+    dt = 0.1, nu = kappa = 1e-4."""
+    import numpy as np
+    scenes = pkg.scenes
+    rng = np.random.default_rng(seed)
+    sizes = [(32, 32, 32), (48, 48, 48), (64, 64, 64), (96, 96, 96), (128, 128, 128), (128, 64, 64)]
+    out = []
+    for i in range(count):
+        nx, ny, nz = sizes[int(rng.integers(0, len(sizes)))]
+        desc = scenes.plume_desc(max(nx, ny, nz), d_iters, p_iters, shape=(nx, ny, nz))
+        radius = float(rng.uniform(2.0, 8.0))
+        centre = (int(rng.integers(int(radius) + 1, nx - int(radius) - 1)), int(rng.integers(int(radius) + 1, max(int(radius) + 2, ny // 4))),
+                  int(rng.integers(int(radius) + 1, nz - int(radius) - 1)))
+        src, fx, fy, fz = scenes.plume_fields(desc.shape, centre, radius, 100.0, float(rng.uniform(1.0, 10.0)))
+        out.append(scenes.Scene(f"batch{i}", desc, src, (fx, fy, fz), None))
+    return out
+
+
+def batch_study(pkg, args):
+    """Aggregate throughput of 64 uneven simulations: one-by-one vs one batched launch per stage."""
+    hip = pkg.hip
+    steps = args.steps if args.steps != 200 else 10
+    scenes_ = batch_scenes(pkg)
+    cells = sum(sc.desc.cells for sc in scenes_)
+
+    def run(make_stream, batched, tune=False):
+        shared = hip.Stream() if make_stream else None
+        ctxs = []
+        for sc in scenes_:
+            ctx = pkg.Context(sc.desc, stream=shared)
+            pkg.scenes.seed_context(ctx, sc)
+            ctxs.append(ctx)
+        batch = pkg.Batch(ctxs) if batched else None
+        info = None
+        if batch is not None and tune:
+            tuned = batch.autotune(args.tuning_cache, repeats=5)
+            info = {"tuned_shapes": tuned}
+        step = batch.step if batch else (lambda: [c.step(1) for c in ctxs])
+        step()
+        hip.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            step()
+        hip.synchronize()
+        dt = time.perf_counter() - t
+        if batch is not None:
+            info = dict(info or {}, launches_per_step=batch.info(0)["launches_per_step"])
+            batch.destroy()
+        for c in ctxs:
+            c.destroy()
+        if shared is not None:
+            shared.destroy()
+        return steps / dt, info
+
+    res = {}
+    for name, shared, batched, tune in (("one_by_one_shared_stream", True, False, False), ("one_by_one_own_streams", False, False, False),
+                                        ("batched", False, True, False), ("batched_autotuned", False, True, True)):
+        sps, info = run(shared, batched, tune)
+        res[name] = {"batch_steps_per_s": round(sps, 3), "gcell_updates_per_s": round(cells * sps / 1e9, 4)}
+        if info:
+            res[name].update(info)
+    print(json.dumps({"study": "batched uneven simulations", "simulations": len(scenes_), "total_cells": cells,
+                      "diffuse_iterations": 40, "pressure_iterations": 80, "steps": steps,
+                      "sizes": sorted({"x".join(map(str, reversed(sc.desc.shape))) for sc in scenes_}),
+                      "results": res, "speedup_batched_vs_shared_stream": round(res["batched_autotuned"]["batch_steps_per_s"] / res["one_by_one_shared_stream"]["batch_steps_per_s"], 3),
+                      "speedup_batched_vs_own_streams": round(res["batched_autotuned"]["batch_steps_per_s"] / res["one_by_one_own_streams"]["batch_steps_per_s"], 3)}), flush=True)
 
 
 if __name__ == "__main__":

@@ -41,7 +41,7 @@ typedef enum StableFluidsExtWarmStart {
      * warm_start_iterations smoother passes. When the guess residual ||b - A p0|| exceeds
      * 10x the zero-guess residual ||b||, the guess is dropped and the step runs the plain
      * pressure_iterations passes from zero (bit-identical to the reference path).
-     * Needs nx, ny, nz multiples of 8, FP32 iterate storage and a model loaded with
+     * Needs extents multiples of 2^(c+2) (c = warm_start_coarsening), FP32 iterate storage and a model loaded with
      * stable_fluids_ext_set_pressure_model_cuda (until then the network is all zeros). */
     STABLE_FLUIDS_EXT_WARM_START_LEARNED = 1,
 } StableFluidsExtWarmStart;
@@ -54,7 +54,9 @@ typedef struct StableFluidsExtOptions {
     uint32_t pressure_warm_start;        /* StableFluidsExtWarmStart */
     uint32_t warm_start_iterations;      /* passes after a learned guess (<= pressure_iterations;
                                             raised by one when pressure_iterations minus it is odd) */
-    uint32_t reserved[2];                /* must be zero */
+    uint32_t warm_start_coarsening;      /* the network runs on the grid coarsened 2^c times per
+                                            axis (c = 1, 2 or 3; 0 = 1); extents multiples of 2^(c+2) */
+    uint32_t reserved[1];                /* must be zero */
 } StableFluidsExtOptions;
 
 /* Same as stable_fluids_create_context_cuda, with options (NULL = reference behaviour).
@@ -101,6 +103,33 @@ STABLE_FLUIDS_API uint64_t stable_fluids_ext_stage_bytes_cuda(void* context, uin
 /* Returns the context's HIP stream (hipStream_t as void*). */
 STABLE_FLUIDS_API void* stable_fluids_ext_get_stream_cuda(void* context);
 
+/* ---- batched simulations ----
+ * A batch steps many contexts (uneven grid sizes allowed) with one kernel launch per
+ * stage instead of one per stage per context: one graph launch runs every context's step
+ * in the reference step order, each context's result bit-identical to
+ * stable_fluids_step_cuda on it. The contexts must agree on diffuse_iterations and
+ * pressure_iterations (both > 0), scalar and vector field counts and the smoother, have
+ * positive viscosity and scalar diffusivities, FP32 storage, no learned warm start, and
+ * (RBGS) even extents on periodic axes. The batch reads the contexts' buffers in place:
+ * views, setters and updates keep working between batch steps; do not step a context
+ * on its own stream while a batch step runs. stream NULL: the batch owns a new stream. */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_create_cuda(void* const* contexts, uint32_t count, void* stream, void** out_batch);
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_destroy_cuda(void* batch);
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_step_cuda(void* batch);
+STABLE_FLUIDS_API void* stable_fluids_ext_batch_get_stream_cuda(void* batch);
+/* Smoother z-chunk length of one context in the batch (0 = default plan); rebuilds the
+ * batch graph. Tiling never changes results. */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_set_z_chunk_cuda(void* batch, uint32_t sim, uint32_t z_chunk);
+/* out_tiling4 = {tiles_x, tiles_y, z chunks, z chunk length} of context `sim`;
+ * out_launches_per_step = kernel launches in one batch step. Either may be NULL. */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_info_cuda(void* batch, uint32_t sim, int32_t* out_tiling4, uint32_t* out_launches_per_step);
+/* Autotuner: for every grid shape in the batch, times `repeats` batched pressure passes
+ * (writing only scratch buffers) for each z-chunk length in {default, 2, 4, ..., 64} and
+ * keeps the fastest; shapes listed in cache_path (lines "nx ny nz smoother zc") are
+ * taken from it, and the file is rewritten with all shapes (cache_path may be NULL).
+ * Rebuilds the batch graph; *out_tuned_shapes = shapes timed (not from the cache). */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_autotune_cuda(void* batch, const char* cache_path, uint32_t repeats, uint32_t* out_tuned_shapes);
+
 /* Device array [nz][ny][nx] FP32 of the last step's pre-projection divergence (the
  * pressure right-hand side; the DIVERGENCE view holds the post-projection field). Valid
  * until the context is destroyed; NULL for a NULL context. */
@@ -120,9 +149,11 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_warm_start_stats_cuda(voi
  * pre-projection divergence div and the exact solution p* of A p = -h^2 div (pressure
  * boundary rules; FP64 conjugate gradients to ||r|| <= rel_tol ||b||, at most
  * max_iterations, started from the previous call's solution; zero-mean when the system
- * has no outflow face). Writes input[v][2] = (div / sigma, wall mask) and target[v] =
- * p* / (h^2 sigma), sigma = RMS(div), device FP32. Synchronous. */
-STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_training_pair_cuda(void* context, float* input, float* target, double rel_tol, uint32_t max_iterations, double* out_rel_residual, uint32_t* out_iterations);
+ * has no outflow face), both averaged over blocks of f^3 cells, f = 2^coarsening
+ * (extents multiples of f). Writes input[v][2] = (div_c / sigma, wall mask) and
+ * target[v] = p*_c / ((f h)^2 sigma), sigma = RMS(div_c), on the coarse grid, device
+ * FP32. Synchronous. */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_training_pair_cuda(void* context, float* input, float* target, uint32_t coarsening, double rel_tol, uint32_t max_iterations, double* out_rel_residual, uint32_t* out_iterations);
 
 #ifdef __cplusplus
 }

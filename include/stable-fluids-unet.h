@@ -41,10 +41,13 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_unet_get_grads_cuda(void* net
 /* output[v] = net(input)[v]; input is device [v][2] FP32 (divergence, wall mask),
  * output device [v] FP32. Asynchronous on the network stream. */
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_unet_forward_cuda(void* net, const float* input, float* output);
-/* Forward, loss L = mean_v (net(input)[v] - target[v])^2 added to *loss (device FP32,
- * may be NULL) and the parameter gradient dL/dparams (replaces the previous one).
+/* Forward, loss L = weight * mean_v (q[v] - target[v])^2 added to *loss (device FP32, may
+ * be NULL) and the parameter gradient dL/dparams (replaces the previous one), q =
+ * net(input). flags bit 0 (STABLE_FLUIDS_UNET_LOSS_CENTERED): q is replaced by
+ * q - mean(q), for zero-mean targets defined up to a constant (closed boxes).
  * Asynchronous. */
-STABLE_FLUIDS_API StableFluidsResult stable_fluids_unet_backward_cuda(void* net, const float* input, const float* target, float* loss);
+#define STABLE_FLUIDS_UNET_LOSS_CENTERED 1u
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_unet_backward_cuda(void* net, const float* input, const float* target, float weight, uint32_t flags, float* loss);
 /* One Adam step on the parameters with the last gradient. Asynchronous. */
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_unet_adam_step_cuda(void* net, float lr, float beta1, float beta2, float eps);
 

@@ -2,10 +2,9 @@
 
 Restates stable-fluids_amd/csrc/sf_warm.hip in numpy: the pressure operator of a step
 (reference backend-cuda.cu:374-384, ghosts by load_pressure :96-128: walls copy the
-cell, outflow faces give their pressure, periodic axes wrap), the half-resolution
-restriction and wall mask of the network input, the trilinear prolongation of its
-output, the guess residual test and the fallback rule, and the exact solve check of a
-training pair. Arrays are [z][y][x].
+cell, outflow faces give their pressure, periodic axes wrap), the block-mean
+restriction (f^3 cells) and wall mask of the network input, the trilinear prolongation
+of its output, the guess residual test and the fallback rule. Arrays are [z][y][x].
 
 Pinned by tests/test_warm_oracle.py: the residual equals 6 (x_jacobi - x) from one
 Jacobi pass of the C oracle (oracle/sf_oracle.cpp), whose ghosts are pinned against
@@ -65,9 +64,10 @@ def apply_homogeneous(p, rules):
     return 6.0 * p - neighbour_sum(np.asarray(p, dtype=np.float64), rules, homogeneous=True)
 
 
-def restrict(div):
-    z, y, x = div.shape
-    return np.asarray(div, dtype=np.float64).reshape(z // 2, 2, y // 2, 2, x // 2, 2).mean(axis=(1, 3, 5))
+def restrict(a, f=2):
+    """Mean over blocks of f^3 cells."""
+    z, y, x = a.shape
+    return np.asarray(a, dtype=np.float64).reshape(z // f, f, y // f, f, x // f, f).mean(axis=(1, 3, 5))
 
 
 def wall_mask(shape, rules):
@@ -82,30 +82,31 @@ def wall_mask(shape, rules):
     return m
 
 
-def prolong(q):
-    """Cell-centred trilinear interpolation to twice the resolution, clamped at the edges."""
+def prolong(q, f=2):
+    """Cell-centred trilinear interpolation to f times the resolution, clamped at the edges."""
     out = np.asarray(q, dtype=np.float64)
     for axis in range(3):
         n = out.shape[axis]
-        c = np.clip(0.5 * np.arange(2 * n) - 0.25, 0.0, n - 1)
+        c = np.clip((np.arange(f * n) + 0.5) / f - 0.5, 0.0, n - 1)
         i0 = np.floor(c).astype(int)
         i1 = np.minimum(i0 + 1, n - 1)
         t = c - i0
         shape = [1, 1, 1]
-        shape[axis] = 2 * n
+        shape[axis] = f * n
         t = t.reshape(shape)
         out = np.take(out, i0, axis=axis) * (1 - t) + np.take(out, i1, axis=axis) * t
     return out
 
 
-def guess(div, params, h, rules, rnd=uo.identity):
-    """Warm-start guess p0 on the full grid and its (sigma, network output q)."""
-    dc = restrict(div)
+def guess(div, params, h, rules, rnd=uo.identity, f=2):
+    """Warm-start guess p0 on the full grid (network on the grid coarsened f times) and
+    its (sigma, network output q)."""
+    dc = restrict(div, f)
     sigma = float(np.sqrt(np.mean(dc * dc)))
     inv = 1.0 / sigma if sigma > 0 else 0.0
     inp = np.stack([dc * inv, wall_mask(dc.shape, rules)], axis=-1)
     q, _ = uo.forward(params, inp, rnd)
-    return 4.0 * h * h * sigma * prolong(q), sigma, q
+    return (f * h) ** 2 * sigma * prolong(q, f), sigma, q
 
 
 def guess_test(p0, div, h2, rules, ratio=10.0):

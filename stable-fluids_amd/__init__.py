@@ -5,5 +5,5 @@ include/stable-fluids-3d-ext.h). This package only binds it for Python callers, 
 the benchmark: `runtime.Context` mirrors the reference's entry points, `scenes` builds
 the workloads, `hip` holds device-buffer plumbing.
 """
-from . import abi, hip, runtime, scenes, unet  # noqa: F401
-from .runtime import Context, StableFluidsError, ext_options, load_library  # noqa: F401
+from . import abi, hip, runtime, scenes, unet, warm_eval  # noqa: F401
+from .runtime import Batch, Context, StableFluidsError, ext_options, load_library  # noqa: F401

@@ -105,7 +105,8 @@ class ExtOptions(C.Structure):
     """StableFluidsExtOptions (stable-fluids-3d-ext.h)."""
     _fields_ = [("struct_size", C.c_uint32), ("smoother", C.c_uint32), ("iterate_storage", C.c_uint32),
                 ("defect_correction_interval", C.c_uint32), ("pressure_warm_start", C.c_uint32),
-                ("warm_start_iterations", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+                ("warm_start_iterations", C.c_uint32), ("warm_start_coarsening", C.c_uint32),
+                ("reserved", C.c_uint32 * 1)]
 
 
 assert C.sizeof(FlowFace) == 20 and C.sizeof(FlowBoundary) == 120

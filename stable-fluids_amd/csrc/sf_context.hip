@@ -16,7 +16,9 @@
 #include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <new>
 #include <vector>
@@ -46,6 +48,7 @@ struct Options {
     int interval          = 10;        // smoother passes per refinement cycle
     bool learned          = false;     // learned pressure warm start (sf_warm.hip)
     int warm_iterations   = 0;         // passes after an accepted guess
+    int warm_factor       = 2;         // restriction factor of the network input
 };
 
 struct ScalarField {
@@ -264,13 +267,14 @@ hipError_t enqueue_learned_pressure(Context& c, const StencilBC& p_bc) {
     const int K    = c.options.warm_iterations;
     const int C    = P - K;
     const float h  = c.config.cell_size;
+    const int fac  = c.options.warm_factor;
     const Dims cd  = unet::net_dims(*w.net);
     const uint64_t vc = static_cast<uint64_t>(cd.nx) * cd.ny * cd.nz;
     float* buf[2]  = {(P % 2 == 0) ? f.p : f.p_aux, (P % 2 == 0) ? f.p_aux : f.p}; // buf[P % 2] = p
-    SF_TRY(warm::launch_restrict_input(c.stream, f.rhs, w.in2, w.partials, w.scal, cd, p_bc));
+    SF_TRY(warm::launch_restrict_input(c.stream, f.rhs, w.in2, w.partials, w.scal, cd, fac, p_bc));
     SF_TRY(unet::launch_pack_input(c.stream, w.in2, w.scal + 1, 1.0f, unet::net_input(*w.net), vc));
     SF_TRY(unet::net_forward_packed(*w.net, false));
-    SF_TRY(warm::launch_prolong(c.stream, unet::net_output(*w.net), w.scal, 4.0f * h * h, buf[0], cd));
+    SF_TRY(warm::launch_prolong(c.stream, unet::net_output(*w.net), w.scal, static_cast<float>(fac * fac) * h * h, buf[0], cd, fac));
     SF_TRY(warm::launch_guess_test(c.stream, buf[0], f.rhs, h * h, 10.0, w.partials, w.stats, w.skip, c.dims, p_bc));
     for (int i = 1; i <= P; ++i) {
         SmoothPass pass{buf[(i - 1) & 1], f.rhs, buf[i & 1], h * h, 1.0f};
@@ -394,9 +398,13 @@ StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const Stabl
         if (options.learned) {
             const int P = cfg.pressure_iterations;
             int k       = static_cast<int>(std::min<uint32_t>(opt->warm_start_iterations, 1u << 20));
-            if (P <= 0 || k > P || options.refine || cfg.nx % 8 || cfg.ny % 8 || cfg.nz % 8) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+            const uint32_t cl = opt->warm_start_coarsening == 0 ? 1u : opt->warm_start_coarsening;
+            if (cl > 3) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+            const int fac = 1 << cl, m = 4 * fac;
+            if (P <= 0 || k > P || options.refine || cfg.nx % m || cfg.ny % m || cfg.nz % m) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
             if ((P - k) % 2) ++k;
             options.warm_iterations = k;
+            options.warm_factor     = fac;
         }
     }
     if (cfg.nx <= 0 || cfg.ny <= 0 || cfg.nz <= 0) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
@@ -447,7 +455,8 @@ StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const Stabl
             if (!c->alloc(*p, c->bytes)) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
         if (c->options.learned) {
             auto& w = c->warm;
-            if (unet::net_create(cfg.nx / 2, cfg.ny / 2, cfg.nz / 2, c->stream, &w.net) != hipSuccess) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
+            const int fac = c->options.warm_factor;
+            if (unet::net_create(cfg.nx / fac, cfg.ny / fac, cfg.nz / fac, c->stream, &w.net) != hipSuccess) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
             if (!c->alloc(w.in2, c->bytes / 4) || !c->alloc(w.scal, 16) || !c->alloc(w.partials, sizeof(double) * 2 * warm::kPartialBlocks) || !c->alloc(w.stats, 4 * sizeof(double)) || !c->alloc(w.skip, 16)) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
         }
         for (auto& s : c->scalars)
@@ -558,6 +567,290 @@ __global__ __launch_bounds__(256) void k_norm_final(double* __restrict__ partial
     block_reduce<4>(acc, true, partials + 4 * kNormBlocks);
 }
 
+
+// ---- batched simulations: one launch per stage for many contexts ----
+//
+// The batch replays each context's step sequence (enqueue_step) with every stage
+// launched once for all simulations: workgroup i runs block items[i].block of the
+// stage descriptor items[i].desc (a simulation, or a simulation and velocity
+// component). The arithmetic of each block is the single-simulation kernel's, so
+// every simulation ends bit-identical to stepping it alone. The smoother's z-chunk
+// length per grid shape is the only tuned parameter (plan_tiles); it changes which
+// workgroup computes a cell, never the value.
+
+struct Batch {
+    std::vector<Context*> sims;
+    hipStream_t stream = nullptr;
+    bool owns_stream   = false;
+    std::vector<int> zc; // smoother z-chunk per simulation (0 = default)
+    hipGraph_t graph         = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<void*> allocations;
+    std::vector<std::function<hipError_t()>> plan; // the step's launches, in order
+    int launches = 0;
+
+    ~Batch() { release(); if (owns_stream && stream) (void)hipStreamDestroy(stream); }
+    void release() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph      = nullptr;
+        for (void* p : allocations) (void)hipFree(p);
+        allocations.clear();
+    }
+    template <class T>
+    T* upload(const std::vector<T>& v) {
+        void* raw = nullptr;
+        if (hipMalloc(&raw, std::max<size_t>(v.size() * sizeof(T), 16)) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        allocations.push_back(raw);
+        if (!v.empty() && hipMemcpy(raw, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+        return static_cast<T*>(raw);
+    }
+};
+
+bool batch_compatible(const Context& a, const Context& b) {
+    const auto &x = a.config, &y = b.config;
+    if (x.diffuse_iterations != y.diffuse_iterations || x.pressure_iterations != y.pressure_iterations) return false;
+    if (a.scalars.size() != b.scalars.size() || a.vectors.size() != b.vectors.size()) return false;
+    if (a.options.smoother != b.options.smoother) return false;
+    return true;
+}
+
+bool batch_supported(const Context& c) {
+    const auto& cfg = c.config;
+    if (c.options.refine || c.options.learned) return false;
+    if (cfg.diffuse_iterations <= 0 || cfg.pressure_iterations <= 0) return false;
+    if (!(cfg.dt * cfg.viscosity / (cfg.cell_size * cfg.cell_size) > 0.0f)) return false;
+    for (const auto& s : c.scalars)
+        if (!(cfg.dt * s.desc.diffusion / (cfg.cell_size * cfg.cell_size) > 0.0f)) return false;
+    const auto& fb = cfg.flow_boundary;
+    if (!smooth_batchable(c.options.smoother, c.dims, pressure_rules(fb))) return false;
+    for (int comp = 0; comp < 3; ++comp)
+        if (!smooth_batchable(c.options.smoother, c.dims, velocity_rules(fb, comp))) return false;
+    for (const auto& s : c.scalars)
+        if (!smooth_batchable(c.options.smoother, c.dims, scalar_rules(s.desc.boundary))) return false;
+    return true;
+}
+
+// Work list of a stage: for each descriptor d, its blocks 0..blocks(d)-1.
+template <class F>
+std::vector<BatchItem> batch_items(const size_t n_desc, F blocks) {
+    std::vector<BatchItem> items;
+    for (size_t d = 0; d < n_desc; ++d)
+        for (uint32_t b = 0, n = blocks(d); b < n; ++b) items.push_back(BatchItem{static_cast<uint32_t>(d), b});
+    return items;
+}
+
+#define SF_BTRY(expr)                                           \
+    do {                                                        \
+        if ((expr) != hipSuccess) return hipErrorUnknown;       \
+    } while (0)
+
+// The step of every simulation (enqueue_step order), one launch per stage: uploads every
+// descriptor table and work list, and records the launches in B.plan (captured later:
+// no allocation or copy may happen during stream capture).
+hipError_t plan_batch_step(Batch& B) {
+    auto& S        = B.sims;
+    const size_t N = S.size();
+    const auto& c0 = *S[0];
+    const int D = c0.config.diffuse_iterations, P = c0.config.pressure_iterations;
+    const SmootherKind sm = c0.options.smoother;
+    B.launches = 0;
+    B.plan.clear();
+    std::vector<TileGeom> geo(N);
+    for (size_t i = 0; i < N; ++i) geo[i] = plan_tiles(S[i]->dims, B.zc[i]);
+
+    const auto elem_items = B.upload(batch_items(N, [&](size_t d) { return elem_block_count(S[d]->cells); }));
+    const auto tile_items = B.upload(batch_items(N, [&](size_t d) { return tile_blocks(S[d]->dims); }));
+    const auto sm1_items  = B.upload(batch_items(N, [&](size_t d) { return smooth_blocks(geo[d]); }));
+    const auto sm3_items  = B.upload(batch_items(3 * N, [&](size_t d) { return smooth_blocks(geo[d / 3]); }));
+    const uint32_t n_elem = static_cast<uint32_t>(batch_items(N, [&](size_t d) { return elem_block_count(S[d]->cells); }).size());
+    const uint32_t n_tile = static_cast<uint32_t>(batch_items(N, [&](size_t d) { return tile_blocks(S[d]->dims); }).size());
+    const uint32_t n_sm1  = static_cast<uint32_t>(batch_items(N, [&](size_t d) { return smooth_blocks(geo[d]); }).size());
+    const uint32_t n_sm3  = static_cast<uint32_t>(batch_items(3 * N, [&](size_t d) { return smooth_blocks(geo[d / 3]); }).size());
+    if (!elem_items || !tile_items || !sm1_items || !sm3_items) return hipErrorOutOfMemory;
+    hipStream_t st = B.stream;
+    auto add       = [&](std::function<hipError_t()> f) {
+        B.plan.push_back(std::move(f));
+        ++B.launches;
+        return hipSuccess;
+    };
+
+    // forces (reference :612-617)
+    for (size_t v = 0; v < c0.vectors.size(); ++v) {
+        std::vector<AddForceDesc> d(N);
+        for (size_t i = 0; i < N; ++i) {
+            auto& c = *S[i];
+            d[i]    = AddForceDesc{{c.flow.vx, c.flow.vy, c.flow.vz}, {c.vectors[v].x, c.vectors[v].y, c.vectors[v].z}, c.config.dt, c.cells};
+        }
+        const auto dd = B.upload(d);
+        if (!dd) return hipErrorOutOfMemory;
+        SF_BTRY(add([=] { return launch_add_force_batched(st, dd, elem_items, n_elem); }));
+    }
+    {
+        std::vector<AdvectVelocityDesc> d(N);
+        for (size_t i = 0; i < N; ++i) {
+            auto& c = *S[i];
+            auto& f = c.flow;
+            d[i]    = AdvectVelocityDesc{{f.tx, f.ty, f.tz}, {f.vx, f.vy, f.vz}, c.config.dt, c.dims, c.config.cell_size, c.config.flow_boundary};
+        }
+        const auto dd = B.upload(d);
+        if (!dd) return hipErrorOutOfMemory;
+        SF_BTRY(add([=] { return launch_advect_velocity_batched(st, dd, tile_items, n_tile); }));
+    }
+    // velocity diffusion: pass i of all three components of all simulations per launch
+    for (int pass = 1; pass <= D; ++pass) {
+        std::vector<SmoothDesc> d(3 * N);
+        for (size_t i = 0; i < N; ++i) {
+            auto& c           = *S[i];
+            auto& f           = c.flow;
+            const float alpha = c.config.dt * c.config.viscosity / (c.config.cell_size * c.config.cell_size);
+            float* t[3] = {f.tx, f.ty, f.tz};
+            float* a[3] = {f.ax, f.ay, f.az};
+            float* v[3] = {f.vx, f.vy, f.vz};
+            for (int comp = 0; comp < 3; ++comp) {
+                float* dst       = ((D - pass) % 2 == 0) ? a[comp] : v[comp];
+                const float* src = pass == 1 ? t[comp] : (((D - pass + 1) % 2 == 0) ? a[comp] : v[comp]);
+                d[3 * i + comp]  = SmoothDesc{src, t[comp], dst, alpha, 1.0f, c.dims, velocity_rules(c.config.flow_boundary, comp), geo[i]};
+            }
+        }
+        const auto dd = B.upload(d);
+        if (!dd) return hipErrorOutOfMemory;
+        SF_BTRY(add([=] { return launch_smooth_batched(st, sm, kSolveDiffusion, false, dd, sm3_items, n_sm3); }));
+    }
+    {
+        std::vector<DivergenceDesc> d(N);
+        for (size_t i = 0; i < N; ++i) {
+            auto& c = *S[i];
+            auto& f = c.flow;
+            d[i]    = DivergenceDesc{f.rhs, {f.ax, f.ay, f.az}, c.dims, c.config.cell_size, divergence_rules(c.config.flow_boundary)};
+        }
+        const auto dd = B.upload(d);
+        if (!dd) return hipErrorOutOfMemory;
+        SF_BTRY(add([=] { return launch_divergence_batched(st, dd, tile_items, n_tile); }));
+    }
+    for (int pass = 1; pass <= P; ++pass) {
+        std::vector<SmoothDesc> d(N);
+        for (size_t i = 0; i < N; ++i) {
+            auto& c          = *S[i];
+            auto& f          = c.flow;
+            float* dst       = ((P - pass) % 2 == 0) ? f.p : f.p_aux;
+            const float* src = pass == 1 ? nullptr : (((P - pass + 1) % 2 == 0) ? f.p : f.p_aux);
+            d[i] = SmoothDesc{src, f.rhs, dst, c.config.cell_size * c.config.cell_size, 1.0f, c.dims, pressure_rules(c.config.flow_boundary), geo[i]};
+        }
+        const auto dd = B.upload(d);
+        if (!dd) return hipErrorOutOfMemory;
+        SF_BTRY(add([=] { return launch_smooth_batched(st, sm, kSolvePoisson, pass == 1, dd, sm1_items, n_sm1); }));
+    }
+    {
+        std::vector<ProjectDesc> d(N);
+        for (size_t i = 0; i < N; ++i) {
+            auto& c = *S[i];
+            auto& f = c.flow;
+            d[i]    = ProjectDesc{{f.vx, f.vy, f.vz}, f.div, f.mag, {f.ax, f.ay, f.az}, f.p, c.dims, c.config.cell_size, pressure_rules(c.config.flow_boundary), divergence_rules(c.config.flow_boundary)};
+        }
+        const auto dd = B.upload(d);
+        if (!dd) return hipErrorOutOfMemory;
+        SF_BTRY(add([=] { return launch_project_batched(st, dd, tile_items, n_tile); }));
+    }
+    // scalars (reference :652-668)
+    for (size_t si = 0; si < c0.scalars.size(); ++si) {
+        std::vector<AddScaledDesc> da(N);
+        std::vector<AdvectScalarDesc> dv(N);
+        for (size_t i = 0; i < N; ++i) {
+            auto& c = *S[i];
+            auto& s = c.scalars[si];
+            auto& f = c.flow;
+            da[i]   = AddScaledDesc{s.data, s.source, c.config.dt, c.cells};
+            dv[i]   = AdvectScalarDesc{s.temp, s.data, {f.vx, f.vy, f.vz}, c.config.dt, c.dims, c.config.cell_size, s.desc.boundary, c.config.flow_boundary};
+        }
+        const auto dda = B.upload(da);
+        const auto ddv = B.upload(dv);
+        if (!dda || !ddv) return hipErrorOutOfMemory;
+        SF_BTRY(add([=] { return launch_add_scaled_batched(st, dda, elem_items, n_elem); }));
+        SF_BTRY(add([=] { return launch_advect_scalar_batched(st, ddv, tile_items, n_tile); }));
+        for (int pass = 1; pass <= D; ++pass) {
+            std::vector<SmoothDesc> d(N);
+            for (size_t i = 0; i < N; ++i) {
+                auto& c              = *S[i];
+                auto& s              = c.scalars[si];
+                const float alpha_s  = c.config.dt * s.desc.diffusion / (c.config.cell_size * c.config.cell_size);
+                const bool dissipate = s.desc.dissipation > 0.0f;
+                const float factor   = dissipate ? 1.0f / (1.0f + c.config.dt * s.desc.dissipation) : 1.0f;
+                float* dst           = ((D - pass) % 2 == 0) ? s.data : s.aux;
+                const float* src     = pass == 1 ? s.temp : (((D - pass + 1) % 2 == 0) ? s.data : s.aux);
+                d[i] = SmoothDesc{src, s.temp, dst, alpha_s, pass == D ? factor : 1.0f, c.dims, scalar_rules(s.desc.boundary), geo[i]};
+            }
+            const auto dd = B.upload(d);
+            if (!dd) return hipErrorOutOfMemory;
+            SF_BTRY(add([=] { return launch_smooth_batched(st, sm, kSolveDiffusion, false, dd, sm1_items, n_sm1); }));
+        }
+    }
+    return hipSuccess;
+}
+
+#undef SF_BTRY
+
+StableFluidsResult build_batch_graph(Batch& B) {
+    B.release();
+    const hipError_t planned = plan_batch_step(B);
+    if (planned != hipSuccess) return planned == hipErrorOutOfMemory ? STABLE_FLUIDS_RESULT_OUT_OF_MEMORY : STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    if (hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    hipError_t enq = hipSuccess;
+    for (auto& launch : B.plan)
+        if (enq == hipSuccess) enq = launch();
+    hipGraph_t graph     = nullptr;
+    const hipError_t end = hipStreamEndCapture(B.stream, &graph);
+    if (enq != hipSuccess || end != hipSuccess || graph == nullptr) {
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    }
+    B.graph = graph;
+    if (hipGraphInstantiate(&B.graph_exec, B.graph, nullptr, nullptr, 0) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    return STABLE_FLUIDS_RESULT_OK;
+}
+
+// Mean time of `repeats` batched pressure passes (read rhs, write the scratch p_aux) under
+// the z-chunk plan B.zc; views are not touched.
+double time_batch_smoother(Batch& B, const int repeats) {
+    const size_t N = B.sims.size();
+    std::vector<TileGeom> geo(N);
+    for (size_t i = 0; i < N; ++i) geo[i] = plan_tiles(B.sims[i]->dims, B.zc[i]);
+    std::vector<SmoothDesc> d(N);
+    for (size_t i = 0; i < N; ++i) {
+        auto& c = *B.sims[i];
+        d[i]    = SmoothDesc{c.flow.rhs, c.flow.rhs, c.flow.p_aux, c.config.cell_size * c.config.cell_size, 1.0f, c.dims, pressure_rules(c.config.flow_boundary), geo[i]};
+    }
+    const auto items = batch_items(N, [&](size_t k) { return smooth_blocks(geo[k]); });
+    void *dd = nullptr, *di = nullptr;
+    double ms = -1.0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipMalloc(&dd, d.size() * sizeof(SmoothDesc)) == hipSuccess && hipMalloc(&di, items.size() * sizeof(BatchItem)) == hipSuccess &&
+        hipMemcpy(dd, d.data(), d.size() * sizeof(SmoothDesc), hipMemcpyHostToDevice) == hipSuccess &&
+        hipMemcpy(di, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice) == hipSuccess &&
+        hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+        const SmootherKind sm = B.sims[0]->options.smoother;
+        const auto* descs     = static_cast<const SmoothDesc*>(dd);
+        const auto* its       = static_cast<const BatchItem*>(di);
+        const uint32_t n      = static_cast<uint32_t>(items.size());
+        bool ok = launch_smooth_batched(B.stream, sm, kSolvePoisson, false, descs, its, n) == hipSuccess;
+        ok      = ok && hipEventRecord(e0, B.stream) == hipSuccess;
+        for (int r = 0; ok && r < repeats; ++r) ok = launch_smooth_batched(B.stream, sm, kSolvePoisson, false, descs, its, n) == hipSuccess;
+        ok       = ok && hipEventRecord(e1, B.stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+        float t  = 0.0f;
+        if (ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms = t / repeats;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (dd) (void)hipFree(dd);
+    if (di) (void)hipFree(di);
+    (void)hipGetLastError();
+    return ms;
+}
 } // namespace
 
 } // namespace sf
@@ -734,10 +1027,12 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_warm_start_stats_cuda(voi
     return sf::status(hipMemcpy(out3, c.warm.stats, 3 * sizeof(double), hipMemcpyDeviceToHost));
 }
 
-STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_training_pair_cuda(void* context, float* input, float* target, double rel_tol, uint32_t max_iterations, double* out_rel_residual, uint32_t* out_iterations) {
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_training_pair_cuda(void* context, float* input, float* target, uint32_t coarsening, double rel_tol, uint32_t max_iterations, double* out_rel_residual, uint32_t* out_iterations) {
     sf::Range r("stable.ext.training_pair");
-    if (context == nullptr || input == nullptr || target == nullptr || !(rel_tol > 0.0)) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
-    auto& c  = *static_cast<Context*>(context);
+    if (context == nullptr || input == nullptr || target == nullptr || !(rel_tol > 0.0) || coarsening > 4) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& c       = *static_cast<Context*>(context);
+    const int fac = 1 << coarsening;
+    if (c.config.nx % fac || c.config.ny % fac || c.config.nz % fac) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     auto& g  = c.cg;
     if (g.x == nullptr) {
         const size_t db = c.cells * sizeof(double);
@@ -749,11 +1044,148 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_training_pair_cuda(void* 
     sf::warm::CgResult res{};
     const int max_it = static_cast<int>(std::min<uint32_t>(max_iterations, 1u << 30));
     if (sf::warm::cg_solve(c.stream, c.flow.rhs, h2, g.x, g.r, g.p, g.ap, g.partials, g.scal, c.dims, bc, rel_tol, max_it, &res) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
-    if (sf::warm::launch_pair(c.stream, c.flow.rhs, g.x, h2, input, target, g.partials, g.scal, c.dims, bc) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    if (sf::warm::launch_pair(c.stream, c.flow.rhs, g.x, h2, input, target, g.partials, g.scal, c.dims, fac, bc) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     if (hipStreamSynchronize(c.stream) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     if (out_rel_residual) *out_rel_residual = res.rel_residual;
     if (out_iterations) *out_iterations = static_cast<uint32_t>(res.iterations);
     return STABLE_FLUIDS_RESULT_OK;
+}
+
+
+// ---- batched simulations (stable-fluids-3d-ext.h) ----
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_create_cuda(void* const* contexts, uint32_t count, void* stream, void** out_batch) {
+    sf::Range r("stable.ext.batch_create");
+    if (out_batch == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    *out_batch = nullptr;
+    if (contexts == nullptr || count == 0) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    std::unique_ptr<sf::Batch> b{new (std::nothrow) sf::Batch{}};
+    if (!b) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
+    for (uint32_t i = 0; i < count; ++i) {
+        if (contexts[i] == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        auto* c = static_cast<Context*>(contexts[i]);
+        for (auto* o : b->sims)
+            if (o == c) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE; // a context appears once
+        if (!sf::batch_supported(*c) || (i > 0 && !sf::batch_compatible(*b->sims[0], *c))) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        b->sims.push_back(c);
+    }
+    b->zc.assign(count, 0);
+    b->stream = static_cast<hipStream_t>(stream);
+    if (b->stream == nullptr) {
+        if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        b->owns_stream = true;
+    }
+    for (auto* c : b->sims)
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    const StableFluidsResult g = sf::build_batch_graph(*b);
+    if (g != STABLE_FLUIDS_RESULT_OK) return g;
+    *out_batch = b.release();
+    return STABLE_FLUIDS_RESULT_OK;
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_destroy_cuda(void* batch) {
+    if (batch == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto* b = static_cast<sf::Batch*>(batch);
+    (void)hipStreamSynchronize(b->stream);
+    delete b;
+    return STABLE_FLUIDS_RESULT_OK;
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_step_cuda(void* batch) {
+    sf::Range r("stable.ext.batch_step");
+    if (batch == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& b = *static_cast<sf::Batch*>(batch);
+    return sf::status(hipGraphLaunch(b.graph_exec, b.stream));
+}
+
+STABLE_FLUIDS_API void* stable_fluids_ext_batch_get_stream_cuda(void* batch) { return batch == nullptr ? nullptr : static_cast<sf::Batch*>(batch)->stream; }
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_set_z_chunk_cuda(void* batch, uint32_t sim, uint32_t z_chunk) {
+    if (batch == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& b = *static_cast<sf::Batch*>(batch);
+    if (sim >= b.sims.size() || z_chunk > static_cast<uint32_t>(b.sims[sim]->dims.nz)) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    if (hipStreamSynchronize(b.stream) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    b.zc[sim] = static_cast<int>(z_chunk);
+    return sf::build_batch_graph(b);
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_info_cuda(void* batch, uint32_t sim, int32_t* out_tiling4, uint32_t* out_launches_per_step) {
+    if (batch == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& b = *static_cast<sf::Batch*>(batch);
+    if (sim >= b.sims.size()) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    const sf::TileGeom g = sf::plan_tiles(b.sims[sim]->dims, b.zc[sim]);
+    if (out_tiling4) {
+        out_tiling4[0] = g.tiles_x;
+        out_tiling4[1] = g.tiles_y;
+        out_tiling4[2] = g.chunks;
+        out_tiling4[3] = g.zc;
+    }
+    if (out_launches_per_step) *out_launches_per_step = static_cast<uint32_t>(b.launches);
+    return STABLE_FLUIDS_RESULT_OK;
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_autotune_cuda(void* batch, const char* cache_path, uint32_t repeats, uint32_t* out_tuned_shapes) {
+    sf::Range r("stable.ext.batch_autotune");
+    if (batch == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& b = *static_cast<sf::Batch*>(batch);
+    if (hipStreamSynchronize(b.stream) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    const int reps   = static_cast<int>(std::min<uint32_t>(std::max<uint32_t>(repeats, 1u), 1000u));
+    const int sm     = static_cast<int>(b.sims[0]->options.smoother);
+    struct Entry {
+        int nx, ny, nz, smoother, zc;
+    };
+    std::vector<Entry> cache;
+    if (cache_path != nullptr) {
+        if (FILE* f = std::fopen(cache_path, "r")) {
+            Entry e{};
+            while (std::fscanf(f, "%d %d %d %d %d", &e.nx, &e.ny, &e.nz, &e.smoother, &e.zc) == 5) cache.push_back(e);
+            std::fclose(f);
+        }
+    }
+    // shapes in the batch, in first-appearance order
+    std::vector<sf::Dims> shapes;
+    for (auto* c : b.sims) {
+        bool seen = false;
+        for (auto& d : shapes) seen = seen || (d.nx == c->dims.nx && d.ny == c->dims.ny && d.nz == c->dims.nz);
+        if (!seen) shapes.push_back(c->dims);
+    }
+    auto assign = [&](const sf::Dims& d, const int zc) {
+        for (size_t i = 0; i < b.sims.size(); ++i)
+            if (b.sims[i]->dims.nx == d.nx && b.sims[i]->dims.ny == d.ny && b.sims[i]->dims.nz == d.nz) b.zc[i] = zc;
+    };
+    uint32_t tuned = 0;
+    for (auto& d : shapes) {
+        bool cached = false;
+        for (auto& e : cache)
+            if (e.nx == d.nx && e.ny == d.ny && e.nz == d.nz && e.smoother == sm) {
+                assign(d, e.zc);
+                cached = true;
+            }
+        if (cached) continue;
+        // coordinate descent: other shapes keep their current chunk lengths
+        int best_zc = 0;
+        double best = 1e300;
+        for (const int cand : {0, 2, 4, 8, 16, 32, 64}) {
+            if (cand > d.nz) continue;
+            assign(d, cand);
+            const double t = sf::time_batch_smoother(b, reps);
+            if (t > 0.0 && t < best) {
+                best    = t;
+                best_zc = sf::plan_tiles(d, cand).zc;
+            }
+        }
+        assign(d, best_zc);
+        cache.push_back(Entry{d.nx, d.ny, d.nz, sm, best_zc});
+        ++tuned;
+    }
+    if (cache_path != nullptr) {
+        if (FILE* f = std::fopen(cache_path, "w")) {
+            for (auto& e : cache) std::fprintf(f, "%d %d %d %d %d\n", e.nx, e.ny, e.nz, e.smoother, e.zc);
+            std::fclose(f);
+        }
+    }
+    if (out_tuned_shapes) *out_tuned_shapes = tuned;
+    return sf::build_batch_graph(b);
 }
 
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_run_stage_cuda(void* context, uint32_t stage, uint32_t repeat) {

@@ -81,10 +81,7 @@ __global__ __launch_bounds__(kElemBlock) void k_add_force(float* __restrict__ vx
 
 // One midpoint trace per cell, three component samples (reference
 // advect_velocity_component_kernel backend-cuda.cu:330-340, launched per component).
-__global__ __launch_bounds__(kTileX* kTileY) void k_advect_velocity(float* __restrict__ tx, float* __restrict__ ty, float* __restrict__ tz, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float dt, const Dims d, const float h, const StableFluidsFlowBoundaryConfig bc) {
-    const int x = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    const int y = static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
-    const int z = static_cast<int>(blockIdx.z);
+__device__ __forceinline__ void advect_velocity_body(float* __restrict__ tx, float* __restrict__ ty, float* __restrict__ tz, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float dt, const Dims d, const float h, const StableFluidsFlowBoundaryConfig& bc, const int x, const int y, const int z) {
     if (x >= d.nx || y >= d.ny) return;
     const float px = (static_cast<float>(x) + 0.5f) * h;
     const float py = (static_cast<float>(y) + 0.5f) * h;
@@ -100,10 +97,7 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_advect_velocity(float* __res
 }
 
 // reference advect_scalar_component_kernel backend-cuda.cu:342-352
-__global__ __launch_bounds__(kTileX* kTileY) void k_advect_scalar(float* __restrict__ out, const float* __restrict__ src, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float dt, const Dims d, const float h, const StableFluidsScalarBoundaryConfig sbc, const StableFluidsFlowBoundaryConfig fbc) {
-    const int x = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    const int y = static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
-    const int z = static_cast<int>(blockIdx.z);
+__device__ __forceinline__ void advect_scalar_body(float* __restrict__ out, const float* __restrict__ src, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float dt, const Dims d, const float h, const StableFluidsScalarBoundaryConfig& sbc, const StableFluidsFlowBoundaryConfig& fbc, const int x, const int y, const int z) {
     if (x >= d.nx || y >= d.ny) return;
     const float px = (static_cast<float>(x) + 0.5f) * h;
     const float py = (static_cast<float>(y) + 0.5f) * h;
@@ -130,10 +124,7 @@ __device__ __forceinline__ float along_axis(const int axis, const int dir, const
 }
 
 // reference compute_divergence_kernel backend-cuda.cu:386-396
-__global__ __launch_bounds__(kTileX* kTileY) void k_divergence(float* __restrict__ div, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const Dims d, const float h, const StencilBC dbc) {
-    const int x = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    const int y = static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
-    const int z = static_cast<int>(blockIdx.z);
+__device__ __forceinline__ void divergence_body(float* __restrict__ div, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const Dims d, const float h, const StencilBC& dbc, const int x, const int y, const int z) {
     if (x >= d.nx || y >= d.ny) return;
     const uint64_t i    = lin(x, y, z, d);
     const float inv_2h  = 0.5f / h;
@@ -164,10 +155,7 @@ __device__ __forceinline__ float grad_p(const float* __restrict__ p, const int c
 // reference project_velocity_kernel :409-422, then compute_divergence_kernel :386-396 and
 // velocity_magnitude_kernel :424-434 on the projected field (the scalar stage between
 // them does not touch the velocity). Out of place: neighbours are re-projected locally.
-__global__ __launch_bounds__(kTileX* kTileY) void k_project(float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz, float* __restrict__ div, float* __restrict__ mag, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float* __restrict__ p, const Dims d, const float h, const StencilBC pbc, const StencilBC dbc) {
-    const int x = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    const int y = static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y);
-    const int z = static_cast<int>(blockIdx.z);
+__device__ __forceinline__ void project_body(float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz, float* __restrict__ div, float* __restrict__ mag, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float* __restrict__ p, const Dims d, const float h, const StencilBC& pbc, const StencilBC& dbc, const int x, const int y, const int z) {
     if (x >= d.nx || y >= d.ny) return;
     const uint64_t i   = lin(x, y, z, d);
     const float inv_2h = 0.5f / h;
@@ -190,6 +178,93 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_project(float* __restrict__ 
     oz[i]  = proj[2];
     div[i] = acc;
     mag[i] = sqrtf(proj[0] * proj[0] + proj[1] * proj[1] + proj[2] * proj[2]);
+}
+
+// Local block b of a tile-kernel launch over d -> (block x, block y, plane).
+__device__ __forceinline__ void tile_block_of(const Dims d, const uint32_t b, int& bx, int& by, int& bz) {
+    const uint32_t gx = static_cast<uint32_t>((d.nx + kTileX - 1) / kTileX), gy = static_cast<uint32_t>((d.ny + kTileY - 1) / kTileY);
+    bx = static_cast<int>(b % gx);
+    by = static_cast<int>((b / gx) % gy);
+    bz = static_cast<int>(b / (gx * gy));
+}
+
+__global__ __launch_bounds__(kTileX* kTileY) void k_advect_velocity(float* __restrict__ tx, float* __restrict__ ty, float* __restrict__ tz, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float dt, const Dims d, const float h, const StableFluidsFlowBoundaryConfig bc) {
+    advect_velocity_body(tx, ty, tz, vx, vy, vz, dt, d, h, bc, static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y), static_cast<int>(blockIdx.z));
+}
+__global__ __launch_bounds__(kTileX* kTileY) void k_advect_velocity_batched(const AdvectVelocityDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
+    const BatchItem it = items[blockIdx.x];
+    const AdvectVelocityDesc sd    = descs[it.desc];
+    int bx, by, bz;
+    tile_block_of(sd.d, it.block, bx, by, bz);
+    advect_velocity_body(sd.t[0], sd.t[1], sd.t[2], sd.v[0], sd.v[1], sd.v[2], sd.dt, sd.d, sd.h, sd.bc, bx * kTileX + static_cast<int>(threadIdx.x), by * kTileY + static_cast<int>(threadIdx.y), bz);
+}
+__global__ __launch_bounds__(kTileX* kTileY) void k_advect_scalar(float* __restrict__ out, const float* __restrict__ src, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float dt, const Dims d, const float h, const StableFluidsScalarBoundaryConfig sbc, const StableFluidsFlowBoundaryConfig fbc) {
+    advect_scalar_body(out, src, vx, vy, vz, dt, d, h, sbc, fbc, static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y), static_cast<int>(blockIdx.z));
+}
+__global__ __launch_bounds__(kTileX* kTileY) void k_advect_scalar_batched(const AdvectScalarDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
+    const BatchItem it = items[blockIdx.x];
+    const AdvectScalarDesc sd    = descs[it.desc];
+    int bx, by, bz;
+    tile_block_of(sd.d, it.block, bx, by, bz);
+    advect_scalar_body(sd.out, sd.src, sd.v[0], sd.v[1], sd.v[2], sd.dt, sd.d, sd.h, sd.sbc, sd.fbc, bx * kTileX + static_cast<int>(threadIdx.x), by * kTileY + static_cast<int>(threadIdx.y), bz);
+}
+__global__ __launch_bounds__(kTileX* kTileY) void k_divergence(float* __restrict__ div, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const Dims d, const float h, const StencilBC dbc) {
+    divergence_body(div, vx, vy, vz, d, h, dbc, static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y), static_cast<int>(blockIdx.z));
+}
+__global__ __launch_bounds__(kTileX* kTileY) void k_divergence_batched(const DivergenceDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
+    const BatchItem it = items[blockIdx.x];
+    const DivergenceDesc sd    = descs[it.desc];
+    int bx, by, bz;
+    tile_block_of(sd.d, it.block, bx, by, bz);
+    divergence_body(sd.div, sd.v[0], sd.v[1], sd.v[2], sd.d, sd.h, sd.dbc, bx * kTileX + static_cast<int>(threadIdx.x), by * kTileY + static_cast<int>(threadIdx.y), bz);
+}
+__global__ __launch_bounds__(kTileX* kTileY) void k_project(float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz, float* __restrict__ div, float* __restrict__ mag, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float* __restrict__ p, const Dims d, const float h, const StencilBC pbc, const StencilBC dbc) {
+    project_body(ox, oy, oz, div, mag, vx, vy, vz, p, d, h, pbc, dbc, static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y), static_cast<int>(blockIdx.z));
+}
+__global__ __launch_bounds__(kTileX* kTileY) void k_project_batched(const ProjectDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
+    const BatchItem it = items[blockIdx.x];
+    const ProjectDesc sd    = descs[it.desc];
+    int bx, by, bz;
+    tile_block_of(sd.d, it.block, bx, by, bz);
+    project_body(sd.o[0], sd.o[1], sd.o[2], sd.div, sd.mag, sd.v[0], sd.v[1], sd.v[2], sd.p, sd.d, sd.h, sd.pbc, sd.dbc, bx * kTileX + static_cast<int>(threadIdx.x), by * kTileY + static_cast<int>(threadIdx.y), bz);
+}
+
+// Batched elementwise stages: local block b covers elements [4 * 256 * b, + 1024).
+__global__ __launch_bounds__(kElemBlock) void k_add_force_batched(const AddForceDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
+    const BatchItem it = items[blockIdx.x];
+    const AddForceDesc sd = descs[it.desc];
+    const uint64_t i = (static_cast<uint64_t>(it.block) * kElemBlock + threadIdx.x) * 4;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (i + 3 < sd.n) {
+            float4 a        = *reinterpret_cast<float4*>(sd.v[c] + i);
+            const float4 fv = *reinterpret_cast<const float4*>(sd.f[c] + i);
+            a.x += sd.dt * fv.x;
+            a.y += sd.dt * fv.y;
+            a.z += sd.dt * fv.z;
+            a.w += sd.dt * fv.w;
+            *reinterpret_cast<float4*>(sd.v[c] + i) = a;
+        } else {
+            for (uint64_t k = i; k < sd.n; ++k) sd.v[c][k] += sd.dt * sd.f[c][k];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kElemBlock) void k_add_scaled_batched(const AddScaledDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
+    const BatchItem it = items[blockIdx.x];
+    const AddScaledDesc sd = descs[it.desc];
+    const uint64_t i = (static_cast<uint64_t>(it.block) * kElemBlock + threadIdx.x) * 4;
+    if (i + 3 < sd.n) {
+        float4 a       = *reinterpret_cast<float4*>(sd.dst + i);
+        const float4 f = *reinterpret_cast<const float4*>(sd.src + i);
+        a.x += sd.dt * f.x;
+        a.y += sd.dt * f.y;
+        a.z += sd.dt * f.z;
+        a.w += sd.dt * f.w;
+        *reinterpret_cast<float4*>(sd.dst + i) = a;
+    } else {
+        for (uint64_t k = i; k < sd.n; ++k) sd.dst[k] += sd.dt * sd.src[k];
+    }
 }
 
 inline unsigned elem_blocks(const uint64_t n) { return static_cast<unsigned>((n + 4 * kElemBlock - 1) / (4 * kElemBlock)); }
@@ -239,6 +314,40 @@ hipError_t launch_divergence(hipStream_t s, float* div, const float* vx, const f
 
 hipError_t launch_project(hipStream_t s, float* ox, float* oy, float* oz, float* div, float* mag, const float* vx, const float* vy, const float* vz, const float* p, const Dims d, const float h, const StencilBC& p_bc, const StencilBC& div_bc) {
     k_project<<<tile_grid(d), tile_block(), 0, s>>>(ox, oy, oz, div, mag, vx, vy, vz, p, d, h, p_bc, div_bc);
+    return hipGetLastError();
+}
+
+uint32_t tile_blocks(const Dims d) { return static_cast<uint32_t>(((d.nx + kTileX - 1) / kTileX) * ((d.ny + kTileY - 1) / kTileY) * d.nz); }
+uint32_t elem_block_count(const uint64_t n) { return static_cast<uint32_t>((n + 4 * kElemBlock - 1) / (4 * kElemBlock)); }
+
+hipError_t launch_add_force_batched(hipStream_t s, const AddForceDesc* descs, const BatchItem* items, const uint32_t n_items) {
+    if (n_items == 0) return hipSuccess;
+    k_add_force_batched<<<n_items, kElemBlock, 0, s>>>(descs, items);
+    return hipGetLastError();
+}
+hipError_t launch_add_scaled_batched(hipStream_t s, const AddScaledDesc* descs, const BatchItem* items, const uint32_t n_items) {
+    if (n_items == 0) return hipSuccess;
+    k_add_scaled_batched<<<n_items, kElemBlock, 0, s>>>(descs, items);
+    return hipGetLastError();
+}
+hipError_t launch_advect_velocity_batched(hipStream_t s, const AdvectVelocityDesc* descs, const BatchItem* items, const uint32_t n_items) {
+    if (n_items == 0) return hipSuccess;
+    k_advect_velocity_batched<<<n_items, tile_block(), 0, s>>>(descs, items);
+    return hipGetLastError();
+}
+hipError_t launch_advect_scalar_batched(hipStream_t s, const AdvectScalarDesc* descs, const BatchItem* items, const uint32_t n_items) {
+    if (n_items == 0) return hipSuccess;
+    k_advect_scalar_batched<<<n_items, tile_block(), 0, s>>>(descs, items);
+    return hipGetLastError();
+}
+hipError_t launch_divergence_batched(hipStream_t s, const DivergenceDesc* descs, const BatchItem* items, const uint32_t n_items) {
+    if (n_items == 0) return hipSuccess;
+    k_divergence_batched<<<n_items, tile_block(), 0, s>>>(descs, items);
+    return hipGetLastError();
+}
+hipError_t launch_project_batched(hipStream_t s, const ProjectDesc* descs, const BatchItem* items, const uint32_t n_items) {
+    if (n_items == 0) return hipSuccess;
+    k_project_batched<<<n_items, tile_block(), 0, s>>>(descs, items);
     return hipGetLastError();
 }
 

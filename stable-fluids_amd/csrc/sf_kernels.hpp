@@ -49,6 +49,95 @@ struct CorrectionPass {
     const float* scale_slot;
 };
 
+// ---- batched launches: one launch runs the same stage for many simulations ----
+
+// Workgroup i of a batched launch runs local block `block` of descriptor `desc`.
+struct BatchItem {
+    uint32_t desc;
+    uint32_t block;
+};
+
+// Smoother tiling: tiles_x * tiles_y tiles of 128 x 16 cells, z split in chunks of zc planes.
+struct TileGeom {
+    int tiles_x, tiles_y, chunks, zc;
+};
+// zc_override > 0 replaces the default chunk length (results do not depend on it).
+TileGeom plan_tiles(Dims d, int zc_override);
+uint32_t smooth_blocks(const TileGeom& g);
+
+struct SmoothDesc {
+    const float* x_old; // nullptr only with a zero-first launch
+    const float* b;
+    float* x_new;
+    float coef;
+    float out_scale;
+    Dims d;
+    StencilBC bc;
+    TileGeom g;
+};
+// RBGS passes need even extents on periodic axes in a batch (no odd-periodic variant).
+bool smooth_batchable(SmootherKind smoother, Dims d, const StencilBC& bc);
+hipError_t launch_smooth_batched(hipStream_t s, SmootherKind smoother, SolveKind solve, bool zero, const SmoothDesc* descs, const BatchItem* items, uint32_t n_items);
+
+struct AdvectVelocityDesc {
+    float* t[3];
+    const float* v[3];
+    float dt;
+    Dims d;
+    float h;
+    StableFluidsFlowBoundaryConfig bc;
+};
+struct AdvectScalarDesc {
+    float* out;
+    const float* src;
+    const float* v[3];
+    float dt;
+    Dims d;
+    float h;
+    StableFluidsScalarBoundaryConfig sbc;
+    StableFluidsFlowBoundaryConfig fbc;
+};
+struct DivergenceDesc {
+    float* div;
+    const float* v[3];
+    Dims d;
+    float h;
+    StencilBC dbc;
+};
+struct ProjectDesc {
+    float* o[3];
+    float* div;
+    float* mag;
+    const float* v[3];
+    const float* p;
+    Dims d;
+    float h;
+    StencilBC pbc;
+    StencilBC dbc;
+};
+struct AddForceDesc {
+    float* v[3];
+    const float* f[3];
+    float dt;
+    uint64_t n;
+};
+struct AddScaledDesc {
+    float* dst;
+    const float* src;
+    float dt;
+    uint64_t n;
+};
+// blocks per simulation of the tile kernels (64 x 4 cells per plane) and of the
+// elementwise kernels (1024 elements)
+uint32_t tile_blocks(Dims d);
+uint32_t elem_block_count(uint64_t n);
+hipError_t launch_add_force_batched(hipStream_t s, const AddForceDesc* descs, const BatchItem* items, uint32_t n_items);
+hipError_t launch_add_scaled_batched(hipStream_t s, const AddScaledDesc* descs, const BatchItem* items, uint32_t n_items);
+hipError_t launch_advect_velocity_batched(hipStream_t s, const AdvectVelocityDesc* descs, const BatchItem* items, uint32_t n_items);
+hipError_t launch_advect_scalar_batched(hipStream_t s, const AdvectScalarDesc* descs, const BatchItem* items, uint32_t n_items);
+hipError_t launch_divergence_batched(hipStream_t s, const DivergenceDesc* descs, const BatchItem* items, uint32_t n_items);
+hipError_t launch_project_batched(hipStream_t s, const ProjectDesc* descs, const BatchItem* items, uint32_t n_items);
+
 hipError_t launch_fill(hipStream_t s, float* p, float value, uint64_t n);
 hipError_t launch_copy_scaled(hipStream_t s, float* dst, const float* src, float factor, uint64_t n);
 hipError_t launch_add_scaled(hipStream_t s, float* dst, const float* src, float dt, uint64_t n);

@@ -51,10 +51,6 @@ constexpr int kJOldPer = (kJOldF + kThreads - 1) / kThreads;
 constexpr int kOutPer  = kPX * kTY / kThreads;
 static_assert(kPX * kTY % kThreads == 0, "whole output rows per lane group");
 
-struct TileGeom {
-    int tiles_x, tiles_y, chunks, zc;
-};
-
 __device__ __forceinline__ int fix_coord(const int c, const int n, const bool periodic) {
     if (periodic) return wrap_index(c, n);
     return c < 0 ? 0 : (c >= n ? n - 1 : c);
@@ -102,16 +98,20 @@ __device__ __forceinline__ float relax_elem(const int j, const float2 L, const f
     sum            = sum + (j ? YP.y : YP.x);
     sum            = sum + (j ? ZM.y : ZM.x);
     sum            = sum + (j ? ZP.y : ZP.x);
-    return relax_value<kSolve>(bv, sum, coef);
+    return relax_fast<kSolve>(bv, sum, coef);
 }
 
-// Workgroup -> (tile x, tile y, z chunk). Workgroups are dispatched round-robin over the
-// 8 XCDs; renumbering gives each XCD (and its L2) a contiguous run of chunks and tiles,
-// so z-halo planes and x-y halo columns are mostly shared within one L2.
-__device__ __forceinline__ void tile_of(const TileGeom g, int& tx, int& ty, int& cz) {
+// Workgroups are dispatched round-robin over the 8 XCDs; renumbering gives each XCD (and
+// its L2) a contiguous run of chunks and tiles, so z-halo planes and x-y halo columns
+// are mostly shared within one L2.
+__device__ __forceinline__ int xcd_block() {
     const int n = static_cast<int>(gridDim.x);
-    int w       = static_cast<int>(blockIdx.x);
-    if ((n & 7) == 0) w = (w & 7) * (n >> 3) + (w >> 3);
+    const int w = static_cast<int>(blockIdx.x);
+    return (n & 7) == 0 ? (w & 7) * (n >> 3) + (w >> 3) : w;
+}
+
+// Block w of a pass -> (tile x, tile y, z chunk).
+__device__ __forceinline__ void tile_of(const TileGeom g, const int w, int& tx, int& ty, int& cz) {
     cz          = w % g.chunks;
     const int t = w / g.chunks;
     tx          = t % g.tiles_x;
@@ -185,14 +185,40 @@ __device__ __forceinline__ void fetch_region(float (&r)[kPer], const typename St
     }
 }
 
+// Pair version of fetch_region for even row lengths: slot i loads the aligned pair at
+// pair_off[i] (-1: nothing) and keeps element pair_sel[i] (0 both, 1 both .x, 2 both .y:
+// a left or right x-ghost pair reads the adjacent edge cell), then applies the ghost
+// rule pair_gc[i] like fetch_region does per element.
+template <int kPer, bool kZero, class St = StoreF32>
+__device__ __forceinline__ void fetch_pairs(float2 (&r)[kPer], const typename St::T* __restrict__ src, const int (&pair_off)[kPer], const int (&pair_sel)[kPer], const int (&pair_gc)[kPer], const int k, const Dims d, const bool pz, const uint64_t plane, const StencilBC& bc) {
+    const bool below = !pz && k < 0, above = !pz && k >= d.nz;
+    const uint64_t base = static_cast<uint64_t>(fix_coord(k, d.nz, pz)) * plane;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        float2 v = make_float2(0.0f, 0.0f);
+        if (!kZero && pair_off[i] >= 0) v = St::load2(src + base + static_cast<uint64_t>(pair_off[i]));
+        if (pair_sel[i] == 1) v.y = v.x;
+        if (pair_sel[i] == 2) v.x = v.y;
+        if (below || above) {
+            const FaceRule f = below ? bc.face[4] : bc.face[5];
+            v                = make_float2(apply_ghost(f, v.x), apply_ghost(f, v.y));
+        } else if (pair_gc[i]) {
+            const FaceRule f = face_rule(bc, pair_gc[i] - 1);
+            v                = make_float2(apply_ghost(f, v.x), apply_ghost(f, v.y));
+        }
+        r[i] = v;
+    }
+}
+
 template <int kSolve, bool kZero>
 __global__ __launch_bounds__(kThreads) void k_rbgs_pass(const float* __restrict__ x_old, const float* __restrict__ b, float* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g, const int* __restrict__ skip) {
     __shared__ float2 s_old[3][kOPX * kOY];
     __shared__ float2 s_h[3][kHPX * kHY];
     if (skip != nullptr && *skip != 0) return;
+    const int w = xcd_block();
 
     int tx, ty, cz;
-    tile_of(g, tx, ty, cz);
+    tile_of(g, w, tx, ty, cz);
     const int tid  = static_cast<int>(threadIdx.x);
     const int x0   = tx * kTX, y0 = ty * kTY;
     const int z0   = cz * g.zc;
@@ -370,14 +396,13 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_pass(const float* __restrict_
     }
 }
 
-template <int kSolve, bool kZero, class St>
-__global__ __launch_bounds__(kThreads) void k_jacobi_pass(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g, const float* __restrict__ scale_slot, const int* __restrict__ skip) {
+template <int kSolve, bool kZero, class St, bool kPairs>
+__device__ __forceinline__ void jacobi_body(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC& bc, const TileGeom& g, const float* __restrict__ scale_slot, const int w) {
     __shared__ float2 s_old[3][kHPX * kHY];
-    if (skip != nullptr && *skip != 0) return;
     const float inv_s = rhs_scale<St>(scale_slot);
 
     int tx, ty, cz;
-    tile_of(g, tx, ty, cz);
+    tile_of(g, w, tx, ty, cz);
     const int tid  = static_cast<int>(threadIdx.x);
     const int x0   = tx * kTX, y0 = ty * kTY;
     const int z0   = cz * g.zc;
@@ -385,16 +410,44 @@ __global__ __launch_bounds__(kThreads) void k_jacobi_pass(const typename St::T* 
     const bool px  = bc.periodic & 1, py = (bc.periodic >> 1) & 1, pz = (bc.periodic >> 2) & 1;
     const uint64_t plane = static_cast<uint64_t>(d.nx) * static_cast<uint64_t>(d.ny);
 
+    // even rows: the old region is loaded as aligned pairs (kHPer per lane); odd rows
+    // element by element (kJOldPer per lane)
+    constexpr bool pairs = kPairs; // caller checked (d.nx & 1) == 0
     int o_off[kJOldPer], o_gc[kJOldPer];
+    int p_off[kHPer], p_sel[kHPer], p_gc[kHPer];
+    if constexpr (pairs) {
 #pragma unroll
-    for (int i = 0; i < kJOldPer; ++i) {
-        const int e = tid + i * kThreads;
-        o_off[i]    = -1;
-        o_gc[i]     = 0;
-        if (e < kJOldF) {
-            const int gx = x0 - 2 + e % (2 * kHPX), gy = y0 - 1 + e / (2 * kHPX);
-            o_off[i]     = fix_coord(gy, d.ny, py) * d.nx + fix_coord(gx, d.nx, px);
-            o_gc[i]      = ghost_code(gx, gy, d, px, py);
+        for (int i = 0; i < kHPer; ++i) {
+            const int e = tid + i * kThreads;
+            p_off[i]    = -1;
+            p_sel[i]    = 0;
+            p_gc[i]     = 0;
+            if (e < kHP) {
+                const int gx = x0 - 2 + 2 * (e % kHPX), gy = y0 - 1 + e / kHPX;
+                const int row = fix_coord(gy, d.ny, py) * d.nx;
+                if (!px && gx < 0) {
+                    p_off[i] = row;
+                    p_sel[i] = 1;
+                } else if (!px && gx >= d.nx) {
+                    p_off[i] = row + d.nx - 2;
+                    p_sel[i] = 2;
+                } else {
+                    p_off[i] = row + fix_coord(gx, d.nx, px);
+                }
+                p_gc[i] = ghost_code(gx, gy, d, px, py);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kJOldPer; ++i) {
+            const int e = tid + i * kThreads;
+            o_off[i]    = -1;
+            o_gc[i]     = 0;
+            if (e < kJOldF) {
+                const int gx = x0 - 2 + e % (2 * kHPX), gy = y0 - 1 + e / (2 * kHPX);
+                o_off[i]     = fix_coord(gy, d.ny, py) * d.nx + fix_coord(gx, d.nx, px);
+                o_gc[i]      = ghost_code(gx, gy, d, px, py);
+            }
         }
     }
     const int fp = tid & (kPX - 1);
@@ -406,6 +459,7 @@ __global__ __launch_bounds__(kThreads) void k_jacobi_pass(const typename St::T* 
         f_info[i]    = (xa < d.nx && y < d.ny ? 1 : 0) | (xa + 1 < d.nx && y < d.ny ? 2 : 0);
     }
     float r_old[kJOldPer];
+    float2 r_pair[kHPer];
     float2 r_b[kOutPer];
     auto fetch_b = [&](const int k) {
         if (k < 0 || k >= d.nz) return;
@@ -414,22 +468,40 @@ __global__ __launch_bounds__(kThreads) void k_jacobi_pass(const typename St::T* 
         for (int i = 0; i < kOutPer; ++i) {
             const int fy  = (tid >> 6) + (kThreads / 64) * i;
             const float* p = b + base + static_cast<uint64_t>((y0 + fy) * d.nx + x0 + 2 * fp);
-            if (f_info[i] & 1) r_b[i].x = p[0] * inv_s;
-            if (f_info[i] & 2) r_b[i].y = p[1] * inv_s;
+            if ((d.nx & 1) == 0 && (f_info[i] & 2)) {
+                const float2 q = *reinterpret_cast<const float2*>(p);
+                r_b[i]         = make_float2(q.x * inv_s, q.y * inv_s);
+            } else {
+                if (f_info[i] & 1) r_b[i].x = p[0] * inv_s;
+                if (f_info[i] & 2) r_b[i].y = p[1] * inv_s;
+            }
         }
     };
     auto stash_old = [&](const int k) {
-        float* s = reinterpret_cast<float*>(s_old[mod3(k)]);
+        if constexpr (pairs) {
+            float2* s = s_old[mod3(k)];
 #pragma unroll
-        for (int i = 0; i < kJOldPer; ++i)
-            if (tid + i * kThreads < kJOldF) s[tid + i * kThreads] = r_old[i];
+            for (int i = 0; i < kHPer; ++i)
+                if (tid + i * kThreads < kHP) s[tid + i * kThreads] = r_pair[i];
+        } else {
+            float* s = reinterpret_cast<float*>(s_old[mod3(k)]);
+#pragma unroll
+            for (int i = 0; i < kJOldPer; ++i)
+                if (tid + i * kThreads < kJOldF) s[tid + i * kThreads] = r_old[i];
+        }
+    };
+    auto fetch_old = [&](const int k) {
+        if constexpr (pairs)
+            fetch_pairs<kHPer, kZero, St>(r_pair, x_old, p_off, p_sel, p_gc, k, d, pz, plane, bc);
+        else
+            fetch_region<kJOldPer, kZero, St>(r_old, x_old, o_off, o_gc, k, d, pz, plane, bc);
     };
 
-    fetch_region<kJOldPer, kZero, St>(r_old, x_old, o_off, o_gc, z0 - 1, d, pz, plane, bc);
+    fetch_old(z0 - 1);
     stash_old(z0 - 1);
-    fetch_region<kJOldPer, kZero, St>(r_old, x_old, o_off, o_gc, z0, d, pz, plane, bc);
+    fetch_old(z0);
     stash_old(z0);
-    fetch_region<kJOldPer, kZero, St>(r_old, x_old, o_off, o_gc, z0 + 1, d, pz, plane, bc);
+    fetch_old(z0 + 1);
     fetch_b(z0);
     for (int k = z0; k < z1; ++k) {
         stash_old(k + 1);
@@ -437,7 +509,7 @@ __global__ __launch_bounds__(kThreads) void k_jacobi_pass(const typename St::T* 
 #pragma unroll
         for (int i = 0; i < kOutPer; ++i) bv[i] = r_b[i];
         if (k + 1 < z1) {
-            fetch_region<kJOldPer, kZero, St>(r_old, x_old, o_off, o_gc, k + 2, d, pz, plane, bc);
+            fetch_old(k + 2);
             fetch_b(k + 1);
         }
         __syncthreads();
@@ -488,14 +560,13 @@ __device__ __forceinline__ int wrap_near(const int k, const int n) {
 }
 
 template <int kSolve, bool kZero, class St>
-__global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g, const float* __restrict__ scale_slot, const int* __restrict__ skip) {
+__device__ __forceinline__ void rbgs_split_body(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC& bc, const TileGeom& g, const float* __restrict__ scale_slot, const int w) {
     __shared__ float s_old[3][kOldPlane];
     __shared__ float s_h[3][kHPlane];
-    if (skip != nullptr && *skip != 0) return;
     const float inv_s = rhs_scale<St>(scale_slot);
 
     int tx, ty, cz;
-    tile_of(g, tx, ty, cz);
+    tile_of(g, w, tx, ty, cz);
     const int tid  = static_cast<int>(threadIdx.x);
     const int x0   = tx * kTX, y0 = ty * kTY;
     const int xs   = x0 - 4, ys = y0 - 2;
@@ -723,11 +794,43 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const typename St:
     if (k <= z1) body(k, Even{});
 }
 
+template <int kSolve, bool kZero, class St, bool kPairs>
+__global__ __launch_bounds__(kThreads) void k_jacobi_pass(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g, const float* __restrict__ scale_slot, const int* __restrict__ skip) {
+    if (skip != nullptr && *skip != 0) return;
+    jacobi_body<kSolve, kZero, St, kPairs>(x_old, b, x_new, coef, out_scale, d, bc, g, scale_slot, xcd_block());
+}
+
+template <int kSolve, bool kZero, class St>
+__global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g, const float* __restrict__ scale_slot, const int* __restrict__ skip) {
+    if (skip != nullptr && *skip != 0) return;
+    rbgs_split_body<kSolve, kZero, St>(x_old, b, x_new, coef, out_scale, d, bc, g, scale_slot, xcd_block());
+}
+
+// Batched passes: workgroup i runs block items[i].block of pass descs[items[i].desc].
+template <int kSolve, bool kZero>
+__global__ __launch_bounds__(kThreads) void k_rbgs_split_batched(const SmoothDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
+    const BatchItem it = items[blockIdx.x];
+    const SmoothDesc sd = descs[it.desc];
+    rbgs_split_body<kSolve, kZero, StoreF32>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
+}
+
+template <int kSolve, bool kZero>
+__global__ __launch_bounds__(kThreads) void k_jacobi_batched(const SmoothDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
+    const BatchItem it = items[blockIdx.x];
+    const SmoothDesc sd = descs[it.desc];
+    if ((sd.d.nx & 1) == 0)
+        jacobi_body<kSolve, kZero, StoreF32, true>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
+    else
+        jacobi_body<kSolve, kZero, StoreF32, false>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
+}
+
 bool has_odd_periodic(const Dims d, const StencilBC& bc) {
     return ((bc.periodic & 1) && (d.nx & 1)) || ((bc.periodic & 2) && (d.ny & 1)) || ((bc.periodic & 4) && (d.nz & 1));
 }
 
-TileGeom plan_tiles(const Dims d) {
+} // namespace
+
+TileGeom plan_tiles(const Dims d, const int zc_override) {
     TileGeom g{};
     g.tiles_x        = (d.nx + kTX - 1) / kTX;
     g.tiles_y        = (d.ny + kTY - 1) / kTY;
@@ -736,19 +839,52 @@ TileGeom plan_tiles(const Dims d) {
     long zc = (static_cast<long>(d.nz) * tiles + 511) / 512;
     if (zc < 4) zc = 4;
     if (zc > 64) zc = 64;
+    if (zc_override > 0) zc = zc_override;
     if (zc > d.nz) zc = d.nz;
     g.zc     = static_cast<int>(zc);
     g.chunks = (d.nz + g.zc - 1) / g.zc;
     return g;
 }
 
-} // namespace
+uint32_t smooth_blocks(const TileGeom& g) { return static_cast<uint32_t>(g.tiles_x * g.tiles_y * g.chunks); }
+
+bool smooth_batchable(const SmootherKind smoother, const Dims d, const StencilBC& bc) { return smoother == kSmoothJacobi || !has_odd_periodic(d, bc); }
+
+hipError_t launch_smooth_batched(hipStream_t s, const SmootherKind smoother, const SolveKind solve, const bool zero, const SmoothDesc* descs, const BatchItem* items, const uint32_t n_items) {
+    if (n_items == 0) return hipSuccess;
+#define SF_B(KERNEL, SO, Z) KERNEL<SO, Z><<<n_items, kThreads, 0, s>>>(descs, items)
+    if (smoother == kSmoothRBGS) {
+        if (solve == kSolvePoisson) {
+            if (zero) SF_B(k_rbgs_split_batched, kSolvePoisson, true);
+            else SF_B(k_rbgs_split_batched, kSolvePoisson, false);
+        } else {
+            if (zero) SF_B(k_rbgs_split_batched, kSolveDiffusion, true);
+            else SF_B(k_rbgs_split_batched, kSolveDiffusion, false);
+        }
+    } else {
+        if (solve == kSolvePoisson) {
+            if (zero) SF_B(k_jacobi_batched, kSolvePoisson, true);
+            else SF_B(k_jacobi_batched, kSolvePoisson, false);
+        } else {
+            if (zero) SF_B(k_jacobi_batched, kSolveDiffusion, true);
+            else SF_B(k_jacobi_batched, kSolveDiffusion, false);
+        }
+    }
+#undef SF_B
+    return hipGetLastError();
+}
+
 
 hipError_t launch_smooth(hipStream_t s, const SmootherKind smoother, const SolveKind solve, const SmoothPass& pass, const Dims d, const StencilBC& bc) {
-    const TileGeom g      = plan_tiles(d);
+    const TileGeom g      = plan_tiles(d, 0);
     const unsigned blocks = static_cast<unsigned>(g.tiles_x * g.tiles_y * g.chunks);
     const bool zero       = pass.x_old == nullptr;
 #define SF_PASS_GENERAL(SO, Z) k_rbgs_pass<SO, Z><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g, pass.skip)
+#define SF_JPASS(SO, Z)                                                                                                                                                            \
+    do {                                                                                                                                                                            \
+        if ((d.nx & 1) == 0) k_jacobi_pass<SO, Z, StoreF32, true><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g, nullptr, pass.skip); \
+        else k_jacobi_pass<SO, Z, StoreF32, false><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g, nullptr, pass.skip);         \
+    } while (0)
 #define SF_PASS(KERNEL, SO, Z) KERNEL<SO, Z, StoreF32><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g, nullptr, pass.skip)
     if (smoother == kSmoothRBGS && !has_odd_periodic(d, bc)) {
         if (solve == kSolvePoisson) {
@@ -768,14 +904,15 @@ hipError_t launch_smooth(hipStream_t s, const SmootherKind smoother, const Solve
         }
     } else {
         if (solve == kSolvePoisson) {
-            if (zero) SF_PASS(k_jacobi_pass, kSolvePoisson, true);
-            else SF_PASS(k_jacobi_pass, kSolvePoisson, false);
+            if (zero) SF_JPASS(kSolvePoisson, true);
+            else SF_JPASS(kSolvePoisson, false);
         } else {
-            if (zero) SF_PASS(k_jacobi_pass, kSolveDiffusion, true);
-            else SF_PASS(k_jacobi_pass, kSolveDiffusion, false);
+            if (zero) SF_JPASS(kSolveDiffusion, true);
+            else SF_JPASS(kSolveDiffusion, false);
         }
     }
 #undef SF_PASS
+#undef SF_JPASS
 #undef SF_PASS_GENERAL
     return hipGetLastError();
 }
@@ -784,11 +921,16 @@ namespace {
 
 template <class St>
 hipError_t correction_pass(hipStream_t s, const SmootherKind smoother, const SolveKind solve, const CorrectionPass& pass, const Dims d, const StencilBC& bc) {
-    const TileGeom g      = plan_tiles(d);
+    const TileGeom g      = plan_tiles(d, 0);
     const unsigned blocks = static_cast<unsigned>(g.tiles_x * g.tiles_y * g.chunks);
     const auto* e_old     = static_cast<const typename St::T*>(pass.e_old);
     auto* e_new           = static_cast<typename St::T*>(pass.e_new);
 #define SF_CORR(KERNEL, SO, Z) KERNEL<SO, Z, St><<<blocks, kThreads, 0, s>>>(e_old, pass.r, e_new, pass.coef, 1.0f, d, bc, g, pass.scale_slot, nullptr)
+#define SF_JCORR(SO, Z)                                                                                                                                      \
+    do {                                                                                                                                                     \
+        if ((d.nx & 1) == 0) k_jacobi_pass<SO, Z, St, true><<<blocks, kThreads, 0, s>>>(e_old, pass.r, e_new, pass.coef, 1.0f, d, bc, g, pass.scale_slot, nullptr); \
+        else k_jacobi_pass<SO, Z, St, false><<<blocks, kThreads, 0, s>>>(e_old, pass.r, e_new, pass.coef, 1.0f, d, bc, g, pass.scale_slot, nullptr);         \
+    } while (0)
     const bool zero = e_old == nullptr;
     if (smoother == kSmoothRBGS) {
         if (solve == kSolvePoisson) {
@@ -800,14 +942,15 @@ hipError_t correction_pass(hipStream_t s, const SmootherKind smoother, const Sol
         }
     } else {
         if (solve == kSolvePoisson) {
-            if (zero) SF_CORR(k_jacobi_pass, kSolvePoisson, true);
-            else SF_CORR(k_jacobi_pass, kSolvePoisson, false);
+            if (zero) SF_JCORR(kSolvePoisson, true);
+            else SF_JCORR(kSolvePoisson, false);
         } else {
-            if (zero) SF_CORR(k_jacobi_pass, kSolveDiffusion, true);
-            else SF_CORR(k_jacobi_pass, kSolveDiffusion, false);
+            if (zero) SF_JCORR(kSolveDiffusion, true);
+            else SF_JCORR(kSolveDiffusion, false);
         }
     }
 #undef SF_CORR
+#undef SF_JCORR
     return hipGetLastError();
 }
 

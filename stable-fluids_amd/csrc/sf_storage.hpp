@@ -12,6 +12,8 @@ struct StoreF32 {
     using T = float;
     static constexpr bool kNarrow = false;
     __device__ __forceinline__ static float load(const T v) { return v; }
+    // aligned pair p[0], p[1]
+    __device__ __forceinline__ static float2 load2(const T* p) { return *reinterpret_cast<const float2*>(p); }
     __device__ __forceinline__ static T store(const float v) { return v; }
 };
 
@@ -19,6 +21,10 @@ struct StoreF16 {
     using T = _Float16;
     static constexpr bool kNarrow = true;
     __device__ __forceinline__ static float load(const T v) { return static_cast<float>(v); }
+    __device__ __forceinline__ static float2 load2(const T* p) {
+        const uint32_t w = *reinterpret_cast<const uint32_t*>(p);
+        return make_float2(static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w & 0xffffu))), static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w >> 16))));
+    }
     __device__ __forceinline__ static T store(const float v) { return static_cast<_Float16>(v); }
 };
 
@@ -26,6 +32,10 @@ struct StoreBF16 {
     using T = uint16_t; // upper half of the FP32 bit pattern
     static constexpr bool kNarrow = true;
     __device__ __forceinline__ static float load(const T v) { return __uint_as_float(static_cast<uint32_t>(v) << 16); }
+    __device__ __forceinline__ static float2 load2(const T* p) {
+        const uint32_t w = *reinterpret_cast<const uint32_t*>(p);
+        return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
+    }
     __device__ __forceinline__ static T store(const float v) {
         const uint32_t u = __float_as_uint(v);
         if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<T>((u >> 16) | 0x40u); // quiet NaN

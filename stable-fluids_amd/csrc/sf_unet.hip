@@ -48,6 +48,8 @@ struct Net {
     bf16_t *a0, *a1, *a2, *p1, *a3, *a4, *p2, *a5, *a6, *u1, *r7, *s7, *a8, *u2, *r9, *s9, *a10;
     float* y = nullptr;    // head output [v][16] FP32
     float* q = nullptr;    // channel 0 of y
+    double* partials = nullptr; // 1024 block sums
+    float* mean = nullptr;      // mean of q (centred loss)
     bf16_t *gy, *g10, *ds9, *g9, *du2, *da8, *g8, *ds7, *g7, *du1, *da6, *g6, *g5, *dp2, *t4, *g4, *g3, *dp1, *t2, *g2, *g1;
     std::vector<void*> allocations;
 
@@ -121,7 +123,7 @@ hipError_t net_create(const int nx, const int ny, const int nz, hipStream_t stre
     for (int l = 0; ok && l < kLayers; ++l) ok = m.alloc(m.wb[l], 27ull * kCin[l] * kCout[l] * 2) && m.alloc(m.wt[l], 27ull * kCin[l] * kCout[l] * 2);
     ok = ok && m.act(m.a0, 0, 16) && m.act(m.a1, 0, 16) && m.act(m.a2, 0, 16) && m.act(m.p1, 1, 16) && m.act(m.a3, 1, 32) && m.act(m.a4, 1, 32) && m.act(m.p2, 2, 32) && m.act(m.a5, 2, 64) && m.act(m.a6, 2, 64) &&
          m.act(m.u1, 1, 64) && m.act(m.r7, 1, 32) && m.act(m.s7, 1, 32) && m.act(m.a8, 1, 32) && m.act(m.u2, 0, 32) && m.act(m.r9, 0, 16) && m.act(m.s9, 0, 16) && m.act(m.a10, 0, 16) &&
-         m.alloc(m.y, m.vox[0] * 16 * 4) && m.alloc(m.q, m.vox[0] * 4);
+         m.alloc(m.y, m.vox[0] * 16 * 4) && m.alloc(m.q, m.vox[0] * 4) && m.alloc(m.partials, 1024 * sizeof(double)) && m.alloc(m.mean, 16);
     ok = ok && m.act(m.gy, 0, 16) && m.act(m.g10, 0, 16) && m.act(m.ds9, 0, 16) && m.act(m.g9, 0, 16) && m.act(m.du2, 0, 32) && m.act(m.da8, 1, 32) && m.act(m.g8, 1, 32) && m.act(m.ds7, 1, 32) &&
          m.act(m.g7, 1, 32) && m.act(m.du1, 1, 64) && m.act(m.da6, 2, 64) && m.act(m.g6, 2, 64) && m.act(m.g5, 2, 64) && m.act(m.dp2, 2, 32) && m.act(m.t4, 1, 32) && m.act(m.g4, 1, 32) &&
          m.act(m.g3, 1, 32) && m.act(m.dp1, 1, 16) && m.act(m.t2, 0, 16) && m.act(m.g2, 0, 16) && m.act(m.g1, 0, 16);
@@ -174,9 +176,10 @@ hipError_t net_forward_packed(Net& n, const bool train) {
 bf16_t* net_input(Net& n) { return n.a0; }
 const float* net_output(const Net& n) { return n.q; }
 
-hipError_t net_backward(Net& n, const float* target, float* loss) {
+hipError_t net_backward(Net& n, const float* target, const float weight, const bool centered, float* loss) {
     SF_TRY(launch_fill_f32(n.stream, n.grads, 0.0f, n.padded));
-    SF_TRY(launch_mse_grad(n.stream, n.y, target, n.gy, loss, n.vox[0]));
+    if (centered) SF_TRY(launch_head_mean(n.stream, n.y, n.partials, n.mean, n.vox[0]));
+    SF_TRY(launch_mse_grad(n.stream, n.y, target, centered ? n.mean : nullptr, weight, n.gy, loss, n.vox[0]));
     SF_TRY(wgrad(n, 10, n.a10, n.gy, 0));
     SF_TRY(dgrad(n, 10, n.gy, n.g10, n.a10, 0));
     SF_TRY(wgrad(n, 9, n.s9, n.g10, 0));
@@ -313,12 +316,12 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_unet_forward_cuda(void* net, 
     return res(hipMemcpyAsync(output, n.q, n.vox[0] * 4, hipMemcpyDeviceToDevice, n.stream));
 }
 
-STABLE_FLUIDS_API StableFluidsResult stable_fluids_unet_backward_cuda(void* net, const float* input, const float* target, float* loss) {
-    if (net == nullptr || input == nullptr || target == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_unet_backward_cuda(void* net, const float* input, const float* target, float weight, uint32_t flags, float* loss) {
+    if (net == nullptr || input == nullptr || target == nullptr || !(weight >= 0.0f) || (flags & ~STABLE_FLUIDS_UNET_LOSS_CENTERED)) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     auto& n = *static_cast<Net*>(net);
     if (sf::unet::launch_pack_input(n.stream, input, nullptr, 1.0f, n.a0, n.vox[0]) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     if (sf::unet::net_forward_packed(n, true) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
-    return res(sf::unet::net_backward(n, target, loss));
+    return res(sf::unet::net_backward(n, target, weight, (flags & STABLE_FLUIDS_UNET_LOSS_CENTERED) != 0, loss));
 }
 
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_unet_adam_step_cuda(void* net, float lr, float beta1, float beta2, float eps) {

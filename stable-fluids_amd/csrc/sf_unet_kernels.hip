@@ -327,12 +327,13 @@ __global__ __launch_bounds__(kElem) void k_pack_input(const float* __restrict__ 
     }
 }
 
-__global__ __launch_bounds__(kElem) void k_mse_grad(const float* __restrict__ y, const float* __restrict__ t, bf16_t* __restrict__ g, float* __restrict__ loss, const uint64_t n) {
+__global__ __launch_bounds__(kElem) void k_mse_grad(const float* __restrict__ y, const float* __restrict__ t, const float* __restrict__ mean, const float w, bf16_t* __restrict__ g, float* __restrict__ loss, const uint64_t n) {
     __shared__ float s[kElem / 64];
-    const float inv_n = 1.0f / static_cast<float>(n);
+    const float inv_n = w / static_cast<float>(n);
+    const float shift = mean ? *mean : 0.0f;
     float acc = 0.0f;
     for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kElem) + threadIdx.x; v < n; v += static_cast<uint64_t>(gridDim.x) * kElem) {
-        const float dv = y[v * 16] - t[v];
+        const float dv = (y[v * 16] - shift) - t[v];
         acc += dv * dv;
         uint4* dst = reinterpret_cast<uint4*>(g + v * 16);
         dst[0]     = make_uint4(static_cast<uint32_t>(f2bf(2.0f * dv * inv_n)), 0u, 0u, 0u);
@@ -344,6 +345,31 @@ __global__ __launch_bounds__(kElem) void k_mse_grad(const float* __restrict__ y,
     if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) atomicAdd(loss, (s[0] + s[1] + s[2] + s[3]) * inv_n);
+}
+
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double s[kElem / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const double t = s[0] + s[1] + s[2] + s[3];
+    __syncthreads();
+    return t;
+}
+
+__global__ __launch_bounds__(kElem) void k_head_partials(const float* __restrict__ y, double* __restrict__ partials, const uint64_t n) {
+    double acc = 0.0;
+    for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kElem) + threadIdx.x; v < n; v += static_cast<uint64_t>(gridDim.x) * kElem) acc += y[v * 16];
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kElem) void k_head_mean(const double* __restrict__ partials, const int nb, const uint64_t n, float* __restrict__ mean) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kElem) acc += partials[b];
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) *mean = static_cast<float>(t / static_cast<double>(n));
 }
 
 __global__ __launch_bounds__(kElem) void k_head_channel0(const float* __restrict__ y, float* __restrict__ out, const uint64_t n) {
@@ -462,8 +488,15 @@ hipError_t launch_pack_input(hipStream_t s, const float* in, const float* scale_
     k_pack_input<<<elem_grid(n), kElem, 0, s>>>(in, scale_ptr, in_scale, a0, n);
     return hipGetLastError();
 }
-hipError_t launch_mse_grad(hipStream_t s, const float* y, const float* t, bf16_t* g, float* loss, const uint64_t n) {
-    k_mse_grad<<<elem_grid(n), kElem, 0, s>>>(y, t, g, loss, n);
+hipError_t launch_mse_grad(hipStream_t s, const float* y, const float* t, const float* mean, const float w, bf16_t* g, float* loss, const uint64_t n) {
+    k_mse_grad<<<elem_grid(n), kElem, 0, s>>>(y, t, mean, w, g, loss, n);
+    return hipGetLastError();
+}
+hipError_t launch_head_mean(hipStream_t s, const float* y, double* partials, float* mean, const uint64_t n) {
+    const uint64_t b  = (n + kElem - 1) / kElem;
+    const unsigned nb = static_cast<unsigned>(b < 1024 ? (b ? b : 1) : 1024);
+    k_head_partials<<<nb, kElem, 0, s>>>(y, partials, n);
+    k_head_mean<<<1, kElem, 0, s>>>(partials, static_cast<int>(nb), n, mean);
     return hipGetLastError();
 }
 hipError_t launch_head_channel0(hipStream_t s, const float* y, float* out, const uint64_t n) {

@@ -48,9 +48,12 @@ hipError_t launch_fill_f32(hipStream_t s, float* p, float value, uint64_t n);
 // in[v][1], channels 2..15 zero. in is [v][2] FP32.
 hipError_t launch_pack_input(hipStream_t s, const float* in, const float* scale_ptr, float in_scale, bf16_t* a0, uint64_t n);
 // Loss of channel 0 of the FP32 head output y[v][16] against target t[v]:
-// loss += sum (y0 - t)^2 / n (into *loss, FP32 atomic, may be null); g[v][0] =
-// 2 (y0 - t) / n as BF16, channels 1..15 zero.
-hipError_t launch_mse_grad(hipStream_t s, const float* y, const float* t, bf16_t* g, float* loss, uint64_t n);
+// loss += w sum (y0 - t)^2 / n (into *loss, FP32 atomic, may be null); g[v][0] =
+// 2 w (y0 - t) / n as BF16, channels 1..15 zero.
+// With `mean` (device scalar, from launch_head_mean) y0 is replaced by y0 - *mean.
+hipError_t launch_mse_grad(hipStream_t s, const float* y, const float* t, const float* mean, float w, bf16_t* g, float* loss, uint64_t n);
+// *mean = mean over v of y[v][0] (FP64 sums; partials: 1024 doubles).
+hipError_t launch_head_mean(hipStream_t s, const float* y, double* partials, float* mean, uint64_t n);
 // out[v] = y[v][0] (channel 0 of the FP32 head output).
 hipError_t launch_head_channel0(hipStream_t s, const float* y, float* out, uint64_t n);
 

@@ -87,15 +87,23 @@ __host__ __device__ inline uint64_t cells_of(const Dims d) { return static_cast<
 
 #define GRID_STRIDE(i, n) for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x; i < (n); i += static_cast<uint64_t>(gridDim.x) * kThreads)
 
-__global__ __launch_bounds__(kThreads) void k_restrict_input(const float* __restrict__ div, float* __restrict__ in2, double* __restrict__ partials, const Dims cd, const StencilBC bc) {
-    const Dims fd{2 * cd.nx, 2 * cd.ny, 2 * cd.nz};
+// mean of the f^3 fine cells of coarse cell (x, y, z)
+template <class T>
+__device__ __forceinline__ double block_mean(const T* __restrict__ v, const int x, const int y, const int z, const int f, const Dims fd) {
+    double s = 0.0;
+    for (int k = 0; k < f; ++k)
+        for (int j = 0; j < f; ++j)
+            for (int i = 0; i < f; ++i) s += v[lin(f * x + i, f * y + j, f * z + k, fd)];
+    return s / static_cast<double>(f * f * f);
+}
+
+__global__ __launch_bounds__(kThreads) void k_restrict_input(const float* __restrict__ div, float* __restrict__ in2, double* __restrict__ partials, const Dims cd, const int f, const StencilBC bc) {
+    const Dims fd{f * cd.nx, f * cd.ny, f * cd.nz};
     double acc = 0.0;
     GRID_STRIDE(v, cells_of(cd)) {
         int x, y, z;
         decode(v, cd, x, y, z);
-        float s = 0.0f;
-        for (int k = 0; k < 8; ++k) s += div[lin(2 * x + (k & 1), 2 * y + ((k >> 1) & 1), 2 * z + (k >> 2), fd)];
-        s *= 0.125f;
+        const float s  = static_cast<float>(block_mean(div, x, y, z, f, fd));
         in2[2 * v]     = s;
         in2[2 * v + 1] = wall_mask(x, y, z, cd, bc);
         acc += static_cast<double>(s) * s;
@@ -115,9 +123,10 @@ __global__ __launch_bounds__(kThreads) void k_sigma(const double* __restrict__ p
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_prolong(const float* __restrict__ q, const float* __restrict__ scal, const float scale, float* __restrict__ p0, const Dims cd) {
-    const Dims fd{2 * cd.nx, 2 * cd.ny, 2 * cd.nz};
-    const float amp = scale * scal[0];
+__global__ __launch_bounds__(kThreads) void k_prolong(const float* __restrict__ q, const float* __restrict__ scal, const float scale, float* __restrict__ p0, const Dims cd, const int f) {
+    const Dims fd{f * cd.nx, f * cd.ny, f * cd.nz};
+    const float amp   = scale * scal[0];
+    const float inv_f = 1.0f / static_cast<float>(f);
     GRID_STRIDE(i, cells_of(fd)) {
         int x, y, z;
         decode(i, fd, x, y, z);
@@ -126,7 +135,7 @@ __global__ __launch_bounds__(kThreads) void k_prolong(const float* __restrict__ 
         float t[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const float cc = fminf(fmaxf(0.5f * c[a] - 0.25f, 0.0f), static_cast<float>(n[a] - 1));
+            const float cc = fminf(fmaxf((static_cast<float>(c[a]) + 0.5f) * inv_f - 0.5f, 0.0f), static_cast<float>(n[a] - 1));
             i0[a]          = static_cast<int>(cc);
             i1[a]          = min(i0[a] + 1, n[a] - 1);
             t[a]           = cc - i0[a];
@@ -274,22 +283,30 @@ __global__ __launch_bounds__(kThreads) void k_moments(const double* __restrict__
     block_sum_to(s2, partials + 2 * blockIdx.x + 1);
 }
 
-// partial sums of div^2 (for sigma)
-__global__ __launch_bounds__(kThreads) void k_sum_sq(const float* __restrict__ div, double* __restrict__ partials, const uint64_t n) {
+// partial sums of the squared coarse divergence (mean of f^3 cells)
+__global__ __launch_bounds__(kThreads) void k_coarse_sum_sq(const float* __restrict__ div, double* __restrict__ partials, const Dims cd, const int f) {
+    const Dims fd{f * cd.nx, f * cd.ny, f * cd.nz};
     double acc = 0.0;
-    GRID_STRIDE(i, n) acc += static_cast<double>(div[i]) * div[i];
+    GRID_STRIDE(v, cells_of(cd)) {
+        int x, y, z;
+        decode(v, cd, x, y, z);
+        const double m = block_mean(div, x, y, z, f, fd);
+        acc += m * m;
+    }
     block_sum_to(acc, partials + blockIdx.x);
 }
 
-__global__ __launch_bounds__(kThreads) void k_pair(const float* __restrict__ div, const double* __restrict__ x, const float h2, float* __restrict__ in2, float* __restrict__ target, const double* __restrict__ scal, const Dims d, const StencilBC bc) {
-    const double sigma = sqrt(scal[5] / static_cast<double>(cells_of(d)));
+// coarse pair: in2 = (div_c / sigma_c, wall mask), target = x_c / (h_c^2 sigma_c)
+__global__ __launch_bounds__(kThreads) void k_pair(const float* __restrict__ div, const double* __restrict__ x, const double hc2, float* __restrict__ in2, float* __restrict__ target, const double* __restrict__ scal, const Dims cd, const int f, const StencilBC bc) {
+    const Dims fd{f * cd.nx, f * cd.ny, f * cd.nz};
+    const double sigma = sqrt(scal[5] / static_cast<double>(cells_of(cd)));
     const double inv   = sigma > 0.0 ? 1.0 / sigma : 0.0;
-    GRID_STRIDE(i, cells_of(d)) {
+    GRID_STRIDE(v, cells_of(cd)) {
         int xx, y, z;
-        decode(i, d, xx, y, z);
-        in2[2 * i]     = static_cast<float>(div[i] * inv);
-        in2[2 * i + 1] = wall_mask(xx, y, z, d, bc);
-        target[i]      = static_cast<float>(x[i] * inv / h2);
+        decode(v, cd, xx, y, z);
+        in2[2 * v]     = static_cast<float>(block_mean(div, xx, y, z, f, fd) * inv);
+        in2[2 * v + 1] = wall_mask(xx, y, z, cd, bc);
+        target[v]      = static_cast<float>(block_mean(x, xx, y, z, f, fd) * inv / hc2);
     }
 }
 
@@ -308,16 +325,16 @@ bool singular(const Dims, const StencilBC& bc) {
 
 } // namespace
 
-hipError_t launch_restrict_input(hipStream_t s, const float* div, float* in2, double* partials, float* scal, const Dims cd, const StencilBC& p_bc) {
+hipError_t launch_restrict_input(hipStream_t s, const float* div, float* in2, double* partials, float* scal, const Dims cd, const int f, const StencilBC& p_bc) {
     const unsigned b = blocks_for(cells_of(cd));
-    k_restrict_input<<<b, kThreads, 0, s>>>(div, in2, partials, cd, p_bc);
+    k_restrict_input<<<b, kThreads, 0, s>>>(div, in2, partials, cd, f, p_bc);
     k_sigma<<<1, kThreads, 0, s>>>(partials, static_cast<int>(b), static_cast<double>(cells_of(cd)), scal);
     return hipGetLastError();
 }
 
-hipError_t launch_prolong(hipStream_t s, const float* q, const float* scal, const float scale, float* p0, const Dims cd) {
-    const Dims fd{2 * cd.nx, 2 * cd.ny, 2 * cd.nz};
-    k_prolong<<<blocks_for(cells_of(fd)), kThreads, 0, s>>>(q, scal, scale, p0, cd);
+hipError_t launch_prolong(hipStream_t s, const float* q, const float* scal, const float scale, float* p0, const Dims cd, const int f) {
+    const Dims fd{f * cd.nx, f * cd.ny, f * cd.nz};
+    k_prolong<<<blocks_for(cells_of(fd)), kThreads, 0, s>>>(q, scal, scale, p0, cd, f);
     return hipGetLastError();
 }
 
@@ -378,11 +395,13 @@ hipError_t cg_solve(hipStream_t s, const float* div, const float h2, double* x, 
     return hipSuccess;
 }
 
-hipError_t launch_pair(hipStream_t s, const float* div, const double* x, const float h2, float* in2, float* target, double* partials, double* scal, const Dims d, const StencilBC& p_bc) {
-    const unsigned b = blocks_for(cells_of(d));
-    k_sum_sq<<<b, kThreads, 0, s>>>(div, partials, cells_of(d));
+hipError_t launch_pair(hipStream_t s, const float* div, const double* x, const float h2, float* in2, float* target, double* partials, double* scal, const Dims d, const int f, const StencilBC& p_bc) {
+    if (f < 1 || d.nx % f || d.ny % f || d.nz % f) return hipErrorInvalidValue;
+    const Dims cd{d.nx / f, d.ny / f, d.nz / f};
+    const unsigned b = blocks_for(cells_of(cd));
+    k_coarse_sum_sq<<<b, kThreads, 0, s>>>(div, partials, cd, f);
     k_reduce<<<1, kThreads, 0, s>>>(partials, static_cast<int>(b), 1, 0, scal, 5);
-    k_pair<<<b, kThreads, 0, s>>>(div, x, h2, in2, target, scal, d, p_bc);
+    k_pair<<<b, kThreads, 0, s>>>(div, x, static_cast<double>(h2) * f * f, in2, target, scal, cd, f, p_bc);
     return hipGetLastError();
 }
 

@@ -15,13 +15,14 @@ namespace sf::warm {
 
 constexpr int kPartialBlocks = 1024;
 
-// Coarse network input in2[v][2] = (mean of the 8 fine div values, wall mask) on the
-// half-resolution grid cd (fine = 2 * cd), and 1/sigma into scal[1], sigma into
-// scal[0] (sigma = RMS of the coarse divergence; 1/sigma = 0 when it is 0).
+// Coarse network input in2[v][2] = (mean of the f^3 fine div values, wall mask) on the
+// coarse grid cd (fine = f * cd), and 1/sigma into scal[1], sigma into scal[0]
+// (sigma = RMS of the coarse divergence; 1/sigma = 0 when it is 0).
 // partials: kPartialBlocks doubles.
-hipError_t launch_restrict_input(hipStream_t s, const float* div, float* in2, double* partials, float* scal, Dims cd, const StencilBC& p_bc);
-// p0[fine] = scale * sigma * trilinear(q)[fine], scale = (2h)^2, sigma = scal[0].
-hipError_t launch_prolong(hipStream_t s, const float* q, const float* scal, float scale, float* p0, Dims cd);
+hipError_t launch_restrict_input(hipStream_t s, const float* div, float* in2, double* partials, float* scal, Dims cd, int f, const StencilBC& p_bc);
+// p0[fine] = scale * sigma * trilinear(q)[fine] (cell-centred, clamped at the edges),
+// scale = (f h)^2, sigma = scal[0].
+hipError_t launch_prolong(hipStream_t s, const float* q, const float* scal, float scale, float* p0, Dims cd, int f);
 // Residual test of the guess p0: g = ||b - A p0||, z = ||b|| (= residual of the zero
 // guess), FP64. stats[0] = g, stats[1] = z, stats[2] = 1 when g > ratio * z (fallback).
 // skip = 0 on fallback (the fallback passes run), 1 otherwise; on fallback p0 is zeroed.
@@ -38,8 +39,9 @@ struct CgResult {
     double rel_residual;
 };
 hipError_t cg_solve(hipStream_t s, const float* div, float h2, double* x, double* r, double* p, double* ap, double* partials, double* scal, Dims d, const StencilBC& p_bc, double tol, int max_iter, CgResult* out);
-// Training pair from the divergence and the solution x: in2[v] = (div / sigma, wall
-// mask), target[v] = x / (h2 sigma), sigma = RMS(div) (FP64 in scal[6]).
-hipError_t launch_pair(hipStream_t s, const float* div, const double* x, float h2, float* in2, float* target, double* partials, double* scal, Dims d, const StencilBC& p_bc);
+// Training pair on the grid coarsened f times from the divergence and the solution x:
+// in2[v] = (div_c / sigma, wall mask), target[v] = x_c / ((f h)^2 sigma), where _c is
+// the mean of the f^3 fine cells and sigma = RMS(div_c).
+hipError_t launch_pair(hipStream_t s, const float* div, const double* x, float h2, float* in2, float* target, double* partials, double* scal, Dims d, int f, const StencilBC& p_bc);
 
 } // namespace sf::warm

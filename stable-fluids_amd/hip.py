@@ -82,6 +82,10 @@ def to_device(dev_ptr, a):
     check(lib().hipMemcpy(C.c_void_p(dev_ptr), a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes), _H2D), "hipMemcpy H2D")
 
 
+def copy_device(dst_ptr, src_ptr, nbytes):
+    check(lib().hipMemcpy(C.c_void_p(dst_ptr), C.c_void_p(src_ptr), C.c_size_t(nbytes), _D2D), "hipMemcpy D2D")
+
+
 def synchronize():
     check(lib().hipDeviceSynchronize(), "hipDeviceSynchronize")
 

@@ -28,6 +28,9 @@ EXT_ENTRY_POINTS = (
     "stable_fluids_ext_run_stage_cuda", "stable_fluids_ext_stage_bytes_cuda", "stable_fluids_ext_get_stream_cuda",
     "stable_fluids_ext_set_pressure_model_cuda", "stable_fluids_ext_warm_start_stats_cuda", "stable_fluids_ext_training_pair_cuda",
     "stable_fluids_ext_get_pressure_rhs_cuda",
+    "stable_fluids_ext_batch_create_cuda", "stable_fluids_ext_batch_destroy_cuda", "stable_fluids_ext_batch_step_cuda",
+    "stable_fluids_ext_batch_get_stream_cuda", "stable_fluids_ext_batch_set_z_chunk_cuda", "stable_fluids_ext_batch_info_cuda",
+    "stable_fluids_ext_batch_autotune_cuda",
 )
 UNET_ENTRY_POINTS = (
     "stable_fluids_unet_param_count", "stable_fluids_unet_create_cuda", "stable_fluids_unet_destroy_cuda",
@@ -75,10 +78,18 @@ def load_library(path=LIB_PATH):
     lib.stable_fluids_ext_get_stream_cuda.restype = vp
     i32, u64, f32 = C.c_int32, C.c_uint64, C.c_float
     lib.stable_fluids_ext_set_pressure_model_cuda.argtypes = [vp, vp, u64]
+    lib.stable_fluids_ext_batch_create_cuda.argtypes = [C.POINTER(vp), u32, vp, C.POINTER(vp)]
+    lib.stable_fluids_ext_batch_destroy_cuda.argtypes = [vp]
+    lib.stable_fluids_ext_batch_step_cuda.argtypes = [vp]
+    lib.stable_fluids_ext_batch_get_stream_cuda.argtypes = [vp]
+    lib.stable_fluids_ext_batch_get_stream_cuda.restype = vp
+    lib.stable_fluids_ext_batch_set_z_chunk_cuda.argtypes = [vp, u32, u32]
+    lib.stable_fluids_ext_batch_info_cuda.argtypes = [vp, u32, C.POINTER(C.c_int32), C.POINTER(u32)]
+    lib.stable_fluids_ext_batch_autotune_cuda.argtypes = [vp, C.c_char_p, u32, C.POINTER(u32)]
     lib.stable_fluids_ext_get_pressure_rhs_cuda.argtypes = [vp]
     lib.stable_fluids_ext_get_pressure_rhs_cuda.restype = vp
     lib.stable_fluids_ext_warm_start_stats_cuda.argtypes = [vp, C.POINTER(C.c_double)]
-    lib.stable_fluids_ext_training_pair_cuda.argtypes = [vp, vp, vp, C.c_double, u32, C.POINTER(C.c_double), C.POINTER(u32)]
+    lib.stable_fluids_ext_training_pair_cuda.argtypes = [vp, vp, vp, u32, C.c_double, u32, C.POINTER(C.c_double), C.POINTER(u32)]
     lib.stable_fluids_unet_param_count.argtypes = []
     lib.stable_fluids_unet_param_count.restype = u64
     lib.stable_fluids_unet_create_cuda.argtypes = [i32, i32, i32, vp, C.POINTER(vp)]
@@ -86,20 +97,20 @@ def load_library(path=LIB_PATH):
     for name in ("set_params", "get_params", "get_grads"):
         getattr(lib, f"stable_fluids_unet_{name}_cuda").argtypes = [vp, vp, u64]
     lib.stable_fluids_unet_forward_cuda.argtypes = [vp, vp, vp]
-    lib.stable_fluids_unet_backward_cuda.argtypes = [vp, vp, vp, vp]
+    lib.stable_fluids_unet_backward_cuda.argtypes = [vp, vp, vp, f32, u32, vp]
     lib.stable_fluids_unet_adam_step_cuda.argtypes = [vp, f32, f32, f32, f32]
     lib.stable_fluids_unet_conv3_cuda.argtypes = [vp, i32, i32, vp, vp, vp, vp, i32, i32, i32, i32]
     lib.stable_fluids_unet_conv3_wgrad_cuda.argtypes = [vp, i32, i32, vp, vp, vp, vp, i32, i32, i32]
     for name in ENTRY_POINTS + EXT_ENTRY_POINTS + UNET_ENTRY_POINTS:
         if name not in ("stable_fluids_ext_stage_bytes_cuda", "stable_fluids_ext_get_stream_cuda", "stable_fluids_unet_param_count",
-                        "stable_fluids_ext_get_pressure_rhs_cuda"):
+                        "stable_fluids_ext_get_pressure_rhs_cuda", "stable_fluids_ext_batch_get_stream_cuda"):
             getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
 
 
 def ext_options(smoother=abi.SMOOTHER_RBGS, iterate_storage=abi.STORAGE_F32, defect_correction_interval=0,
-                pressure_warm_start=abi.WARM_START_ZERO, warm_start_iterations=0):
+                pressure_warm_start=abi.WARM_START_ZERO, warm_start_iterations=0, warm_start_coarsening=0):
     o = abi.ExtOptions()
     o.struct_size = C.sizeof(abi.ExtOptions)
     o.smoother = smoother
@@ -107,6 +118,7 @@ def ext_options(smoother=abi.SMOOTHER_RBGS, iterate_storage=abi.STORAGE_F32, def
     o.defect_correction_interval = defect_correction_interval
     o.pressure_warm_start = pressure_warm_start
     o.warm_start_iterations = warm_start_iterations
+    o.warm_start_coarsening = warm_start_coarsening
     return o
 
 
@@ -125,8 +137,8 @@ class Context:
         lib = load_library()
         self.lib = lib
         self.desc = desc
-        if stream is not None:
-            desc.desc.stream = stream.handle if isinstance(stream, hip.Stream) else stream
+        # the desc may be reused: always write this context's stream (NULL: the library's own)
+        desc.desc.stream = (stream.handle if isinstance(stream, hip.Stream) else stream) if stream is not None else None
         self.scalar_handles = (C.c_uint32 * max(1, desc.n_scalars))()
         self.vector_handles = (C.c_uint32 * max(1, desc.n_vectors))()
         handle = C.c_void_p()
@@ -206,10 +218,11 @@ class Context:
         ptr = self.lib.stable_fluids_ext_get_pressure_rhs_cuda(self.handle)
         return hip.to_host(ptr, self.desc.cells).reshape(self.desc.shape)
 
-    def training_pair(self, inp, target, rel_tol=1e-6, max_iterations=20000):
-        """Fills device buffers inp [v][2] and target [v]; returns (relative residual, iterations)."""
+    def training_pair(self, inp, target, coarsening=0, rel_tol=1e-6, max_iterations=20000):
+        """Fills device buffers inp [v][2] and target [v] on the grid coarsened 2^coarsening
+        times; returns (relative residual, iterations)."""
         res, its = C.c_double(), C.c_uint32()
-        self._call("stable_fluids_ext_training_pair_cuda", _ptr(inp), _ptr(target), rel_tol, max_iterations, C.byref(res), C.byref(its))
+        self._call("stable_fluids_ext_training_pair_cuda", _ptr(inp), _ptr(target), coarsening, rel_tol, max_iterations, C.byref(res), C.byref(its))
         return res.value, its.value
 
     def stage_bytes(self, stage):
@@ -243,6 +256,61 @@ class Context:
         if r != abi.RESULT_OK:
             raise StableFluidsError(name, r)
         return r
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+
+class Batch:
+    """Many contexts stepped together, one kernel launch per stage
+    (stable_fluids_ext_batch_*_cuda). The contexts stay owned by the caller."""
+
+    def __init__(self, contexts, stream=None):
+        self.lib = load_library()
+        self.contexts = list(contexts)
+        arr = (C.c_void_p * len(self.contexts))(*[c.handle for c in self.contexts])
+        h = C.c_void_p()
+        s = stream.handle if isinstance(stream, hip.Stream) else stream
+        r = self.lib.stable_fluids_ext_batch_create_cuda(arr, len(self.contexts), s, C.byref(h))
+        if r != abi.RESULT_OK:
+            raise StableFluidsError("stable_fluids_ext_batch_create_cuda", r)
+        self.handle = h.value
+
+    def _call(self, name, *args):
+        r = getattr(self.lib, name)(self.handle, *args)
+        if r != abi.RESULT_OK:
+            raise StableFluidsError(name, r)
+        return r
+
+    def step(self, n=1):
+        for _ in range(n):
+            self._call("stable_fluids_ext_batch_step_cuda")
+
+    @property
+    def stream(self):
+        return self.lib.stable_fluids_ext_batch_get_stream_cuda(self.handle)
+
+    def set_z_chunk(self, sim, z_chunk):
+        self._call("stable_fluids_ext_batch_set_z_chunk_cuda", sim, z_chunk)
+
+    def info(self, sim):
+        t = (C.c_int32 * 4)()
+        n = C.c_uint32()
+        self._call("stable_fluids_ext_batch_info_cuda", sim, t, C.byref(n))
+        return {"tiles_x": t[0], "tiles_y": t[1], "chunks": t[2], "z_chunk": t[3], "launches_per_step": n.value}
+
+    def autotune(self, cache_path=None, repeats=10):
+        n = C.c_uint32()
+        self._call("stable_fluids_ext_batch_autotune_cuda", cache_path.encode() if cache_path else None, repeats, C.byref(n))
+        return n.value
+
+    def destroy(self):
+        if self.handle:
+            self.lib.stable_fluids_ext_batch_destroy_cuda(self.handle)
+            self.handle = None
 
     def __enter__(self):
         return self

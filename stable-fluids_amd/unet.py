@@ -12,6 +12,8 @@ import numpy as np
 from . import abi, hip
 from .runtime import StableFluidsError, _ptr, load_library
 
+LOSS_CENTERED = 1  # STABLE_FLUIDS_UNET_LOSS_CENTERED
+
 
 def param_count():
     return int(load_library().stable_fluids_unet_param_count())
@@ -65,8 +67,9 @@ class UNet:
         """inp: device [v][2] float32; out: device [v] float32 (asynchronous)."""
         _check("forward", self.lib.stable_fluids_unet_forward_cuda(self.handle, _ptr(inp), _ptr(out)))
 
-    def backward(self, inp, target, loss=None):
-        _check("backward", self.lib.stable_fluids_unet_backward_cuda(self.handle, _ptr(inp), _ptr(target), _ptr(loss)))
+    def backward(self, inp, target, loss=None, weight=1.0, centered=False):
+        flags = LOSS_CENTERED if centered else 0
+        _check("backward", self.lib.stable_fluids_unet_backward_cuda(self.handle, _ptr(inp), _ptr(target), weight, flags, _ptr(loss)))
 
     def adam_step(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
         _check("adam_step", self.lib.stable_fluids_unet_adam_step_cuda(self.handle, lr, beta1, beta2, eps))

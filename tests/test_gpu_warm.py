@@ -47,8 +47,12 @@ def test_training_pair_is_exact_solve(walls_only):
         rel, its = ctx.training_pair(inp, tgt, rel_tol=1e-6, max_iterations=5000)
         x2 = hip.to_host(inp.value, 2 * desc.cells).reshape(desc.shape + (2,))
         t = hip.to_host(tgt.value, desc.cells).reshape(desc.shape).astype(np.float64)
-        # a second call starts from the first solution and converges at once
-        rel2, its2 = ctx.training_pair(inp, tgt, rel_tol=1e-6, max_iterations=5000)
+        # a second call starts from the first solution and converges at once; its pair is
+        # the same solve averaged over 2^3 blocks
+        rel2, its2 = ctx.training_pair(inp, tgt, coarsening=1, rel_tol=1e-6, max_iterations=5000)
+        cells_c = desc.cells // 8
+        x2c = hip.to_host(inp.value, 2 * cells_c).reshape(tuple(n // 2 for n in desc.shape) + (2,))
+        tc = hip.to_host(tgt.value, cells_c).reshape(tuple(n // 2 for n in desc.shape)).astype(np.float64)
     print(f"walls_only={walls_only}: cg iterations {its} rel residual {rel:.2e}; warm restart {its2}", flush=True)
     assert rel <= 1e-6 and its > 0 and its2 <= 10
     rules = wo.pressure_rules(desc.desc.config.flow_boundary)
@@ -66,18 +70,24 @@ def test_training_pair_is_exact_solve(walls_only):
     err = np.sqrt(np.sum(r * r)) / np.sqrt(np.sum(b * b))
     print(f"residual of the FP32 target {err:.2e}", flush=True)
     assert err <= 1e-5
+    dc = wo.restrict(rhs, 2)
+    sc = np.sqrt(np.mean(dc * dc))
+    assert np.max(np.abs(x2c[..., 0] - dc / sc)) <= 1e-5 * np.max(np.abs(dc / sc))
+    assert np.array_equal(x2c[..., 1], wo.wall_mask(dc.shape, rules))
+    ref_tc = wo.restrict(p, 2) / ((2.0 / 16) ** 2 * sc)
+    assert np.max(np.abs(tc - ref_tc)) <= 1e-4 * np.max(np.abs(ref_tc))
 
 
 def plume(n=32, p_iters=12):
     return scenes.plume_scene(n, diffuse_iterations=4, pressure_iterations=p_iters, noise_seed=1)
 
 
-def learned_opts(k):
-    return sf.ext_options(pressure_warm_start=abi.WARM_START_LEARNED, warm_start_iterations=k)
+def learned_opts(k, coarsening=1):
+    return sf.ext_options(pressure_warm_start=abi.WARM_START_LEARNED, warm_start_iterations=k, warm_start_coarsening=coarsening)
 
 
-def run_learned(scene, steps, params, k):
-    with sf.Context(scene.desc, options=learned_opts(k)) as ctx:
+def run_learned(scene, steps, params, k, coarsening=1):
+    with sf.Context(scene.desc, options=learned_opts(k, coarsening)) as ctx:
         ctx.set_pressure_model(params)
         scenes.seed_context(ctx, scene)
         ctx.step(steps)
@@ -112,21 +122,47 @@ def test_zero_network_is_the_plain_k_pass_step_bitwise():
     assert_same(got, plain, "zero net vs plain K")
 
 
-def test_accepted_guess_matches_oracle():
+@pytest.mark.parametrize("coarsening", [1, 2, 3])
+def test_accepted_guess_matches_oracle(coarsening):
     """K = 0 and every pass skipped: the pressure view is the guess itself."""
     scene = plume(p_iters=12)
     layers = uo.init_params(4)
     w, b = layers[-1]
     layers[-1] = (w * 0.05, b)
     params = uo.join_params(layers).astype(np.float32)
-    got, stats, rhs = run_learned(scene, 1, params, k=0)
+    got, stats, rhs = run_learned(scene, 1, params, k=0, coarsening=coarsening)
     assert not stats["fallback"]
     h = float(np.float32(1.0 / 32))
     rules = wo.pressure_rules(scene.desc.desc.config.flow_boundary)
     p_bf = [(uo.bf16_round(w), np.asarray(b, np.float32).astype(np.float64)) for w, b in uo.split_params(params.astype(np.float64))]
-    p0, sigma, _ = wo.guess(rhs.astype(np.float64), p_bf, h, rules, uo.bf16_round)
+    p0, sigma, _ = wo.guess(rhs.astype(np.float64), p_bf, h, rules, uo.bf16_round, f=2 ** coarsening)
     err = np.max(np.abs(got["pressure"] - p0)) / np.max(np.abs(p0))
     g, z, fb = wo.guess_test(got["pressure"], rhs.astype(np.float64), h * h, rules)
     print(f"guess err={err:.2e} sigma={sigma:.3e} stats={stats} oracle g={g:.6e} z={z:.6e}", flush=True)
     assert err <= 1e-2
     assert abs(stats["guess_residual"] - g) <= 1e-5 * g and abs(stats["zero_residual"] - z) <= 1e-6 * z and not fb
+
+
+def test_trained_network_gain():
+    """The trained network (stable-fluids_amd/weights) against the method without it: one
+    step of the 128^3 plume from a common state, post-projection divergence L2 with the
+    network guess + K = 8 passes below the zero guess + 8 passes (plain, K passes) and
+    below an untrained (He-initialised) network + 8 passes; the plain 40-pass value is
+    reported beside them."""
+    weights = sf.warm_eval.load_model()
+    if weights is None:
+        pytest.skip("no trained parameters under stable-fluids_amd/weights")
+    scene = scenes.plume_scene(128, diffuse_iterations=20, pressure_iterations=40)
+    states = sf.warm_eval.checkpoints(scene, (12,))
+    k = 8
+    learned = lambda: sf.ext_options(pressure_warm_start=abi.WARM_START_LEARNED, warm_start_iterations=k, warm_start_coarsening=2)
+    short = scenes.plume_scene(128, diffuse_iterations=20, pressure_iterations=k)
+    d_plain_k, _ = sf.warm_eval.one_step_div(short, states, sf.ext_options())
+    d_plain_40, _ = sf.warm_eval.one_step_div(scene, states, sf.ext_options())
+    untrained = uo.join_params(uo.init_params(0)).astype(np.float32)
+    d_untrained, st_u = sf.warm_eval.one_step_div(scene, states, learned(), untrained)
+    d_trained, st_t = sf.warm_eval.one_step_div(scene, states, learned(), weights)
+    print(f"div L2: plain {k} passes {d_plain_k[0]:.4e}, untrained+{k} {d_untrained[0]:.4e} {st_u[0]}, "
+          f"trained+{k} {d_trained[0]:.4e} {st_t[0]}, plain 40 passes {d_plain_40[0]:.4e}", flush=True)
+    assert not st_t[0]["fallback"]
+    assert d_trained[0] < d_plain_k[0] and d_trained[0] < d_untrained[0]

@@ -50,6 +50,13 @@ def test_prolong_and_restrict():
     assert np.allclose(p[0, 0, 1:-1], expect[1:-1], atol=1e-12)
     assert p[0, 0, 0] == 0.0 and p[0, 0, -1] == 15.0  # clamped edge cells
     assert np.allclose(wo.restrict(np.full((4, 6, 8), -1.5)), -1.5)
+    assert np.allclose(wo.restrict(np.full((8, 8, 12), 0.75), 4), 0.75)
+    q = np.broadcast_to(ic[None, None, :] * 2.0, (2, 2, 6))
+    p4 = wo.prolong(q, 4)
+    xf = np.arange(24)
+    expect4 = 2.0 * ((xf + 0.5) / 4 - 0.5)
+    inner = (xf >= 2) & (xf <= 21)
+    assert np.allclose(p4[0, 0, inner], expect4[inner], atol=1e-12)
 
 
 def test_wall_mask_marks_wall_faces_only():
