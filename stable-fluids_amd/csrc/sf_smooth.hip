@@ -552,6 +552,8 @@ constexpr int kOldPlane = 2 * kArr;
 constexpr int kHPlane   = kArr;
 constexpr int kHVals    = (kTY + 2) * (kPX + 2);                      // red values per H plane
 constexpr int kHSlots3  = (kHVals + kThreads - 1) / kThreads;
+constexpr int kOldPairs = kArr;                                        // pairs per old plane
+constexpr int kOldPairPer = (kOldPairs + kThreads - 1) / kThreads;
 
 // Periodic plane index for k in [-2, n + 1] without a division (n >= 2); n = 1 maps to 0.
 __device__ __forceinline__ int wrap_near(const int k, const int n) {
@@ -559,7 +561,7 @@ __device__ __forceinline__ int wrap_near(const int k, const int n) {
     return k < 0 ? k + n : (k >= n ? k - n : k);
 }
 
-template <int kSolve, bool kZero, class St>
+template <int kSolve, bool kZero, class St, bool kPairs>
 __device__ __forceinline__ void rbgs_split_body(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC& bc, const TileGeom& g, const float* __restrict__ scale_slot, const int w) {
     __shared__ float s_old[3][kOldPlane];
     __shared__ float s_h[3][kHPlane];
@@ -577,19 +579,42 @@ __device__ __forceinline__ void rbgs_split_body(const typename St::T* __restrict
 
     // Old-plane load slots: contiguous floats of the (2*kW)-wide region rows, stored to the
     // even or odd LDS array by column parity. Lanes past the region repeat its last slot
-    // (same address, same value), so no lane needs a branch.
-    uint32_t o_off[kOldPer];
-    int o_gc[kOldPer], o_lds[kOldPer];
+    // (same address, same value), so no lane needs a branch. With even rows (kPairs) a slot
+    // is an aligned pair: .x goes to the even array, .y to the odd one; x-ghost pairs read
+    // the adjacent edge cell (sel 1: both from .x, sel 2: both from .y).
+    constexpr int kSlots = kPairs ? kOldPairPer : kOldPer;
+    uint32_t o_off[kSlots];
+    int o_gc[kSlots], o_lds[kSlots], o_sel[kSlots];
     bool any_gc = false;
 #pragma unroll
-    for (int j = 0; j < kOldPer; ++j) {
-        const int e  = min(tid + j * kThreads, kOldPlane - 1);
-        const int r  = e / (2 * kW), c = e % (2 * kW);
-        const int gx = xs + c, gy = ys + r;
-        o_off[j]     = static_cast<uint32_t>(fix_coord(gy, d.ny, py) * d.nx + fix_coord(gx, d.nx, px));
-        o_gc[j]      = ghost_code(gx, gy, d, px, py);
-        o_lds[j]     = (c & 1) * kArr + r * kW + (c >> 1);
-        any_gc       = any_gc || o_gc[j] != 0;
+    for (int j = 0; j < kSlots; ++j) {
+        if constexpr (kPairs) {
+            const int e   = min(tid + j * kThreads, kOldPairs - 1);
+            const int r   = e / kW, i = e % kW;
+            const int gx  = xs + 2 * i, gy = ys + r;
+            const int row = fix_coord(gy, d.ny, py) * d.nx;
+            o_sel[j]      = 0;
+            if (!px && gx < 0) {
+                o_off[j] = static_cast<uint32_t>(row);
+                o_sel[j] = 1;
+            } else if (!px && gx >= d.nx) {
+                o_off[j] = static_cast<uint32_t>(row + d.nx - 2);
+                o_sel[j] = 2;
+            } else {
+                o_off[j] = static_cast<uint32_t>(row + fix_coord(gx, d.nx, px));
+            }
+            o_gc[j]  = ghost_code(gx, gy, d, px, py);
+            o_lds[j] = r * kW + i;
+        } else {
+            const int e  = min(tid + j * kThreads, kOldPlane - 1);
+            const int r  = e / (2 * kW), c = e % (2 * kW);
+            const int gx = xs + c, gy = ys + r;
+            o_off[j]     = static_cast<uint32_t>(fix_coord(gy, d.ny, py) * d.nx + fix_coord(gx, d.nx, px));
+            o_gc[j]      = ghost_code(gx, gy, d, px, py);
+            o_lds[j]     = (c & 1) * kArr + r * kW + (c >> 1);
+            o_sel[j]     = 0;
+        }
+        any_gc = any_gc || o_gc[j] != 0;
     }
     const bool wave_gc = __builtin_amdgcn_readfirstlane(__any(any_gc)) != 0;
 
@@ -635,20 +660,43 @@ __device__ __forceinline__ void rbgs_split_body(const typename St::T* __restrict
     auto plane_of = [&](const int k) { return pz ? wrap_near(k, d.nz) : (k < 0 ? 0 : (k >= d.nz ? d.nz - 1 : k)); };
     auto real_plane = [&](const int k) { return pz || (k >= 0 && k < d.nz); };
 
-    float r_old[kOldPer], r_bh[kHSlots3], r_bf[kOutPer];
+    float r_old[kPairs ? 1 : kOldPer], r_bh[kHSlots3], r_bf[kOutPer];
+    float2 r_pair[kPairs ? kOldPairPer : 1];
     auto fetch_old = [&](const int k) {
         const typename St::T* src = x_old + static_cast<uint64_t>(plane_of(k)) * plane;
+        if constexpr (kPairs) {
 #pragma unroll
-        for (int j = 0; j < kOldPer; ++j) r_old[j] = kZero ? 0.0f : St::load(src[o_off[j]]);
-        if (!pz && (k < 0 || k >= d.nz)) {
-            const FaceRule f = k < 0 ? FaceRule{bc.face[4].mode, bc.face[4].value} : FaceRule{bc.face[5].mode, bc.face[5].value};
+            for (int j = 0; j < kOldPairPer; ++j) {
+                float2 v = kZero ? make_float2(0.0f, 0.0f) : St::load2(src + o_off[j]);
+                if (o_sel[j] == 1) v.y = v.x;
+                if (o_sel[j] == 2) v.x = v.y;
+                r_pair[j] = v;
+            }
+            if (!pz && (k < 0 || k >= d.nz)) {
+                const FaceRule f = k < 0 ? FaceRule{bc.face[4].mode, bc.face[4].value} : FaceRule{bc.face[5].mode, bc.face[5].value};
 #pragma unroll
-            for (int j = 0; j < kOldPer; ++j) r_old[j] = apply_ghost(f, r_old[j]);
-        } else if (wave_gc) {
+                for (int j = 0; j < kOldPairPer; ++j) r_pair[j] = make_float2(apply_ghost(f, r_pair[j].x), apply_ghost(f, r_pair[j].y));
+            } else if (wave_gc) {
 #pragma unroll
-            for (int j = 0; j < kOldPer; ++j) {
-                const float gv = apply_ghost(face_rule(bc, o_gc[j] - 1), r_old[j]);
-                r_old[j]       = o_gc[j] ? gv : r_old[j];
+                for (int j = 0; j < kOldPairPer; ++j) {
+                    const FaceRule f = face_rule(bc, o_gc[j] - 1);
+                    const float2 gv  = make_float2(apply_ghost(f, r_pair[j].x), apply_ghost(f, r_pair[j].y));
+                    r_pair[j]        = o_gc[j] ? gv : r_pair[j];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kOldPer; ++j) r_old[j] = kZero ? 0.0f : St::load(src[o_off[j]]);
+            if (!pz && (k < 0 || k >= d.nz)) {
+                const FaceRule f = k < 0 ? FaceRule{bc.face[4].mode, bc.face[4].value} : FaceRule{bc.face[5].mode, bc.face[5].value};
+#pragma unroll
+                for (int j = 0; j < kOldPer; ++j) r_old[j] = apply_ghost(f, r_old[j]);
+            } else if (wave_gc) {
+#pragma unroll
+                for (int j = 0; j < kOldPer; ++j) {
+                    const float gv = apply_ghost(face_rule(bc, o_gc[j] - 1), r_old[j]);
+                    r_old[j]       = o_gc[j] ? gv : r_old[j];
+                }
             }
         }
     };
@@ -672,8 +720,16 @@ __device__ __forceinline__ void rbgs_split_body(const typename St::T* __restrict
         }
     };
     auto stash_old = [&](float* s) {
+        if constexpr (kPairs) {
 #pragma unroll
-        for (int j = 0; j < kOldPer; ++j) s[o_lds[j]] = r_old[j];
+            for (int j = 0; j < kOldPairPer; ++j) {
+                s[o_lds[j]]        = r_pair[j].x;
+                s[kArr + o_lds[j]] = r_pair[j].y;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kOldPer; ++j) s[o_lds[j]] = r_old[j];
+        }
     };
 
     // LDS rings, rotated each plane: old planes k-1, k, k+1 and H planes k-2, k-1, k
@@ -800,10 +856,10 @@ __global__ __launch_bounds__(kThreads) void k_jacobi_pass(const typename St::T* 
     jacobi_body<kSolve, kZero, St, kPairs>(x_old, b, x_new, coef, out_scale, d, bc, g, scale_slot, xcd_block());
 }
 
-template <int kSolve, bool kZero, class St>
+template <int kSolve, bool kZero, class St, bool kPairs>
 __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g, const float* __restrict__ scale_slot, const int* __restrict__ skip) {
     if (skip != nullptr && *skip != 0) return;
-    rbgs_split_body<kSolve, kZero, St>(x_old, b, x_new, coef, out_scale, d, bc, g, scale_slot, xcd_block());
+    rbgs_split_body<kSolve, kZero, St, kPairs>(x_old, b, x_new, coef, out_scale, d, bc, g, scale_slot, xcd_block());
 }
 
 // Batched passes: workgroup i runs block items[i].block of pass descs[items[i].desc].
@@ -811,7 +867,10 @@ template <int kSolve, bool kZero>
 __global__ __launch_bounds__(kThreads) void k_rbgs_split_batched(const SmoothDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
     const BatchItem it = items[blockIdx.x];
     const SmoothDesc sd = descs[it.desc];
-    rbgs_split_body<kSolve, kZero, StoreF32>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
+    if ((sd.d.nx & 1) == 0)
+        rbgs_split_body<kSolve, kZero, StoreF32, true>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
+    else
+        rbgs_split_body<kSolve, kZero, StoreF32, false>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
 }
 
 template <int kSolve, bool kZero>
@@ -885,14 +944,19 @@ hipError_t launch_smooth(hipStream_t s, const SmootherKind smoother, const Solve
         if ((d.nx & 1) == 0) k_jacobi_pass<SO, Z, StoreF32, true><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g, nullptr, pass.skip); \
         else k_jacobi_pass<SO, Z, StoreF32, false><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g, nullptr, pass.skip);         \
     } while (0)
+#define SF_SPASS(SO, Z)                                                                                                                                                                \
+    do {                                                                                                                                                                                \
+        if ((d.nx & 1) == 0) k_rbgs_split_pass<SO, Z, StoreF32, true><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g, nullptr, pass.skip); \
+        else k_rbgs_split_pass<SO, Z, StoreF32, false><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g, nullptr, pass.skip);         \
+    } while (0)
 #define SF_PASS(KERNEL, SO, Z) KERNEL<SO, Z, StoreF32><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g, nullptr, pass.skip)
     if (smoother == kSmoothRBGS && !has_odd_periodic(d, bc)) {
         if (solve == kSolvePoisson) {
-            if (zero) SF_PASS(k_rbgs_split_pass, kSolvePoisson, true);
-            else SF_PASS(k_rbgs_split_pass, kSolvePoisson, false);
+            if (zero) SF_SPASS(kSolvePoisson, true);
+            else SF_SPASS(kSolvePoisson, false);
         } else {
-            if (zero) SF_PASS(k_rbgs_split_pass, kSolveDiffusion, true);
-            else SF_PASS(k_rbgs_split_pass, kSolveDiffusion, false);
+            if (zero) SF_SPASS(kSolveDiffusion, true);
+            else SF_SPASS(kSolveDiffusion, false);
         }
     } else if (smoother == kSmoothRBGS) {
         if (solve == kSolvePoisson) {
@@ -912,6 +976,7 @@ hipError_t launch_smooth(hipStream_t s, const SmootherKind smoother, const Solve
         }
     }
 #undef SF_PASS
+#undef SF_SPASS
 #undef SF_JPASS
 #undef SF_PASS_GENERAL
     return hipGetLastError();
@@ -925,7 +990,11 @@ hipError_t correction_pass(hipStream_t s, const SmootherKind smoother, const Sol
     const unsigned blocks = static_cast<unsigned>(g.tiles_x * g.tiles_y * g.chunks);
     const auto* e_old     = static_cast<const typename St::T*>(pass.e_old);
     auto* e_new           = static_cast<typename St::T*>(pass.e_new);
-#define SF_CORR(KERNEL, SO, Z) KERNEL<SO, Z, St><<<blocks, kThreads, 0, s>>>(e_old, pass.r, e_new, pass.coef, 1.0f, d, bc, g, pass.scale_slot, nullptr)
+#define SF_SCORR(SO, Z)                                                                                                                                      \
+    do {                                                                                                                                                     \
+        if ((d.nx & 1) == 0) k_rbgs_split_pass<SO, Z, St, true><<<blocks, kThreads, 0, s>>>(e_old, pass.r, e_new, pass.coef, 1.0f, d, bc, g, pass.scale_slot, nullptr); \
+        else k_rbgs_split_pass<SO, Z, St, false><<<blocks, kThreads, 0, s>>>(e_old, pass.r, e_new, pass.coef, 1.0f, d, bc, g, pass.scale_slot, nullptr);         \
+    } while (0)
 #define SF_JCORR(SO, Z)                                                                                                                                      \
     do {                                                                                                                                                     \
         if ((d.nx & 1) == 0) k_jacobi_pass<SO, Z, St, true><<<blocks, kThreads, 0, s>>>(e_old, pass.r, e_new, pass.coef, 1.0f, d, bc, g, pass.scale_slot, nullptr); \
@@ -934,11 +1003,11 @@ hipError_t correction_pass(hipStream_t s, const SmootherKind smoother, const Sol
     const bool zero = e_old == nullptr;
     if (smoother == kSmoothRBGS) {
         if (solve == kSolvePoisson) {
-            if (zero) SF_CORR(k_rbgs_split_pass, kSolvePoisson, true);
-            else SF_CORR(k_rbgs_split_pass, kSolvePoisson, false);
+            if (zero) SF_SCORR(kSolvePoisson, true);
+            else SF_SCORR(kSolvePoisson, false);
         } else {
-            if (zero) SF_CORR(k_rbgs_split_pass, kSolveDiffusion, true);
-            else SF_CORR(k_rbgs_split_pass, kSolveDiffusion, false);
+            if (zero) SF_SCORR(kSolveDiffusion, true);
+            else SF_SCORR(kSolveDiffusion, false);
         }
     } else {
         if (solve == kSolvePoisson) {
@@ -949,7 +1018,7 @@ hipError_t correction_pass(hipStream_t s, const SmootherKind smoother, const Sol
             else SF_JCORR(kSolveDiffusion, false);
         }
     }
-#undef SF_CORR
+#undef SF_SCORR
 #undef SF_JCORR
     return hipGetLastError();
 }
