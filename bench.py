@@ -35,6 +35,8 @@ sys.path.insert(0, ROOT)
 import __graft_entry__ as entry  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# the velocity diffusion RBGS pass (FP32 iterates, even rows loaded as aligned pairs)
+VELOCITY_PASS_KERNEL = "k_rbgs_split_pass<0, false, sf::StoreF32, true>"
 STORAGE = {"f32": 0, "f16": 1, "bf16": 2, "f32-refined": 3}
 METRIC = "steps/s at 256³; Jacobi HBM GB/s; learned-vs-plain speedup at equal div L2"
 
@@ -133,7 +135,7 @@ def main():
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_plume256_kernels.json")
     if n == 256 and args.smoother == "rbgs" and os.path.exists(tfile):
-        rec = json.load(open(tfile))["kernels"].get("k_rbgs_split_pass<0, false>")
+        rec = json.load(open(tfile))["kernels"].get(VELOCITY_PASS_KERNEL)
         traffic = rec["traffic_bytes"] if rec else None
 
     # learned warm start vs plain pressure passes at equal post-projection divergence L2

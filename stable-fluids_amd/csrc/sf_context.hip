@@ -644,6 +644,39 @@ std::vector<BatchItem> batch_items(const size_t n_desc, F blocks) {
     return items;
 }
 
+// Smoother work lists split by row parity (the pair-load kernels need even nx): [0] the
+// items of descriptors with odd nx, [1] those with even nx; one launch per non-empty list.
+struct SmoothLists {
+    const BatchItem* items[2] = {nullptr, nullptr};
+    uint32_t n[2]             = {0, 0};
+    int launches() const { return (n[0] > 0) + (n[1] > 0); }
+};
+
+template <class Nx, class F>
+bool smooth_lists(Batch& B, const size_t n_desc, Nx nx_of, F blocks, SmoothLists* out) {
+    for (int parity = 0; parity < 2; ++parity) {
+        std::vector<BatchItem> items;
+        for (size_t d = 0; d < n_desc; ++d) {
+            if (((nx_of(d) & 1) == 0) != (parity == 1)) continue;
+            for (uint32_t b = 0, n = blocks(d); b < n; ++b) items.push_back(BatchItem{static_cast<uint32_t>(d), b});
+        }
+        out->n[parity] = static_cast<uint32_t>(items.size());
+        if (items.empty()) continue;
+        out->items[parity] = B.upload(items);
+        if (!out->items[parity]) return false;
+    }
+    return true;
+}
+
+hipError_t launch_smooth_lists(hipStream_t s, const SmootherKind sm, const SolveKind solve, const bool zero, const SmoothDesc* descs, const SmoothLists& L) {
+    for (int parity = 0; parity < 2; ++parity) {
+        if (L.n[parity] == 0) continue;
+        const hipError_t e = launch_smooth_batched(s, sm, solve, zero, parity == 1, descs, L.items[parity], L.n[parity]);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 #define SF_BTRY(expr)                                           \
     do {                                                        \
         if ((expr) != hipSuccess) return hipErrorUnknown;       \
@@ -665,17 +698,16 @@ hipError_t plan_batch_step(Batch& B) {
 
     const auto elem_items = B.upload(batch_items(N, [&](size_t d) { return elem_block_count(S[d]->cells); }));
     const auto tile_items = B.upload(batch_items(N, [&](size_t d) { return tile_blocks(S[d]->dims); }));
-    const auto sm1_items  = B.upload(batch_items(N, [&](size_t d) { return smooth_blocks(geo[d]); }));
-    const auto sm3_items  = B.upload(batch_items(3 * N, [&](size_t d) { return smooth_blocks(geo[d / 3]); }));
     const uint32_t n_elem = static_cast<uint32_t>(batch_items(N, [&](size_t d) { return elem_block_count(S[d]->cells); }).size());
     const uint32_t n_tile = static_cast<uint32_t>(batch_items(N, [&](size_t d) { return tile_blocks(S[d]->dims); }).size());
-    const uint32_t n_sm1  = static_cast<uint32_t>(batch_items(N, [&](size_t d) { return smooth_blocks(geo[d]); }).size());
-    const uint32_t n_sm3  = static_cast<uint32_t>(batch_items(3 * N, [&](size_t d) { return smooth_blocks(geo[d / 3]); }).size());
-    if (!elem_items || !tile_items || !sm1_items || !sm3_items) return hipErrorOutOfMemory;
+    SmoothLists sm1, sm3;
+    if (!elem_items || !tile_items) return hipErrorOutOfMemory;
+    if (!smooth_lists(B, N, [&](size_t d) { return S[d]->dims.nx; }, [&](size_t d) { return smooth_blocks(geo[d]); }, &sm1)) return hipErrorOutOfMemory;
+    if (!smooth_lists(B, 3 * N, [&](size_t d) { return S[d / 3]->dims.nx; }, [&](size_t d) { return smooth_blocks(geo[d / 3]); }, &sm3)) return hipErrorOutOfMemory;
     hipStream_t st = B.stream;
-    auto add       = [&](std::function<hipError_t()> f) {
+    auto add       = [&](std::function<hipError_t()> f, const int launches = 1) {
         B.plan.push_back(std::move(f));
-        ++B.launches;
+        B.launches += launches;
         return hipSuccess;
     };
 
@@ -719,7 +751,7 @@ hipError_t plan_batch_step(Batch& B) {
         }
         const auto dd = B.upload(d);
         if (!dd) return hipErrorOutOfMemory;
-        SF_BTRY(add([=] { return launch_smooth_batched(st, sm, kSolveDiffusion, false, dd, sm3_items, n_sm3); }));
+        SF_BTRY(add([=] { return launch_smooth_lists(st, sm, kSolveDiffusion, false, dd, sm3); }, sm3.launches()));
     }
     {
         std::vector<DivergenceDesc> d(N);
@@ -743,7 +775,7 @@ hipError_t plan_batch_step(Batch& B) {
         }
         const auto dd = B.upload(d);
         if (!dd) return hipErrorOutOfMemory;
-        SF_BTRY(add([=] { return launch_smooth_batched(st, sm, kSolvePoisson, pass == 1, dd, sm1_items, n_sm1); }));
+        SF_BTRY(add([=] { return launch_smooth_lists(st, sm, kSolvePoisson, pass == 1, dd, sm1); }, sm1.launches()));
     }
     {
         std::vector<ProjectDesc> d(N);
@@ -786,7 +818,7 @@ hipError_t plan_batch_step(Batch& B) {
             }
             const auto dd = B.upload(d);
             if (!dd) return hipErrorOutOfMemory;
-            SF_BTRY(add([=] { return launch_smooth_batched(st, sm, kSolveDiffusion, false, dd, sm1_items, n_sm1); }));
+            SF_BTRY(add([=] { return launch_smooth_lists(st, sm, kSolveDiffusion, false, dd, sm1); }, sm1.launches()));
         }
     }
     return hipSuccess;
@@ -825,7 +857,18 @@ double time_batch_smoother(Batch& B, const int repeats) {
         auto& c = *B.sims[i];
         d[i]    = SmoothDesc{c.flow.rhs, c.flow.rhs, c.flow.p_aux, c.config.cell_size * c.config.cell_size, 1.0f, c.dims, pressure_rules(c.config.flow_boundary), geo[i]};
     }
-    const auto items = batch_items(N, [&](size_t k) { return smooth_blocks(geo[k]); });
+    // even-nx items first, then odd-nx ones: one launch per parity group
+    std::vector<BatchItem> items;
+    uint32_t n_even = 0;
+    for (int parity = 1; parity >= 0; --parity) {
+        std::vector<BatchItem> group;
+        for (size_t k = 0; k < N; ++k) {
+            if (((B.sims[k]->dims.nx & 1) == 0) != (parity == 1)) continue;
+            for (uint32_t b = 0, n = smooth_blocks(geo[k]); b < n; ++b) group.push_back(BatchItem{static_cast<uint32_t>(k), b});
+        }
+        if (parity == 1) n_even = static_cast<uint32_t>(group.size());
+        items.insert(items.end(), group.begin(), group.end());
+    }
     void *dd = nullptr, *di = nullptr;
     double ms = -1.0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -836,10 +879,14 @@ double time_batch_smoother(Batch& B, const int repeats) {
         const SmootherKind sm = B.sims[0]->options.smoother;
         const auto* descs     = static_cast<const SmoothDesc*>(dd);
         const auto* its       = static_cast<const BatchItem*>(di);
-        const uint32_t n      = static_cast<uint32_t>(items.size());
-        bool ok = launch_smooth_batched(B.stream, sm, kSolvePoisson, false, descs, its, n) == hipSuccess;
-        ok      = ok && hipEventRecord(e0, B.stream) == hipSuccess;
-        for (int r = 0; ok && r < repeats; ++r) ok = launch_smooth_batched(B.stream, sm, kSolvePoisson, false, descs, its, n) == hipSuccess;
+        SmoothLists L;
+        L.items[1] = its;
+        L.n[1]     = n_even;
+        L.items[0] = its + n_even;
+        L.n[0]     = static_cast<uint32_t>(items.size()) - n_even;
+        bool ok    = launch_smooth_lists(B.stream, sm, kSolvePoisson, false, descs, L) == hipSuccess;
+        ok         = ok && hipEventRecord(e0, B.stream) == hipSuccess;
+        for (int r = 0; ok && r < repeats; ++r) ok = launch_smooth_lists(B.stream, sm, kSolvePoisson, false, descs, L) == hipSuccess;
         ok       = ok && hipEventRecord(e1, B.stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
         float t  = 0.0f;
         if (ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms = t / repeats;

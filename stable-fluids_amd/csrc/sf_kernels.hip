@@ -193,7 +193,7 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_advect_velocity(float* __res
 }
 __global__ __launch_bounds__(kTileX* kTileY) void k_advect_velocity_batched(const AdvectVelocityDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
     const BatchItem it = items[blockIdx.x];
-    const AdvectVelocityDesc sd    = descs[it.desc];
+    const AdvectVelocityDesc& sd   = descs[it.desc];
     int bx, by, bz;
     tile_block_of(sd.d, it.block, bx, by, bz);
     advect_velocity_body(sd.t[0], sd.t[1], sd.t[2], sd.v[0], sd.v[1], sd.v[2], sd.dt, sd.d, sd.h, sd.bc, bx * kTileX + static_cast<int>(threadIdx.x), by * kTileY + static_cast<int>(threadIdx.y), bz);
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_advect_scalar(float* __restr
 }
 __global__ __launch_bounds__(kTileX* kTileY) void k_advect_scalar_batched(const AdvectScalarDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
     const BatchItem it = items[blockIdx.x];
-    const AdvectScalarDesc sd    = descs[it.desc];
+    const AdvectScalarDesc& sd   = descs[it.desc];
     int bx, by, bz;
     tile_block_of(sd.d, it.block, bx, by, bz);
     advect_scalar_body(sd.out, sd.src, sd.v[0], sd.v[1], sd.v[2], sd.dt, sd.d, sd.h, sd.sbc, sd.fbc, bx * kTileX + static_cast<int>(threadIdx.x), by * kTileY + static_cast<int>(threadIdx.y), bz);
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_divergence(float* __restrict
 }
 __global__ __launch_bounds__(kTileX* kTileY) void k_divergence_batched(const DivergenceDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
     const BatchItem it = items[blockIdx.x];
-    const DivergenceDesc sd    = descs[it.desc];
+    const DivergenceDesc& sd   = descs[it.desc];
     int bx, by, bz;
     tile_block_of(sd.d, it.block, bx, by, bz);
     divergence_body(sd.div, sd.v[0], sd.v[1], sd.v[2], sd.d, sd.h, sd.dbc, bx * kTileX + static_cast<int>(threadIdx.x), by * kTileY + static_cast<int>(threadIdx.y), bz);
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_project(float* __restrict__ 
 }
 __global__ __launch_bounds__(kTileX* kTileY) void k_project_batched(const ProjectDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
     const BatchItem it = items[blockIdx.x];
-    const ProjectDesc sd    = descs[it.desc];
+    const ProjectDesc& sd   = descs[it.desc];
     int bx, by, bz;
     tile_block_of(sd.d, it.block, bx, by, bz);
     project_body(sd.o[0], sd.o[1], sd.o[2], sd.div, sd.mag, sd.v[0], sd.v[1], sd.v[2], sd.p, sd.d, sd.h, sd.pbc, sd.dbc, bx * kTileX + static_cast<int>(threadIdx.x), by * kTileY + static_cast<int>(threadIdx.y), bz);
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_project_batched(const Projec
 // Batched elementwise stages: local block b covers elements [4 * 256 * b, + 1024).
 __global__ __launch_bounds__(kElemBlock) void k_add_force_batched(const AddForceDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
     const BatchItem it = items[blockIdx.x];
-    const AddForceDesc sd = descs[it.desc];
+    const AddForceDesc& sd = descs[it.desc];
     const uint64_t i = (static_cast<uint64_t>(it.block) * kElemBlock + threadIdx.x) * 4;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(kElemBlock) void k_add_force_batched(const AddForce
 
 __global__ __launch_bounds__(kElemBlock) void k_add_scaled_batched(const AddScaledDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
     const BatchItem it = items[blockIdx.x];
-    const AddScaledDesc sd = descs[it.desc];
+    const AddScaledDesc& sd = descs[it.desc];
     const uint64_t i = (static_cast<uint64_t>(it.block) * kElemBlock + threadIdx.x) * 4;
     if (i + 3 < sd.n) {
         float4 a       = *reinterpret_cast<float4*>(sd.dst + i);

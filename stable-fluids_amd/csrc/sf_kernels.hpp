@@ -77,7 +77,8 @@ struct SmoothDesc {
 };
 // RBGS passes need even extents on periodic axes in a batch (no odd-periodic variant).
 bool smooth_batchable(SmootherKind smoother, Dims d, const StencilBC& bc);
-hipError_t launch_smooth_batched(hipStream_t s, SmootherKind smoother, SolveKind solve, bool zero, const SmoothDesc* descs, const BatchItem* items, uint32_t n_items);
+// Every item's descriptor must have even nx when `pairs`, odd nx otherwise.
+hipError_t launch_smooth_batched(hipStream_t s, SmootherKind smoother, SolveKind solve, bool zero, bool pairs, const SmoothDesc* descs, const BatchItem* items, uint32_t n_items);
 
 struct AdvectVelocityDesc {
     float* t[3];

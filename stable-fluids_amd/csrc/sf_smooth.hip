@@ -862,25 +862,21 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const typename St:
     rbgs_split_body<kSolve, kZero, St, kPairs>(x_old, b, x_new, coef, out_scale, d, bc, g, scale_slot, xcd_block());
 }
 
-// Batched passes: workgroup i runs block items[i].block of pass descs[items[i].desc].
-template <int kSolve, bool kZero>
+// Batched passes: workgroup i runs block items[i].block of pass descs[items[i].desc]. The
+// descriptor is read in place (a local copy of it would be indexed into scratch memory);
+// all items of one launch share the row parity kPairs.
+template <int kSolve, bool kZero, bool kPairs>
 __global__ __launch_bounds__(kThreads) void k_rbgs_split_batched(const SmoothDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
-    const BatchItem it = items[blockIdx.x];
-    const SmoothDesc sd = descs[it.desc];
-    if ((sd.d.nx & 1) == 0)
-        rbgs_split_body<kSolve, kZero, StoreF32, true>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
-    else
-        rbgs_split_body<kSolve, kZero, StoreF32, false>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
+    const BatchItem it   = items[blockIdx.x];
+    const SmoothDesc& sd = descs[it.desc];
+    rbgs_split_body<kSolve, kZero, StoreF32, kPairs>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
 }
 
-template <int kSolve, bool kZero>
+template <int kSolve, bool kZero, bool kPairs>
 __global__ __launch_bounds__(kThreads) void k_jacobi_batched(const SmoothDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
-    const BatchItem it = items[blockIdx.x];
-    const SmoothDesc sd = descs[it.desc];
-    if ((sd.d.nx & 1) == 0)
-        jacobi_body<kSolve, kZero, StoreF32, true>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
-    else
-        jacobi_body<kSolve, kZero, StoreF32, false>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
+    const BatchItem it   = items[blockIdx.x];
+    const SmoothDesc& sd = descs[it.desc];
+    jacobi_body<kSolve, kZero, StoreF32, kPairs>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
 }
 
 bool has_odd_periodic(const Dims d, const StencilBC& bc) {
@@ -909,9 +905,13 @@ uint32_t smooth_blocks(const TileGeom& g) { return static_cast<uint32_t>(g.tiles
 
 bool smooth_batchable(const SmootherKind smoother, const Dims d, const StencilBC& bc) { return smoother == kSmoothJacobi || !has_odd_periodic(d, bc); }
 
-hipError_t launch_smooth_batched(hipStream_t s, const SmootherKind smoother, const SolveKind solve, const bool zero, const SmoothDesc* descs, const BatchItem* items, const uint32_t n_items) {
+hipError_t launch_smooth_batched(hipStream_t s, const SmootherKind smoother, const SolveKind solve, const bool zero, const bool pairs, const SmoothDesc* descs, const BatchItem* items, const uint32_t n_items) {
     if (n_items == 0) return hipSuccess;
-#define SF_B(KERNEL, SO, Z) KERNEL<SO, Z><<<n_items, kThreads, 0, s>>>(descs, items)
+#define SF_B(KERNEL, SO, Z)                                                          \
+    do {                                                                             \
+        if (pairs) KERNEL<SO, Z, true><<<n_items, kThreads, 0, s>>>(descs, items);   \
+        else KERNEL<SO, Z, false><<<n_items, kThreads, 0, s>>>(descs, items);        \
+    } while (0)
     if (smoother == kSmoothRBGS) {
         if (solve == kSolvePoisson) {
             if (zero) SF_B(k_rbgs_split_batched, kSolvePoisson, true);
