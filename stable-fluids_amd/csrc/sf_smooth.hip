@@ -27,6 +27,7 @@
 // odd periodic axis, where a pair can hold two cells of one colour, takes a second relax.
 #include <type_traits>
 
+#include "sf_buffer.hpp"
 #include "sf_kernels.hpp"
 #include "sf_storage.hpp"
 
@@ -396,9 +397,180 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_pass(const float* __restrict_
     }
 }
 
-template <int kSolve, bool kZero, class St, bool kPairs>
+// Jacobi pass for even nx. Each wave fetches whole rows of the old region (lane l holds
+// the pair x0 + 2l, x0 + 2l + 1), so ghost rules are wave-uniform per row; lanes 0..3
+// of every wave fetch the two halo columns (x0 - 1 and x0 + 128) of two rows. Only the
+// cells the 7-point stencil reads are filled: rows y0 - 1 .. y0 + 16 at the 64 interior
+// pairs, halo columns on rows y0 .. y0 + 15; corner cells stay unset.
+//
+// Planes are fetched two ahead into alternating register sets and staged in a ring of
+// four LDS slots, so one barrier per plane orders the ring (a fast wave writes slot
+// k + 2 while a slow one still reads k - 1 .. k + 1). All loads and stores go through
+// plane buffer resources with out-of-range offsets for idle lanes and 0-byte resources
+// for idle planes: no memory instruction sits under a branch, the compiler's wait counts
+// stay exact and a plane's loads are only waited for when it is staged two planes later.
+// Ghost rules and the right-hand-side scale are applied at staging / relaxing time.
+template <int kSolve, bool kZero, class St>
+__device__ __forceinline__ void jacobi_rows_body(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC& bc, const TileGeom& g, const float* __restrict__ scale_slot, const int w) {
+    using T                = typename St::T;
+    constexpr int kWaves   = kThreads / 64;
+    constexpr int kRowsPer = (kHY + kWaves - 1) / kWaves; // 3
+    __shared__ float2 s_old[4][kHY * kHPX];
+    const float inv_s = rhs_scale<St>(scale_slot);
+
+    int tx, ty, cz;
+    tile_of(g, w, tx, ty, cz);
+    const int tid  = static_cast<int>(threadIdx.x);
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int x0 = tx * kTX, y0 = ty * kTY;
+    const int z0 = cz * g.zc;
+    const int z1 = min(z0 + g.zc, d.nz);
+    const bool px = bc.periodic & 1, py = (bc.periodic >> 1) & 1, pz = (bc.periodic >> 2) & 1;
+    const uint32_t plane = static_cast<uint32_t>(d.nx) * static_cast<uint32_t>(d.ny);
+    const bool full_x    = x0 + kTX <= d.nx;
+
+    // this lane's pair of a row; in a tile that crosses x = nx, the pair at nx carries the
+    // x+ neighbour of the last cell in .x (ghost or wrapped cell 0), later pairs are unused
+    const int xa    = x0 + 2 * lane;
+    int lane_src    = xa;
+    bool lane_ghost = false;
+    if (!full_x && xa >= d.nx) {
+        lane_src   = (xa == d.nx && px) ? 0 : d.nx - 2;
+        lane_ghost = xa == d.nx && !px;
+    }
+    int row_off[kRowsPer]; // byte offsets in a plane
+    bool row_ghost[kRowsPer];
+    FaceRule row_rule[kRowsPer];
+#pragma unroll
+    for (int j = 0; j < kRowsPer; ++j) {
+        const int r  = wave + kWaves * j;
+        const int gy = y0 - 1 + r;
+        row_ghost[j] = !py && (gy < 0 || gy >= d.ny);
+        row_rule[j]  = gy < 0 ? bc.face[2] : bc.face[3];
+        row_off[j]   = r < kHY ? static_cast<int>(sizeof(T)) * (fix_coord(gy, d.ny, py) * d.nx + lane_src) : kOutOfRange;
+    }
+    // halo columns: lane bit 0 = side (0: x0 - 1, 1: x0 + 128), bit 1 = row of the pair
+    // of tile rows 2 wave, 2 wave + 1; columns beyond a partial tile are unused
+    const int h_side = lane & 1;
+    const int h_row  = 2 * wave + ((lane >> 1) & 1);
+    int h_off        = kOutOfRange;
+    bool h_ghost     = false;
+    {
+        const int gy = min(y0 + h_row, d.ny - 1);
+        int gx       = h_side ? x0 + kTX : x0 - 1;
+        if (gx < 0) {
+            h_ghost = !px;
+            gx      = px ? d.nx - 1 : 0;
+        } else if (gx >= d.nx) {
+            h_ghost = gx == d.nx && !px;
+            gx      = (gx == d.nx && px) ? 0 : d.nx - 1;
+        }
+        if (lane < 4) h_off = static_cast<int>(sizeof(T)) * (gy * d.nx + gx);
+    }
+    const FaceRule h_rule = h_side ? bc.face[1] : bc.face[0];
+    const int h_lds       = 2 * ((1 + h_row) * kHPX + (h_side ? kHPX - 1 : 0)) + (h_side ? 0 : 1);
+
+    int b_off[kOutPer], x_off[kOutPer];
+#pragma unroll
+    for (int i = 0; i < kOutPer; ++i) {
+        const int y    = y0 + wave + kWaves * i;
+        const bool ok  = xa < d.nx && y < d.ny;
+        b_off[i]       = ok ? static_cast<int>(sizeof(float)) * (y * d.nx + xa) : kOutOfRange;
+        x_off[i]       = ok ? static_cast<int>(sizeof(T)) * (y * d.nx + xa) : kOutOfRange;
+    }
+    const uint32_t x_bytes = plane * static_cast<uint32_t>(sizeof(T));
+    const uint32_t b_bytes = plane * static_cast<uint32_t>(sizeof(float));
+
+    struct Regs {
+        float2 row[kRowsPer];
+        float h;
+    };
+    Regs ra, rb;
+    float2 ba[kOutPer], bb[kOutPer];
+    // raw values of plane k (nothing when k is past the last plane the chunk reads)
+    auto fetch_old = [&](const int k, Regs& r) {
+        if constexpr (kZero) return;
+        const Rsrc src = plane_rsrc(x_old + static_cast<uint64_t>(fix_coord(k, d.nz, pz)) * plane, k <= z1 ? x_bytes : 0u);
+#pragma unroll
+        for (int j = 0; j < kRowsPer; ++j) r.row[j] = buf_load2<St>(src, row_off[j]);
+        r.h = buf_load1<St>(src, h_off);
+    };
+    auto stash_old = [&](const int k, const Regs& r) {
+        const bool below = !pz && k < 0, above = !pz && k >= d.nz;
+        const FaceRule zr = below ? bc.face[4] : bc.face[5];
+        float2* s         = s_old[k & 3];
+#pragma unroll
+        for (int j = 0; j < kRowsPer; ++j) {
+            if (wave + kWaves * j >= kHY) continue;
+            float2 v = kZero ? make_float2(0.0f, 0.0f) : r.row[j];
+            if (below || above) {
+                v = make_float2(apply_ghost(zr, v.x), apply_ghost(zr, v.y));
+            } else if (row_ghost[j]) {
+                v = make_float2(apply_ghost(row_rule[j], v.x), apply_ghost(row_rule[j], v.y));
+            } else if (!full_x && lane_ghost) {
+                v.x = apply_ghost(bc.face[1], v.y);
+            }
+            s[(wave + kWaves * j) * kHPX + 1 + lane] = v;
+        }
+        if (lane < 4) {
+            float v = kZero ? 0.0f : r.h;
+            if (below || above) v = apply_ghost(zr, v);
+            else if (h_ghost) v = apply_ghost(h_rule, v);
+            reinterpret_cast<float*>(s)[h_lds] = v;
+        }
+    };
+    // raw b of plane k (nothing past the chunk)
+    auto fetch_b = [&](const int k, float2 (&r_b)[kOutPer]) {
+        const Rsrc src = plane_rsrc(b + static_cast<uint64_t>(k) * plane, k < z1 ? b_bytes : 0u);
+#pragma unroll
+        for (int i = 0; i < kOutPer; ++i) r_b[i] = buf_load2<StoreF32>(src, b_off[i]);
+    };
+    // relax plane k (stores dropped past the chunk)
+    auto compute = [&](const int k, const float2 (&bv)[kOutPer]) {
+        const float2* so = s_old[k & 3];
+        const float2* sm = s_old[(k - 1) & 3];
+        const float2* sp = s_old[(k + 1) & 3];
+        const Rsrc dst   = plane_rsrc(x_new + static_cast<uint64_t>(k) * plane, k < z1 ? x_bytes : 0u);
+#pragma unroll
+        for (int i = 0; i < kOutPer; ++i) {
+            const int c    = (wave + kWaves * i + 1) * kHPX + lane + 1;
+            const float2 L = so[c - 1], C = so[c], R = so[c + 1], YM = so[c - kHPX], YP = so[c + kHPX], ZM = sm[c], ZP = sp[c];
+            const float v0 = relax_elem<kSolve>(0, L, C, R, YM, YP, ZM, ZP, bv[i].x * inv_s, coef) * out_scale;
+            const float v1 = relax_elem<kSolve>(1, L, C, R, YM, YP, ZM, ZP, bv[i].y * inv_s, coef) * out_scale;
+            buf_store2<St>(dst, x_off[i], v0, v1);
+        }
+    };
+    // plane k: stage k + 1 (fetched two steps ago into r) and refill r with k + 3; after the
+    // barrier relax k with b of k (fetched two steps ago into bk) and refill bk with k + 2
+    auto plane_step = [&](const int k, Regs& r, float2 (&bk)[kOutPer]) {
+        stash_old(k + 1, r);
+        fetch_old(k + 3, r);
+        __syncthreads();
+        compute(k, bk);
+        fetch_b(k + 2, bk);
+    };
+
+    fetch_old(z0 - 1, ra);
+    stash_old(z0 - 1, ra);
+    fetch_old(z0, ra);
+    stash_old(z0, ra);
+    fetch_old(z0 + 1, rb); // plane z0 + 1 + 2j in rb, z0 + 2j in ra (j >= 1)
+    fetch_old(z0 + 2, ra);
+    fetch_b(z0, ba);       // b of z0 + 2j in ba, of z0 + 1 + 2j in bb
+    fetch_b(z0 + 1, bb);
+    // an odd chunk ends with one idle step (its loads and stores are switched off)
+    for (int k = z0; k < z1; k += 2) {
+        plane_step(k, rb, ba);
+        plane_step(k + 1, ra, bb);
+    }
+}
+
+// Jacobi pass for odd nx: the old region is fetched element by element (kJOldPer per
+// lane) with per-element ghost codes; same ring and prefetch as jacobi_rows_body.
+template <int kSolve, bool kZero, class St>
 __device__ __forceinline__ void jacobi_body(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC& bc, const TileGeom& g, const float* __restrict__ scale_slot, const int w) {
-    __shared__ float2 s_old[3][kHPX * kHY];
+    __shared__ float2 s_old[4][kHPX * kHY];
     const float inv_s = rhs_scale<St>(scale_slot);
 
     int tx, ty, cz;
@@ -410,44 +582,16 @@ __device__ __forceinline__ void jacobi_body(const typename St::T* __restrict__ x
     const bool px  = bc.periodic & 1, py = (bc.periodic >> 1) & 1, pz = (bc.periodic >> 2) & 1;
     const uint64_t plane = static_cast<uint64_t>(d.nx) * static_cast<uint64_t>(d.ny);
 
-    // even rows: the old region is loaded as aligned pairs (kHPer per lane); odd rows
-    // element by element (kJOldPer per lane)
-    constexpr bool pairs = kPairs; // caller checked (d.nx & 1) == 0
     int o_off[kJOldPer], o_gc[kJOldPer];
-    int p_off[kHPer], p_sel[kHPer], p_gc[kHPer];
-    if constexpr (pairs) {
 #pragma unroll
-        for (int i = 0; i < kHPer; ++i) {
-            const int e = tid + i * kThreads;
-            p_off[i]    = -1;
-            p_sel[i]    = 0;
-            p_gc[i]     = 0;
-            if (e < kHP) {
-                const int gx = x0 - 2 + 2 * (e % kHPX), gy = y0 - 1 + e / kHPX;
-                const int row = fix_coord(gy, d.ny, py) * d.nx;
-                if (!px && gx < 0) {
-                    p_off[i] = row;
-                    p_sel[i] = 1;
-                } else if (!px && gx >= d.nx) {
-                    p_off[i] = row + d.nx - 2;
-                    p_sel[i] = 2;
-                } else {
-                    p_off[i] = row + fix_coord(gx, d.nx, px);
-                }
-                p_gc[i] = ghost_code(gx, gy, d, px, py);
-            }
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < kJOldPer; ++i) {
-            const int e = tid + i * kThreads;
-            o_off[i]    = -1;
-            o_gc[i]     = 0;
-            if (e < kJOldF) {
-                const int gx = x0 - 2 + e % (2 * kHPX), gy = y0 - 1 + e / (2 * kHPX);
-                o_off[i]     = fix_coord(gy, d.ny, py) * d.nx + fix_coord(gx, d.nx, px);
-                o_gc[i]      = ghost_code(gx, gy, d, px, py);
-            }
+    for (int i = 0; i < kJOldPer; ++i) {
+        const int e = tid + i * kThreads;
+        o_off[i]    = -1;
+        o_gc[i]     = 0;
+        if (e < kJOldF) {
+            const int gx = x0 - 2 + e % (2 * kHPX), gy = y0 - 1 + e / (2 * kHPX);
+            o_off[i]     = fix_coord(gy, d.ny, py) * d.nx + fix_coord(gx, d.nx, px);
+            o_gc[i]      = ghost_code(gx, gy, d, px, py);
         }
     }
     const int fp = tid & (kPX - 1);
@@ -458,11 +602,12 @@ __device__ __forceinline__ void jacobi_body(const typename St::T* __restrict__ x
         const int xa = x0 + 2 * fp, y = y0 + fy;
         f_info[i]    = (xa < d.nx && y < d.ny ? 1 : 0) | (xa + 1 < d.nx && y < d.ny ? 2 : 0);
     }
-    float r_old[kJOldPer];
-    float2 r_pair[kHPer];
-    float2 r_b[kOutPer];
-    auto fetch_b = [&](const int k) {
-        if (k < 0 || k >= d.nz) return;
+    struct OldRegs {
+        float f[kJOldPer];
+    };
+    OldRegs ra, rb;
+    float2 ba[kOutPer], bb[kOutPer];
+    auto fetch_b = [&](const int k, float2 (&r_b)[kOutPer]) {
         const uint64_t base = static_cast<uint64_t>(k) * plane;
 #pragma unroll
         for (int i = 0; i < kOutPer; ++i) {
@@ -477,45 +622,17 @@ __device__ __forceinline__ void jacobi_body(const typename St::T* __restrict__ x
             }
         }
     };
-    auto stash_old = [&](const int k) {
-        if constexpr (pairs) {
-            float2* s = s_old[mod3(k)];
+    auto stash_old = [&](const int k, const OldRegs& r) {
+        float* s = reinterpret_cast<float*>(s_old[k & 3]);
 #pragma unroll
-            for (int i = 0; i < kHPer; ++i)
-                if (tid + i * kThreads < kHP) s[tid + i * kThreads] = r_pair[i];
-        } else {
-            float* s = reinterpret_cast<float*>(s_old[mod3(k)]);
-#pragma unroll
-            for (int i = 0; i < kJOldPer; ++i)
-                if (tid + i * kThreads < kJOldF) s[tid + i * kThreads] = r_old[i];
-        }
+        for (int i = 0; i < kJOldPer; ++i)
+            if (tid + i * kThreads < kJOldF) s[tid + i * kThreads] = r.f[i];
     };
-    auto fetch_old = [&](const int k) {
-        if constexpr (pairs)
-            fetch_pairs<kHPer, kZero, St>(r_pair, x_old, p_off, p_sel, p_gc, k, d, pz, plane, bc);
-        else
-            fetch_region<kJOldPer, kZero, St>(r_old, x_old, o_off, o_gc, k, d, pz, plane, bc);
-    };
-
-    fetch_old(z0 - 1);
-    stash_old(z0 - 1);
-    fetch_old(z0);
-    stash_old(z0);
-    fetch_old(z0 + 1);
-    fetch_b(z0);
-    for (int k = z0; k < z1; ++k) {
-        stash_old(k + 1);
-        float2 bv[kOutPer];
-#pragma unroll
-        for (int i = 0; i < kOutPer; ++i) bv[i] = r_b[i];
-        if (k + 1 < z1) {
-            fetch_old(k + 2);
-            fetch_b(k + 1);
-        }
-        __syncthreads();
-        const float2* so = s_old[mod3(k)];
-        const float2* sm = s_old[mod3(k - 1)];
-        const float2* sp = s_old[mod3(k + 1)];
+    auto fetch_old = [&](const int k, OldRegs& r) { fetch_region<kJOldPer, kZero, St>(r.f, x_old, o_off, o_gc, k, d, pz, plane, bc); };
+    auto compute = [&](const int k, const float2 (&bv)[kOutPer]) {
+        const float2* so    = s_old[k & 3];
+        const float2* sm    = s_old[(k - 1) & 3];
+        const float2* sp    = s_old[(k + 1) & 3];
         const uint64_t base = static_cast<uint64_t>(k) * plane;
 #pragma unroll
         for (int i = 0; i < kOutPer; ++i) {
@@ -534,7 +651,31 @@ __device__ __forceinline__ void jacobi_body(const typename St::T* __restrict__ x
                 if (info & 2) dst[1] = St::store(relax_elem<kSolve>(1, L, C, R, YM, YP, ZM, ZP, bv[i].y, coef) * out_scale);
             }
         }
+    };
+    // plane k: stage k+1 (fetched two planes ago into `r`), refill `r` with k+3 and `bn`
+    // with b of k+2, then relax plane k with b of k (held in `bn` until now)
+    auto plane_step = [&](const int k, OldRegs& r, float2 (&bn)[kOutPer]) {
+        stash_old(k + 1, r);
+        float2 bv[kOutPer];
+#pragma unroll
+        for (int i = 0; i < kOutPer; ++i) bv[i] = bn[i];
+        if (k + 3 <= z1) fetch_old(k + 3, r);
+        if (k + 2 < z1) fetch_b(k + 2, bn);
         __syncthreads();
+        compute(k, bv);
+    };
+
+    fetch_old(z0 - 1, ra);
+    stash_old(z0 - 1, ra);
+    fetch_old(z0, ra);
+    stash_old(z0, ra);
+    fetch_old(z0 + 1, rb); // plane z0 + 1 + 2j in rb, z0 + 2j in ra (j >= 1)
+    if (z0 + 2 <= z1) fetch_old(z0 + 2, ra);
+    fetch_b(z0, ba);       // b of z0 + 2j in ba, of z0 + 1 + 2j in bb
+    if (z0 + 1 < z1) fetch_b(z0 + 1, bb);
+    for (int k = z0; k < z1; k += 2) {
+        plane_step(k, rb, ba);
+        if (k + 1 < z1) plane_step(k + 1, ra, bb);
     }
 }
 
@@ -853,7 +994,10 @@ __device__ __forceinline__ void rbgs_split_body(const typename St::T* __restrict
 template <int kSolve, bool kZero, class St, bool kPairs>
 __global__ __launch_bounds__(kThreads) void k_jacobi_pass(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g, const float* __restrict__ scale_slot, const int* __restrict__ skip) {
     if (skip != nullptr && *skip != 0) return;
-    jacobi_body<kSolve, kZero, St, kPairs>(x_old, b, x_new, coef, out_scale, d, bc, g, scale_slot, xcd_block());
+    if constexpr (kPairs)
+        jacobi_rows_body<kSolve, kZero, St>(x_old, b, x_new, coef, out_scale, d, bc, g, scale_slot, xcd_block());
+    else
+        jacobi_body<kSolve, kZero, St>(x_old, b, x_new, coef, out_scale, d, bc, g, scale_slot, xcd_block());
 }
 
 template <int kSolve, bool kZero, class St, bool kPairs>
@@ -876,7 +1020,10 @@ template <int kSolve, bool kZero, bool kPairs>
 __global__ __launch_bounds__(kThreads) void k_jacobi_batched(const SmoothDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
     const BatchItem it   = items[blockIdx.x];
     const SmoothDesc& sd = descs[it.desc];
-    jacobi_body<kSolve, kZero, StoreF32, kPairs>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
+    if constexpr (kPairs)
+        jacobi_rows_body<kSolve, kZero, StoreF32>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
+    else
+        jacobi_body<kSolve, kZero, StoreF32>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
 }
 
 bool has_odd_periodic(const Dims d, const StencilBC& bc) {

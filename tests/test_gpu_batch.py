@@ -85,13 +85,15 @@ def test_batch_equals_each_context_alone(smoother):
     assert_same(got, ref, "batch vs alone")
 
 
-def test_tiling_never_changes_results(tmp_path):
-    base = make()
+@pytest.mark.parametrize("smoother", [abi.SMOOTHER_RBGS, abi.SMOOTHER_JACOBI])
+def test_tiling_never_changes_results(tmp_path, smoother):
+    opts = sf.ext_options(smoother=smoother)
+    base = make(opts)
     with sf.Batch([ctx for _, ctx in base]) as b:
         b.step(4)
         ref = fields(base)
     destroy(base)
-    tuned = make()
+    tuned = make(opts)
     cache = str(tmp_path / "tiles.txt")
     with sf.Batch([ctx for _, ctx in tuned]) as b:
         for i, zc in enumerate([2, 3, 5, 1, 9, 20]):
@@ -108,7 +110,7 @@ def test_tiling_never_changes_results(tmp_path):
     assert len([l for l in lines if l.strip()]) == len(SIMS)
     print("tuned plans", [(p["z_chunk"], p["chunks"]) for p in plans], flush=True)
     # a second batch takes every shape from the cache
-    again = make()
+    again = make(opts)
     with sf.Batch([ctx for _, ctx in again]) as b:
         assert b.autotune(cache, repeats=3) == 0
         assert [b.info(i)["z_chunk"] for i in range(len(SIMS))] == [p["z_chunk"] for p in plans]

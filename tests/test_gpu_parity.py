@@ -130,6 +130,43 @@ def test_jacobi_smoother_bitexact():
     assert_fp64(gpu1, run_oracle(desc, smooth.seed_oracle, 1, 64, smoother=sf_oracle.JACOBI), "jacobi smooth inputs 1 step vs fp64")
 
 
+def walls_desc(n=(12, 10, 8), d_iters=7, p_iters=9):
+    """No-slip x- wall with a moving lid (reflect ghosts), outflow y+ and z+ (constant
+    pressure ghosts), free-slip elsewhere; fixed-value and zero-flux scalar faces."""
+    nx, ny, nz = n
+    wall_v = [(0.0, 0.2, -0.1), (0, 0, 0), (0, 0, 0), (0, 0, 0), (0.05, -0.2, 0.0), (0, 0, 0)]
+    flow = abi.flow_boundary([abi.FLOW_NO_SLIP, abi.FLOW_FREE_SLIP, abi.FLOW_FREE_SLIP, abi.FLOW_OUTFLOW,
+                              abi.FLOW_NO_SLIP, abi.FLOW_OUTFLOW], velocity=wall_v, pressure=[0, 0, 0, 0.5, 0, -0.25])
+    s1 = abi.scalar_boundary([abi.SCALAR_FIXED, abi.SCALAR_ZERO_FLUX, abi.SCALAR_ZERO_FLUX, abi.SCALAR_FIXED,
+                              abi.SCALAR_FIXED, abi.SCALAR_ZERO_FLUX], values=[1.0, 0, 0, -0.5, 0.25, 0])
+    scalars = [abi.scalar_desc(1e-3, 0.2, 0.25, s1)]
+    vectors = [abi.force_desc(0.1, 0.0, -0.2)]
+    return abi.Desc(nx, ny, nz, 1.0 / 16, 0.05, 2e-3, d_iters, p_iters, flow, scalars=scalars, vectors=vectors)
+
+
+# smoother tile shapes: 128 x 16 cells per tile, z chunks of the default plan
+TILE_CASES = {
+    "periodic_one_full_tile": (mixed_desc, (128, 6, 3)),     # halo columns wrap
+    "periodic_partial_tile": (mixed_desc, (130, 3, 4)),      # pair at nx carries the wrapped cell
+    "walls_two_full_tiles": (walls_desc, (256, 18, 5)),      # wall halo columns, partial y tile
+    "walls_partial_tile": (walls_desc, (130, 3, 4)),         # pair at nx carries the x+ ghost
+    "walls_odd_nx": (walls_desc, (9, 7, 6)),
+    "walls_slab_nz1": (walls_desc, (16, 12, 1)),             # one-plane chunks
+}
+
+
+@pytest.mark.parametrize("smoother", ["rbgs", "jacobi"])
+@pytest.mark.parametrize("case", sorted(TILE_CASES))
+def test_tile_shapes_bitexact(case, smoother):
+    make, n = TILE_CASES[case]
+    desc = make(n=n)
+    inputs = Inputs(desc, seed=zlib.crc32(case.encode()) % 1000, velocity_scale=0.5)
+    kind = abi.SMOOTHER_JACOBI if smoother == "jacobi" else abi.SMOOTHER_RBGS
+    ref_kind = sf_oracle.JACOBI if smoother == "jacobi" else sf_oracle.RBGS
+    gpu = run_gpu(desc, inputs.seed_gpu, 3, options=sf.ext_options(smoother=kind))
+    assert_bitexact(gpu, run_oracle(desc, inputs.seed_oracle, 3, 32, smoother=ref_kind), f"{case} {smoother} 3 steps vs fp32")
+
+
 def test_plume256_noise_one_step_bitexact():
     """The 256^3 throughput workload (seeded divergent initial velocity, D=40, P=80), one step."""
     scene = scenes.plume_scene(256, diffuse_iterations=40, pressure_iterations=80, noise_seed=0)
