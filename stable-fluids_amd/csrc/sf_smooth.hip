@@ -78,14 +78,101 @@ __device__ __forceinline__ float div6(const float x) {
     return q;
 }
 
-// relax_value with the Poisson division done by div6 (bit-identical)
+// Division by the diffusion denominator den = 1 + 6 coef, done the same way: x * RN(1/den)
+// plus one FMA correction. By Markstein's theorem the corrected quotient is the IEEE one
+// whenever the reciprocal is correctly rounded and nothing under- or overflows; an
+// exhaustive check over every finite float x for eleven divisors in [1, 2^20] (among them
+// all-ones mantissas) finds no difference for 2^-100 <= |x| <= 2^100. Other x, and
+// divisors outside [1, 2^20], take the division instruction sequence.
+struct DiffusionDiv {
+    float den, inv;
+    bool fast;
+};
+
+__device__ __forceinline__ DiffusionDiv diffusion_div(const float coef) {
+    const float den = 1.0f + 6.0f * coef;
+    return DiffusionDiv{den, 1.0f / den, den >= 1.0f && den <= 0x1p20f};
+}
+
+__device__ __forceinline__ float div_by(const DiffusionDiv& dv, const float x) {
+    const float q1 = x * dv.inv;
+    const float e  = __builtin_fmaf(-q1, dv.den, x);
+    float q        = __builtin_fmaf(e, dv.inv, q1);
+    const float ax = __builtin_fabsf(x);
+    if (__builtin_expect(!(dv.fast && ax >= 0x1p-100f && ax <= 0x1p100f), 0)) q = x / dv.den;
+    return q;
+}
+
+// relax_value with both divisions done by the corrected reciprocal (bit-identical); the
+// divisor set-up is loop-invariant and hoisted out of the plane loops
 template <int kSolve>
 __device__ __forceinline__ float relax_fast(const float bv, const float sum, const float coef) {
     if constexpr (kSolve == kSolvePoisson)
         return div6(sum - coef * bv);
     else
-        return (bv + coef * sum) / (1.0f + 6.0f * coef);
+        return div_by(diffusion_div(coef), bv + coef * sum);
 }
+
+// A ghost rule in the form g(v) = k ? c : fma(a, v, c): copy (1, -0), reflect (-1, value),
+// const (value). fma(1, v, -0) = v and fma(-1, v, value) = value - v exactly, so this is
+// apply_ghost bit for bit; (1, -0) is also the identity for cells that are not ghosts.
+struct LinGhost {
+    float a, c;
+    bool k;
+};
+
+__device__ __forceinline__ LinGhost lin_identity() { return LinGhost{1.0f, -0.0f, false}; }
+
+__device__ __forceinline__ LinGhost lin_ghost(const FaceRule f) {
+    if (f.mode == kGhostCopy) return lin_identity();
+    if (f.mode == kGhostReflect) return LinGhost{-1.0f, f.value, false};
+    return LinGhost{0.0f, f.value, true};
+}
+
+__device__ __forceinline__ float lin_apply(const LinGhost g, const float v) { return g.k ? g.c : __builtin_fmaf(g.a, v, g.c); }
+
+// Relaxation quotients of a stage with the range guard of div6 / div_by deferred: each
+// quotient is taken by the corrected reciprocal (its sign copied from the numerator, which
+// makes zeros exact), the numerator's |x| bit pattern is folded into a running max and
+// min, and one test per stage sends the rare stage with an out-of-range numerator back
+// through the exact quotient. Results are those of relax_fast bit for bit.
+template <int kSolve>
+struct StageDiv {
+    // |x| in [2^-110, FLT_MAX] (x / 6) or [2^-100, 2^100] (x / den), or x = 0
+    static constexpr uint32_t kLo = kSolve == kSolvePoisson ? 0x08800000u : 0x0d800000u;
+    static constexpr uint32_t kHi = kSolve == kSolvePoisson ? 0x7f7fffffu : 0x71800000u;
+    DiffusionDiv dv;
+    uint32_t hi = 0u, lo = 0xffffffffu;
+    __device__ __forceinline__ explicit StageDiv(const float coef) : dv(diffusion_div(coef)) {}
+    __device__ __forceinline__ float num(const float bv, const float sum, const float coef) const {
+        if constexpr (kSolve == kSolvePoisson) return sum - coef * bv;
+        else return bv + coef * sum;
+    }
+    __device__ __forceinline__ float quot(const float x) {
+        const uint32_t bits = __float_as_uint(x) & 0x7fffffffu;
+        hi                  = max(hi, bits);
+        lo                  = min(lo, bits - 1u);
+        float q;
+        if constexpr (kSolve == kSolvePoisson) {
+            constexpr float r = 0x1.555556p-3f;
+            const float q1    = x * r;
+            q                 = __builtin_fmaf(__builtin_fmaf(-q1, 6.0f, x), r, q1);
+        } else {
+            const float q1 = x * dv.inv;
+            q              = __builtin_fmaf(__builtin_fmaf(-q1, dv.den, x), dv.inv, q1);
+        }
+        return __builtin_copysignf(q, x);
+    }
+    // true when some quotient of this lane needs the exact path
+    __device__ __forceinline__ bool bad() const {
+        const bool fast = kSolve == kSolvePoisson || dv.fast;
+        return !fast || hi > kHi || lo < kLo - 1u;
+    }
+    __device__ __forceinline__ float exact(const float x) const {
+        if constexpr (kSolve == kSolvePoisson) return x / 6.0f;
+        else return x / dv.den;
+    }
+};
 
 // Relax element j of the pair C, whose row neighbours are the pairs L and R, from the
 // six neighbours in the reference's order x-, x+, y-, y+, z-, z+.
@@ -702,6 +789,321 @@ __device__ __forceinline__ int wrap_near(const int k, const int n) {
     return k < 0 ? k + n : (k >= n ? k - n : k);
 }
 
+// RBGS split pass for even nx. Each wave owns two tile rows (y0 + wave, y0 + wave + 8):
+// lane l holds the pair x0 + 2l (LDS pair i = l + 2) of those rows in the old region, the
+// H stage and the final stage. The rest of the old region (halo rows 0, 1, 18, 19 and the
+// four halo pairs of every row) and of the H region (halo rows 1, 18 and the halo pairs
+// 1, 66) is spread evenly as one extra slot per lane, with per-lane ghost codes, so every
+// wave does the same work between barriers. The red cell of a pair is the even or the odd
+// one by the parity of y + z (x of a pair start is even), so on the tile rows every LDS
+// index is a row-uniform base plus the lane. Ghost rules run as one FMA (LinGhost) and
+// only where a tile touches a ghost; the right-hand-side scale is applied at use. Loads
+// and stores go through plane buffer resources (out-of-range offsets for idle lanes,
+// 0-byte resources for idle planes) and planes are fetched two ahead into alternating
+// register sets. Two barriers per plane: after staging old(k + 1) and after H(k); the
+// next plane's first barrier already orders the reuse of the H ring, the second one that
+// of the old ring.
+template <int kSolve, bool kZero, class St>
+__device__ __forceinline__ void rbgs_rows_body(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC& bc, const TileGeom& g, const float* __restrict__ scale_slot, const int w) {
+    using T              = typename St::T;
+    constexpr int kT     = static_cast<int>(sizeof(T));
+    constexpr int kWaves = kThreads / 64;
+    constexpr int kRowsW = kTY / kWaves;                                    // tile rows per wave
+    constexpr int kOMisc = (4 * kPX + 4 * kR) / kWaves;                     // extra old pairs per wave
+    constexpr int kHAll  = 2 * (kPX + 2) + 2 * kTY;                         // extra H pairs
+    constexpr int kHMisc = (kHAll + kWaves - 1) / kWaves;                   // per wave
+    static_assert(kRowsW == kOutPer && kOMisc * kWaves == 4 * kPX + 4 * kR && kOMisc <= 64 && kHMisc <= 64, "tile split");
+    __shared__ float s_old[3][kOldPlane];
+    __shared__ float s_h[3][kHPlane];
+    const float inv_s = rhs_scale<St>(scale_slot);
+
+    int tx, ty, cz;
+    tile_of(g, w, tx, ty, cz);
+    const int tid  = static_cast<int>(threadIdx.x);
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int x0 = tx * kTX, y0 = ty * kTY;
+    const int xs = x0 - 4, ys = y0 - 2;
+    const int z0 = cz * g.zc;
+    const int z1 = min(z0 + g.zc, d.nz);
+    const bool px = bc.periodic & 1, py = (bc.periodic >> 1) & 1, pz = (bc.periodic >> 2) & 1;
+    const uint32_t plane   = static_cast<uint32_t>(d.nx) * static_cast<uint32_t>(d.ny);
+    const uint32_t x_bytes = plane * static_cast<uint32_t>(sizeof(T));
+    const uint32_t b_bytes = plane * static_cast<uint32_t>(sizeof(float));
+    const bool full_x      = x0 + kTX <= d.nx;
+    const bool full_y      = y0 + kTY <= d.ny;
+
+    // the lane's pair on the tile rows; past a wall (partial tile) both cells take the x+
+    // ghost of cell nx - 1, across a periodic x face the pair wraps
+    const int xa    = x0 + 2 * lane;
+    int lane_src    = xa;
+    bool lane_xg    = false;
+    if (!full_x && xa >= d.nx) {
+        if (px) {
+            lane_src = wrap_index(xa, d.nx);
+        } else {
+            lane_src = d.nx - 2;
+            lane_xg  = true;
+        }
+    }
+    const bool lane_xreal = px || xa < d.nx;
+    const LinGhost g_xp   = lin_ghost(bc.face[1]);
+    const LinGhost g_yp   = lin_ghost(bc.face[3]);
+
+    int t_off[kRowsW], hb_off[kRowsW], bf_off[kRowsW], st_off[kRowsW];
+    bool t_yg[kRowsW], t_rl[kRowsW];
+#pragma unroll
+    for (int j = 0; j < kRowsW; ++j) {
+        const int y   = y0 + wave + kWaves * j;
+        const int yy  = fix_coord(y, d.ny, py);
+        t_yg[j]       = !py && y >= d.ny;
+        t_off[j]      = kT * (yy * d.nx + lane_src);
+        t_rl[j]       = (py || y < d.ny) && lane_xreal;
+        hb_off[j]     = t_rl[j] ? 4 * (yy * d.nx + fix_coord(xa, d.nx, px)) : kOutOfRange;
+        const bool ok = xa < d.nx && y < d.ny;
+        bf_off[j]     = ok ? 4 * (y * d.nx + xa) : kOutOfRange;
+        st_off[j]     = ok ? kT * (y * d.nx + xa) : kOutOfRange;
+    }
+    // extra old pairs: halo rows 0, 1, 18, 19 at the 64 tile pairs, then the halo pairs
+    // 0, 1, 66, 67 of all 20 rows
+    // (lanes past a wave's share repeat an earlier lane's item: same load, same LDS value)
+    int m_off = kOutOfRange, m_sel = 0, m_lds = 0;
+    LinGhost m_g = lin_identity();
+    {
+        const int m = kOMisc * wave + lane % kOMisc;
+        int r, i;
+        if (m < 4 * kPX) {
+            const int hr = m / kPX;
+            r            = hr < 2 ? hr : hr + kR - 4;
+            i            = m % kPX + 2;
+        } else {
+            const int m2 = m - 4 * kPX, q = m2 & 3;
+            r            = m2 >> 2;
+            i            = q < 2 ? q : kW - 4 + q;
+        }
+        {
+            const int gx = xs + 2 * i, gy = ys + r;
+            const int row = fix_coord(gy, d.ny, py) * d.nx;
+            int off;
+            if (!px && gx < 0) {
+                off   = row;
+                m_sel = 1;
+            } else if (!px && gx >= d.nx) {
+                off   = row + d.nx - 2;
+                m_sel = 2;
+            } else {
+                off = row + fix_coord(gx, d.nx, px);
+            }
+            m_off        = kT * off;
+            const int gc = ghost_code(gx, gy, d, px, py);
+            if (gc) m_g = lin_ghost(face_rule(bc, gc - 1));
+            m_lds = r * kW + i;
+        }
+    }
+    // extra H pairs: halo rows 1 and 18 (pairs 1 .. 66), then the halo pairs 1 and 66 of
+    // rows 2 .. 17
+    int n_off = kOutOfRange, n_base = 0, n_ypar = 0;
+    bool n_rl = false;
+    {
+        const int n_cnt = min(kHMisc, kHAll - kHMisc * wave);
+        const int m     = kHMisc * wave + lane % n_cnt;
+        int r, i;
+        if (m < 2 * (kPX + 2)) {
+            r = m < kPX + 2 ? 1 : kTY + 2;
+            i = 1 + m % (kPX + 2);
+        } else {
+            const int m2 = m - 2 * (kPX + 2);
+            r            = 2 + (m2 >> 1);
+            i            = (m2 & 1) ? kPX + 2 : 1;
+        }
+        {
+            const int y = ys + r, gx = xs + 2 * i;
+            n_rl        = (py || (y >= 0 && y < d.ny)) && (px || (gx >= 0 && gx < d.nx));
+            if (n_rl) n_off = 4 * (fix_coord(y, d.ny, py) * d.nx + fix_coord(gx, d.nx, px));
+            n_base = r * kW + i;
+            n_ypar = y & 1;
+        }
+    }
+
+    auto plane_of = [&](const int k) { return pz ? wrap_near(k, d.nz) : (k < 0 ? 0 : (k >= d.nz ? d.nz - 1 : k)); };
+
+    struct OldRegs {
+        float2 t[kRowsW];
+        float2 m;
+    };
+    struct HRegs {
+        float t[kRowsW];
+        float m;
+    };
+    OldRegs oa, ob;
+    HRegs ha, hb;
+    float fa[kOutPer], fb[kOutPer];
+
+    auto fetch_old = [&](const int k, OldRegs& r) {
+        if constexpr (kZero) return;
+        const Rsrc src = plane_rsrc(x_old + static_cast<uint64_t>(plane_of(k)) * plane, k <= z1 + 1 ? x_bytes : 0u);
+#pragma unroll
+        for (int j = 0; j < kRowsW; ++j) r.t[j] = buf_load2<St>(src, t_off[j]);
+        r.m = buf_load2<St>(src, m_off);
+    };
+    auto stash_old = [&](const int k, float* s, const OldRegs& r) {
+        const bool zg      = !pz && (k < 0 || k >= d.nz);
+        const LinGhost g_z = lin_ghost(k < 0 ? bc.face[4] : bc.face[5]);
+#pragma unroll
+        for (int j = 0; j < kRowsW; ++j) {
+            float2 v = kZero ? make_float2(0.0f, 0.0f) : r.t[j];
+            if (zg || t_yg[j] || !full_x) {
+                if (lane_xg) v.x = v.y;
+                const LinGhost gr = zg ? g_z : (lane_xg ? g_xp : (t_yg[j] ? g_yp : lin_identity()));
+                v                 = make_float2(lin_apply(gr, v.x), lin_apply(gr, v.y));
+            }
+            const int c = (2 + wave + kWaves * j) * kW + lane + 2;
+            s[c]        = v.x;
+            s[kArr + c] = v.y;
+        }
+        {
+            float2 v = kZero ? make_float2(0.0f, 0.0f) : r.m;
+            if (m_sel == 1) v.y = v.x;
+            if (m_sel == 2) v.x = v.y;
+            const LinGhost gm = zg ? g_z : m_g;
+            v                 = make_float2(lin_apply(gm, v.x), lin_apply(gm, v.y));
+            s[m_lds]          = v.x;
+            s[kArr + m_lds]   = v.y;
+        }
+    };
+    // b of the red cells of H plane k (H planes z0 - 1 .. z1, real planes only)
+    auto fetch_bh = [&](const int k, HRegs& r) {
+        const bool live = k >= z0 - 1 && k <= z1 && (pz || (k >= 0 && k < d.nz));
+        const Rsrc src  = plane_rsrc(b + static_cast<uint64_t>(plane_of(k)) * plane, live ? b_bytes : 0u);
+        const int a_t   = (y0 + wave + k) & 1; // 1: the red cell of a tile-row pair is the odd one
+#pragma unroll
+        for (int j = 0; j < kRowsW; ++j) r.t[j] = buf_load1<StoreF32>(src, hb_off[j] + 4 * a_t);
+        r.m = buf_load1<StoreF32>(src, n_off + 4 * ((n_ypar + k) & 1));
+    };
+    // b of the black cells of final plane k
+    auto fetch_bf = [&](const int k, float (&r)[kOutPer]) {
+        const Rsrc src = plane_rsrc(b + static_cast<uint64_t>(k) * plane, (k >= z0 && k < z1) ? b_bytes : 0u);
+        const int e    = ((y0 + wave + k) & 1) == 0 ? 1 : 0;
+#pragma unroll
+        for (int j = 0; j < kOutPer; ++j) r[j] = buf_load1<StoreF32>(src, bf_off[j] + 4 * e);
+    };
+    // H(k): red value of every pair on the tile plus a one-pair halo; ghosts keep the old value
+    auto compute_h = [&](const int k, const float* om, const float* o0, const float* op, float* h0, const HRegs& bh) {
+        const bool preal    = pz || (k >= 0 && k < d.nz);
+        const bool all_real = preal && (full_x || px) && (full_y || py); // every tile-row pair real
+        StageDiv<kSolve> dvs(coef);
+        auto num = [&](const int base, const int a, const float bv) {
+            const int ca = base + a * kArr;
+            const int cx = base + (1 - a) * (kArr - 1);
+            float sum    = o0[cx];
+            sum          = sum + o0[cx + 1];
+            sum          = sum + o0[ca - kW];
+            sum          = sum + o0[ca + kW];
+            sum          = sum + om[ca];
+            sum          = sum + op[ca];
+            return dvs.num(bv * inv_s, sum, coef);
+        };
+        const int a_t = (y0 + wave + k) & 1;
+        const int a_m = (n_ypar + k) & 1;
+        float x[kRowsW + 1], q[kRowsW + 1];
+        int base[kRowsW + 1], av[kRowsW + 1];
+#pragma unroll
+        for (int j = 0; j < kRowsW; ++j) {
+            base[j] = (2 + wave + kWaves * j) * kW + lane + 2;
+            av[j]   = a_t;
+            x[j]    = num(base[j], a_t, bh.t[j]);
+        }
+        base[kRowsW] = n_base;
+        av[kRowsW]   = a_m;
+        x[kRowsW]    = num(n_base, a_m, bh.m);
+#pragma unroll
+        for (int j = 0; j <= kRowsW; ++j) q[j] = dvs.quot(x[j]);
+        if (dvs.bad()) {
+#pragma unroll
+            for (int j = 0; j <= kRowsW; ++j) q[j] = dvs.exact(x[j]);
+        }
+        if (!all_real) { // tile rows past a wall or off a ghost plane
+#pragma unroll
+            for (int j = 0; j < kRowsW; ++j)
+                if (!(preal && t_rl[j])) q[j] = o0[base[j] + a_t * kArr];
+        }
+        if (!(preal && n_rl)) q[kRowsW] = o0[n_base + a_m * kArr]; // halo pairs are ghosts at walls
+#pragma unroll
+        for (int j = 0; j <= kRowsW; ++j) h0[base[j]] = q[j];
+    };
+    // final plane k - 1: black cells relaxed from the red values, red cells copied; stores
+    // switched off outside the chunk
+    auto compute_final = [&](const int k, const float* hm2, const float* hm1, const float* h0, const float (&bf)[kOutPer]) {
+        const int kf     = k - 1;
+        const bool live  = kf >= z0 && kf < z1;
+        const Rsrc dst   = plane_rsrc(x_new + static_cast<uint64_t>(live ? kf : 0) * plane, live ? x_bytes : 0u);
+        const int e      = ((y0 + wave + kf) & 1) == 0 ? 1 : 0; // 1: the red cell is the even one
+        StageDiv<kSolve> dvs(coef);
+        float x[kOutPer], q[kOutPer], red[kOutPer];
+#pragma unroll
+        for (int j = 0; j < kOutPer; ++j) {
+            const int c = (2 + wave + kWaves * j) * kW + lane + 2;
+            float sum   = hm1[c - 1 + e];
+            sum         = sum + hm1[c + e];
+            sum         = sum + hm1[c - kW];
+            sum         = sum + hm1[c + kW];
+            sum         = sum + hm2[c];
+            sum         = sum + h0[c];
+            x[j]        = dvs.num(bf[j] * inv_s, sum, coef);
+            red[j]      = hm1[c];
+            q[j]        = dvs.quot(x[j]);
+        }
+        if (dvs.bad()) {
+#pragma unroll
+            for (int j = 0; j < kOutPer; ++j) q[j] = dvs.exact(x[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < kOutPer; ++j) {
+            const float ve = (e ? red[j] : q[j]) * out_scale;
+            const float vo = (e ? q[j] : red[j]) * out_scale;
+            buf_store2<St>(dst, st_off[j], ve, vo);
+        }
+    };
+    // plane k; register sets alternate with k - z0 (set 0 at k = z0 - 1): old(m), bh(m)
+    // and bf(m) live in set (m - z0 + 1) & 1. Ring slots rotate: old planes k - 1, k,
+    // k + 1 in (o_a, o_b, o_c), H planes k - 2, k - 1, k in (h_a, h_b, h_c); old(k + 1)
+    // and H(k) take the slots that held old(k - 2) and H(k - 3).
+    float *o_a = s_old[0], *o_b = s_old[1], *o_c = s_old[2];
+    float *h_a = s_h[0], *h_b = s_h[1], *h_c = s_h[2];
+    auto body = [&](const int k, OldRegs& o_next, HRegs& h_cur, float (&f_prev)[kOutPer]) {
+        stash_old(k + 1, o_c, o_next);
+        fetch_old(k + 3, o_next);
+        __syncthreads();
+        compute_h(k, o_a, o_b, o_c, h_c, h_cur);
+        fetch_bh(k + 2, h_cur);
+        __syncthreads();
+        compute_final(k, h_a, h_b, h_c, f_prev);
+        fetch_bf(k + 1, f_prev);
+        float* o_t = o_a;
+        float* h_t = h_a;
+        o_a           = o_b;
+        o_b           = o_c;
+        o_c           = o_t;
+        h_a           = h_b;
+        h_b           = h_c;
+        h_c           = h_t;
+    };
+
+    fetch_old(z0 - 2, oa);
+    stash_old(z0 - 2, s_old[0], oa);
+    fetch_old(z0 - 1, oa);
+    stash_old(z0 - 1, s_old[1], oa);
+    fetch_old(z0, ob);     // set 1
+    fetch_old(z0 + 1, oa); // set 0
+    fetch_bh(z0 - 1, ha);
+    fetch_bh(z0, hb);
+    // an odd number of planes z0 - 1 .. z1 ends with one idle plane
+    for (int k = z0 - 1; k <= z1; k += 2) {
+        body(k, ob, ha, fb);
+        body(k + 1, oa, hb, fa);
+    }
+}
+
 template <int kSolve, bool kZero, class St, bool kPairs>
 __device__ __forceinline__ void rbgs_split_body(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC& bc, const TileGeom& g, const float* __restrict__ scale_slot, const int w) {
     __shared__ float s_old[3][kOldPlane];
@@ -1003,7 +1405,10 @@ __global__ __launch_bounds__(kThreads) void k_jacobi_pass(const typename St::T* 
 template <int kSolve, bool kZero, class St, bool kPairs>
 __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g, const float* __restrict__ scale_slot, const int* __restrict__ skip) {
     if (skip != nullptr && *skip != 0) return;
-    rbgs_split_body<kSolve, kZero, St, kPairs>(x_old, b, x_new, coef, out_scale, d, bc, g, scale_slot, xcd_block());
+    if constexpr (kPairs)
+        rbgs_rows_body<kSolve, kZero, St>(x_old, b, x_new, coef, out_scale, d, bc, g, scale_slot, xcd_block());
+    else
+        rbgs_split_body<kSolve, kZero, St, false>(x_old, b, x_new, coef, out_scale, d, bc, g, scale_slot, xcd_block());
 }
 
 // Batched passes: workgroup i runs block items[i].block of pass descs[items[i].desc]. The
@@ -1013,7 +1418,10 @@ template <int kSolve, bool kZero, bool kPairs>
 __global__ __launch_bounds__(kThreads) void k_rbgs_split_batched(const SmoothDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
     const BatchItem it   = items[blockIdx.x];
     const SmoothDesc& sd = descs[it.desc];
-    rbgs_split_body<kSolve, kZero, StoreF32, kPairs>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
+    if constexpr (kPairs)
+        rbgs_rows_body<kSolve, kZero, StoreF32>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
+    else
+        rbgs_split_body<kSolve, kZero, StoreF32, false>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
 }
 
 template <int kSolve, bool kZero, bool kPairs>
