@@ -50,16 +50,17 @@ __device__ __forceinline__ float buf_load1(const Rsrc r, const int off) {
     }
 }
 
-// Aligned pair (a, b) at byte offset `off`, rounded to the storage type.
-template <class St>
+// Aligned pair (a, b) at byte offset `off`, rounded to the storage type. (Non-temporal
+// stores, aux bit NT, measured slower for the smoother outputs: the next pass reads them.)
+template <class St, int kPolicy = 0>
 __device__ __forceinline__ void buf_store2(const Rsrc r, const int off, const float a, const float b) {
     if constexpr (St::kNarrow) {
         const uint32_t w = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, St::store(a))) | (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, St::store(b))) << 16);
-        __builtin_amdgcn_raw_buffer_store_b32(w, r, off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b32(w, r, off, 0, kPolicy);
     } else {
         typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
         const u32x2 v = {__float_as_uint(a), __float_as_uint(b)};
-        __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, kPolicy);
     }
 }
 

@@ -535,10 +535,13 @@ __device__ __forceinline__ void jacobi_rows_body(const typename St::T* __restric
         const int gy = y0 - 1 + r;
         row_ghost[j] = !py && (gy < 0 || gy >= d.ny);
         row_rule[j]  = gy < 0 ? bc.face[2] : bc.face[3];
-        row_off[j]   = r < kHY ? static_cast<int>(sizeof(T)) * (fix_coord(gy, d.ny, py) * d.nx + lane_src) : kOutOfRange;
+        // a wave without a third row re-reads its last one (never staged)
+        const int gr = r < kHY ? gy : gy - kWaves;
+        row_off[j]   = static_cast<int>(sizeof(T)) * (fix_coord(gr, d.ny, py) * d.nx + lane_src);
     }
     // halo columns: lane bit 0 = side (0: x0 - 1, 1: x0 + 128), bit 1 = row of the pair
-    // of tile rows 2 wave, 2 wave + 1; columns beyond a partial tile are unused
+    // of tile rows 2 wave, 2 wave + 1; columns beyond a partial tile are unused. Lanes 4 ..
+    // 63 repeat lanes 0 .. 3 (same addresses, never staged).
     const int h_side = lane & 1;
     const int h_row  = 2 * wave + ((lane >> 1) & 1);
     int h_off        = kOutOfRange;
@@ -553,7 +556,7 @@ __device__ __forceinline__ void jacobi_rows_body(const typename St::T* __restric
             h_ghost = gx == d.nx && !px;
             gx      = (gx == d.nx && px) ? 0 : d.nx - 1;
         }
-        if (lane < 4) h_off = static_cast<int>(sizeof(T)) * (gy * d.nx + gx);
+        h_off = static_cast<int>(sizeof(T)) * (gy * d.nx + gx);
     }
     const FaceRule h_rule = h_side ? bc.face[1] : bc.face[0];
     const int h_lds       = 2 * ((1 + h_row) * kHPX + (h_side ? kHPX - 1 : 0)) + (h_side ? 0 : 1);
@@ -1027,7 +1030,8 @@ __device__ __forceinline__ void rbgs_rows_body(const typename St::T* __restrict_
             for (int j = 0; j < kRowsW; ++j)
                 if (!(preal && t_rl[j])) q[j] = o0[base[j] + a_t * kArr];
         }
-        if (!(preal && n_rl)) q[kRowsW] = o0[n_base + a_m * kArr]; // halo pairs are ghosts at walls
+        const float n_old = o0[n_base + a_m * kArr]; // halo pairs are ghosts at walls
+        q[kRowsW]         = (preal && n_rl) ? q[kRowsW] : n_old;
 #pragma unroll
         for (int j = 0; j <= kRowsW; ++j) h0[base[j]] = q[j];
     };
