@@ -298,11 +298,13 @@ hipError_t launch_add_force(hipStream_t s, float* vx, float* vy, float* vz, cons
 }
 
 hipError_t launch_advect_velocity(hipStream_t s, float* tx, float* ty, float* tz, const float* vx, const float* vy, const float* vz, const float dt, const Dims d, const float h, const StableFluidsFlowBoundaryConfig& bc) {
+    if (advect_buffers_fit(d)) return launch_advect_velocity_buffers(s, tx, ty, tz, vx, vy, vz, dt, d, h, bc);
     k_advect_velocity<<<tile_grid(d), tile_block(), 0, s>>>(tx, ty, tz, vx, vy, vz, dt, d, h, bc);
     return hipGetLastError();
 }
 
 hipError_t launch_advect_scalar(hipStream_t s, float* out, const float* src, const float* vx, const float* vy, const float* vz, const float dt, const Dims d, const float h, const StableFluidsScalarBoundaryConfig& sbc, const StableFluidsFlowBoundaryConfig& fbc) {
+    if (advect_buffers_fit(d)) return launch_advect_scalar_buffers(s, out, src, vx, vy, vz, dt, d, h, sbc, fbc);
     k_advect_scalar<<<tile_grid(d), tile_block(), 0, s>>>(out, src, vx, vy, vz, dt, d, h, sbc, fbc);
     return hipGetLastError();
 }

@@ -144,6 +144,10 @@ hipError_t launch_copy_scaled(hipStream_t s, float* dst, const float* src, float
 hipError_t launch_add_scaled(hipStream_t s, float* dst, const float* src, float dt, uint64_t n);
 hipError_t launch_add_force(hipStream_t s, float* vx, float* vy, float* vz, const float* fx, const float* fy, const float* fz, float dt, uint64_t n);
 hipError_t launch_advect_velocity(hipStream_t s, float* tx, float* ty, float* tz, const float* vx, const float* vy, const float* vz, float dt, Dims d, float h, const StableFluidsFlowBoundaryConfig& bc);
+// sf_advect.hip: the same stages through field buffer resources (fields below 2 GiB)
+bool advect_buffers_fit(Dims d);
+hipError_t launch_advect_velocity_buffers(hipStream_t s, float* tx, float* ty, float* tz, const float* vx, const float* vy, const float* vz, float dt, Dims d, float h, const StableFluidsFlowBoundaryConfig& bc);
+hipError_t launch_advect_scalar_buffers(hipStream_t s, float* out, const float* src, const float* vx, const float* vy, const float* vz, float dt, Dims d, float h, const StableFluidsScalarBoundaryConfig& sbc, const StableFluidsFlowBoundaryConfig& fbc);
 hipError_t launch_advect_scalar(hipStream_t s, float* out, const float* src, const float* vx, const float* vy, const float* vz, float dt, Dims d, float h, const StableFluidsScalarBoundaryConfig& sbc, const StableFluidsFlowBoundaryConfig& fbc);
 hipError_t launch_smooth(hipStream_t s, SmootherKind smoother, SolveKind solve, const SmoothPass& pass, Dims d, const StencilBC& bc);
 hipError_t launch_correction(hipStream_t s, SmootherKind smoother, SolveKind solve, StorageKind storage, const CorrectionPass& pass, Dims d, const StencilBC& homogeneous_bc);
