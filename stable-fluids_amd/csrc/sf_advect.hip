@@ -218,11 +218,19 @@ __device__ __forceinline__ void trace_cell(const VelocityField& v, const int x, 
 
 __device__ __forceinline__ Grid grid_of(const Dims d, const float h, const float inv_h) { return Grid{d.nx, d.ny, d.nz, 4 * d.nx, 4 * d.nx * d.ny, h, inv_h}; }
 
+// Blocks run on a 1D grid renumbered per XCD: workgroups are dealt round-robin over the 8
+// XCDs, so XCD k receives the contiguous block range [k n / 8, (k + 1) n / 8) in x, y, z
+// order, and the tap halos shared by neighbouring blocks meet in the same L2.
 __device__ __forceinline__ bool cell_of(const Dims d, int& x, int& y, int& z) {
+    const int n    = static_cast<int>(gridDim.x);
+    const int w    = static_cast<int>(blockIdx.x);
+    const int b    = (n & 7) == 0 ? (w & 7) * (n >> 3) + (w >> 3) : w;
+    const int bx_n = (d.nx + kAX - 1) / kAX, by_n = (d.ny + kAY - 1) / kAY;
+    const int bx = b % bx_n, by = (b / bx_n) % by_n;
     const int lane = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
-    x              = static_cast<int>(blockIdx.x) * kAX + (lane & 15);
-    y              = static_cast<int>(blockIdx.y) * kAY + 4 * wave + (lane >> 4);
-    z              = static_cast<int>(blockIdx.z);
+    x              = bx * kAX + (lane & 15);
+    y              = by * kAY + 4 * wave + (lane >> 4);
+    z              = b / (bx_n * by_n);
     return x < d.nx && y < d.ny;
 }
 
@@ -324,7 +332,7 @@ bool pow2_spacing(const float h) {
     return h > 0.0f && std::isfinite(h) && std::frexp(h, &e) == 0.5f;
 }
 
-inline dim3 adv_grid(const Dims d) { return dim3(static_cast<unsigned>((d.nx + kAX - 1) / kAX), static_cast<unsigned>((d.ny + kAY - 1) / kAY), static_cast<unsigned>(d.nz)); }
+inline unsigned adv_grid(const Dims d) { return static_cast<unsigned>(((d.nx + kAX - 1) / kAX) * ((d.ny + kAY - 1) / kAY) * d.nz); }
 
 } // namespace
 

@@ -23,6 +23,8 @@ ALGORITHMIC_BYTES_PER_CELL = {
     "k_copy_scaled": 8,           # read src; write dst
     "k_advect_velocity": 24,      # read 3 velocity components; write 3
     "k_advect_scalar": 20,        # read the scalar and 3 velocity components; write 1
+    "k_advect_velocity_b": 24,    # buffer-resource variants of the two above
+    "k_advect_scalar_b": 20,
     "k_project": 36,              # read a (3 components), p; write v (3), div, |v|
     "k_divergence": 16,           # read 3 velocity components; write div
     "k_add_force": 36,            # read 3 velocity + 3 force components; write 3
