@@ -32,6 +32,14 @@ __device__ __forceinline__ void block_sum_to(double v, double* out) {
 }
 
 __device__ __forceinline__ void decode(const uint64_t i, const Dims d, int& x, int& y, int& z) {
+    if (i <= 0xffffffffull) { // 32-bit division: a 64-bit one is a long instruction sequence
+        const uint32_t j = static_cast<uint32_t>(i), nx = static_cast<uint32_t>(d.nx), ny = static_cast<uint32_t>(d.ny);
+        const uint32_t r = j / nx;
+        x                = static_cast<int>(j - r * nx);
+        y                = static_cast<int>(r % ny);
+        z                = static_cast<int>(r / ny);
+        return;
+    }
     x = static_cast<int>(i % static_cast<uint64_t>(d.nx));
     const uint64_t r = i / static_cast<uint64_t>(d.nx);
     y = static_cast<int>(r % static_cast<uint64_t>(d.ny));
@@ -97,16 +105,34 @@ __device__ __forceinline__ double block_mean(const T* __restrict__ v, const int 
     return s / static_cast<double>(f * f * f);
 }
 
+// Eight lanes per coarse cell: lane s sums the z-slices k = s, s + 8, ... of its f^3
+// block (rows of f consecutive cells, so the eight cells of a wave read contiguous
+// rows), and the group adds its partial sums by shuffles.
+constexpr int kGroup = 8;
+
 __global__ __launch_bounds__(kThreads) void k_restrict_input(const float* __restrict__ div, float* __restrict__ in2, double* __restrict__ partials, const Dims cd, const int f, const StencilBC bc) {
     const Dims fd{f * cd.nx, f * cd.ny, f * cd.nz};
-    double acc = 0.0;
-    GRID_STRIDE(v, cells_of(cd)) {
+    const int lane_s   = static_cast<int>(threadIdx.x) & (kGroup - 1);
+    const uint64_t n   = cells_of(cd);
+    const uint64_t row = static_cast<uint64_t>(fd.nx), plane = row * static_cast<uint64_t>(fd.ny);
+    double acc         = 0.0;
+    for (uint64_t g = (static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x) / kGroup; g < n; g += static_cast<uint64_t>(gridDim.x) * (kThreads / kGroup)) {
         int x, y, z;
-        decode(v, cd, x, y, z);
-        const float s  = static_cast<float>(block_mean(div, x, y, z, f, fd));
-        in2[2 * v]     = s;
-        in2[2 * v + 1] = wall_mask(x, y, z, cd, bc);
-        acc += static_cast<double>(s) * s;
+        decode(g, cd, x, y, z);
+        double s = 0.0;
+        for (int k = lane_s; k < f; k += kGroup)
+            for (int j = 0; j < f; ++j) {
+                const float* r = div + static_cast<uint64_t>(f * z + k) * plane + static_cast<uint64_t>(f * y + j) * row + static_cast<uint64_t>(f * x);
+                for (int i = 0; i < f; ++i) s += r[i];
+            }
+#pragma unroll
+        for (int o = 1; o < kGroup; o <<= 1) s += __shfl_xor(s, o, kGroup);
+        if (lane_s == 0) {
+            const float m  = static_cast<float>(s / static_cast<double>(f * f * f));
+            in2[2 * g]     = m;
+            in2[2 * g + 1] = wall_mask(x, y, z, cd, bc);
+            acc += static_cast<double>(m) * m;
+        }
     }
     block_sum_to(acc, partials + blockIdx.x);
 }
@@ -123,42 +149,100 @@ __global__ __launch_bounds__(kThreads) void k_sigma(const double* __restrict__ p
     }
 }
 
+// Rows of the fine grid handled by this block: a contiguous chunk, so the row's (y, z)
+// comes from one division per block and then steps along.
+struct Rows {
+    uint64_t r, end;
+    int y, z;
+};
+
+__device__ __forceinline__ Rows block_rows(const int ny, const int nz) {
+    const uint64_t rows = static_cast<uint64_t>(ny) * static_cast<uint64_t>(nz);
+    const uint64_t per  = (rows + gridDim.x - 1) / gridDim.x;
+    Rows w;
+    w.r   = min(rows, static_cast<uint64_t>(blockIdx.x) * per);
+    w.end = min(rows, w.r + per);
+    w.y   = static_cast<int>(w.r % static_cast<uint64_t>(ny));
+    w.z   = static_cast<int>(w.r / static_cast<uint64_t>(ny));
+    return w;
+}
+
+__device__ __forceinline__ void next_row(Rows& w, const int ny) {
+    ++w.r;
+    if (++w.y == ny) {
+        w.y = 0;
+        ++w.z;
+    }
+}
+
+// Cell-centred trilinear prolongation, separable: for each fine row (one per block at a
+// time) the block first interpolates the coarse line at the row's y and z into LDS, then
+// every fine cell of the row lerps in x. Coarse lines up to kLine cells.
+constexpr int kLine = 2048;
+
+__device__ __forceinline__ void axis_weights(const int c, const int n, const float inv_f, int& i0, int& i1, float& t) {
+    const float cc = fminf(fmaxf((static_cast<float>(c) + 0.5f) * inv_f - 0.5f, 0.0f), static_cast<float>(n - 1));
+    i0             = static_cast<int>(cc);
+    i1             = min(i0 + 1, n - 1);
+    t              = cc - static_cast<float>(i0);
+}
+
 __global__ __launch_bounds__(kThreads) void k_prolong(const float* __restrict__ q, const float* __restrict__ scal, const float scale, float* __restrict__ p0, const Dims cd, const int f) {
+    __shared__ float line[kLine];
     const Dims fd{f * cd.nx, f * cd.ny, f * cd.nz};
-    const float amp   = scale * scal[0];
-    const float inv_f = 1.0f / static_cast<float>(f);
-    GRID_STRIDE(i, cells_of(fd)) {
-        int x, y, z;
-        decode(i, fd, x, y, z);
-        const int c[3] = {x, y, z}, n[3] = {cd.nx, cd.ny, cd.nz};
-        int i0[3], i1[3];
-        float t[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const float cc = fminf(fmaxf((static_cast<float>(c[a]) + 0.5f) * inv_f - 0.5f, 0.0f), static_cast<float>(n[a] - 1));
-            i0[a]          = static_cast<int>(cc);
-            i1[a]          = min(i0[a] + 1, n[a] - 1);
-            t[a]           = cc - i0[a];
+    const float amp     = scale * scal[0];
+    const float inv_f   = 1.0f / static_cast<float>(f);
+    const uint64_t cpl  = static_cast<uint64_t>(cd.nx) * static_cast<uint64_t>(cd.ny);
+    for (Rows w = block_rows(fd.ny, fd.nz); w.r < w.end; next_row(w, fd.ny)) {
+        const uint64_t rw = w.r;
+        const int y = w.y, z = w.z;
+        int y0, y1, z0, z1;
+        float ty, tz;
+        axis_weights(y, cd.ny, inv_f, y0, y1, ty);
+        axis_weights(z, cd.nz, inv_f, z0, z1, tz);
+        __syncthreads(); // the previous row's line is no longer read
+        for (int cx = static_cast<int>(threadIdx.x); cx < cd.nx; cx += kThreads) {
+            const float* b = q + static_cast<uint64_t>(cx);
+            const float a00 = b[static_cast<uint64_t>(z0) * cpl + static_cast<uint64_t>(y0) * cd.nx];
+            const float a10 = b[static_cast<uint64_t>(z0) * cpl + static_cast<uint64_t>(y1) * cd.nx];
+            const float a01 = b[static_cast<uint64_t>(z1) * cpl + static_cast<uint64_t>(y0) * cd.nx];
+            const float a11 = b[static_cast<uint64_t>(z1) * cpl + static_cast<uint64_t>(y1) * cd.nx];
+            line[cx]        = ((1.0f - ty) * a00 + ty * a10) * (1.0f - tz) + ((1.0f - ty) * a01 + ty * a11) * tz;
         }
-        float v = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int xx = (k & 1) ? i1[0] : i0[0], yy = (k & 2) ? i1[1] : i0[1], zz = (k & 4) ? i1[2] : i0[2];
-            const float w = ((k & 1) ? t[0] : 1.0f - t[0]) * ((k & 2) ? t[1] : 1.0f - t[1]) * ((k & 4) ? t[2] : 1.0f - t[2]);
-            v += w * q[lin(xx, yy, zz, cd)];
+        __syncthreads();
+        float* out = p0 + rw * static_cast<uint64_t>(fd.nx);
+        for (int x = static_cast<int>(threadIdx.x); x < fd.nx; x += kThreads) {
+            int x0, x1;
+            float tx;
+            axis_weights(x, cd.nx, inv_f, x0, x1, tx);
+            out[x] = amp * ((1.0f - tx) * line[x0] + tx * line[x1]);
         }
-        p0[i] = amp * v;
     }
 }
 
 __global__ __launch_bounds__(kThreads) void k_guess_partials(const float* __restrict__ p0, const float* __restrict__ div, const float h2, double* __restrict__ partials, const Dims d, const StencilBC bc) {
     double acc_g = 0.0, acc_z = 0.0;
-    GRID_STRIDE(i, cells_of(d)) {
-        int x, y, z;
-        decode(i, d, x, y, z);
-        const double b  = -static_cast<double>(h2) * div[i] + ghost_constant(x, y, z, d, bc);
+    for (Rows w = block_rows(d.ny, d.nz); w.r < w.end; next_row(w, d.ny))
+    for (int x = static_cast<int>(threadIdx.x); x < d.nx; x += kThreads) {
+        const int y      = w.y, z = w.z;
+        const uint64_t i = w.r * static_cast<uint64_t>(d.nx) + static_cast<uint64_t>(x);
+        double b, ap;
         const double pc = p0[i];
-        const double ap = 6.0 * pc - static_cast<double>(neighbour_sum<float>(p0, x, y, z, d, bc, p0[i], true));
+        if (x > 0 && x < d.nx - 1 && y > 0 && y < d.ny - 1 && z > 0 && z < d.nz - 1) {
+            // no face: no ghost constant, six direct neighbours (same order as neighbour_sum)
+            const uint64_t sy = static_cast<uint64_t>(d.nx), sz = sy * static_cast<uint64_t>(d.ny);
+            float sum = p0[i - 1];
+            sum += p0[i + 1];
+            sum += p0[i - sy];
+            sum += p0[i + sy];
+            sum += p0[i - sz];
+            sum += p0[i + sz];
+            b  = -static_cast<double>(h2) * div[i];
+            ap = 6.0 * pc - static_cast<double>(sum);
+        } else {
+            b  = -static_cast<double>(h2) * div[i] + ghost_constant(x, y, z, d, bc);
+            ap = 6.0 * pc - static_cast<double>(neighbour_sum<float>(p0, x, y, z, d, bc, p0[i], true));
+        }
         acc_g += (b - ap) * (b - ap);
         acc_z += b * b;
     }
@@ -326,7 +410,7 @@ bool singular(const Dims, const StencilBC& bc) {
 } // namespace
 
 hipError_t launch_restrict_input(hipStream_t s, const float* div, float* in2, double* partials, float* scal, const Dims cd, const int f, const StencilBC& p_bc) {
-    const unsigned b = blocks_for(cells_of(cd));
+    const unsigned b = blocks_for(cells_of(cd) * kGroup);
     k_restrict_input<<<b, kThreads, 0, s>>>(div, in2, partials, cd, f, p_bc);
     k_sigma<<<1, kThreads, 0, s>>>(partials, static_cast<int>(b), static_cast<double>(cells_of(cd)), scal);
     return hipGetLastError();
@@ -334,6 +418,7 @@ hipError_t launch_restrict_input(hipStream_t s, const float* div, float* in2, do
 
 hipError_t launch_prolong(hipStream_t s, const float* q, const float* scal, const float scale, float* p0, const Dims cd, const int f) {
     const Dims fd{f * cd.nx, f * cd.ny, f * cd.nz};
+    if (cd.nx > kLine) return hipErrorInvalidValue;
     k_prolong<<<blocks_for(cells_of(fd)), kThreads, 0, s>>>(q, scal, scale, p0, cd, f);
     return hipGetLastError();
 }

@@ -13,7 +13,7 @@
 
 namespace sf::warm {
 
-constexpr int kPartialBlocks = 1024;
+constexpr int kPartialBlocks = 8192; // enough waves in flight for the grid-stride stages
 
 // Coarse network input in2[v][2] = (mean of the f^3 fine div values, wall mask) on the
 // coarse grid cd (fine = f * cd), and 1/sigma into scal[1], sigma into scal[0]
@@ -21,7 +21,7 @@ constexpr int kPartialBlocks = 1024;
 // partials: kPartialBlocks doubles.
 hipError_t launch_restrict_input(hipStream_t s, const float* div, float* in2, double* partials, float* scal, Dims cd, int f, const StencilBC& p_bc);
 // p0[fine] = scale * sigma * trilinear(q)[fine] (cell-centred, clamped at the edges),
-// scale = (f h)^2, sigma = scal[0].
+// scale = (f h)^2, sigma = scal[0]. The coarse grid is at most 2048 cells wide.
 hipError_t launch_prolong(hipStream_t s, const float* q, const float* scal, float scale, float* p0, Dims cd, int f);
 // Residual test of the guess p0: g = ||b - A p0||, z = ||b|| (= residual of the zero
 // guess), FP64. stats[0] = g, stats[1] = z, stats[2] = 1 when g > ratio * z (fallback).
