@@ -20,8 +20,17 @@ constexpr int kOutOfRange   = 0x7ffffff0; // byte offset past any plane
 
 using Rsrc = __amdgpu_buffer_rsrc_t;
 
+// The base and size are wave-uniform by construction; passing them through readfirstlane
+// keeps the descriptor in scalar registers even where the compiler has parked a uniform
+// value in a vector register (a descriptor in vector registers turns every access into a
+// loop over the lanes' distinct descriptors).
 __device__ __forceinline__ Rsrc plane_rsrc(const void* base, const uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), kBufferConfig);
+    const uint64_t p  = reinterpret_cast<uint64_t>(base);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(p));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(p >> 32));
+    const uint32_t n  = __builtin_amdgcn_readfirstlane(bytes);
+    void* q           = reinterpret_cast<void*>((static_cast<uint64_t>(hi) << 32) | lo);
+    return __builtin_amdgcn_make_buffer_rsrc(q, 0, static_cast<int>(n), kBufferConfig);
 }
 
 // Aligned pair of elements at byte offset `off`.
