@@ -46,6 +46,14 @@ typedef enum StableFluidsExtWarmStart {
     STABLE_FLUIDS_EXT_WARM_START_LEARNED = 1,
 } StableFluidsExtWarmStart;
 
+typedef enum StableFluidsExtFlags {
+    /* Run FP32 RBGS solves two passes per kernel launch (even nx, even extents on periodic
+     * axes, nz >= 12 on a periodic z, fields below 2 GiB; other grids keep one pass per
+     * launch): the old iterate is read once and the second iterate written once, half the
+     * HBM traffic of two launches. Results are bit-identical either way. */
+    STABLE_FLUIDS_EXT_FLAG_PASS_PAIRS = 1,
+} StableFluidsExtFlags;
+
 typedef struct StableFluidsExtOptions {
     uint32_t struct_size;                /* sizeof(StableFluidsExtOptions) */
     uint32_t smoother;                   /* StableFluidsExtSmoother */
@@ -56,7 +64,7 @@ typedef struct StableFluidsExtOptions {
                                             raised by one when pressure_iterations minus it is odd) */
     uint32_t warm_start_coarsening;      /* the network runs on the grid coarsened 2^c times per
                                             axis (c = 1, 2 or 3; 0 = 1); extents multiples of 2^(c+2) */
-    uint32_t reserved[1];                /* must be zero */
+    uint32_t flags;                      /* StableFluidsExtFlags bits; other bits must be zero */
 } StableFluidsExtOptions;
 
 /* Same as stable_fluids_create_context_cuda, with options (NULL = reference behaviour).
@@ -89,6 +97,8 @@ typedef enum StableFluidsExtStage {
     STABLE_FLUIDS_EXT_STAGE_PROJECT        = 3, /* projection + divergence + |v| */
     STABLE_FLUIDS_EXT_STAGE_JACOBI_PASS    = 4, /* one Jacobi pass of the pressure solve, whatever the smoother option */
     STABLE_FLUIDS_EXT_STAGE_COPY           = 5, /* plain copy of one field (8 B/cell), to calibrate byte counters */
+    STABLE_FLUIDS_EXT_STAGE_PRESSURE_PAIR  = 6, /* two RBGS passes of the pressure solve in one launch */
+    STABLE_FLUIDS_EXT_STAGE_VELOCITY_PAIR  = 7, /* two RBGS passes of one velocity component's diffusion in one launch */
 } StableFluidsExtStage;
 
 /* Enqueues `repeat` launches of one stage kernel on the context stream, outside the step

@@ -49,6 +49,15 @@ STORAGE_BF16 = 2
 STORAGE_F32_REFINED = 3
 WARM_START_ZERO = 0
 WARM_START_LEARNED = 1
+FLAG_PASS_PAIRS = 1
+STAGE_PRESSURE_PASS = 0
+STAGE_VELOCITY_PASS = 1
+STAGE_ADVECT = 2
+STAGE_PROJECT = 3
+STAGE_JACOBI_PASS = 4
+STAGE_COPY = 5
+STAGE_PRESSURE_PAIR = 6
+STAGE_VELOCITY_PAIR = 7
 
 
 class FlowFace(C.Structure):
@@ -106,7 +115,7 @@ class ExtOptions(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("smoother", C.c_uint32), ("iterate_storage", C.c_uint32),
                 ("defect_correction_interval", C.c_uint32), ("pressure_warm_start", C.c_uint32),
                 ("warm_start_iterations", C.c_uint32), ("warm_start_coarsening", C.c_uint32),
-                ("reserved", C.c_uint32 * 1)]
+                ("flags", C.c_uint32)]
 
 
 assert C.sizeof(FlowFace) == 20 and C.sizeof(FlowBoundary) == 120

@@ -49,6 +49,7 @@ struct Options {
     bool learned          = false;     // learned pressure warm start (sf_warm.hip)
     int warm_iterations   = 0;         // passes after an accepted guess
     int warm_factor       = 2;         // restriction factor of the network input
+    bool pairs            = false;     // two RBGS passes per launch where supported
 };
 
 struct ScalarField {
@@ -242,13 +243,23 @@ hipError_t enqueue_solve_mixed(Context& c, SolveKind solve, const float* b, cons
     return launch_finalize(c.stream, c.options.storage, out, x, e, e_slot, last_scale, c.cells);
 }
 
-// K smoother passes from x0 (nullptr = zero guess) into `out`, ping-ponging with `aux`.
+// Whether this context's FP32 smoother passes run two per launch under these rules.
+bool use_pairs(const Context& c, const StencilBC& bc) { return c.options.pairs && !c.options.refine && smooth2_supported(c.options.smoother, c.dims, bc); }
+
+// K smoother passes from x0 (nullptr = zero guess) into `out`, ping-ponging with `aux`:
+// launch l of L writes out when L - 1 - l is even, so the last one lands in out. With
+// pass pairs, an odd K starts with one single pass.
 hipError_t enqueue_solve(Context& c, SolveKind solve, const float* b, const float* x0, float* out, float* aux, int passes, float coef, float last_scale, const StencilBC& bc, int slot_base) {
     if (c.options.refine) return enqueue_solve_mixed(c, solve, b, x0, out, aux, passes, coef, last_scale, bc, slot_base);
-    const float* src = x0;
-    for (int i = 1; i <= passes; ++i) {
-        float* dst = ((passes - i) % 2 == 0) ? out : aux;
-        SF_TRY(launch_smooth(c.stream, c.options.smoother, solve, SmoothPass{src, b, dst, coef, i == passes ? last_scale : 1.0f}, c.dims, bc));
+    const int pairs    = use_pairs(c, bc) ? passes / 2 : 0;
+    const int singles  = passes - 2 * pairs;
+    const int launches = singles + pairs;
+    const float* src   = x0;
+    for (int l = 0; l < launches; ++l) {
+        float* dst = ((launches - 1 - l) % 2 == 0) ? out : aux;
+        const SmoothPass pass{src, b, dst, coef, l == launches - 1 ? last_scale : 1.0f};
+        if (l < singles) SF_TRY(launch_smooth(c.stream, c.options.smoother, solve, pass, c.dims, bc));
+        else SF_TRY(launch_smooth2(c.stream, solve, pass, c.dims, bc));
         src = dst;
     }
     return hipSuccess;
@@ -270,12 +281,29 @@ hipError_t enqueue_learned_pressure(Context& c, const StencilBC& p_bc) {
     const int fac  = c.options.warm_factor;
     const Dims cd  = unet::net_dims(*w.net);
     const uint64_t vc = static_cast<uint64_t>(cd.nx) * cd.ny * cd.nz;
-    float* buf[2]  = {(P % 2 == 0) ? f.p : f.p_aux, (P % 2 == 0) ? f.p_aux : f.p}; // buf[P % 2] = p
+    // Pass pairs (P and C even): launch j of P / 2 runs passes 2j + 1, 2j + 2 into dst(j),
+    // the last into p; the first C / 2 are skipped on an accepted guess, which therefore
+    // goes where launch C / 2 reads it, dst(C / 2 - 1). Launch 0 starts from zero when it
+    // can be skipped (it only runs on fallback, as the plain solve's first launch).
+    const bool pairs = use_pairs(c, p_bc) && P % 2 == 0;
+    const int L      = P / 2;
+    auto dst         = [&](const int j) { return ((L - 1 - j) & 1) == 0 ? f.p : f.p_aux; };
+    float* buf[2]    = {(P % 2 == 0) ? f.p : f.p_aux, (P % 2 == 0) ? f.p_aux : f.p}; // buf[P % 2] = p
+    float* guess     = pairs ? dst(C / 2 - 1) : buf[0];
     SF_TRY(warm::launch_restrict_input(c.stream, f.rhs, w.in2, w.partials, w.scal, cd, fac, p_bc));
     SF_TRY(unet::launch_pack_input(c.stream, w.in2, w.scal + 1, 1.0f, unet::net_input(*w.net), vc));
     SF_TRY(unet::net_forward_packed(*w.net, false));
-    SF_TRY(warm::launch_prolong(c.stream, unet::net_output(*w.net), w.scal, static_cast<float>(fac * fac) * h * h, buf[0], cd, fac));
-    SF_TRY(warm::launch_guess_test(c.stream, buf[0], f.rhs, h * h, 10.0, w.partials, w.stats, w.skip, c.dims, p_bc));
+    SF_TRY(warm::launch_prolong(c.stream, unet::net_output(*w.net), w.scal, static_cast<float>(fac * fac) * h * h, guess, cd, fac));
+    SF_TRY(warm::launch_guess_test(c.stream, guess, f.rhs, h * h, 10.0, w.partials, w.stats, w.skip, c.dims, p_bc));
+    if (pairs) {
+        for (int j = 0; j < L; ++j) {
+            const float* src = j == 0 ? (C > 0 ? nullptr : guess) : dst(j - 1);
+            SmoothPass pass{src, f.rhs, dst(j), h * h, 1.0f};
+            if (2 * j < C) pass.skip = w.skip;
+            SF_TRY(launch_smooth2(c.stream, kSolvePoisson, pass, c.dims, p_bc));
+        }
+        return hipSuccess;
+    }
     for (int i = 1; i <= P; ++i) {
         SmoothPass pass{buf[(i - 1) & 1], f.rhs, buf[i & 1], h * h, 1.0f};
         if (i <= C) pass.skip = w.skip;
@@ -388,8 +416,8 @@ StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const Stabl
         if (opt->iterate_storage > STABLE_FLUIDS_EXT_STORAGE_F32_REFINED) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         if (opt->defect_correction_interval > 4096) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         if (opt->pressure_warm_start > STABLE_FLUIDS_EXT_WARM_START_LEARNED) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
-        for (uint32_t v : opt->reserved)
-            if (v != 0) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        if (opt->flags & ~static_cast<uint32_t>(STABLE_FLUIDS_EXT_FLAG_PASS_PAIRS)) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        options.pairs = (opt->flags & STABLE_FLUIDS_EXT_FLAG_PASS_PAIRS) != 0;
         options.smoother = opt->smoother == STABLE_FLUIDS_EXT_SMOOTHER_JACOBI ? kSmoothJacobi : kSmoothRBGS;
         options.refine   = opt->iterate_storage != STABLE_FLUIDS_EXT_STORAGE_F32;
         options.storage  = opt->iterate_storage == STABLE_FLUIDS_EXT_STORAGE_F16 ? kStoreF16 : (opt->iterate_storage == STABLE_FLUIDS_EXT_STORAGE_BF16 ? kStoreBF16 : kStoreF32);
@@ -1046,7 +1074,9 @@ STABLE_FLUIDS_API uint64_t stable_fluids_ext_stage_bytes_cuda(void* context, uin
     switch (stage) {
     case STABLE_FLUIDS_EXT_STAGE_PRESSURE_PASS:
     case STABLE_FLUIDS_EXT_STAGE_VELOCITY_PASS:
-    case STABLE_FLUIDS_EXT_STAGE_JACOBI_PASS: return 12ull * c.cells; // read x_old, b; write x_new
+    case STABLE_FLUIDS_EXT_STAGE_JACOBI_PASS:
+    case STABLE_FLUIDS_EXT_STAGE_PRESSURE_PAIR:
+    case STABLE_FLUIDS_EXT_STAGE_VELOCITY_PAIR: return 12ull * c.cells; // read x_old, b; write x_new (a pair: two passes)
     case STABLE_FLUIDS_EXT_STAGE_ADVECT: return 24ull * c.cells;        // read v (3); write t (3)
     case STABLE_FLUIDS_EXT_STAGE_PROJECT: return 36ull * c.cells;       // read a (3), p; write v (3), div, |v|
     case STABLE_FLUIDS_EXT_STAGE_COPY: return 8ull * c.cells;           // read one field, write one
@@ -1259,6 +1289,19 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_run_stage_cuda(void* cont
         case STABLE_FLUIDS_EXT_STAGE_JACOBI_PASS: {
             const bool odd = i & 1;
             e = sf::launch_smooth(c.stream, sf::kSmoothJacobi, sf::kSolvePoisson, sf::SmoothPass{odd ? f.p : f.p_aux, f.div, odd ? f.p_aux : f.p, h * h, 1.0f}, c.dims, sf::pressure_rules(fb));
+            break;
+        }
+        case STABLE_FLUIDS_EXT_STAGE_PRESSURE_PAIR: {
+            const bool odd = i & 1;
+            if (!sf::smooth2_supported(sf::kSmoothRBGS, c.dims, sf::pressure_rules(fb))) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+            e = sf::launch_smooth2(c.stream, sf::kSolvePoisson, sf::SmoothPass{odd ? f.p : f.p_aux, f.div, odd ? f.p_aux : f.p, h * h, 1.0f}, c.dims, sf::pressure_rules(fb));
+            break;
+        }
+        case STABLE_FLUIDS_EXT_STAGE_VELOCITY_PAIR: {
+            const bool odd    = i & 1;
+            const float alpha = c.config.dt * c.config.viscosity / (h * h);
+            if (!sf::smooth2_supported(sf::kSmoothRBGS, c.dims, sf::velocity_rules(fb, 0))) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+            e = sf::launch_smooth2(c.stream, sf::kSolveDiffusion, sf::SmoothPass{odd ? f.ax : f.vx, f.tx, odd ? f.vx : f.ax, alpha, 1.0f}, c.dims, sf::velocity_rules(fb, 0));
             break;
         }
         case STABLE_FLUIDS_EXT_STAGE_COPY: e = sf::launch_copy_scaled(c.stream, f.ax, f.vx, 1.0f, c.cells); break;

@@ -110,7 +110,8 @@ def load_library(path=LIB_PATH):
 
 
 def ext_options(smoother=abi.SMOOTHER_RBGS, iterate_storage=abi.STORAGE_F32, defect_correction_interval=0,
-                pressure_warm_start=abi.WARM_START_ZERO, warm_start_iterations=0, warm_start_coarsening=0):
+                pressure_warm_start=abi.WARM_START_ZERO, warm_start_iterations=0, warm_start_coarsening=0,
+                pass_pairs=False):
     o = abi.ExtOptions()
     o.struct_size = C.sizeof(abi.ExtOptions)
     o.smoother = smoother
@@ -119,6 +120,7 @@ def ext_options(smoother=abi.SMOOTHER_RBGS, iterate_storage=abi.STORAGE_F32, def
     o.pressure_warm_start = pressure_warm_start
     o.warm_start_iterations = warm_start_iterations
     o.warm_start_coarsening = warm_start_coarsening
+    o.flags = abi.FLAG_PASS_PAIRS if pass_pairs else 0
     return o
 
 
