@@ -5,7 +5,8 @@
 // stencil kernels bound their lanes (and switch whole planes off by giving the resource
 // 0 bytes) without branches around memory instructions. Branch-free memory keeps the
 // compiler's wait counts exact, which is what lets loads stay in flight across planes.
-// Offsets are 32-bit byte offsets within one plane.
+// Offsets are 32-bit byte offsets within one plane, or within a whole field below 2 GiB
+// when the plane is chosen by the scalar offset.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -33,43 +34,43 @@ __device__ __forceinline__ Rsrc plane_rsrc(const void* base, const uint32_t byte
     return __builtin_amdgcn_make_buffer_rsrc(q, 0, static_cast<int>(n), kBufferConfig);
 }
 
-// Aligned pair of elements at byte offset `off`.
+// Aligned pair of elements at byte offset `off` (+ the wave-uniform `soff`).
 template <class St>
-__device__ __forceinline__ float2 buf_load2(const Rsrc r, const int off) {
+__device__ __forceinline__ float2 buf_load2(const Rsrc r, const int off, const int soff = 0) {
     if constexpr (St::kNarrow) {
-        const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+        const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0);
         if constexpr (__is_same(typename St::T, _Float16))
             return make_float2(static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w & 0xffffu))), static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(w >> 16))));
         else
             return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
     } else {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0);
         return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
     }
 }
 
-// One element at byte offset `off`.
+// One element at byte offset `off` (+ the wave-uniform `soff`).
 template <class St>
-__device__ __forceinline__ float buf_load1(const Rsrc r, const int off) {
+__device__ __forceinline__ float buf_load1(const Rsrc r, const int off, const int soff = 0) {
     if constexpr (St::kNarrow) {
-        const uint16_t h = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+        const uint16_t h = __builtin_amdgcn_raw_buffer_load_b16(r, off, soff, 0);
         return St::load(__builtin_bit_cast(typename St::T, h));
     } else {
-        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
     }
 }
 
 // Aligned pair (a, b) at byte offset `off`, rounded to the storage type. (Non-temporal
 // stores, aux bit NT, measured slower for the smoother outputs: the next pass reads them.)
 template <class St, int kPolicy = 0>
-__device__ __forceinline__ void buf_store2(const Rsrc r, const int off, const float a, const float b) {
+__device__ __forceinline__ void buf_store2(const Rsrc r, const int off, const float a, const float b, const int soff = 0) {
     if constexpr (St::kNarrow) {
         const uint32_t w = static_cast<uint32_t>(__builtin_bit_cast(uint16_t, St::store(a))) | (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, St::store(b))) << 16);
-        __builtin_amdgcn_raw_buffer_store_b32(w, r, off, 0, kPolicy);
+        __builtin_amdgcn_raw_buffer_store_b32(w, r, off, soff, kPolicy);
     } else {
         typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
         const u32x2 v = {__float_as_uint(a), __float_as_uint(b)};
-        __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, kPolicy);
+        __builtin_amdgcn_raw_buffer_store_b64(v, r, off, soff, kPolicy);
     }
 }
 

@@ -806,7 +806,7 @@ __device__ __forceinline__ int wrap_near(const int k, const int n) {
 // register sets. Two barriers per plane: after staging old(k + 1) and after H(k); the
 // next plane's first barrier already orders the reuse of the H ring, the second one that
 // of the old ring.
-template <int kSolve, bool kZero, class St>
+template <int kSolve, bool kZero, class St, bool kWhole>
 __device__ __forceinline__ void rbgs_rows_body(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC& bc, const TileGeom& g, const float* __restrict__ scale_slot, const int w) {
     using T              = typename St::T;
     constexpr int kT     = static_cast<int>(sizeof(T));
@@ -835,6 +835,12 @@ __device__ __forceinline__ void rbgs_rows_body(const typename St::T* __restrict_
     const uint32_t b_bytes = plane * static_cast<uint32_t>(sizeof(float));
     const bool full_x      = x0 + kTX <= d.nx;
     const bool full_y      = y0 + kTY <= d.ny;
+    // kWhole (fields below 2 GiB): one resource per field, the plane chosen by the scalar
+    // offset, and a 0-byte resource for idle planes; otherwise one resource per plane
+    // (a resource is rebuilt from the field base with the size selected, which keeps it in
+    // scalar registers; selecting between two resources would not)
+    const uint32_t x_field = kWhole ? x_bytes * static_cast<uint32_t>(d.nz) : 0u;
+    const uint32_t b_field = kWhole ? b_bytes * static_cast<uint32_t>(d.nz) : 0u;
 
     // the lane's pair on the tile rows; past a wall (partial tile) both cells take the x+
     // ghost of cell nx - 1, across a periodic x face the pair wraps
@@ -944,10 +950,17 @@ __device__ __forceinline__ void rbgs_rows_body(const typename St::T* __restrict_
 
     auto fetch_old = [&](const int k, OldRegs& r) {
         if constexpr (kZero) return;
-        const Rsrc src = plane_rsrc(x_old + static_cast<uint64_t>(plane_of(k)) * plane, k <= z1 + 1 ? x_bytes : 0u);
+        Rsrc src;
+        int so = 0;
+        if constexpr (kWhole) {
+            src = plane_rsrc(x_old, k <= z1 + 1 ? x_field : 0u);
+            so  = plane_of(k) * static_cast<int>(x_bytes);
+        } else {
+            src = plane_rsrc(x_old + static_cast<uint64_t>(plane_of(k)) * plane, k <= z1 + 1 ? x_bytes : 0u);
+        }
 #pragma unroll
-        for (int j = 0; j < kRowsW; ++j) r.t[j] = buf_load2<St>(src, t_off[j]);
-        r.m = buf_load2<St>(src, m_off);
+        for (int j = 0; j < kRowsW; ++j) r.t[j] = buf_load2<St>(src, t_off[j], so);
+        r.m = buf_load2<St>(src, m_off, so);
     };
     auto stash_old = [&](const int k, float* s, const OldRegs& r) {
         const bool zg      = !pz && (k < 0 || k >= d.nz);
@@ -977,18 +990,33 @@ __device__ __forceinline__ void rbgs_rows_body(const typename St::T* __restrict_
     // b of the red cells of H plane k (H planes z0 - 1 .. z1, real planes only)
     auto fetch_bh = [&](const int k, HRegs& r) {
         const bool live = k >= z0 - 1 && k <= z1 && (pz || (k >= 0 && k < d.nz));
-        const Rsrc src  = plane_rsrc(b + static_cast<uint64_t>(plane_of(k)) * plane, live ? b_bytes : 0u);
         const int a_t   = (y0 + wave + k) & 1; // 1: the red cell of a tile-row pair is the odd one
+        Rsrc src;
+        int so = 4 * a_t;
+        if constexpr (kWhole) {
+            src = plane_rsrc(b, live ? b_field : 0u);
+            so += plane_of(k) * static_cast<int>(b_bytes);
+        } else {
+            src = plane_rsrc(b + static_cast<uint64_t>(plane_of(k)) * plane, live ? b_bytes : 0u);
+        }
 #pragma unroll
-        for (int j = 0; j < kRowsW; ++j) r.t[j] = buf_load1<StoreF32>(src, hb_off[j] + 4 * a_t);
-        r.m = buf_load1<StoreF32>(src, n_off + 4 * ((n_ypar + k) & 1));
+        for (int j = 0; j < kRowsW; ++j) r.t[j] = buf_load1<StoreF32>(src, hb_off[j], so);
+        r.m = buf_load1<StoreF32>(src, n_off + 4 * ((n_ypar + k) & 1), so - 4 * a_t);
     };
     // b of the black cells of final plane k
     auto fetch_bf = [&](const int k, float (&r)[kOutPer]) {
-        const Rsrc src = plane_rsrc(b + static_cast<uint64_t>(k) * plane, (k >= z0 && k < z1) ? b_bytes : 0u);
-        const int e    = ((y0 + wave + k) & 1) == 0 ? 1 : 0;
+        const bool live = k >= z0 && k < z1;
+        const int e     = ((y0 + wave + k) & 1) == 0 ? 1 : 0;
+        Rsrc src;
+        int so = 4 * e;
+        if constexpr (kWhole) {
+            src = plane_rsrc(b, live ? b_field : 0u);
+            so += (live ? k : 0) * static_cast<int>(b_bytes);
+        } else {
+            src = plane_rsrc(b + static_cast<uint64_t>(k) * plane, live ? b_bytes : 0u);
+        }
 #pragma unroll
-        for (int j = 0; j < kOutPer; ++j) r[j] = buf_load1<StoreF32>(src, bf_off[j] + 4 * e);
+        for (int j = 0; j < kOutPer; ++j) r[j] = buf_load1<StoreF32>(src, bf_off[j], so);
     };
     // H(k): red value of every pair on the tile plus a one-pair halo; ghosts keep the old value
     auto compute_h = [&](const int k, const float* om, const float* o0, const float* op, float* h0, const HRegs& bh) {
@@ -1040,7 +1068,14 @@ __device__ __forceinline__ void rbgs_rows_body(const typename St::T* __restrict_
     auto compute_final = [&](const int k, const float* hm2, const float* hm1, const float* h0, const float (&bf)[kOutPer]) {
         const int kf     = k - 1;
         const bool live  = kf >= z0 && kf < z1;
-        const Rsrc dst   = plane_rsrc(x_new + static_cast<uint64_t>(live ? kf : 0) * plane, live ? x_bytes : 0u);
+        Rsrc dst;
+        int so = 0;
+        if constexpr (kWhole) {
+            dst = plane_rsrc(x_new, live ? x_field : 0u);
+            so  = (live ? kf : 0) * static_cast<int>(x_bytes);
+        } else {
+            dst = plane_rsrc(x_new + static_cast<uint64_t>(live ? kf : 0) * plane, live ? x_bytes : 0u);
+        }
         const int e      = ((y0 + wave + kf) & 1) == 0 ? 1 : 0; // 1: the red cell is the even one
         StageDiv<kSolve> dvs(coef);
         float x[kOutPer], q[kOutPer], red[kOutPer];
@@ -1065,7 +1100,7 @@ __device__ __forceinline__ void rbgs_rows_body(const typename St::T* __restrict_
         for (int j = 0; j < kOutPer; ++j) {
             const float ve = (e ? red[j] : q[j]) * out_scale;
             const float vo = (e ? q[j] : red[j]) * out_scale;
-            buf_store2<St>(dst, st_off[j], ve, vo);
+            buf_store2<St>(dst, st_off[j], ve, vo, so);
         }
     };
     // plane k; register sets alternate with k - z0 (set 0 at k = z0 - 1): old(m), bh(m)
@@ -1406,11 +1441,11 @@ __global__ __launch_bounds__(kThreads) void k_jacobi_pass(const typename St::T* 
         jacobi_body<kSolve, kZero, St>(x_old, b, x_new, coef, out_scale, d, bc, g, scale_slot, xcd_block());
 }
 
-template <int kSolve, bool kZero, class St, bool kPairs>
+template <int kSolve, bool kZero, class St, bool kPairs, bool kWhole = false>
 __global__ __launch_bounds__(kThreads) void k_rbgs_split_pass(const typename St::T* __restrict__ x_old, const float* __restrict__ b, typename St::T* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g, const float* __restrict__ scale_slot, const int* __restrict__ skip) {
     if (skip != nullptr && *skip != 0) return;
     if constexpr (kPairs)
-        rbgs_rows_body<kSolve, kZero, St>(x_old, b, x_new, coef, out_scale, d, bc, g, scale_slot, xcd_block());
+        rbgs_rows_body<kSolve, kZero, St, kWhole>(x_old, b, x_new, coef, out_scale, d, bc, g, scale_slot, xcd_block());
     else
         rbgs_split_body<kSolve, kZero, St, false>(x_old, b, x_new, coef, out_scale, d, bc, g, scale_slot, xcd_block());
 }
@@ -1423,7 +1458,7 @@ __global__ __launch_bounds__(kThreads) void k_rbgs_split_batched(const SmoothDes
     const BatchItem it   = items[blockIdx.x];
     const SmoothDesc& sd = descs[it.desc];
     if constexpr (kPairs)
-        rbgs_rows_body<kSolve, kZero, StoreF32>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
+        rbgs_rows_body<kSolve, kZero, StoreF32, false>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
     else
         rbgs_split_body<kSolve, kZero, StoreF32, false>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
 }
@@ -1515,8 +1550,6 @@ __device__ __forceinline__ void rbgs2_body(const float* __restrict__ x_old, cons
     const uint32_t fbyte = static_cast<uint32_t>(pbyte) * static_cast<uint32_t>(nz);
     const Rsrc r_old     = plane_rsrc(x_old, kZero ? 0u : fbyte);
     const Rsrc r_b       = plane_rsrc(b, fbyte);
-    const Rsrc r_new     = plane_rsrc(x_new, fbyte);
-    const Rsrc r_none    = plane_rsrc(x_new, 0u);
 
     // ghost rules in LDS (a, c, k of faces 0..5, identity at 6): read only where a tile
     // touches a face, so they do not occupy scalar registers in the march
@@ -1807,7 +1840,7 @@ __device__ __forceinline__ void rbgs2_body(const float* __restrict__ x_old, cons
         __syncthreads();
         const bool live = k >= z0 + 4;
         if (live) stage.template operator()<4, (U + 2) % 3, (U + 0) % 3, (U + 1) % 3, 0>(k - 4, 1, p4);
-        const Rsrc dst = live ? r_new : r_none;
+        const Rsrc dst = plane_rsrc(x_new, live ? fbyte : 0u); // a size select keeps it scalar
         const int t    = (((p4 >> 2) ^ k) & 1) * 4; // black cell of the wave's rows (plane k - 4)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
@@ -1892,10 +1925,14 @@ hipError_t launch_smooth_batched(hipStream_t s, const SmootherKind smoother, con
 }
 
 
+// Whether a field of `elem` byte elements is below 2 GiB (one buffer resource spans it).
+static bool field_below_2g(const Dims d, const int elem) { return static_cast<uint64_t>(elem) * static_cast<uint64_t>(d.nx) * static_cast<uint64_t>(d.ny) * static_cast<uint64_t>(d.nz) < (1ull << 31); }
+
 hipError_t launch_smooth(hipStream_t s, const SmootherKind smoother, const SolveKind solve, const SmoothPass& pass, const Dims d, const StencilBC& bc) {
     const TileGeom g      = plan_tiles(d, 0);
     const unsigned blocks = static_cast<unsigned>(g.tiles_x * g.tiles_y * g.chunks);
     const bool zero       = pass.x_old == nullptr;
+    const bool whole      = field_below_2g(d, 4);
 #define SF_PASS_GENERAL(SO, Z) k_rbgs_pass<SO, Z><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g, pass.skip)
 #define SF_JPASS(SO, Z)                                                                                                                                                            \
     do {                                                                                                                                                                            \
@@ -1904,7 +1941,8 @@ hipError_t launch_smooth(hipStream_t s, const SmootherKind smoother, const Solve
     } while (0)
 #define SF_SPASS(SO, Z)                                                                                                                                                                \
     do {                                                                                                                                                                                \
-        if ((d.nx & 1) == 0) k_rbgs_split_pass<SO, Z, StoreF32, true><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g, nullptr, pass.skip); \
+        if ((d.nx & 1) == 0 && whole) k_rbgs_split_pass<SO, Z, StoreF32, true, true><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g, nullptr, pass.skip); \
+        else if ((d.nx & 1) == 0) k_rbgs_split_pass<SO, Z, StoreF32, true><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g, nullptr, pass.skip); \
         else k_rbgs_split_pass<SO, Z, StoreF32, false><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g, nullptr, pass.skip);         \
     } while (0)
 #define SF_PASS(KERNEL, SO, Z) KERNEL<SO, Z, StoreF32><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g, nullptr, pass.skip)
@@ -1971,9 +2009,11 @@ hipError_t correction_pass(hipStream_t s, const SmootherKind smoother, const Sol
     const unsigned blocks = static_cast<unsigned>(g.tiles_x * g.tiles_y * g.chunks);
     const auto* e_old     = static_cast<const typename St::T*>(pass.e_old);
     auto* e_new           = static_cast<typename St::T*>(pass.e_new);
+    const bool whole      = field_below_2g(d, 4); // the right-hand side r is FP32
 #define SF_SCORR(SO, Z)                                                                                                                                      \
     do {                                                                                                                                                     \
-        if ((d.nx & 1) == 0) k_rbgs_split_pass<SO, Z, St, true><<<blocks, kThreads, 0, s>>>(e_old, pass.r, e_new, pass.coef, 1.0f, d, bc, g, pass.scale_slot, nullptr); \
+        if ((d.nx & 1) == 0 && whole) k_rbgs_split_pass<SO, Z, St, true, true><<<blocks, kThreads, 0, s>>>(e_old, pass.r, e_new, pass.coef, 1.0f, d, bc, g, pass.scale_slot, nullptr); \
+        else if ((d.nx & 1) == 0) k_rbgs_split_pass<SO, Z, St, true><<<blocks, kThreads, 0, s>>>(e_old, pass.r, e_new, pass.coef, 1.0f, d, bc, g, pass.scale_slot, nullptr); \
         else k_rbgs_split_pass<SO, Z, St, false><<<blocks, kThreads, 0, s>>>(e_old, pass.r, e_new, pass.coef, 1.0f, d, bc, g, pass.scale_slot, nullptr);         \
     } while (0)
 #define SF_JCORR(SO, Z)                                                                                                                                      \
