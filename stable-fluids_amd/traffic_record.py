@@ -19,6 +19,7 @@ import json
 # bytes per cell a launch must move at minimum (4-byte fields)
 ALGORITHMIC_BYTES_PER_CELL = {
     "k_rbgs_split_pass": 12,      # read x_old, b; write x_new
+    "k_rbgs2_pass": 12,           # two passes: read x_old, b; write x_new once
     "k_jacobi_pass": 12,          # read x_old, b; write x_new
     "k_copy_scaled": 8,           # read src; write dst
     "k_advect_velocity": 24,      # read 3 velocity components; write 3
@@ -43,7 +44,7 @@ def algorithmic_bytes(name, cells):
     per = ALGORITHMIC_BYTES_PER_CELL.get(base)
     if per is None:
         return None
-    if base in ("k_rbgs_split_pass", "k_jacobi_pass") and name.split("<")[1].split(",")[1].strip() == "true":
+    if base in ("k_rbgs_split_pass", "k_rbgs2_pass", "k_jacobi_pass") and name.split("<")[1].split(",")[1].strip() == "true":
         per = ZERO_GUESS_BYTES_PER_CELL
     return per * cells
 
