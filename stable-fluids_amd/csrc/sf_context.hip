@@ -267,10 +267,8 @@ hipError_t enqueue_solve(Context& c, SolveKind solve, const float* b, const floa
 
 // Learned warm start in place of the zero-initialised pressure solve (reference
 // backend-cuda.cu:643-650): network guess from the half-resolution divergence, residual
-// test, then P passes of which the first C = P - K are skipped when the guess is
-// accepted. C is even, so the skipped passes leave the guess in the buffer the first
-// unskipped pass reads; on fallback the guess is zeroed and all P passes run, exactly the
-// plain solve (its first pass reads zeros instead of assuming them).
+// test, then P passes of which the first C = P - K (even) are skipped when the guess is
+// accepted; on fallback all P passes run from zero, exactly the plain solve.
 hipError_t enqueue_learned_pressure(Context& c, const StencilBC& p_bc) {
     auto& f        = c.flow;
     auto& w        = c.warm;
@@ -281,33 +279,38 @@ hipError_t enqueue_learned_pressure(Context& c, const StencilBC& p_bc) {
     const int fac  = c.options.warm_factor;
     const Dims cd  = unet::net_dims(*w.net);
     const uint64_t vc = static_cast<uint64_t>(cd.nx) * cd.ny * cd.nz;
-    // Pass pairs (P and C even): launch j of P / 2 runs passes 2j + 1, 2j + 2 into dst(j),
-    // the last into p; the first C / 2 are skipped on an accepted guess, which therefore
-    // goes where launch C / 2 reads it, dst(C / 2 - 1). Launch 0 starts from zero when it
-    // can be skipped (it only runs on fallback, as the plain solve's first launch).
-    const bool pairs = use_pairs(c, p_bc) && P % 2 == 0;
-    const int L      = P / 2;
-    auto dst         = [&](const int j) { return ((L - 1 - j) & 1) == 0 ? f.p : f.p_aux; };
-    float* buf[2]    = {(P % 2 == 0) ? f.p : f.p_aux, (P % 2 == 0) ? f.p_aux : f.p}; // buf[P % 2] = p
-    float* guess     = pairs ? dst(C / 2 - 1) : buf[0];
+    // The P passes run as a list of launches of one or two passes (RBGS pass pairs where
+    // the grid allows them: all of them with the pass-pair option, otherwise only the C
+    // fallback-only passes, whose skipped launches each cost a dispatch). Launch l of L
+    // writes dst(l), the last one p. The launches covering the first C passes are skipped
+    // on an accepted guess, which therefore goes where the next launch reads it, and the
+    // first launch starts from zero when it can be skipped (it only runs on fallback, as
+    // the plain solve's first launch). Pair and single launches give identical results.
+    const bool two = smooth2_supported(c.options.smoother, c.dims, p_bc);
+    std::vector<int> per; // passes per launch
+    if (two && c.options.pairs && P % 2 == 0) per.assign(P / 2, 2);
+    else if (two) {
+        per.assign(C / 2, 2); // C is even
+        per.insert(per.end(), K, 1);
+    } else {
+        per.assign(P, 1);
+    }
+    const int L  = static_cast<int>(per.size());
+    int skipped  = 0; // launches covering the first C passes
+    for (int acc = 0; acc < C;) acc += per[skipped++];
+    auto dst     = [&](const int l) { return ((L - 1 - l) & 1) == 0 ? f.p : f.p_aux; };
+    float* guess = dst(skipped - 1);
     SF_TRY(warm::launch_restrict_input(c.stream, f.rhs, w.in2, w.partials, w.scal, cd, fac, p_bc));
     SF_TRY(unet::launch_pack_input(c.stream, w.in2, w.scal + 1, 1.0f, unet::net_input(*w.net), vc));
     SF_TRY(unet::net_forward_packed(*w.net, false));
     SF_TRY(warm::launch_prolong(c.stream, unet::net_output(*w.net), w.scal, static_cast<float>(fac * fac) * h * h, guess, cd, fac));
     SF_TRY(warm::launch_guess_test(c.stream, guess, f.rhs, h * h, 10.0, w.partials, w.stats, w.skip, c.dims, p_bc));
-    if (pairs) {
-        for (int j = 0; j < L; ++j) {
-            const float* src = j == 0 ? (C > 0 ? nullptr : guess) : dst(j - 1);
-            SmoothPass pass{src, f.rhs, dst(j), h * h, 1.0f};
-            if (2 * j < C) pass.skip = w.skip;
-            SF_TRY(launch_smooth2(c.stream, kSolvePoisson, pass, c.dims, p_bc));
-        }
-        return hipSuccess;
-    }
-    for (int i = 1; i <= P; ++i) {
-        SmoothPass pass{buf[(i - 1) & 1], f.rhs, buf[i & 1], h * h, 1.0f};
-        if (i <= C) pass.skip = w.skip;
-        SF_TRY(launch_smooth(c.stream, c.options.smoother, kSolvePoisson, pass, c.dims, p_bc));
+    for (int l = 0; l < L; ++l) {
+        const float* src = l == 0 ? (C > 0 ? nullptr : guess) : dst(l - 1);
+        SmoothPass pass{src, f.rhs, dst(l), h * h, 1.0f};
+        if (l < skipped) pass.skip = w.skip;
+        if (per[l] == 2) SF_TRY(launch_smooth2(c.stream, kSolvePoisson, pass, c.dims, p_bc));
+        else SF_TRY(launch_smooth(c.stream, c.options.smoother, kSolvePoisson, pass, c.dims, p_bc));
     }
     return hipSuccess;
 }
