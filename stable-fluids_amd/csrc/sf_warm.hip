@@ -220,32 +220,78 @@ __global__ __launch_bounds__(kThreads) void k_prolong(const float* __restrict__ 
     }
 }
 
+// Residual sums of the guess and of the zero guess over the block's rows. Cells off the
+// faces take six direct neighbours (same order as neighbour_sum); rows come four at a time
+// and the loads of all four are issued before their FP64 arithmetic (one row at a time
+// left every thread waiting out the memory latency once per row). Face cells take the
+// ghost rules in a second phase with one lane per cell: left in the row loop, the two x
+// faces of every row sent half of all waves through the ghost path for a single lane.
 __global__ __launch_bounds__(kThreads) void k_guess_partials(const float* __restrict__ p0, const float* __restrict__ div, const float h2, double* __restrict__ partials, const Dims d, const StencilBC bc) {
-    double acc_g = 0.0, acc_z = 0.0;
-    for (Rows w = block_rows(d.ny, d.nz); w.r < w.end; next_row(w, d.ny))
-    for (int x = static_cast<int>(threadIdx.x); x < d.nx; x += kThreads) {
-        const int y      = w.y, z = w.z;
-        const uint64_t i = w.r * static_cast<uint64_t>(d.nx) + static_cast<uint64_t>(x);
-        double b, ap;
-        const double pc = p0[i];
-        if (x > 0 && x < d.nx - 1 && y > 0 && y < d.ny - 1 && z > 0 && z < d.nz - 1) {
-            // no face: no ghost constant, six direct neighbours (same order as neighbour_sum)
-            const uint64_t sy = static_cast<uint64_t>(d.nx), sz = sy * static_cast<uint64_t>(d.ny);
-            float sum = p0[i - 1];
-            sum += p0[i + 1];
-            sum += p0[i - sy];
-            sum += p0[i + sy];
-            sum += p0[i - sz];
-            sum += p0[i + sz];
-            b  = -static_cast<double>(h2) * div[i];
-            ap = 6.0 * pc - static_cast<double>(sum);
-        } else {
-            b  = -static_cast<double>(h2) * div[i] + ghost_constant(x, y, z, d, bc);
-            ap = 6.0 * pc - static_cast<double>(neighbour_sum<float>(p0, x, y, z, d, bc, p0[i], true));
-        }
+    constexpr int kU  = 4;
+    double acc_g      = 0.0, acc_z = 0.0;
+    const uint64_t sy = static_cast<uint64_t>(d.nx), sz = sy * static_cast<uint64_t>(d.ny);
+    auto face_cell    = [&](const int x, const int y, const int z) {
+        const uint64_t i = (static_cast<uint64_t>(z) * static_cast<uint64_t>(d.ny) + static_cast<uint64_t>(y)) * sy + static_cast<uint64_t>(x);
+        const float c    = p0[i];
+        const double b   = -static_cast<double>(h2) * div[i] + ghost_constant(x, y, z, d, bc);
+        const double ap  = 6.0 * c - static_cast<double>(neighbour_sum<float>(p0, x, y, z, d, bc, c, true));
         acc_g += (b - ap) * (b - ap);
         acc_z += b * b;
+    };
+    const Rows w0 = block_rows(d.ny, d.nz);
+    Rows w        = w0;
+    while (w.r < w.end) {
+        uint64_t ri[kU];
+        bool rv[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            rv[u] = w.r < w.end && w.y > 0 && w.y < d.ny - 1 && w.z > 0 && w.z < d.nz - 1;
+            ri[u] = w.r;
+            if (w.r < w.end) next_row(w, d.ny);
+        }
+        for (int x = static_cast<int>(threadIdx.x); x < d.nx; x += kThreads) {
+            float c[kU], n0[kU], n1[kU], n2[kU], n3[kU], n4[kU], n5[kU], dv[kU];
+            bool fast[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                fast[u]           = rv[u] && x > 0 && x < d.nx - 1;
+                const uint64_t i  = fast[u] ? ri[u] * sy + static_cast<uint64_t>(x) : 0;
+                const uint64_t ox = fast[u] ? 1 : 0, oy = fast[u] ? sy : 0, oz = fast[u] ? sz : 0;
+                c[u]              = p0[i];
+                n0[u]             = p0[i - ox];
+                n1[u]             = p0[i + ox];
+                n2[u]             = p0[i - oy];
+                n3[u]             = p0[i + oy];
+                n4[u]             = p0[i - oz];
+                n5[u]             = p0[i + oz];
+                dv[u]             = div[i];
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                if (!fast[u]) continue;
+                float sum = n0[u];
+                sum += n1[u];
+                sum += n2[u];
+                sum += n3[u];
+                sum += n4[u];
+                sum += n5[u];
+                const double b  = -static_cast<double>(h2) * dv[u];
+                const double ap = 6.0 * c[u] - static_cast<double>(sum);
+                acc_g += (b - ap) * (b - ap);
+                acc_z += b * b;
+            }
+        }
     }
+    // face cells: x = 0 and x = nx - 1 of every row, then the rest of the rows on a y or z face
+    const uint32_t nrows = static_cast<uint32_t>(w0.end - w0.r);
+    const uint32_t per   = d.nx > 1 ? 2u : 1u;
+    for (uint32_t e = threadIdx.x; e < per * nrows; e += kThreads) {
+        const uint32_t r = static_cast<uint32_t>(w0.r) + e / per;
+        face_cell((e % per) ? d.nx - 1 : 0, static_cast<int>(r % static_cast<uint32_t>(d.ny)), static_cast<int>(r / static_cast<uint32_t>(d.ny)));
+    }
+    for (Rows v = w0; v.r < v.end; next_row(v, d.ny))
+        if (v.y == 0 || v.y == d.ny - 1 || v.z == 0 || v.z == d.nz - 1)
+            for (int x = 1 + static_cast<int>(threadIdx.x); x < d.nx - 1; x += kThreads) face_cell(x, v.y, v.z);
     block_sum_to(acc_g, partials + 2 * blockIdx.x);
     block_sum_to(acc_z, partials + 2 * blockIdx.x + 1);
 }
