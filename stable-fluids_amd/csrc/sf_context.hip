@@ -103,7 +103,8 @@ struct Context {
         float* scal      = nullptr; // sigma, 1/sigma
         double* partials = nullptr;
         double* stats    = nullptr; // guess residual, zero residual, fallback
-        int* skip        = nullptr; // 1: skip the fallback passes
+        int* skip        = nullptr; // [0] 1: skip the fallback passes; [1] grid-barrier count; [2] barrier timeout
+        uint32_t chain_grid = 0;    // workgroups of the one-launch fallback solve (0: not used)
     } warm;
     struct {
         double *x = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *partials = nullptr, *scal = nullptr;
@@ -279,17 +280,26 @@ hipError_t enqueue_learned_pressure(Context& c, const StencilBC& p_bc) {
     const int fac  = c.options.warm_factor;
     const Dims cd  = unet::net_dims(*w.net);
     const uint64_t vc = static_cast<uint64_t>(cd.nx) * cd.ny * cd.nz;
-    // The P passes run as a list of launches of one or two passes (RBGS pass pairs where
-    // the grid allows them: all of them with the pass-pair option, otherwise only the C
-    // fallback-only passes, whose skipped launches each cost a dispatch). Launch l of L
-    // writes dst(l), the last one p. The launches covering the first C passes are skipped
-    // on an accepted guess, which therefore goes where the next launch reads it, and the
-    // first launch starts from zero when it can be skipped (it only runs on fallback, as
-    // the plain solve's first launch). Pair and single launches give identical results.
-    const bool two = smooth2_supported(c.options.smoother, c.dims, p_bc);
+    // The P passes run as a list of launches. The C fallback-only passes come first: one
+    // launch of the grid-barrier solve where the grid allows it (the accepted guess then
+    // costs one exiting dispatch for all C passes), else RBGS pass pairs where the grid
+    // allows them, else single passes; each skipped dispatch costs a few microseconds.
+    // The K passes after it run single, or in pairs with the pass-pair option. Launch l
+    // of L writes dst(l), the last one p. The launches covering the first C passes are
+    // skipped on an accepted guess, which therefore goes where the next launch reads it,
+    // and the first launch starts from zero when it can be skipped (it only runs on
+    // fallback, as the plain solve's first launch). Chain, pair and single launches give
+    // identical results.
+    const bool two   = smooth2_supported(c.options.smoother, c.dims, p_bc);
+    const bool chain = C > 0 && w.chain_grid > 0;
     std::vector<int> per; // passes per launch
-    if (two && c.options.pairs && P % 2 == 0) per.assign(P / 2, 2);
-    else if (two) {
+    if (chain) {
+        per.push_back(C);
+        if (two && c.options.pairs && K % 2 == 0) per.insert(per.end(), K / 2, 2);
+        else per.insert(per.end(), K, 1);
+    } else if (two && c.options.pairs && P % 2 == 0) {
+        per.assign(P / 2, 2);
+    } else if (two) {
         per.assign(C / 2, 2); // C is even
         per.insert(per.end(), K, 1);
     } else {
@@ -304,8 +314,13 @@ hipError_t enqueue_learned_pressure(Context& c, const StencilBC& p_bc) {
     SF_TRY(unet::launch_pack_input(c.stream, w.in2, w.scal + 1, 1.0f, unet::net_input(*w.net), vc));
     SF_TRY(unet::net_forward_packed(*w.net, false));
     SF_TRY(warm::launch_prolong(c.stream, unet::net_output(*w.net), w.scal, static_cast<float>(fac * fac) * h * h, guess, cd, fac));
-    SF_TRY(warm::launch_guess_test(c.stream, guess, f.rhs, h * h, 10.0, w.partials, w.stats, w.skip, c.dims, p_bc));
+    // (the chain zeroes the guess itself on fallback)
+    SF_TRY(warm::launch_guess_test(c.stream, guess, f.rhs, h * h, 10.0, w.partials, w.stats, w.skip, c.dims, p_bc, !chain));
     for (int l = 0; l < L; ++l) {
+        if (chain && l == 0) {
+            SF_TRY(launch_smooth_chain(c.stream, kSolvePoisson, guess, guess == f.p ? f.p_aux : f.p, f.rhs, h * h, C, c.dims, p_bc, w.skip, w.chain_grid));
+            continue;
+        }
         const float* src = l == 0 ? (C > 0 ? nullptr : guess) : dst(l - 1);
         SmoothPass pass{src, f.rhs, dst(l), h * h, 1.0f};
         if (l < skipped) pass.skip = w.skip;
@@ -489,6 +504,8 @@ StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const Stabl
             const int fac = c->options.warm_factor;
             if (unet::net_create(cfg.nx / fac, cfg.ny / fac, cfg.nz / fac, c->stream, &w.net) != hipSuccess) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
             if (!c->alloc(w.in2, c->bytes / 4) || !c->alloc(w.scal, 16) || !c->alloc(w.partials, sizeof(double) * 2 * warm::kPartialBlocks) || !c->alloc(w.stats, 4 * sizeof(double)) || !c->alloc(w.skip, 16)) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
+            const bool chain = cfg.pressure_iterations > c->options.warm_iterations && smooth_chain_supported(c->options.smoother, c->dims, pressure_rules(cfg.flow_boundary));
+            if (chain && smooth_chain_grid(c->dims, &w.chain_grid) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         }
         for (auto& s : c->scalars)
             for (float** p : {&s.data, &s.temp, &s.source, &s.aux})
@@ -514,6 +531,7 @@ StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const Stabl
         for (float* p : {f.vx, f.vy, f.vz, f.tx, f.ty, f.tz, f.ax, f.ay, f.az, f.p, f.p_aux, f.div, f.mag, f.rhs})
             if (launch_fill(c->stream, p, 0.0f, c->cells) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         if (c->options.learned && launch_fill(c->stream, reinterpret_cast<float*>(c->warm.stats), 0.0f, 8) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        if (c->options.learned && launch_fill(c->stream, reinterpret_cast<float*>(c->warm.skip), 0.0f, 4) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         for (auto& s : c->scalars) {
             if (launch_fill(c->stream, s.data, s.desc.initial_value, c->cells) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
             if (launch_fill(c->stream, s.temp, s.desc.initial_value, c->cells) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
@@ -1104,6 +1122,10 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_warm_start_stats_cuda(voi
     auto& c = *static_cast<Context*>(context);
     if (c.warm.stats == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     if (hipStreamSynchronize(c.stream) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    // a grid barrier of the one-launch fallback solve that timed out (its workgroups were
+    // not all resident) left an invalid pressure: reported as a failure
+    int sync[3] = {0, 0, 0};
+    if (hipMemcpy(sync, c.warm.skip, sizeof(sync), hipMemcpyDeviceToHost) != hipSuccess || sync[2] != 0) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     return sf::status(hipMemcpy(out3, c.warm.stats, 3 * sizeof(double), hipMemcpyDeviceToHost));
 }
 

@@ -25,6 +25,7 @@
 // grid planes below/above a non-periodic z range hold the z-face rule of the edge plane.
 // Periodic axes load the wrapped cell. Red/black parity uses wrapped coordinates, and an
 // odd periodic axis, where a pair can hold two cells of one colour, takes a second relax.
+#include <algorithm>
 #include <type_traits>
 
 #include "sf_buffer.hpp"
@@ -1870,6 +1871,61 @@ __global__ __launch_bounds__(kThreads) void k_rbgs2_pass(const float* __restrict
     rbgs2_body<kSolve, kZero>(x_old, b, x_new, coef, out_scale, d, bc, g, xcd_block());
 }
 
+// Spins of one grid-barrier wait (s_sleep 4, about 0.1 us each): about 50 ms, far above a
+// pass, so only a grid that is not all resident runs into it.
+constexpr unsigned kBarrierSpins = 1u << 19;
+
+// Grid barrier: every wave publishes its stores at device scope (the XCDs' L2s are not
+// coherent with each other), one lane per workgroup counts in and waits until `target`
+// workgroups have; then every wave drops stale cached lines. A wait that runs past
+// kBarrierSpins sets *fault and goes on, so the kernel always ends.
+__device__ __forceinline__ void grid_barrier(unsigned* count, const unsigned target, int* fault) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned spins = 0;
+        while (__hip_atomic_load(count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++spins > kBarrierSpins) {
+                __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(4);
+        }
+    }
+    __syncthreads();
+    __threadfence();
+}
+
+// `passes` RBGS passes from zero in one launch, run only when sync[0] == 0: zero `io`,
+// then io -> other -> io -> ... (passes even: the result lands in io). Workgroup i runs
+// pass blocks i, i + G, ... (G = gridDim.x, each renumbered as the single pass does) and
+// the workgroups meet at a grid barrier between passes, counting on sync[1] (zeroed by an
+// earlier kernel on the stream); sync[2] flags a barrier that did not complete. Every relaxation is
+// the single pass's: a zero field gives the bits of its implicit zero iterate.
+template <int kSolve>
+__global__ __launch_bounds__(kThreads) void k_rbgs_chain(float* io, float* other, const float* __restrict__ b, const float coef, const int passes, const Dims d, const StencilBC bc, const TileGeom g, int* __restrict__ sync) {
+    if (sync[0] != 0) return;
+    unsigned* count = reinterpret_cast<unsigned*>(sync + 1);
+    int* fault      = sync + 2;
+    const unsigned G = gridDim.x;
+    const int n      = g.tiles_x * g.tiles_y * g.chunks;
+    const uint64_t pairs = static_cast<uint64_t>(d.nx / 2) * static_cast<uint64_t>(d.ny) * static_cast<uint64_t>(d.nz);
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x; i < pairs; i += static_cast<uint64_t>(G) * kThreads)
+        reinterpret_cast<float2*>(io)[i] = make_float2(0.0f, 0.0f);
+    grid_barrier(count, G, fault);
+    for (int j = 0; j < passes; ++j) {
+        const float* src = (j & 1) ? other : io;
+        float* dst       = (j & 1) ? io : other;
+        for (int t = static_cast<int>(blockIdx.x); t < n; t += static_cast<int>(G)) {
+            const int w = (n & 7) == 0 ? (t & 7) * (n >> 3) + (t >> 3) : t;
+            rbgs_rows_body<kSolve, false, StoreF32, true>(src, b, dst, coef, 1.0f, d, bc, g, nullptr, w);
+            __syncthreads(); // the LDS rings are reused by the next block
+        }
+        if (j + 1 < passes) grid_barrier(count, G * static_cast<unsigned>(j + 2), fault);
+    }
+}
+
 bool has_odd_periodic(const Dims d, const StencilBC& bc) {
     return ((bc.periodic & 1) && (d.nx & 1)) || ((bc.periodic & 2) && (d.ny & 1)) || ((bc.periodic & 4) && (d.nz & 1));
 }
@@ -1998,6 +2054,38 @@ hipError_t launch_smooth2(hipStream_t s, const SolveKind solve, const SmoothPass
         else SF_P2(kSolveDiffusion, false);
     }
 #undef SF_P2
+    return hipGetLastError();
+}
+
+bool smooth_chain_supported(const SmootherKind smoother, const Dims d, const StencilBC& bc) {
+    return smoother == kSmoothRBGS && (d.nx & 1) == 0 && !has_odd_periodic(d, bc) && field_below_2g(d, 4);
+}
+
+hipError_t smooth_chain_grid(const Dims d, uint32_t* out_grid) {
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rbgs_chain<kSolvePoisson>, kThreads, 0);
+    if (e != hipSuccess) return e;
+    int per_cu_d = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_d, k_rbgs_chain<kSolveDiffusion>, kThreads, 0);
+    if (e != hipSuccess) return e;
+    const uint64_t resident = static_cast<uint64_t>(cus) * static_cast<uint64_t>(std::min(per_cu, per_cu_d));
+    if (resident == 0) return hipErrorInvalidConfiguration;
+    const uint64_t n = smooth_blocks(plan_tiles(d, 0));
+    // all workgroups of the grid must be resident together (they wait for each other); a
+    // multiple of 8 keeps every block on the XCD the single pass would give it
+    uint64_t grid = std::min(n, resident);
+    if (grid < n && grid >= 8) grid &= ~7ull;
+    *out_grid = static_cast<uint32_t>(grid);
+    return hipSuccess;
+}
+
+hipError_t launch_smooth_chain(hipStream_t s, const SolveKind solve, float* io, float* other, const float* b, const float coef, const int passes, const Dims d, const StencilBC& bc, int* sync, const uint32_t grid) {
+    if (passes <= 0 || (passes & 1) != 0 || io == other || grid == 0) return hipErrorInvalidValue;
+    const TileGeom g = plan_tiles(d, 0);
+    if (solve == kSolvePoisson) k_rbgs_chain<kSolvePoisson><<<grid, kThreads, 0, s>>>(io, other, b, coef, passes, d, bc, g, sync);
+    else k_rbgs_chain<kSolveDiffusion><<<grid, kThreads, 0, s>>>(io, other, b, coef, passes, d, bc, g, sync);
     return hipGetLastError();
 }
 

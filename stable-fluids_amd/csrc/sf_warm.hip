@@ -175,9 +175,11 @@ __device__ __forceinline__ void next_row(Rows& w, const int ny) {
     }
 }
 
-// Cell-centred trilinear prolongation, separable: for each fine row (one per block at a
-// time) the block first interpolates the coarse line at the row's y and z into LDS, then
-// every fine cell of the row lerps in x. Coarse lines up to kLine cells.
+// Cell-centred trilinear prolongation, separable: a block takes its fine rows in batches;
+// the coarse line of every row of the batch (the coarse field interpolated at the row's y
+// and z) goes to LDS first, one line per wave at a time and four lines' loads in flight
+// (one line at a time per barrier left the block waiting out the load latency once per
+// row), then every fine cell of the batch lerps its line in x. Coarse lines up to kLine.
 constexpr int kLine = 2048;
 
 __device__ __forceinline__ void axis_weights(const int c, const int n, const float inv_f, int& i0, int& i1, float& t) {
@@ -188,34 +190,56 @@ __device__ __forceinline__ void axis_weights(const int c, const int n, const flo
 }
 
 __global__ __launch_bounds__(kThreads) void k_prolong(const float* __restrict__ q, const float* __restrict__ scal, const float scale, float* __restrict__ p0, const Dims cd, const int f) {
-    __shared__ float line[kLine];
+    constexpr int kWaves = kThreads / 64;
+    __shared__ float lines[kLine];
     const Dims fd{f * cd.nx, f * cd.ny, f * cd.nz};
-    const float amp     = scale * scal[0];
-    const float inv_f   = 1.0f / static_cast<float>(f);
-    const uint64_t cpl  = static_cast<uint64_t>(cd.nx) * static_cast<uint64_t>(cd.ny);
-    for (Rows w = block_rows(fd.ny, fd.nz); w.r < w.end; next_row(w, fd.ny)) {
-        const uint64_t rw = w.r;
-        const int y = w.y, z = w.z;
-        int y0, y1, z0, z1;
-        float ty, tz;
-        axis_weights(y, cd.ny, inv_f, y0, y1, ty);
-        axis_weights(z, cd.nz, inv_f, z0, z1, tz);
-        __syncthreads(); // the previous row's line is no longer read
-        for (int cx = static_cast<int>(threadIdx.x); cx < cd.nx; cx += kThreads) {
-            const float* b = q + static_cast<uint64_t>(cx);
-            const float a00 = b[static_cast<uint64_t>(z0) * cpl + static_cast<uint64_t>(y0) * cd.nx];
-            const float a10 = b[static_cast<uint64_t>(z0) * cpl + static_cast<uint64_t>(y1) * cd.nx];
-            const float a01 = b[static_cast<uint64_t>(z1) * cpl + static_cast<uint64_t>(y0) * cd.nx];
-            const float a11 = b[static_cast<uint64_t>(z1) * cpl + static_cast<uint64_t>(y1) * cd.nx];
-            line[cx]        = ((1.0f - ty) * a00 + ty * a10) * (1.0f - tz) + ((1.0f - ty) * a01 + ty * a11) * tz;
+    const float amp    = scale * scal[0];
+    const float inv_f  = 1.0f / static_cast<float>(f);
+    const uint64_t cpl = static_cast<uint64_t>(cd.nx) * static_cast<uint64_t>(cd.ny);
+    const int tid      = static_cast<int>(threadIdx.x);
+    const int wave     = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int batch    = kLine / cd.nx;
+    const Rows w0      = block_rows(fd.ny, fd.nz);
+    for (uint64_t r0 = w0.r; r0 < w0.end; r0 += static_cast<uint64_t>(batch)) {
+        const int nb = static_cast<int>(min(static_cast<uint64_t>(batch), w0.end - r0));
+        const int yb = static_cast<int>(r0 % static_cast<uint64_t>(fd.ny)), zb = static_cast<int>(r0 / static_cast<uint64_t>(fd.ny));
+        __syncthreads(); // the previous batch's lines are no longer read
+        for (int cx0 = 0; cx0 < cd.nx; cx0 += 64) {
+            const int cx  = cx0 + lane;
+            const int cxl = min(cx, cd.nx - 1);
+            for (int j0 = wave; j0 < nb; j0 += 4 * kWaves) {
+                float a[4][4], wy[4], wz[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int sr = min(j0 + u * kWaves, nb - 1);
+                    const int yy = yb + sr;
+                    int y0, y1, z0, z1;
+                    axis_weights(yy % fd.ny, cd.ny, inv_f, y0, y1, wy[u]);
+                    axis_weights(zb + yy / fd.ny, cd.nz, inv_f, z0, z1, wz[u]);
+                    const float* b = q + static_cast<uint64_t>(cxl);
+                    a[u][0]        = b[static_cast<uint64_t>(z0) * cpl + static_cast<uint64_t>(y0) * cd.nx];
+                    a[u][1]        = b[static_cast<uint64_t>(z0) * cpl + static_cast<uint64_t>(y1) * cd.nx];
+                    a[u][2]        = b[static_cast<uint64_t>(z1) * cpl + static_cast<uint64_t>(y0) * cd.nx];
+                    a[u][3]        = b[static_cast<uint64_t>(z1) * cpl + static_cast<uint64_t>(y1) * cd.nx];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int sr   = j0 + u * kWaves;
+                    const float ty = wy[u], tz = wz[u];
+                    if (sr < nb && cx < cd.nx) lines[sr * cd.nx + cx] = ((1.0f - ty) * a[u][0] + ty * a[u][1]) * (1.0f - tz) + ((1.0f - ty) * a[u][2] + ty * a[u][3]) * tz;
+                }
+            }
         }
         __syncthreads();
-        float* out = p0 + rw * static_cast<uint64_t>(fd.nx);
-        for (int x = static_cast<int>(threadIdx.x); x < fd.nx; x += kThreads) {
+        for (int x = tid; x < fd.nx; x += kThreads) {
             int x0, x1;
             float tx;
             axis_weights(x, cd.nx, inv_f, x0, x1, tx);
-            out[x] = amp * ((1.0f - tx) * line[x0] + tx * line[x1]);
+            float* out = p0 + r0 * static_cast<uint64_t>(fd.nx) + static_cast<uint64_t>(x);
+            for (int sr = 0; sr < nb; ++sr) {
+                const float* l = lines + sr * cd.nx;
+                out[static_cast<uint64_t>(sr) * static_cast<uint64_t>(fd.nx)] = amp * ((1.0f - tx) * l[x0] + tx * l[x1]);
+            }
         }
     }
 }
@@ -297,21 +321,29 @@ __global__ __launch_bounds__(kThreads) void k_guess_partials(const float* __rest
 }
 
 __global__ __launch_bounds__(kThreads) void k_guess_decide(const double* __restrict__ partials, const int n_partials, const double ratio, double* __restrict__ stats, int* __restrict__ skip) {
-    double g = 0.0, zz = 0.0;
-    for (int b = threadIdx.x; b < n_partials; b += kThreads) {
-        g += partials[2 * b];
-        zz += partials[2 * b + 1];
+    // four independent sums per thread, so that four pairs of loads are in flight
+    double g[4] = {0.0, 0.0, 0.0, 0.0}, zz[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < n_partials; b += 4 * kThreads) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int bu = b + u * kThreads;
+            if (bu < n_partials) {
+                g[u] += partials[2 * bu];
+                zz[u] += partials[2 * bu + 1];
+            }
+        }
     }
     __shared__ double tg, tz;
-    block_sum_to(g, &tg);
-    block_sum_to(zz, &tz);
+    block_sum_to((g[0] + g[1]) + (g[2] + g[3]), &tg);
+    block_sum_to((zz[0] + zz[1]) + (zz[2] + zz[3]), &tz);
     if (threadIdx.x == 0) {
         const double rg = sqrt(tg), rz = sqrt(tz);
         const bool fallback = !(rg <= ratio * rz); // NaN guesses fall back too
         stats[0]            = rg;
         stats[1]            = rz;
         stats[2]            = fallback ? 1.0 : 0.0;
-        *skip               = fallback ? 0 : 1;
+        skip[0]             = fallback ? 0 : 1;
+        skip[1]             = 0; // grid-barrier count of a one-launch fallback solve
     }
 }
 
@@ -469,11 +501,11 @@ hipError_t launch_prolong(hipStream_t s, const float* q, const float* scal, cons
     return hipGetLastError();
 }
 
-hipError_t launch_guess_test(hipStream_t s, float* p0, const float* div, const float h2, const double ratio, double* partials, double* stats, int* skip, const Dims d, const StencilBC& p_bc) {
+hipError_t launch_guess_test(hipStream_t s, float* p0, const float* div, const float h2, const double ratio, double* partials, double* stats, int* skip, const Dims d, const StencilBC& p_bc, const bool zero_on_fallback) {
     const unsigned b = blocks_for(cells_of(d));
     k_guess_partials<<<b, kThreads, 0, s>>>(p0, div, h2, partials, d, p_bc);
     k_guess_decide<<<1, kThreads, 0, s>>>(partials, static_cast<int>(b), ratio, stats, skip);
-    k_zero_on_fallback<<<b, kThreads, 0, s>>>(p0, stats, cells_of(d));
+    if (zero_on_fallback) k_zero_on_fallback<<<b, kThreads, 0, s>>>(p0, stats, cells_of(d));
     return hipGetLastError();
 }
 

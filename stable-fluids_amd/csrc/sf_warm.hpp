@@ -25,9 +25,10 @@ hipError_t launch_restrict_input(hipStream_t s, const float* div, float* in2, do
 hipError_t launch_prolong(hipStream_t s, const float* q, const float* scal, float scale, float* p0, Dims cd, int f);
 // Residual test of the guess p0: g = ||b - A p0||, z = ||b|| (= residual of the zero
 // guess), FP64. stats[0] = g, stats[1] = z, stats[2] = 1 when g > ratio * z (fallback).
-// skip = 0 on fallback (the fallback passes run), 1 otherwise; on fallback p0 is zeroed.
-// partials: 2 * kPartialBlocks doubles.
-hipError_t launch_guess_test(hipStream_t s, float* p0, const float* div, float h2, double ratio, double* partials, double* stats, int* skip, Dims d, const StencilBC& p_bc);
+// skip[0] = 0 on fallback (the fallback passes run), 1 otherwise; skip[1] = 0 (the grid-
+// barrier count of a one-launch fallback solve); on fallback p0 is zeroed when
+// zero_on_fallback. partials: 2 * kPartialBlocks doubles.
+hipError_t launch_guess_test(hipStream_t s, float* p0, const float* div, float h2, double ratio, double* partials, double* stats, int* skip, Dims d, const StencilBC& p_bc, bool zero_on_fallback);
 
 // FP64 conjugate gradients on A p = b (b from div as above); x holds the start value
 // and receives the solution. Pure Neumann/periodic systems (no outflow face) are

@@ -112,6 +112,25 @@ def test_fallback_is_the_plain_step_bitwise():
     assert_same(got, plain, "fallback vs plain P")
 
 
+def test_fallback_in_rounds_is_the_plain_step_bitwise():
+    """A 2048x1024x8 slab has 1024 smoother blocks, more than the workgroups that fit on the
+    GPU at once (3 per CU): the one-launch fallback solve runs its blocks in rounds between
+    grid barriers and still gives the plain step bit for bit."""
+    desc = flow_desc(n=(2048, 1024, 8), walls_only=True, p_iters=12, d_iters=2)
+    inputs = Inputs(desc, seed=5, velocity_scale=1.0)
+    wild = uo.join_params(uo.init_params(3, scale=300.0)).astype(np.float32)
+    with sf.Context(desc, options=learned_opts(4)) as ctx:
+        ctx.set_pressure_model(wild)
+        inputs.seed_gpu(ctx)
+        ctx.step(2)
+        stats = ctx.warm_start_stats()
+        got = gpu_fields(ctx, desc)
+    print("fallback stats", stats, flush=True)
+    assert stats["fallback"]
+    plain = run_gpu(desc, inputs.seed_gpu, 2)
+    assert_same(got, plain, "fallback in rounds vs plain P")
+
+
 def test_zero_network_is_the_plain_k_pass_step_bitwise():
     scene = plume(p_iters=12)
     zero = np.zeros(uo.param_count(), np.float32)
