@@ -50,17 +50,32 @@ inline uint32_t tile_count(const Dims d) {
     return static_cast<uint32_t>(((d.nx + kTX - 1) / kTX) * ((d.ny + kTY - 1) / kTY) * ((d.nz + kTZ - 1) / kTZ));
 }
 
-// x[tile + halo][C] -> LDS [hz][hy][hx][C]; zeros outside the grid (zero padding).
+// x[tile + halo][C] -> LDS [hz][hy][hx][C]; zeros outside the grid (zero padding). Four
+// 16-byte loads per thread are issued before their stores (voxels outside the grid load
+// voxel 0 and store zeros), so the staging waits out the load latency once per four.
 template <int C>
 __device__ __forceinline__ void stage_halo(const bf16_t* __restrict__ x, bf16_t* s, const Origin o, const Dims d) {
     constexpr int kParts = C / 8;
-    for (int c = threadIdx.x; c < kHalo * kParts; c += kThreads) {
-        const int vox = c / kParts, part = c % kParts;
-        const int hx = vox % kHX, hy = (vox / kHX) % kHY, hz = vox / (kHX * kHY);
-        const int gx = o.x0 - 1 + hx, gy = o.y0 - 1 + hy, gz = o.z0 - 1 + hz;
-        uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        if (gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz) v = *reinterpret_cast<const uint4*>(x + lin(gx, gy, gz, d) * C + part * 8);
-        *reinterpret_cast<uint4*>(s + vox * C + part * 8) = v;
+    constexpr int kTotal = kHalo * kParts;
+    constexpr int kU     = 4;
+    for (int c0 = static_cast<int>(threadIdx.x); c0 < kTotal; c0 += kU * kThreads) {
+        uint4 v[kU];
+        bool in[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int c    = min(c0 + u * kThreads, kTotal - 1);
+            const int vox  = c / kParts, part = c % kParts;
+            const int hx = vox % kHX, hy = (vox / kHX) % kHY, hz = vox / (kHX * kHY);
+            const int gx = o.x0 - 1 + hx, gy = o.y0 - 1 + hy, gz = o.z0 - 1 + hz;
+            in[u]          = gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz;
+            const uint64_t src = in[u] ? lin(gx, gy, gz, d) * C + part * 8 : 0;
+            v[u]           = *reinterpret_cast<const uint4*>(x + src);
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int c = c0 + u * kThreads;
+            if (c < kTotal) *reinterpret_cast<uint4*>(s + c * 8) = in[u] ? v[u] : make_uint4(0u, 0u, 0u, 0u);
+        }
     }
 }
 
@@ -78,46 +93,73 @@ __device__ __forceinline__ void stage_tile(const bf16_t* __restrict__ g, bf16_t*
     }
 }
 
-template <int CI, int CO, bool kF32>
+// NB output-channel blocks of 16 per workgroup; with NB < CO / 16 the CO / 16 / NB
+// workgroups of one tile split the output channels (for grids of few tiles, where one
+// workgroup per tile leaves the GPU nearly idle); every output keeps the same K order.
+template <int CI, int CO, bool kF32, int NB>
 __global__ __launch_bounds__(kThreads) void k_conv3(const bf16_t* __restrict__ x, const bf16_t* __restrict__ w, const float* __restrict__ bias, void* __restrict__ y, const bf16_t* __restrict__ aux, const int epi, const Dims d) {
+    static_assert(NB >= 1 && (CO / 16) % NB == 0, "channel split");
+    constexpr int kSplit = CO / 16 / NB;
     __shared__ __attribute__((aligned(16))) bf16_t s_x[kHalo * CI];
-    const Origin o = tile_origin(static_cast<int>(blockIdx.x), d);
+    const Origin o = tile_origin(static_cast<int>(blockIdx.x) / kSplit, d);
+    const int nb0  = (static_cast<int>(blockIdx.x) % kSplit) * NB;
     stage_halo<CI>(x, s_x, o, d);
     __syncthreads();
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int grp = lane >> 4, m = lane & 15;
-    constexpr int NB     = CO / 16;
     constexpr int KSTEPS = (27 * CI + 31) / 32;
+    constexpr bool kWholeSteps = (27 * CI) % 32 == 0; // no step runs past the last tap
     f32x4 acc[2][NB];
 #pragma unroll
     for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
-    for (int ks = 0; ks < KSTEPS; ++ks) {
+    // Operands of step ks: A from the staged tile in LDS, B (the weights) from L2. They are
+    // fetched two steps ahead into a ring of three register sets (compile-time slots):
+    // fetched at their use, each step waited out the weights' L2 latency. The
+    // accumulation order is that of the plain step loop.
+    struct Ops {
+        bf16x8 a[2], b[NB];
+    };
+    auto fetch = [&](const int ks, Ops& op) {
         const int k = ks * 32 + 8 * grp;
         const int tap = k / CI, ch = k % CI;
-        bf16x8 a[2], b[NB];
-        if (tap < 27) {
+        if (kWholeSteps || tap < 27) {
             const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;
 #pragma unroll
             for (int mb = 0; mb < 2; ++mb) {
                 const int rb = 2 * wave + mb, cy = rb % kTY, cz = rb / kTY;
-                a[mb]        = *reinterpret_cast<const bf16x8*>(s_x + (((cz + dz) * kHY + (cy + dy)) * kHX + (m + dx)) * CI + ch);
+                op.a[mb]     = *reinterpret_cast<const bf16x8*>(s_x + (((cz + dz) * kHY + (cy + dy)) * kHX + (m + dx)) * CI + ch);
             }
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) b[nb] = *reinterpret_cast<const bf16x8*>(w + static_cast<size_t>(tap * CO + nb * 16 + m) * CI + ch);
+            for (int nb = 0; nb < NB; ++nb) op.b[nb] = *reinterpret_cast<const bf16x8*>(w + static_cast<size_t>(tap * CO + (nb0 + nb) * 16 + m) * CI + ch);
         } else {
 #pragma unroll
-            for (int mb = 0; mb < 2; ++mb) a[mb] = bf16x8{};
+            for (int mb = 0; mb < 2; ++mb) op.a[mb] = bf16x8{};
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) b[nb] = bf16x8{};
+            for (int nb = 0; nb < NB; ++nb) op.b[nb] = bf16x8{};
         }
+    };
+    auto mma = [&](const Ops& op) {
 #pragma unroll
         for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mb], b[nb], acc[mb][nb], 0, 0, 0);
+            for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(op.a[mb], op.b[nb], acc[mb][nb], 0, 0, 0);
+    };
+    Ops o0, o1, o2;
+    fetch(0, o0);
+    if (KSTEPS > 1) fetch(1, o1);
+    for (int ks = 0; ks < KSTEPS; ks += 3) {
+        if (ks + 2 < KSTEPS) fetch(ks + 2, o2);
+        mma(o0);
+        if (ks + 1 >= KSTEPS) break;
+        if (ks + 3 < KSTEPS) fetch(ks + 3, o0);
+        mma(o1);
+        if (ks + 2 >= KSTEPS) break;
+        if (ks + 4 < KSTEPS) fetch(ks + 4, o1);
+        mma(o2);
     }
 
 #pragma unroll
@@ -127,7 +169,7 @@ __global__ __launch_bounds__(kThreads) void k_conv3(const bf16_t* __restrict__ x
         if (gy >= d.ny || gz >= d.nz) continue;
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) {
-            const int co    = nb * 16 + m;
+            const int co    = (nb0 + nb) * 16 + m;
             const float bco = bias ? bias[co] : 0.0f;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -385,9 +427,17 @@ bool valid_c(const int c) { return c == 16 || c == 32 || c == 64; }
 
 template <int CI, int CO>
 hipError_t conv3_t(hipStream_t s, const bf16_t* x, const bf16_t* w, const float* bias, bf16_t* yb, float* yf, const bf16_t* aux, const int epi, const Dims d) {
-    const unsigned blocks = tile_count(d);
-    if (yf) k_conv3<CI, CO, true><<<blocks, kThreads, 0, s>>>(x, w, bias, yf, aux, epi, d);
-    else k_conv3<CI, CO, false><<<blocks, kThreads, 0, s>>>(x, w, bias, yb, aux, epi, d);
+    const unsigned tiles = tile_count(d);
+    constexpr int kNB    = CO / 16;
+    // fewer tiles than CUs: one workgroup per 16 output channels of a tile
+    if (kNB > 1 && tiles * kNB <= 1024) {
+        if (yf) k_conv3<CI, CO, true, 1><<<tiles * kNB, kThreads, 0, s>>>(x, w, bias, yf, aux, epi, d);
+        else k_conv3<CI, CO, false, 1><<<tiles * kNB, kThreads, 0, s>>>(x, w, bias, yb, aux, epi, d);
+    } else if (yf) {
+        k_conv3<CI, CO, true, kNB><<<tiles, kThreads, 0, s>>>(x, w, bias, yf, aux, epi, d);
+    } else {
+        k_conv3<CI, CO, false, kNB><<<tiles, kThreads, 0, s>>>(x, w, bias, yb, aux, epi, d);
+    }
     return hipGetLastError();
 }
 
