@@ -71,6 +71,9 @@ def main():
     ap.add_argument("--precision", action="store_true", help="run the 512^3 storage precision study instead")
     ap.add_argument("--batch", action="store_true", help="run the 64-simulation batched study instead")
     ap.add_argument("--tuning-cache", default=None, help="autotuner cache file for --batch")
+    ap.add_argument("--learned-step", type=int, default=None, metavar="K",
+                    help="time only the learned-warm-start step (network guess + K RBGS passes, P = 80) on the 256^3 "
+                         "plume from its step-10 state, --steps steps after --warmup (for kernel traces)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the stable-fluids step here runs one simulation per GPU; "
@@ -83,6 +86,9 @@ def main():
         return
     if args.batch:
         batch_study(pkg, args)
+        return
+    if args.learned_step is not None:
+        learned_step_study(pkg, args)
         return
     abi, hip, scenes = pkg.abi, pkg.hip, pkg.scenes
     n = args.grid
@@ -208,6 +214,23 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def learned_step_study(pkg, args):
+    """Steps/s of the learned-warm-start step alone (the evaluation's timing setup)."""
+    we, abi, scenes = pkg.warm_eval, pkg.abi, pkg.scenes
+    params = we.load_model()
+    if params is None:
+        print("no trained warm-start parameters under stable-fluids_amd/weights", file=sys.stderr)
+        sys.exit(2)
+    scene = scenes.plume_scene(256, diffuse_iterations=20, pressure_iterations=80)
+    state = we.checkpoints(scene, (10,))[-1]
+    opts = pkg.ext_options(smoother=abi.SMOOTHER_RBGS, pressure_warm_start=abi.WARM_START_LEARNED,
+                           warm_start_iterations=args.learned_step, warm_start_coarsening=3)
+    sps = we.steps_per_second(scene, state, opts, params, steps=args.steps, warmup=args.warmup)
+    print(json.dumps({"workload": "learned warm-start step, plume 256^3 from its step-10 state, D=20 P=80, "
+                                  "network on the 32^3 restriction", "k": args.learned_step, "steps": args.steps,
+                      "steps_per_s": round(sps, 3)}), flush=True)
 
 
 def precision_study(pkg, args):

@@ -7,6 +7,8 @@
 //   prolong:        trilinear guess on the full grid, scaled by (2h)^2 sigma;
 //   guess_test:     ||b - A p0|| against ||b|| (FP64), fallback flag, skip flag.
 // All reductions are deterministic two-pass sums (per-block partials, then one block).
+#include <algorithm>
+
 #include "sf_warm.hpp"
 
 namespace sf::warm {
@@ -123,7 +125,17 @@ __global__ __launch_bounds__(kThreads) void k_restrict_input(const float* __rest
         for (int k = lane_s; k < f; k += kGroup)
             for (int j = 0; j < f; ++j) {
                 const float* r = div + static_cast<uint64_t>(f * z + k) * plane + static_cast<uint64_t>(f * y + j) * row + static_cast<uint64_t>(f * x);
-                for (int i = 0; i < f; ++i) s += r[i];
+                // the row of f cells in 16-byte loads where f allows (same summation order)
+                if (f == 8) {
+                    const float4 a = reinterpret_cast<const float4*>(r)[0], b = reinterpret_cast<const float4*>(r)[1];
+                    s += a.x; s += a.y; s += a.z; s += a.w;
+                    s += b.x; s += b.y; s += b.z; s += b.w;
+                } else if (f == 4) {
+                    const float4 a = reinterpret_cast<const float4*>(r)[0];
+                    s += a.x; s += a.y; s += a.z; s += a.w;
+                } else {
+                    for (int i = 0; i < f; ++i) s += r[i];
+                }
             }
 #pragma unroll
         for (int o = 1; o < kGroup; o <<= 1) s += __shfl_xor(s, o, kGroup);
@@ -244,16 +256,16 @@ __global__ __launch_bounds__(kThreads) void k_prolong(const float* __restrict__ 
     }
 }
 
-// Residual sums of the guess and of the zero guess over the block's rows. Cells off the
-// faces take six direct neighbours (same order as neighbour_sum); rows come four at a time
-// and the loads of all four are issued before their FP64 arithmetic (one row at a time
-// left every thread waiting out the memory latency once per row). Face cells take the
-// ghost rules in a second phase with one lane per cell: left in the row loop, the two x
-// faces of every row sent half of all waves through the ghost path for a single lane.
-__global__ __launch_bounds__(kThreads) void k_guess_partials(const float* __restrict__ p0, const float* __restrict__ div, const float h2, double* __restrict__ partials, const Dims d, const StencilBC bc) {
-    constexpr int kU  = 4;
-    double acc_g      = 0.0, acc_z = 0.0;
-    const uint64_t sy = static_cast<uint64_t>(d.nx), sz = sy * static_cast<uint64_t>(d.ny);
+// Residual sums of the guess and of the zero guess. A block takes one row y and a chunk of
+// zc planes and marches up z with the lane's column in registers (z - 1, z, z + 1,
+// and z + 2 in flight), so each plane of p0 is fetched once from HBM; the y -+ 1 rows
+// come from the L2 the neighbouring blocks share (blocks renumbered so that consecutive
+// rows run on one XCD) and x -+ 1 from the lane's own cache lines. Interior cells take
+// the six direct neighbours (same order as neighbour_sum); cells on a face or a periodic
+// boundary take the ghost rules in a second phase.
+__global__ __launch_bounds__(kThreads) void k_guess_partials(const float* __restrict__ p0, const float* __restrict__ div, const float h2, double* __restrict__ partials, const Dims d, const StencilBC bc, const int zc) {
+    double acc_g = 0.0, acc_z = 0.0;
+    const uint64_t sy = static_cast<uint64_t>(d.nx);
     auto face_cell    = [&](const int x, const int y, const int z) {
         const uint64_t i = (static_cast<uint64_t>(z) * static_cast<uint64_t>(d.ny) + static_cast<uint64_t>(y)) * sy + static_cast<uint64_t>(x);
         const float c    = p0[i];
@@ -262,60 +274,56 @@ __global__ __launch_bounds__(kThreads) void k_guess_partials(const float* __rest
         acc_g += (b - ap) * (b - ap);
         acc_z += b * b;
     };
-    const Rows w0 = block_rows(d.ny, d.nz);
-    Rows w        = w0;
-    while (w.r < w.end) {
-        uint64_t ri[kU];
-        bool rv[kU];
-#pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            rv[u] = w.r < w.end && w.y > 0 && w.y < d.ny - 1 && w.z > 0 && w.z < d.nz - 1;
-            ri[u] = w.r;
-            if (w.r < w.end) next_row(w, d.ny);
-        }
+    const int n = static_cast<int>(gridDim.x), bid = static_cast<int>(blockIdx.x);
+    const int wid = (n & 7) == 0 ? (bid & 7) * (n >> 3) + (bid >> 3) : bid;
+    const int y   = wid % d.ny;
+    const int z0  = (wid / d.ny) * zc;
+    const int z1  = min(z0 + zc, d.nz);
+    const bool y_in = y > 0 && y < d.ny - 1;
+    auto row = [&](const int z) { return (static_cast<uint64_t>(z) * static_cast<uint64_t>(d.ny) + static_cast<uint64_t>(y)) * sy; };
+    if (y_in && d.nz > 1) {
         for (int x = static_cast<int>(threadIdx.x); x < d.nx; x += kThreads) {
-            float c[kU], n0[kU], n1[kU], n2[kU], n3[kU], n4[kU], n5[kU], dv[kU];
-            bool fast[kU];
-#pragma unroll
-            for (int u = 0; u < kU; ++u) {
-                fast[u]           = rv[u] && x > 0 && x < d.nx - 1;
-                const uint64_t i  = fast[u] ? ri[u] * sy + static_cast<uint64_t>(x) : 0;
-                const uint64_t ox = fast[u] ? 1 : 0, oy = fast[u] ? sy : 0, oz = fast[u] ? sz : 0;
-                c[u]              = p0[i];
-                n0[u]             = p0[i - ox];
-                n1[u]             = p0[i + ox];
-                n2[u]             = p0[i - oy];
-                n3[u]             = p0[i + oy];
-                n4[u]             = p0[i - oz];
-                n5[u]             = p0[i + oz];
-                dv[u]             = div[i];
-            }
-#pragma unroll
-            for (int u = 0; u < kU; ++u) {
-                if (!fast[u]) continue;
-                float sum = n0[u];
-                sum += n1[u];
-                sum += n2[u];
-                sum += n3[u];
-                sum += n4[u];
-                sum += n5[u];
-                const double b  = -static_cast<double>(h2) * dv[u];
-                const double ap = 6.0 * c[u] - static_cast<double>(sum);
-                acc_g += (b - ap) * (b - ap);
-                acc_z += b * b;
+            const bool x_in = x > 0 && x < d.nx - 1;
+            const int xl = x_in ? x - 1 : x, xr = x_in ? x + 1 : x;
+            float zm = p0[row(max(z0 - 1, 0)) + x];
+            float c  = p0[row(z0) + x];
+            float zp = p0[row(min(z0 + 1, d.nz - 1)) + x];
+            for (int z = z0; z < z1; ++z) {
+                const uint64_t r = row(z);
+                const float nxt  = p0[row(min(z + 2, d.nz - 1)) + x];
+                const float ym = p0[r - sy + x], yp = p0[r + sy + x];
+                const float xm = p0[r + xl], xp = p0[r + xr];
+                const float dv = div[r + x];
+                if (x_in && z > 0 && z < d.nz - 1) {
+                    float sum = xm;
+                    sum += xp;
+                    sum += ym;
+                    sum += yp;
+                    sum += zm;
+                    sum += zp;
+                    const double b  = -static_cast<double>(h2) * dv;
+                    const double ap = 6.0 * c - static_cast<double>(sum);
+                    acc_g += (b - ap) * (b - ap);
+                    acc_z += b * b;
+                }
+                zm = c;
+                c  = zp;
+                zp = nxt;
             }
         }
     }
-    // face cells: x = 0 and x = nx - 1 of every row, then the rest of the rows on a y or z face
-    const uint32_t nrows = static_cast<uint32_t>(w0.end - w0.r);
-    const uint32_t per   = d.nx > 1 ? 2u : 1u;
-    for (uint32_t e = threadIdx.x; e < per * nrows; e += kThreads) {
-        const uint32_t r = static_cast<uint32_t>(w0.r) + e / per;
-        face_cell((e % per) ? d.nx - 1 : 0, static_cast<int>(r % static_cast<uint32_t>(d.ny)), static_cast<int>(r / static_cast<uint32_t>(d.ny)));
+    // face cells of the block: all of them on a y face row; else x = 0 and nx - 1 of every
+    // plane, then the rest of the rows on a z face
+    const int planes = z1 - z0;
+    if (!y_in || d.nz == 1) {
+        for (int e = static_cast<int>(threadIdx.x); e < planes * d.nx; e += kThreads) face_cell(e % d.nx, y, z0 + e / d.nx);
+    } else {
+        const int per = d.nx > 1 ? 2 : 1;
+        for (int e = static_cast<int>(threadIdx.x); e < per * planes; e += kThreads) face_cell((e % per) ? d.nx - 1 : 0, y, z0 + e / per);
+        for (int z = z0; z < z1; ++z)
+            if (z == 0 || z == d.nz - 1)
+                for (int x = 1 + static_cast<int>(threadIdx.x); x < d.nx - 1; x += kThreads) face_cell(x, y, z);
     }
-    for (Rows v = w0; v.r < v.end; next_row(v, d.ny))
-        if (v.y == 0 || v.y == d.ny - 1 || v.z == 0 || v.z == d.nz - 1)
-            for (int x = 1 + static_cast<int>(threadIdx.x); x < d.nx - 1; x += kThreads) face_cell(x, v.y, v.z);
     block_sum_to(acc_g, partials + 2 * blockIdx.x);
     block_sum_to(acc_z, partials + 2 * blockIdx.x + 1);
 }
@@ -497,13 +505,22 @@ hipError_t launch_restrict_input(hipStream_t s, const float* div, float* in2, do
 hipError_t launch_prolong(hipStream_t s, const float* q, const float* scal, const float scale, float* p0, const Dims cd, const int f) {
     const Dims fd{f * cd.nx, f * cd.ny, f * cd.nz};
     if (cd.nx > kLine) return hipErrorInvalidValue;
-    k_prolong<<<blocks_for(cells_of(fd)), kThreads, 0, s>>>(q, scal, scale, p0, cd, f);
+    // about 32 fine rows per block: one round of blocks on the GPU, each streaming its
+    // stores after one batch of line loads
+    const uint64_t rows = static_cast<uint64_t>(fd.ny) * static_cast<uint64_t>(fd.nz);
+    const unsigned b    = static_cast<unsigned>(std::min<uint64_t>(std::max<uint64_t>((rows + 31) / 32, 1), kPartialBlocks));
+    k_prolong<<<b, kThreads, 0, s>>>(q, scal, scale, p0, cd, f);
     return hipGetLastError();
 }
 
 hipError_t launch_guess_test(hipStream_t s, float* p0, const float* div, const float h2, const double ratio, double* partials, double* stats, int* skip, const Dims d, const StencilBC& p_bc, const bool zero_on_fallback) {
-    const unsigned b = blocks_for(cells_of(d));
-    k_guess_partials<<<b, kThreads, 0, s>>>(p0, div, h2, partials, d, p_bc);
+    // 16-plane chunks, longer where the rows times chunks would exceed the partial slots
+    int zc = 16;
+    while (zc < d.nz && static_cast<uint64_t>(d.ny) * static_cast<uint64_t>((d.nz + zc - 1) / zc) > static_cast<uint64_t>(kPartialBlocks)) zc *= 2;
+    const uint64_t nb = static_cast<uint64_t>(d.ny) * static_cast<uint64_t>((d.nz + zc - 1) / zc);
+    if (nb > static_cast<uint64_t>(kPartialBlocks)) return hipErrorInvalidValue;
+    const unsigned b = static_cast<unsigned>(nb);
+    k_guess_partials<<<b, kThreads, 0, s>>>(p0, div, h2, partials, d, p_bc, zc);
     k_guess_decide<<<1, kThreads, 0, s>>>(partials, static_cast<int>(b), ratio, stats, skip);
     if (zero_on_fallback) k_zero_on_fallback<<<b, kThreads, 0, s>>>(p0, stats, cells_of(d));
     return hipGetLastError();
