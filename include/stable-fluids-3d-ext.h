@@ -152,7 +152,9 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_set_pressure_model_cuda(v
 
 /* Warm-start outcome of the last step: out[0] = ||b - A p0|| (guess), out[1] = ||b||
  * (zero guess), out[2] = 1 when the step fell back to the plain passes. Synchronises the
- * context stream. */
+ * context stream. Returns BACKEND_FAILURE when a fallback solve since creation could not
+ * complete a grid barrier (its workgroups were not all resident on the GPU at once): the
+ * pressure of that step is invalid. */
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_warm_start_stats_cuda(void* context, double* out3);
 
 /* Training pair of the warm-start network from the last step's pressure system: the
