@@ -75,6 +75,37 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_create_context_cuda(const
  * context stream. Needed to start from a non-zero velocity and to resume a checkpoint. */
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_set_flow_velocity_cuda(void* context, const float* values_x, const float* values_y, const float* values_z);
 
+/* Animated force computed on the device inside the step graph. The reference's plume scene
+ * recomputes its swirl and drift forcing on the host and uploads force x and z before every
+ * step (vulkan-app/scene_plume.cpp:323-335, masks built at :241-289); this replaces that
+ * per-step upload. Before the force of step n is added (n counts from start_step, one per
+ * step):
+ *   phase = n * phase_rate                      (n converted to float)
+ *   ax = drift_x_amplitude * sin(phase),  az = drift_z_amplitude * cos(phase * drift_z_rate)
+ *   s  = swirl_amplitude * cos(phase * swirl_rate)
+ *   force.x = ax * drift + s * swirl_x,  force.z = az * drift + s * swirl_z
+ * in FP32, in this operation order, into components x and z of force field `field` (its y
+ * component keeps what was set). The scene's constants are phase_rate 0.045, amplitudes
+ * 1.15 / 0.85 / 1.65, rates 0.71 / 0.53. The masks are device arrays of nx*ny*nz floats,
+ * copied by the call. NULL `animation` turns the animation off. Synchronous: it waits for
+ * the context stream and rebuilds the step graph. Contexts with an animation cannot join a
+ * batch. */
+typedef struct StableFluidsExtForceAnimation {
+    float phase_rate;
+    float drift_x_amplitude;
+    float drift_z_amplitude;
+    float drift_z_rate;
+    float swirl_amplitude;
+    float swirl_rate;
+    uint32_t start_step;
+} StableFluidsExtForceAnimation;
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_set_force_animation_cuda(void* context, StableFluidsVectorFieldHandle field, const StableFluidsExtForceAnimation* animation, const float* drift, const float* swirl_x, const float* swirl_z);
+
+/* Step counter of the force animation (the n of the next step). Synchronises the context
+ * stream. */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_force_animation_step_cuda(void* context, uint32_t* out_step);
+
 /* Sets the pressure field from a device array (NULL = zeros). The next step overwrites it
  * unless a warm start uses it; provided for checkpoint/resume of every viewable field. */
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_set_flow_pressure_cuda(void* context, const float* values);

@@ -118,10 +118,22 @@ class ExtOptions(C.Structure):
                 ("flags", C.c_uint32)]
 
 
+class ForceAnimation(C.Structure):
+    """StableFluidsExtForceAnimation (stable-fluids-3d-ext.h)."""
+    _fields_ = [("phase_rate", C.c_float), ("drift_x_amplitude", C.c_float), ("drift_z_amplitude", C.c_float),
+                ("drift_z_rate", C.c_float), ("swirl_amplitude", C.c_float), ("swirl_rate", C.c_float),
+                ("start_step", C.c_uint32)]
+
+
+def plume_scene_animation(start_step=0):
+    """The swirl/drift schedule of the reference's plume scene (vulkan-app/scene_plume.cpp:327-330)."""
+    return ForceAnimation(0.045, 1.15, 0.85, 0.71, 1.65, 0.53, start_step)
+
+
 assert C.sizeof(FlowFace) == 20 and C.sizeof(FlowBoundary) == 120
 assert C.sizeof(ScalarBoundary) == 48 and C.sizeof(SimulationConfig) == 152
 assert C.sizeof(ContextCreateDesc) == 192 and ContextCreateDesc.stream.offset == 152
-assert C.sizeof(ExtOptions) == 32
+assert C.sizeof(ExtOptions) == 32 and C.sizeof(ForceAnimation) == 28
 
 
 def flow_boundary(types, velocity=None, pressure=None):

@@ -65,6 +65,12 @@ struct VectorField {
     float* x = nullptr;
     float* y = nullptr;
     float* z = nullptr;
+    // animated force (stable_fluids_ext_set_force_animation_cuda); masks allocated on first use
+    bool animated = false;
+    ForceAnimation anim{};
+    float* drift   = nullptr;
+    float* swirl_x = nullptr;
+    float* swirl_z = nullptr;
 };
 
 constexpr int kNormBlocks = 1024;
@@ -88,6 +94,7 @@ struct Context {
     } flow;
     std::vector<ScalarField> scalars;
     std::vector<VectorField> vectors;
+    uint32_t* anim_step   = nullptr; // step counter of the force animations (allocated on first use)
     double* norm_partials = nullptr; // kNormBlocks * 4 partial sums, then 4 results
     struct {
         float* r     = nullptr; // FP32 defect
@@ -338,10 +345,14 @@ hipError_t enqueue_step(Context& c) {
     const float h   = cfg.cell_size;
     auto& f         = c.flow;
 
+    bool animated = false;
     for (auto& v : c.vectors) {
         if (v.desc.usage != STABLE_FLUIDS_VECTOR_FIELD_FORCE) continue;
+        if (v.animated) SF_TRY(launch_animate_force(c.stream, v.x, v.z, v.drift, v.swirl_x, v.swirl_z, v.anim, c.anim_step, c.cells));
+        animated = animated || v.animated;
         SF_TRY(launch_add_force(c.stream, f.vx, f.vy, f.vz, v.x, v.y, v.z, dt, c.cells));
     }
+    if (animated) SF_TRY(launch_count_step(c.stream, c.anim_step));
     SF_TRY(launch_advect_velocity(c.stream, f.tx, f.ty, f.tz, f.vx, f.vy, f.vz, dt, c.dims, h, fb));
 
     // viscosity; alpha <= 0 keeps the forced velocity (reference :621-642)
@@ -671,6 +682,8 @@ bool batch_compatible(const Context& a, const Context& b) {
 bool batch_supported(const Context& c) {
     const auto& cfg = c.config;
     if (c.options.refine || c.options.learned) return false;
+    for (const auto& v : c.vectors)
+        if (v.animated) return false;
     if (cfg.diffuse_iterations <= 0 || cfg.pressure_iterations <= 0) return false;
     if (!(cfg.dt * cfg.viscosity / (cfg.cell_size * cfg.cell_size) > 0.0f)) return false;
     for (const auto& s : c.scalars)
@@ -996,6 +1009,44 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_update_vector_field_cuda(void
     auto& v = c.vectors[field - 1u];
     if (sf::upload(c, v.x, x) != hipSuccess || sf::upload(c, v.y, y) != hipSuccess || sf::upload(c, v.z, z) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     return STABLE_FLUIDS_RESULT_OK;
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_set_force_animation_cuda(void* context, StableFluidsVectorFieldHandle field, const StableFluidsExtForceAnimation* animation, const float* drift, const float* swirl_x, const float* swirl_z) {
+    sf::Range r("stable.ext.set_force_animation");
+    if (context == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& c = *static_cast<Context*>(context);
+    if (field == 0 || field > c.vectors.size()) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& v = c.vectors[field - 1u];
+    if (v.desc.usage != STABLE_FLUIDS_VECTOR_FIELD_FORCE) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    if (animation != nullptr && (drift == nullptr || swirl_x == nullptr || swirl_z == nullptr)) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    if (hipStreamSynchronize(c.stream) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    if (animation != nullptr) {
+        if (v.drift == nullptr && (!c.alloc(v.drift, c.bytes) || !c.alloc(v.swirl_x, c.bytes) || !c.alloc(v.swirl_z, c.bytes))) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
+        if (c.anim_step == nullptr && !c.alloc(c.anim_step, sizeof(uint32_t))) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
+        if (hipMemcpy(v.drift, drift, c.bytes, hipMemcpyDeviceToDevice) != hipSuccess || hipMemcpy(v.swirl_x, swirl_x, c.bytes, hipMemcpyDeviceToDevice) != hipSuccess ||
+            hipMemcpy(v.swirl_z, swirl_z, c.bytes, hipMemcpyDeviceToDevice) != hipSuccess)
+            return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        const uint32_t start = animation->start_step;
+        if (hipMemcpy(c.anim_step, &start, sizeof(start), hipMemcpyHostToDevice) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        v.anim     = sf::ForceAnimation{animation->phase_rate, animation->drift_x_amplitude, animation->drift_z_amplitude, animation->drift_z_rate, animation->swirl_amplitude, animation->swirl_rate};
+        v.animated = true;
+    } else {
+        v.animated = false;
+    }
+    // the step graph is captured with the animation kernels in it (or without them)
+    if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
+    if (c.graph) (void)hipGraphDestroy(c.graph);
+    c.graph_exec = nullptr;
+    c.graph      = nullptr;
+    return sf::build_graph(c);
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_force_animation_step_cuda(void* context, uint32_t* out_step) {
+    if (context == nullptr || out_step == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& c = *static_cast<Context*>(context);
+    if (c.anim_step == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    if (hipStreamSynchronize(c.stream) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    return sf::status(hipMemcpy(out_step, c.anim_step, sizeof(uint32_t), hipMemcpyDeviceToHost));
 }
 
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_step_cuda(void* context) {

@@ -6,6 +6,8 @@
 //  * one RK2 trace per cell serves all three velocity components (reference traces 3x);
 //  * smoother passes live in sf_smooth.hip (LDS z-marching tiles);
 //  * projection, divergence of the projected field and |v| run in one pass.
+#include <algorithm>
+
 #include "sf_kernels.hpp"
 
 namespace sf {
@@ -76,6 +78,23 @@ __global__ __launch_bounds__(kElemBlock) void k_add_force(float* __restrict__ vx
         }
     }
 }
+
+// Animated force of the plume scene (vulkan-app/scene_plume.cpp:323-335, see
+// stable_fluids_ext_set_force_animation_cuda): coefficients from the step counter, then
+// x = ax * drift + s * swirl_x and z = az * drift + s * swirl_z per cell.
+__global__ __launch_bounds__(kElemBlock) void k_animate_force(float* __restrict__ fx, float* __restrict__ fz, const float* __restrict__ drift, const float* __restrict__ swirl_x, const float* __restrict__ swirl_z, const ForceAnimation a, const uint32_t* __restrict__ counter, const uint64_t n) {
+    const float phase = static_cast<float>(*counter) * a.phase_rate;
+    const float ax    = a.drift_x_amplitude * sinf(phase);
+    const float az    = a.drift_z_amplitude * cosf(phase * a.drift_z_rate);
+    const float sw    = a.swirl_amplitude * cosf(phase * a.swirl_rate);
+    for (uint64_t i = global_tid(); i < n; i += static_cast<uint64_t>(gridDim.x) * kElemBlock) {
+        const float m = drift[i];
+        fx[i]         = ax * m + sw * swirl_x[i];
+        fz[i]         = az * m + sw * swirl_z[i];
+    }
+}
+
+__global__ void k_count_step(uint32_t* __restrict__ counter) { *counter += 1u; }
 
 // ---------------- advection ----------------
 
@@ -294,6 +313,18 @@ hipError_t launch_add_scaled(hipStream_t s, float* dst, const float* src, const 
 hipError_t launch_add_force(hipStream_t s, float* vx, float* vy, float* vz, const float* fx, const float* fy, const float* fz, const float dt, const uint64_t n) {
     if (n == 0) return hipSuccess;
     k_add_force<<<elem_blocks(n), kElemBlock, 0, s>>>(vx, vy, vz, fx, fy, fz, dt, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_animate_force(hipStream_t s, float* fx, float* fz, const float* drift, const float* swirl_x, const float* swirl_z, const ForceAnimation& a, const uint32_t* counter, const uint64_t n) {
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = static_cast<unsigned>(std::min<uint64_t>((n + kElemBlock - 1) / kElemBlock, 4096));
+    k_animate_force<<<blocks, kElemBlock, 0, s>>>(fx, fz, drift, swirl_x, swirl_z, a, counter, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_step(hipStream_t s, uint32_t* counter) {
+    k_count_step<<<1, 1, 0, s>>>(counter);
     return hipGetLastError();
 }
 

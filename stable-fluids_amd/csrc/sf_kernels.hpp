@@ -142,6 +142,13 @@ hipError_t launch_project_batched(hipStream_t s, const ProjectDesc* descs, const
 hipError_t launch_fill(hipStream_t s, float* p, float value, uint64_t n);
 hipError_t launch_copy_scaled(hipStream_t s, float* dst, const float* src, float factor, uint64_t n);
 hipError_t launch_add_scaled(hipStream_t s, float* dst, const float* src, float dt, uint64_t n);
+// Coefficients of an animated force (stable_fluids_ext_set_force_animation_cuda).
+struct ForceAnimation {
+    float phase_rate, drift_x_amplitude, drift_z_amplitude, drift_z_rate, swirl_amplitude, swirl_rate;
+};
+// fx, fz of step *counter from the masks; launch_count_step adds one to *counter.
+hipError_t launch_animate_force(hipStream_t s, float* fx, float* fz, const float* drift, const float* swirl_x, const float* swirl_z, const ForceAnimation& a, const uint32_t* counter, uint64_t n);
+hipError_t launch_count_step(hipStream_t s, uint32_t* counter);
 hipError_t launch_add_force(hipStream_t s, float* vx, float* vy, float* vz, const float* fx, const float* fy, const float* fz, float dt, uint64_t n);
 hipError_t launch_advect_velocity(hipStream_t s, float* tx, float* ty, float* tz, const float* vx, const float* vy, const float* vz, float dt, Dims d, float h, const StableFluidsFlowBoundaryConfig& bc);
 // sf_advect.hip: the same stages through field buffer resources (fields below 2 GiB)

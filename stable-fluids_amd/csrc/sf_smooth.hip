@@ -1519,13 +1519,6 @@ __device__ __forceinline__ int frame_index(const int s, const int r, const int p
     if (s == 3) off = kOff[3], h = kH[3], w = kWp[3], px = kPx[3];
     return off + (r + h) * w + p + px;
 }
-
-// Wave-uniform row facts of one stage, packed: bits 0-1 row count, bit 2 parity of the
-// first row's y, bits 3-5 ghost rows, bits 6-8 ghost rows below y = 0.
-__device__ __forceinline__ int row_count(const int info) { return info & 3; }
-__device__ __forceinline__ int row_parity(const int info) { return (info >> 2) & 1; }
-__device__ __forceinline__ bool row_ghost(const int info, const int j) { return (info >> (3 + j)) & 1; }
-__device__ __forceinline__ bool row_below(const int info, const int j) { return (info >> (6 + j)) & 1; }
 } // namespace two
 
 template <int kSolve, bool kZero>

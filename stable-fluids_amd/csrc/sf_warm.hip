@@ -161,8 +161,7 @@ __global__ __launch_bounds__(kThreads) void k_sigma(const double* __restrict__ p
     }
 }
 
-// Rows of the fine grid handled by this block: a contiguous chunk, so the row's (y, z)
-// comes from one division per block and then steps along.
+// Rows of the fine grid handled by this block: a contiguous chunk.
 struct Rows {
     uint64_t r, end;
     int y, z;
@@ -177,14 +176,6 @@ __device__ __forceinline__ Rows block_rows(const int ny, const int nz) {
     w.y   = static_cast<int>(w.r % static_cast<uint64_t>(ny));
     w.z   = static_cast<int>(w.r / static_cast<uint64_t>(ny));
     return w;
-}
-
-__device__ __forceinline__ void next_row(Rows& w, const int ny) {
-    ++w.r;
-    if (++w.y == ny) {
-        w.y = 0;
-        ++w.z;
-    }
 }
 
 // Cell-centred trilinear prolongation, separable: a block takes its fine rows in batches;

@@ -30,7 +30,8 @@ EXT_ENTRY_POINTS = (
     "stable_fluids_ext_get_pressure_rhs_cuda",
     "stable_fluids_ext_batch_create_cuda", "stable_fluids_ext_batch_destroy_cuda", "stable_fluids_ext_batch_step_cuda",
     "stable_fluids_ext_batch_get_stream_cuda", "stable_fluids_ext_batch_set_z_chunk_cuda", "stable_fluids_ext_batch_info_cuda",
-    "stable_fluids_ext_batch_autotune_cuda",
+    "stable_fluids_ext_batch_autotune_cuda", "stable_fluids_ext_set_force_animation_cuda",
+    "stable_fluids_ext_force_animation_step_cuda",
 )
 UNET_ENTRY_POINTS = (
     "stable_fluids_unet_param_count", "stable_fluids_unet_create_cuda", "stable_fluids_unet_destroy_cuda",
@@ -69,6 +70,8 @@ def load_library(path=LIB_PATH):
     lib.stable_fluids_step_cuda.argtypes = [vp]
     lib.stable_fluids_get_view_cuda.argtypes = [vp, C.POINTER(abi.ViewRequest), C.POINTER(abi.View)]
     lib.stable_fluids_ext_set_flow_velocity_cuda.argtypes = [vp, vp, vp, vp]
+    lib.stable_fluids_ext_set_force_animation_cuda.argtypes = [vp, u32, C.POINTER(abi.ForceAnimation), vp, vp, vp]
+    lib.stable_fluids_ext_force_animation_step_cuda.argtypes = [vp, C.POINTER(u32)]
     lib.stable_fluids_ext_set_flow_pressure_cuda.argtypes = [vp, vp]
     lib.stable_fluids_ext_field_norms_cuda.argtypes = [vp, C.POINTER(ExtNorms)]
     lib.stable_fluids_ext_run_stage_cuda.argtypes = [vp, u32, u32]
@@ -188,6 +191,17 @@ class Context:
         return view
 
     # ---- extensions ----
+    def set_force_animation(self, handle, animation, drift=None, swirl_x=None, swirl_z=None):
+        """Animated force on field `handle` (abi.ForceAnimation, or None to stop); the masks
+        are device buffers (see upload)."""
+        a = C.byref(animation) if animation is not None else None
+        return self._call("stable_fluids_ext_set_force_animation_cuda", handle, a, _ptr(drift), _ptr(swirl_x), _ptr(swirl_z))
+
+    def force_animation_step(self):
+        out = C.c_uint32(0)
+        self._call("stable_fluids_ext_force_animation_step_cuda", C.byref(out))
+        return out.value
+
     def set_flow_velocity(self, x, y, z):
         return self._call("stable_fluids_ext_set_flow_velocity_cuda", _ptr(x), _ptr(y), _ptr(z))
 

@@ -141,6 +141,60 @@ def reference_benchmark_scene(nx=128, ny=160, nz=128, diffuse_iterations=24, pre
     return Scene("reference-benchmark", desc, src, (fx, fy, fz))
 
 
+def viewer_plume_scene(nx=96, ny=128, nz=96, diffuse_iterations=24, pressure_iterations=96, cell_size=0.01,
+                       dt=1.0 / 90.0, viscosity=0.00012, diffusion=0.00005, dissipation=0.35):
+    """The visualizer's plume scene (vulkan-app/scene_plume.ixx:34-50, scene_plume.cpp:185-289):
+    periodic x/z, no-slip floor, outflow top; a source sphere of radius 6h with source 32 w
+    and upward force 7.6 w (w = 1 - d^2/r^2), plus the masks of its animated forcing: the
+    drift mask (radius 2.1 r, centred 1.35 r above the source) and the swirl masks (radius
+    1.65 r, 0.9 r above; the unit tangent (-dz, dx)/|(dx, dz)| times the weight). Returns
+    (scene, (drift, swirl_x, swirl_z)); the forces' x and z start at zero."""
+    flow = abi.flow_boundary([abi.FLOW_PERIODIC, abi.FLOW_PERIODIC, abi.FLOW_NO_SLIP, abi.FLOW_OUTFLOW,
+                              abi.FLOW_PERIODIC, abi.FLOW_PERIODIC])
+    sb = abi.scalar_boundary([abi.SCALAR_PERIODIC, abi.SCALAR_PERIODIC, abi.SCALAR_ZERO_FLUX, abi.SCALAR_ZERO_FLUX,
+                              abi.SCALAR_PERIODIC, abi.SCALAR_PERIODIC])
+    desc = abi.Desc(nx, ny, nz, cell_size, dt, viscosity, diffuse_iterations, pressure_iterations, flow,
+                    scalars=[abi.scalar_desc(diffusion, dissipation, 0.0, sb)], vectors=[abi.force_desc()])
+    f32 = np.float32
+    h = f32(cell_size)
+    sx, sy, sz = f32(nx) * h * f32(0.5), f32(ny) * h * f32(0.13), f32(nz) * h * f32(0.5)
+    r = h * f32(6.0)
+    swirl_y, drift_y = sy + r * f32(0.9), sy + r * f32(1.35)
+    k, j, i = np.meshgrid(np.arange(nz, dtype=f32), np.arange(ny, dtype=f32), np.arange(nx, dtype=f32), indexing="ij")
+    px, py, pz = (i + f32(0.5)) * h, (j + f32(0.5)) * h, (k + f32(0.5)) * h
+
+    def radial_weight(cy, radius):
+        dx, dy, dz = px - sx, py - cy, pz - sz
+        r2 = radius * radius
+        d2 = dx * dx + dy * dy + dz * dz
+        return np.where((d2 >= r2) | (r2 <= f32(0)), f32(0), f32(1) - d2 / r2).astype(f32)
+
+    w_src = radial_weight(sy, r)
+    w_swirl = radial_weight(swirl_y, r * f32(1.65))
+    w_drift = radial_weight(drift_y, r * f32(2.1))
+    dx, dz = px - sx, pz - sz
+    radial = np.sqrt(dx * dx + dz * dz).astype(f32)
+    with np.errstate(divide="ignore"):
+        inv = np.where(radial > f32(1e-5), f32(1) / radial, f32(0)).astype(f32)
+    swirl_x = (-dz * inv * w_swirl).astype(f32)
+    swirl_z = (dx * inv * w_swirl).astype(f32)
+    zero = np.zeros(desc.shape, f32)
+    scene = Scene("viewer-plume", desc, (f32(32.0) * w_src).astype(f32), (zero, (f32(7.6) * w_src).astype(f32), zero.copy()))
+    return scene, (w_drift, swirl_x, swirl_z)
+
+
+def animated_force(masks, step, animation):
+    """Host restatement of the per-step forcing (vulkan-app/scene_plume.cpp:327-334), FP32:
+    force x and z of step `step` from the masks and an abi.ForceAnimation's constants."""
+    f32 = np.float32
+    drift, swirl_x, swirl_z = masks
+    phase = f32(step) * f32(animation.phase_rate)
+    ax = f32(animation.drift_x_amplitude) * f32(np.sin(phase))
+    az = f32(animation.drift_z_amplitude) * f32(np.cos(phase * f32(animation.drift_z_rate)))
+    sw = f32(animation.swirl_amplitude) * f32(np.cos(phase * f32(animation.swirl_rate)))
+    return (ax * drift + sw * swirl_x).astype(f32), (az * drift + sw * swirl_z).astype(f32)
+
+
 def seed_context(ctx, scene):
     """Uploads the scene's source, force (and initial velocity) into a Context."""
     ctx.update_scalar_field_source(ctx.scalar_handles[0], ctx.upload(scene.source))

@@ -48,6 +48,7 @@ int main(void) {
   P(StableFluidsFlowBoundaryFaceDesc) P(StableFluidsFlowBoundaryConfig) P(StableFluidsScalarBoundaryConfig)
   P(StableFluidsSimulationConfig) P(StableFluidsScalarFieldDesc) P(StableFluidsVectorFieldDesc)
   P(StableFluidsContextCreateDesc) P(StableFluidsViewRequest) P(StableFluidsView) P(StableFluidsExtOptions)
+  P(StableFluidsExtForceAnimation) O(StableFluidsExtForceAnimation, start_step)
   O(StableFluidsContextCreateDesc, stream) O(StableFluidsContextCreateDesc, scalar_fields)
   O(StableFluidsContextCreateDesc, vector_field_count) O(StableFluidsSimulationConfig, flow_boundary)
   O(StableFluidsView, data0) O(StableFluidsViewRequest, consumer_stream) O(StableFluidsScalarFieldDesc, boundary)
@@ -69,6 +70,8 @@ def test_struct_layout_matches_c_compiler(tmp_path):
         "StableFluidsScalarFieldDesc": C.sizeof(a.ScalarFieldDesc), "StableFluidsVectorFieldDesc": C.sizeof(a.VectorFieldDesc),
         "StableFluidsContextCreateDesc": C.sizeof(a.ContextCreateDesc), "StableFluidsViewRequest": C.sizeof(a.ViewRequest),
         "StableFluidsView": C.sizeof(a.View), "StableFluidsExtOptions": C.sizeof(a.ExtOptions),
+        "StableFluidsExtForceAnimation": C.sizeof(a.ForceAnimation),
+        "StableFluidsExtForceAnimation.start_step": a.ForceAnimation.start_step.offset,
         "StableFluidsContextCreateDesc.stream": a.ContextCreateDesc.stream.offset,
         "StableFluidsContextCreateDesc.scalar_fields": a.ContextCreateDesc.scalar_fields.offset,
         "StableFluidsContextCreateDesc.vector_field_count": a.ContextCreateDesc.vector_field_count.offset,
@@ -155,6 +158,8 @@ def test_null_context_calls_fail_cleanly(abi):
     assert lib.stable_fluids_update_scalar_field_source_cuda(None, 1, None) == abi.RESULT_BACKEND_FAILURE
     assert lib.stable_fluids_update_vector_field_cuda(None, 1, None, None, None) == abi.RESULT_BACKEND_FAILURE
     assert lib.stable_fluids_ext_set_flow_velocity_cuda(None, None, None, None) == abi.RESULT_BACKEND_FAILURE
+    assert lib.stable_fluids_ext_set_force_animation_cuda(None, 1, None, None, None, None) == abi.RESULT_BACKEND_FAILURE
+    assert lib.stable_fluids_ext_force_animation_step_cuda(None, None) == abi.RESULT_BACKEND_FAILURE
     assert lib.stable_fluids_ext_run_stage_cuda(None, 0, 1) == abi.RESULT_BACKEND_FAILURE
     assert lib.stable_fluids_ext_stage_bytes_cuda(None, 0) == 0
     assert lib.stable_fluids_create_context_cuda(None, None, None, 0, None, 0) == abi.RESULT_BACKEND_FAILURE
