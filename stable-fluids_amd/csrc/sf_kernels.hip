@@ -149,6 +149,17 @@ __device__ __forceinline__ void divergence_body(float* __restrict__ div, const f
     const float inv_2h  = 0.5f / h;
     const float* v[3]   = {vx, vy, vz};
     float acc           = 0.0f;
+    if (x > 0 && x < d.nx - 1 && y > 0 && y < d.ny - 1 && z > 0 && z < d.nz - 1) {
+        // interior: every tap is the direct neighbour (the same expressions, no ghost rule)
+        const uint64_t st[3] = {1, static_cast<uint64_t>(d.nx), static_cast<uint64_t>(d.nx) * static_cast<uint64_t>(d.ny)};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float dd = (v[c][i + st[c]] - v[c][i - st[c]]) * inv_2h;
+            acc            = c == 0 ? dd : acc + dd;
+        }
+        div[i] = acc;
+        return;
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const float* f     = v[c];
@@ -180,6 +191,29 @@ __device__ __forceinline__ void project_body(float* __restrict__ ox, float* __re
     const float inv_2h = 0.5f / h;
     const float* v[3]  = {vx, vy, vz};
     float proj[3];
+    if (x > 1 && x < d.nx - 2 && y > 1 && y < d.ny - 2 && z > 1 && z < d.nz - 2) {
+        // two cells from every face: every velocity and pressure tap (pressure two cells out,
+        // through the projected neighbours) is a direct load; the same expressions, no ghost rule
+        const uint64_t st[3] = {1, static_cast<uint64_t>(d.nx), static_cast<uint64_t>(d.nx) * static_cast<uint64_t>(d.ny)};
+        const float pc       = p[i];
+        float acc            = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const uint64_t s1 = st[c], s2 = 2 * st[c];
+            const float pp = p[i + s1], pm = p[i - s1];
+            proj[c]          = v[c][i] - (pp - pm) * inv_2h;
+            const float plus  = v[c][i + s1] - (p[i + s2] - pc) * inv_2h;
+            const float minus = v[c][i - s1] - (pc - p[i - s2]) * inv_2h;
+            const float dd    = (plus - minus) * inv_2h;
+            acc               = c == 0 ? dd : acc + dd;
+        }
+        ox[i]  = proj[0];
+        oy[i]  = proj[1];
+        oz[i]  = proj[2];
+        div[i] = acc;
+        mag[i] = sqrtf(proj[0] * proj[0] + proj[1] * proj[1] + proj[2] * proj[2]);
+        return;
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) proj[c] = v[c][i] - grad_p(p, c, x, y, z, d, pbc, inv_2h);
     float acc = 0.0f;
@@ -227,8 +261,20 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_advect_scalar_batched(const 
     tile_block_of(sd.d, it.block, bx, by, bz);
     advect_scalar_body(sd.out, sd.src, sd.v[0], sd.v[1], sd.v[2], sd.dt, sd.d, sd.h, sd.sbc, sd.fbc, bx * kTileX + static_cast<int>(threadIdx.x), by * kTileY + static_cast<int>(threadIdx.y), bz);
 }
+// Cell of this thread in a 64 x 4 tile: wave w covers x in [16 w, 16 w + 16) of all four
+// rows (four 64-byte row segments), so only the waves on the outermost 16 columns of a row
+// meet the x faces (a 64-cell row per wave put half of the waves there).
+__device__ __forceinline__ void cell_of_thread(const int bx, const int by, int& x, int& y) {
+    const int t = static_cast<int>(threadIdx.y) * kTileX + static_cast<int>(threadIdx.x);
+    const int w = t >> 6, l = t & 63;
+    x           = bx * kTileX + 16 * w + (l & 15);
+    y           = by * kTileY + (l >> 4);
+}
+
 __global__ __launch_bounds__(kTileX* kTileY) void k_divergence(float* __restrict__ div, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const Dims d, const float h, const StencilBC dbc) {
-    divergence_body(div, vx, vy, vz, d, h, dbc, static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y), static_cast<int>(blockIdx.z));
+    int x, y;
+    cell_of_thread(static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), x, y);
+    divergence_body(div, vx, vy, vz, d, h, dbc, x, y, static_cast<int>(blockIdx.z));
 }
 __global__ __launch_bounds__(kTileX* kTileY) void k_divergence_batched(const DivergenceDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
     const BatchItem it = items[blockIdx.x];
@@ -238,7 +284,9 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_divergence_batched(const Div
     divergence_body(sd.div, sd.v[0], sd.v[1], sd.v[2], sd.d, sd.h, sd.dbc, bx * kTileX + static_cast<int>(threadIdx.x), by * kTileY + static_cast<int>(threadIdx.y), bz);
 }
 __global__ __launch_bounds__(kTileX* kTileY) void k_project(float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz, float* __restrict__ div, float* __restrict__ mag, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float* __restrict__ p, const Dims d, const float h, const StencilBC pbc, const StencilBC dbc) {
-    project_body(ox, oy, oz, div, mag, vx, vy, vz, p, d, h, pbc, dbc, static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y), static_cast<int>(blockIdx.z));
+    int x, y;
+    cell_of_thread(static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), x, y);
+    project_body(ox, oy, oz, div, mag, vx, vy, vz, p, d, h, pbc, dbc, x, y, static_cast<int>(blockIdx.z));
 }
 __global__ __launch_bounds__(kTileX* kTileY) void k_project_batched(const ProjectDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
     const BatchItem it = items[blockIdx.x];
