@@ -60,8 +60,7 @@ typedef struct StableFluidsExtOptions {
     uint32_t iterate_storage;            /* StableFluidsExtIterateStorage */
     uint32_t defect_correction_interval; /* m, passes per refinement cycle; 0 = 10, at most 4096 */
     uint32_t pressure_warm_start;        /* StableFluidsExtWarmStart */
-    uint32_t warm_start_iterations;      /* passes after a learned guess (<= pressure_iterations;
-                                            raised by one when pressure_iterations minus it is odd) */
+    uint32_t warm_start_iterations;      /* passes after a learned guess (<= pressure_iterations) */
     uint32_t warm_start_coarsening;      /* the network runs on the grid coarsened 2^c times per
                                             axis (c = 1, 2 or 3; 0 = 1); extents multiples of 2^(c+2) */
     uint32_t flags;                      /* StableFluidsExtFlags bits; other bits must be zero */

@@ -275,7 +275,7 @@ hipError_t enqueue_solve(Context& c, SolveKind solve, const float* b, const floa
 
 // Learned warm start in place of the zero-initialised pressure solve (reference
 // backend-cuda.cu:643-650): network guess from the half-resolution divergence, residual
-// test, then P passes of which the first C = P - K (even) are skipped when the guess is
+// test, then P passes of which the first C = P - K are skipped when the guess is
 // accepted; on fallback all P passes run from zero, exactly the plain solve.
 hipError_t enqueue_learned_pressure(Context& c, const StencilBC& p_bc) {
     auto& f        = c.flow;
@@ -304,10 +304,11 @@ hipError_t enqueue_learned_pressure(Context& c, const StencilBC& p_bc) {
         per.push_back(C);
         if (two && c.options.pairs && K % 2 == 0) per.insert(per.end(), K / 2, 2);
         else per.insert(per.end(), K, 1);
-    } else if (two && c.options.pairs && P % 2 == 0) {
+    } else if (two && c.options.pairs && C % 2 == 0 && K % 2 == 0) {
         per.assign(P / 2, 2);
     } else if (two) {
-        per.assign(C / 2, 2); // C is even
+        per.assign(C % 2, 1); // an odd C starts with one single pass
+        per.insert(per.end(), C / 2, 2);
         per.insert(per.end(), K, 1);
     } else {
         per.assign(P, 1);
@@ -325,7 +326,11 @@ hipError_t enqueue_learned_pressure(Context& c, const StencilBC& p_bc) {
     SF_TRY(warm::launch_guess_test(c.stream, guess, f.rhs, h * h, 10.0, w.partials, w.stats, w.skip, c.dims, p_bc, !chain));
     for (int l = 0; l < L; ++l) {
         if (chain && l == 0) {
-            SF_TRY(launch_smooth_chain(c.stream, kSolvePoisson, guess, guess == f.p ? f.p_aux : f.p, f.rhs, h * h, C, c.dims, p_bc, w.skip, w.chain_grid));
+            // the chain's result lands in its first buffer for an even pass count, in its
+            // second for an odd one: either way in the guess buffer
+            float* other = guess == f.p ? f.p_aux : f.p;
+            if (C % 2 == 0) SF_TRY(launch_smooth_chain(c.stream, kSolvePoisson, guess, other, f.rhs, h * h, C, c.dims, p_bc, w.skip, w.chain_grid));
+            else SF_TRY(launch_smooth_chain(c.stream, kSolvePoisson, other, guess, f.rhs, h * h, C, c.dims, p_bc, w.skip, w.chain_grid));
             continue;
         }
         const float* src = l == 0 ? (C > 0 ? nullptr : guess) : dst(l - 1);
@@ -459,7 +464,6 @@ StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const Stabl
             if (cl > 3) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
             const int fac = 1 << cl, m = 4 * fac;
             if (P <= 0 || k > P || options.refine || cfg.nx % m || cfg.ny % m || cfg.nz % m) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
-            if ((P - k) % 2) ++k;
             options.warm_iterations = k;
             options.warm_factor     = fac;
         }

@@ -161,10 +161,10 @@ hipError_t launch_smooth(hipStream_t s, SmootherKind smoother, SolveKind solve, 
 // second), bit-identical to two launch_smooth calls; x_old and x_new must differ.
 bool smooth2_supported(SmootherKind smoother, Dims d, const StencilBC& bc);
 hipError_t launch_smooth2(hipStream_t s, SolveKind solve, const SmoothPass& pass, Dims d, const StencilBC& bc);
-// The first `passes` (even, > 0) passes of an RBGS solve from zero in one launch that runs
-// only when sync[0] == 0 (the learned warm start's fallback): zeros io, then ping-pongs
-// io -> other -> ... -> io, bit-identical to `passes` launch_smooth calls from zero with
-// the result in io. Its `grid` workgroups (smooth_chain_grid, computed outside stream
+// The first `passes` (> 0) passes of an RBGS solve from zero in one launch that runs only
+// when sync[0] == 0 (the learned warm start's fallback): zeros io, then ping-pongs
+// io -> other -> io ..., bit-identical to `passes` launch_smooth calls from zero, with the
+// result in io for an even count and in other for an odd one. Its `grid` workgroups (smooth_chain_grid, computed outside stream
 // capture) meet at grid barriers counting on sync[1], which an earlier kernel on the
 // stream must zero; a barrier that does not complete in bounded time sets sync[2] (the
 // result is then invalid) instead of hanging.

@@ -1891,7 +1891,8 @@ __device__ __forceinline__ void grid_barrier(unsigned* count, const unsigned tar
 }
 
 // `passes` RBGS passes from zero in one launch, run only when sync[0] == 0: zero `io`,
-// then io -> other -> io -> ... (passes even: the result lands in io). Workgroup i runs
+// then io -> other -> io -> ... (the result lands in io for an even count, in other for
+// an odd one). Workgroup i runs
 // pass blocks i, i + G, ... (G = gridDim.x, each renumbered as the single pass does) and
 // the workgroups meet at a grid barrier between passes, counting on sync[1] (zeroed by an
 // earlier kernel on the stream); sync[2] flags a barrier that did not complete. Every relaxation is
@@ -2075,7 +2076,7 @@ hipError_t smooth_chain_grid(const Dims d, uint32_t* out_grid) {
 }
 
 hipError_t launch_smooth_chain(hipStream_t s, const SolveKind solve, float* io, float* other, const float* b, const float coef, const int passes, const Dims d, const StencilBC& bc, int* sync, const uint32_t grid) {
-    if (passes <= 0 || (passes & 1) != 0 || io == other || grid == 0) return hipErrorInvalidValue;
+    if (passes <= 0 || io == other || grid == 0) return hipErrorInvalidValue;
     const TileGeom g = plan_tiles(d, 0);
     if (solve == kSolvePoisson) k_rbgs_chain<kSolvePoisson><<<grid, kThreads, 0, s>>>(io, other, b, coef, passes, d, bc, g, sync);
     else k_rbgs_chain<kSolveDiffusion><<<grid, kThreads, 0, s>>>(io, other, b, coef, passes, d, bc, g, sync);

@@ -66,10 +66,10 @@ def run_learned(scene, params, k, pairs):
         return gpu_fields(ctx, scene.desc), stats
 
 
-@pytest.mark.parametrize("k", [0, 4, 6])
+@pytest.mark.parametrize("k", [0, 4, 5, 6])
 @pytest.mark.parametrize("net", ["accepted", "fallback"])
 def test_learned_warm_start_pairs_bitwise(k, net):
-    """Accepted guesses with C = P - K skipped passes (C / 2 even and odd) and fallbacks."""
+    """Accepted guesses with C = P - K skipped passes (C even and odd) and fallbacks."""
     scene = plume()
     if net == "accepted":
         layers = uo.init_params(4)
