@@ -318,8 +318,9 @@ hipError_t enqueue_learned_pressure(Context& c, const StencilBC& p_bc) {
     for (int acc = 0; acc < C;) acc += per[skipped++];
     auto dst     = [&](const int l) { return ((L - 1 - l) & 1) == 0 ? f.p : f.p_aux; };
     float* guess = dst(skipped - 1);
-    SF_TRY(warm::launch_restrict_input(c.stream, f.rhs, w.in2, w.partials, w.scal, cd, fac, p_bc));
-    SF_TRY(unet::launch_pack_input(c.stream, w.in2, w.scal + 1, 1.0f, unet::net_input(*w.net), vc));
+    // restriction, then sigma and the packing of the network input in one launch
+    SF_TRY(warm::launch_restrict_input(c.stream, f.rhs, w.in2, w.partials, cd, fac, p_bc));
+    SF_TRY(unet::launch_pack_input_sigma(c.stream, w.in2, w.partials, static_cast<int>(warm::restrict_partials(cd)), w.scal, unet::net_input(*w.net), vc));
     SF_TRY(unet::net_forward_packed(*w.net, false));
     SF_TRY(warm::launch_prolong(c.stream, unet::net_output(*w.net), w.scal, static_cast<float>(fac * fac) * h * h, guess, cd, fac));
     // (the chain zeroes the guess itself on fallback)

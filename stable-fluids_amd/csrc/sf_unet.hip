@@ -153,22 +153,26 @@ hipError_t net_forward_packed(Net& n, const bool train) {
     SF_TRY(launch_pool2(n.stream, 32, n.a4, n.p2, n.d[2]));
     SF_TRY(conv(n, 4, n.p2, n.a5, kEpiRelu, nullptr, 2));
     SF_TRY(conv(n, 5, n.a5, n.a6, kEpiRelu, nullptr, 2));
-    SF_TRY(launch_up2(n.stream, 64, n.a6, n.u1, n.d[1]));
+    // training keeps the upsampled inputs and the 16-channel head output for the backward
+    // pass; inference stages the upsampled inputs straight from the coarse tensors and
+    // writes only the head's channel 0 (the same values)
     if (train) {
+        SF_TRY(launch_up2(n.stream, 64, n.a6, n.u1, n.d[1]));
         SF_TRY(conv(n, 6, n.u1, n.r7, kEpiRelu, nullptr, 1));
         SF_TRY(launch_add_mask(n.stream, n.r7, n.a4, nullptr, n.s7, n.vox[1] * 32));
     } else {
-        SF_TRY(conv(n, 6, n.u1, n.s7, kEpiRelu | kEpiAdd, n.a4, 1));
+        SF_TRY(conv(n, 6, n.a6, n.s7, kEpiRelu | kEpiAdd | kEpiUpInput, n.a4, 1));
     }
     SF_TRY(conv(n, 7, n.s7, n.a8, kEpiRelu, nullptr, 1));
-    SF_TRY(launch_up2(n.stream, 32, n.a8, n.u2, n.d[0]));
     if (train) {
+        SF_TRY(launch_up2(n.stream, 32, n.a8, n.u2, n.d[0]));
         SF_TRY(conv(n, 8, n.u2, n.r9, kEpiRelu, nullptr, 0));
         SF_TRY(launch_add_mask(n.stream, n.r9, n.a2, nullptr, n.s9, n.vox[0] * 16));
     } else {
-        SF_TRY(conv(n, 8, n.u2, n.s9, kEpiRelu | kEpiAdd, n.a2, 0));
+        SF_TRY(conv(n, 8, n.a8, n.s9, kEpiRelu | kEpiAdd | kEpiUpInput, n.a2, 0));
     }
     SF_TRY(conv(n, 9, n.s9, n.a10, kEpiRelu, nullptr, 0));
+    if (!train) return launch_conv3(n.stream, kCin[10], kCout[10], n.a10, n.wb[10], n.params + n.b_off[10], nullptr, n.q, nullptr, kEpiDense0, n.d[0]);
     SF_TRY(launch_conv3(n.stream, kCin[10], kCout[10], n.a10, n.wb[10], n.params + n.b_off[10], nullptr, n.y, nullptr, kEpiNone, n.d[0]));
     return launch_head_channel0(n.stream, n.y, n.q, n.vox[0]);
 }

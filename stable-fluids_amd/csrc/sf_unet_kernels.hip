@@ -53,11 +53,15 @@ inline uint32_t tile_count(const Dims d) {
 // x[tile + halo][C] -> LDS [hz][hy][hx][C]; zeros outside the grid (zero padding). Four
 // 16-byte loads per thread are issued before their stores (voxels outside the grid load
 // voxel 0 and store zeros), so the staging waits out the load latency once per four.
+// With up = 1, x holds the input at half resolution (extents d / 2) and the staged voxel
+// (gx, gy, gz) of the full-resolution input is x's voxel (gx / 2, gy / 2, gz / 2): the
+// nearest upsampling, fused.
 template <int C>
-__device__ __forceinline__ void stage_halo(const bf16_t* __restrict__ x, bf16_t* s, const Origin o, const Dims d) {
+__device__ __forceinline__ void stage_halo(const bf16_t* __restrict__ x, bf16_t* s, const Origin o, const Dims d, const int up = 0) {
     constexpr int kParts = C / 8;
     constexpr int kTotal = kHalo * kParts;
     constexpr int kU     = 4;
+    const Dims ds{d.nx >> up, d.ny >> up, d.nz >> up};
     for (int c0 = static_cast<int>(threadIdx.x); c0 < kTotal; c0 += kU * kThreads) {
         uint4 v[kU];
         bool in[kU];
@@ -68,7 +72,7 @@ __device__ __forceinline__ void stage_halo(const bf16_t* __restrict__ x, bf16_t*
             const int hx = vox % kHX, hy = (vox / kHX) % kHY, hz = vox / (kHX * kHY);
             const int gx = o.x0 - 1 + hx, gy = o.y0 - 1 + hy, gz = o.z0 - 1 + hz;
             in[u]          = gx >= 0 && gx < d.nx && gy >= 0 && gy < d.ny && gz >= 0 && gz < d.nz;
-            const uint64_t src = in[u] ? lin(gx, gy, gz, d) * C + part * 8 : 0;
+            const uint64_t src = in[u] ? lin(gx >> up, gy >> up, gz >> up, ds) * C + part * 8 : 0;
             v[u]           = *reinterpret_cast<const uint4*>(x + src);
         }
 #pragma unroll
@@ -103,7 +107,7 @@ __global__ __launch_bounds__(kThreads) void k_conv3(const bf16_t* __restrict__ x
     __shared__ __attribute__((aligned(16))) bf16_t s_x[kHalo * CI];
     const Origin o = tile_origin(static_cast<int>(blockIdx.x) / kSplit, d);
     const int nb0  = (static_cast<int>(blockIdx.x) % kSplit) * NB;
-    stage_halo<CI>(x, s_x, o, d);
+    stage_halo<CI>(x, s_x, o, d, (epi & kEpiUpInput) ? 1 : 0);
     __syncthreads();
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -171,11 +175,12 @@ __global__ __launch_bounds__(kThreads) void k_conv3(const bf16_t* __restrict__ x
         for (int nb = 0; nb < NB; ++nb) {
             const int co    = (nb0 + nb) * 16 + m;
             const float bco = bias ? bias[co] : 0.0f;
+            if ((epi & kEpiDense0) && co != 0) continue;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int gx = o.x0 + 4 * grp + i;
                 if (gx >= d.nx) continue;
-                const uint64_t idx = lin(gx, gy, gz, d) * CO + co;
+                const uint64_t idx = (epi & kEpiDense0) ? lin(gx, gy, gz, d) : lin(gx, gy, gz, d) * CO + co;
                 float v            = acc[mb][nb][i] + bco;
                 if (epi & kEpiMask) v = bf2f(aux[idx]) > 0.0f ? v : 0.0f;
                 if (epi & kEpiRelu) v = fmaxf(v, 0.0f);
@@ -414,6 +419,26 @@ __global__ __launch_bounds__(kElem) void k_head_mean(const double* __restrict__ 
     if (threadIdx.x == 0) *mean = static_cast<float>(t / static_cast<double>(n));
 }
 
+// Input packing with the normalisation computed in place: every block adds the per-block
+// sums of squares of the restricted divergence (the same order as a one-block reduction),
+// so all blocks get the same sigma; block 0 stores sigma and 1/sigma for the later stages.
+__global__ __launch_bounds__(kElem) void k_pack_input_sigma(const float* __restrict__ in, const double* __restrict__ partials, const int n_partials, float* __restrict__ scal, bf16_t* __restrict__ a0, const uint64_t n) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < n_partials; b += kElem) acc += partials[b];
+    const double sigma = sqrt(block_sum(acc) / static_cast<double>(n));
+    const float inv    = sigma > 0.0 ? static_cast<float>(1.0 / sigma) : 0.0f;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[0] = static_cast<float>(sigma);
+        scal[1] = inv;
+    }
+    for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kElem) + threadIdx.x; v < n; v += static_cast<uint64_t>(gridDim.x) * kElem) {
+        const uint32_t c01 = static_cast<uint32_t>(f2bf(in[2 * v] * inv)) | (static_cast<uint32_t>(f2bf(in[2 * v + 1])) << 16);
+        uint4* dst = reinterpret_cast<uint4*>(a0 + v * 16);
+        dst[0]     = make_uint4(c01, 0u, 0u, 0u);
+        dst[1]     = make_uint4(0u, 0u, 0u, 0u);
+    }
+}
+
 __global__ __launch_bounds__(kElem) void k_head_channel0(const float* __restrict__ y, float* __restrict__ out, const uint64_t n) {
     for (uint64_t v = blockIdx.x * static_cast<uint64_t>(kElem) + threadIdx.x; v < n; v += static_cast<uint64_t>(gridDim.x) * kElem) out[v] = y[v * 16];
 }
@@ -478,6 +503,8 @@ hipError_t wgrad_t(hipStream_t s, const bf16_t* x, const bf16_t* g, float* dw, f
 hipError_t launch_conv3(hipStream_t s, const int cin, const int cout, const bf16_t* x, const bf16_t* w, const float* bias, bf16_t* y_bf16, float* y_f32, const bf16_t* aux, const int epilogue, const Dims d) {
     if ((y_bf16 == nullptr) == (y_f32 == nullptr)) return hipErrorInvalidValue;
     if ((epilogue & (kEpiAdd | kEpiMask)) && aux == nullptr) return hipErrorInvalidValue;
+    if ((epilogue & kEpiDense0) && y_f32 == nullptr) return hipErrorInvalidValue;
+    if ((epilogue & kEpiUpInput) && ((d.nx | d.ny | d.nz) & 1)) return hipErrorInvalidValue;
     if (d.nx <= 0 || d.ny <= 0 || d.nz <= 0) return hipErrorInvalidValue;
     SF_CONV_CASES(conv3_t, s, x, w, bias, y_bf16, y_f32, aux, epilogue, d)
 }
@@ -534,6 +561,11 @@ hipError_t launch_fill_f32(hipStream_t s, float* p, const float value, const uin
     k_fill_f32<<<elem_grid(n), kElem, 0, s>>>(p, value, n);
     return hipGetLastError();
 }
+hipError_t launch_pack_input_sigma(hipStream_t s, const float* in, const double* partials, const int n_partials, float* scal, bf16_t* a0, const uint64_t n) {
+    k_pack_input_sigma<<<elem_grid(n), kElem, 0, s>>>(in, partials, n_partials, scal, a0, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_pack_input(hipStream_t s, const float* in, const float* scale_ptr, const float in_scale, bf16_t* a0, const uint64_t n) {
     k_pack_input<<<elem_grid(n), kElem, 0, s>>>(in, scale_ptr, in_scale, a0, n);
     return hipGetLastError();

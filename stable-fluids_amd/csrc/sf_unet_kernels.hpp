@@ -19,6 +19,8 @@ enum ConvEpilogue : int {
     kEpiRelu = 1, // y = max(acc + bias, 0)
     kEpiAdd  = 2, // y = (...) + aux   (after the ReLU when both are set)
     kEpiMask = 4, // y = (acc + bias) * (aux > 0)
+    kEpiUpInput = 8,  // x holds the input at half resolution: nearest upsampling fused into the staging
+    kEpiDense0  = 16, // FP32 output of channel 0 only, dense (y[voxel])
 };
 
 // y[v][co] = epilogue(sum_tap sum_ci w[tap][co][ci] * x[v + off(tap)][ci] + bias[co]).
@@ -47,6 +49,9 @@ hipError_t launch_fill_f32(hipStream_t s, float* p, float value, uint64_t n);
 // Network input: a0[v][0] = in[v][0] * in_scale (*scale_ptr when given), a0[v][1] =
 // in[v][1], channels 2..15 zero. in is [v][2] FP32.
 hipError_t launch_pack_input(hipStream_t s, const float* in, const float* scale_ptr, float in_scale, bf16_t* a0, uint64_t n);
+// The same with scale 1/sigma for the pair-channel input in, sigma = sqrt(sum of the
+// n_partials per-block sums / n); sigma and 1/sigma are stored in scal[0], scal[1].
+hipError_t launch_pack_input_sigma(hipStream_t s, const float* in, const double* partials, int n_partials, float* scal, bf16_t* a0, uint64_t n);
 // Loss of channel 0 of the FP32 head output y[v][16] against target t[v]:
 // loss += w sum (y0 - t)^2 / n (into *loss, FP32 atomic, may be null); g[v][0] =
 // 2 w (y0 - t) / n as BF16, channels 1..15 zero.

@@ -2,8 +2,8 @@
 //
 // No counterpart in the reference (its projection restarts pressure from zero,
 // backend-cuda.cu:644). Stage kernels of the warm start, in graph order:
-//   restrict_input: half-resolution divergence (mean of 8 cells) and wall mask, sigma;
-//   (network forward, sf_unet.hip)
+//   restrict_input: coarse divergence (mean of f^3 cells), wall mask, sums of squares;
+//   (sigma and the input packing, network forward: sf_unet_kernels.hip, sf_unet.hip)
 //   prolong:        trilinear guess on the full grid, scaled by (2h)^2 sigma;
 //   guess_test:     ||b - A p0|| against ||b|| (FP64), fallback flag, skip flag.
 // All reductions are deterministic two-pass sums (per-block partials, then one block).
@@ -149,18 +149,6 @@ __global__ __launch_bounds__(kThreads) void k_restrict_input(const float* __rest
     block_sum_to(acc, partials + blockIdx.x);
 }
 
-__global__ __launch_bounds__(kThreads) void k_sigma(const double* __restrict__ partials, const int n_partials, const double n, float* __restrict__ scal) {
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < n_partials; b += kThreads) acc += partials[b];
-    __shared__ double total;
-    block_sum_to(acc, &total);
-    if (threadIdx.x == 0) {
-        const double sigma = sqrt(total / n);
-        scal[0]            = static_cast<float>(sigma);
-        scal[1]            = sigma > 0.0 ? static_cast<float>(1.0 / sigma) : 0.0f;
-    }
-}
-
 // Rows of the fine grid handled by this block: a contiguous chunk.
 struct Rows {
     uint64_t r, end;
@@ -279,12 +267,19 @@ __global__ __launch_bounds__(kThreads) void k_guess_partials(const float* __rest
             float zm = p0[row(max(z0 - 1, 0)) + x];
             float c  = p0[row(z0) + x];
             float zp = p0[row(min(z0 + 1, d.nz - 1)) + x];
+            // the taps of planes z + 1 and z + 2 are in flight while plane z is summed
+            struct Taps {
+                float ym, yp, xm, xp, dv;
+            };
+            auto taps = [&](const int z) {
+                const uint64_t r = row(min(z, d.nz - 1));
+                return Taps{p0[r - sy + x], p0[r + sy + x], p0[r + xl], p0[r + xr], div[r + x]};
+            };
+            Taps t0 = taps(z0), t1 = taps(z0 + 1);
             for (int z = z0; z < z1; ++z) {
-                const uint64_t r = row(z);
-                const float nxt  = p0[row(min(z + 2, d.nz - 1)) + x];
-                const float ym = p0[r - sy + x], yp = p0[r + sy + x];
-                const float xm = p0[r + xl], xp = p0[r + xr];
-                const float dv = div[r + x];
+                const float nxt = p0[row(min(z + 2, d.nz - 1)) + x];
+                const Taps t2   = taps(z + 2);
+                const float ym = t0.ym, yp = t0.yp, xm = t0.xm, xp = t0.xp, dv = t0.dv;
                 if (x_in && z > 0 && z < d.nz - 1) {
                     float sum = xm;
                     sum += xp;
@@ -300,6 +295,8 @@ __global__ __launch_bounds__(kThreads) void k_guess_partials(const float* __rest
                 zm = c;
                 c  = zp;
                 zp = nxt;
+                t0 = t1;
+                t1 = t2;
             }
         }
     }
@@ -486,10 +483,10 @@ bool singular(const Dims, const StencilBC& bc) {
 
 } // namespace
 
-hipError_t launch_restrict_input(hipStream_t s, const float* div, float* in2, double* partials, float* scal, const Dims cd, const int f, const StencilBC& p_bc) {
-    const unsigned b = blocks_for(cells_of(cd) * kGroup);
-    k_restrict_input<<<b, kThreads, 0, s>>>(div, in2, partials, cd, f, p_bc);
-    k_sigma<<<1, kThreads, 0, s>>>(partials, static_cast<int>(b), static_cast<double>(cells_of(cd)), scal);
+unsigned restrict_partials(const Dims cd) { return blocks_for(cells_of(cd) * kGroup); }
+
+hipError_t launch_restrict_input(hipStream_t s, const float* div, float* in2, double* partials, const Dims cd, const int f, const StencilBC& p_bc) {
+    k_restrict_input<<<restrict_partials(cd), kThreads, 0, s>>>(div, in2, partials, cd, f, p_bc);
     return hipGetLastError();
 }
 

@@ -16,10 +16,11 @@ namespace sf::warm {
 constexpr int kPartialBlocks = 8192; // enough waves in flight for the grid-stride stages
 
 // Coarse network input in2[v][2] = (mean of the f^3 fine div values, wall mask) on the
-// coarse grid cd (fine = f * cd), and 1/sigma into scal[1], sigma into scal[0]
-// (sigma = RMS of the coarse divergence; 1/sigma = 0 when it is 0).
-// partials: kPartialBlocks doubles.
-hipError_t launch_restrict_input(hipStream_t s, const float* div, float* in2, double* partials, float* scal, Dims cd, int f, const StencilBC& p_bc);
+// coarse grid cd (fine = f * cd), and the per-block sums of squares of the means in
+// partials (restrict_partials(cd) of them; unet::launch_pack_input_sigma turns them into
+// sigma = RMS of the coarse divergence). partials: kPartialBlocks doubles.
+hipError_t launch_restrict_input(hipStream_t s, const float* div, float* in2, double* partials, Dims cd, int f, const StencilBC& p_bc);
+unsigned restrict_partials(Dims cd);
 // p0[fine] = scale * sigma * trilinear(q)[fine] (cell-centred, clamped at the edges),
 // scale = (f h)^2, sigma = scal[0]. The coarse grid is at most 2048 cells wide.
 hipError_t launch_prolong(hipStream_t s, const float* q, const float* scal, float scale, float* p0, Dims cd, int f);

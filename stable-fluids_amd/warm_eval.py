@@ -71,7 +71,7 @@ def steps_per_second(scene, state, options, params=None, steps=20, warmup=3):
 
 
 def evaluate(n=256, plain_passes=80, k_list=(2, 4, 6, 8, 10, 12, 13, 14, 15, 16, 17, 18, 19, 20, 24, 32, 40, 56, 80), at_steps=(10, 20, 30),
-             coarsening=3, params=None, time_steps=20):
+             coarsening=3, params=None, time_steps=60):
     """Equal-quality comparison on the plume at n^3 (centre (n/2, 3n/32, n/2), radius n/16,
     zero initial velocity, D = 20). Returns a dict (None when no model is available)."""
     params = load_model() if params is None else params
