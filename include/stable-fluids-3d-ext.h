@@ -88,7 +88,8 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_set_flow_velocity_cuda(vo
  * 1.15 / 0.85 / 1.65, rates 0.71 / 0.53. The masks are device arrays of nx*ny*nz floats,
  * copied by the call. NULL `animation` turns the animation off. Synchronous: it waits for
  * the context stream and rebuilds the step graph. Contexts with an animation cannot join a
- * batch. */
+ * batch, and a context that is in a batch must not be given one (the batch's own step graph
+ * would not run it). */
 typedef struct StableFluidsExtForceAnimation {
     float phase_rate;
     float drift_x_amplitude;
