@@ -174,12 +174,13 @@ def main():
         oracle = entry.load_oracle()
         sim = oracle.OracleSim(abi, scene.desc, precision=64)
         scenes.seed_oracle(sim, scene)
+        cpu_steps = 2
         t = time.perf_counter()
-        sim.step(1)
+        sim.step(cpu_steps)
         cpu_s = time.perf_counter() - t
         sim.destroy()
-        cpu = {"value": round(1.0 / cpu_s, 5), "unit": "steps/s", "cores": oracle.threads(abi), "kind": "port",
-               "sample": f"1 step of the same {n}^3 workload (FP64 host loops, OpenMP), {cpu_s:.1f} s"}
+        cpu = {"value": round(cpu_steps / cpu_s, 5), "unit": "steps/s", "cores": oracle.threads(abi), "kind": "port",
+               "sample": f"{cpu_steps} steps of the same {n}^3 workload (FP64 host loops, OpenMP), {cpu_s:.1f} s"}
 
     out = {
         "metric": METRIC,
