@@ -4,7 +4,8 @@
   the pinned FP64 residual (oracle/warm_oracle.py), on mixed faces and on all walls
   (singular system, zero-mean solution);
 * fallback (guess residual > 10x the zero-guess residual) gives the plain step bit for
-  bit: P passes from zero;
+  bit: P passes from zero (through the one-launch fallback solve with an even and an odd
+  pass count, in rounds of blocks on a wide slab, and without it: K = P, Jacobi);
 * an all-zero network (guess 0, residual equal to the zero guess, accepted) gives the
   plain step with K passes bit for bit;
 * an accepted guess equals the FP64 restatement of restrict -> normalise -> network ->
@@ -112,6 +113,25 @@ def test_fallback_is_the_plain_step_bitwise(k):
     assert stats["fallback"]
     plain = run_gpu(scene.desc, lambda ctx: scenes.seed_context(ctx, scene), 4)
     assert_same(got, plain, "fallback vs plain P")
+
+
+@pytest.mark.parametrize("smoother, k", [("rbgs", 12), ("jacobi", 4), ("jacobi", 5)])
+def test_fallback_without_the_one_launch_solve(smoother, k):
+    """K = P (no fallback-only passes: the test zeroes the guess itself) and the Jacobi
+    smoother (skipped single launches): a fallback still gives the plain step bit for bit."""
+    scene = plume(p_iters=12)
+    sm = abi.SMOOTHER_JACOBI if smoother == "jacobi" else abi.SMOOTHER_RBGS
+    wild = uo.join_params(uo.init_params(3, scale=300.0)).astype(np.float32)
+    opts = sf.ext_options(smoother=sm, pressure_warm_start=abi.WARM_START_LEARNED, warm_start_iterations=k, warm_start_coarsening=1)
+    with sf.Context(scene.desc, options=opts) as ctx:
+        ctx.set_pressure_model(wild)
+        scenes.seed_context(ctx, scene)
+        ctx.step(3)
+        stats = ctx.warm_start_stats()
+        got = gpu_fields(ctx, scene.desc)
+    assert stats["fallback"]
+    plain = run_gpu(scene.desc, lambda ctx: scenes.seed_context(ctx, scene), 3, options=sf.ext_options(smoother=sm))
+    assert_same(got, plain, f"{smoother} fallback, K={k}")
 
 
 def test_fallback_in_rounds_is_the_plain_step_bitwise():
