@@ -78,7 +78,7 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_set_flow_velocity_cuda(vo
  * recomputes its swirl and drift forcing on the host and uploads force x and z before every
  * step (vulkan-app/scene_plume.cpp:323-335, masks built at :241-289); this replaces that
  * per-step upload. Before the force of step n is added (n counts from start_step, one per
- * step):
+ * step; the context has one counter, which every call sets to its start_step):
  *   phase = n * phase_rate                      (n converted to float)
  *   ax = drift_x_amplitude * sin(phase),  az = drift_z_amplitude * cos(phase * drift_z_rate)
  *   s  = swirl_amplitude * cos(phase * swirl_rate)
