@@ -14,7 +14,9 @@ and the host-loop baseline (FP64 oracle on this host's cores, a bounded sample).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid N] [--smoother rbgs|jacobi]
                   [--storage f32|f16|bf16|f32-refined] [--interval M]
-                  [--no-cpu-baseline] [--stage-repeat R]
+                  [--no-cpu-baseline] [--no-learned] [--stage-repeat R]
+  python bench.py --learned-step K [--steps S] [--warmup W]   (the learned step alone, for traces)
+  python bench.py --batch [--tuning-cache FILE]                (64 uneven simulations, below)
 
 Precision study (--precision): the plume at 512^3 (centre (256,48,256), radius 32,
 D=20, P=40, zero initial velocity), 50 steps with FP32, FP16 and BF16 iterate storage
