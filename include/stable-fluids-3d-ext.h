@@ -121,6 +121,20 @@ typedef struct StableFluidsExtNorms {
  * accumulation) and waits for the result. */
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_field_norms_cuda(void* context, StableFluidsExtNorms* out_norms);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * reduction of stable_fluids_ext_field_norms_cuda over caller-given device fields of
+ * `cells` floats each (for example a context's views), bit-identical to it on the same
+ * fields. It allocates nothing and does not synchronise: the partial sums live in a
+ * static device array of the library (32,800 bytes, one per GPU, held until the process
+ * ends), and the result is written to out_norms_device (device memory) on `stream`
+ * (NULL: the null stream), which must belong to the current device. Every cell count
+ * fits the array (at most 1024 partial sums); NULL pointers, cells = 0 and a stream of
+ * another device return STABLE_FLUIDS_RESULT_BACKEND_FAILURE. All calls on one GPU share
+ * its array: calls that may overlap on one GPU (unordered streams, threads) race and give
+ * wrong results; order them on one stream. */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_field_norms_noalloc_cuda(const float* divergence, const float* velocity_x, const float* velocity_y, const float* velocity_z, const float* pressure, uint64_t cells, void* stream, StableFluidsExtNorms* out_norms_device);
+
 typedef enum StableFluidsExtStage {
     STABLE_FLUIDS_EXT_STAGE_PRESSURE_PASS  = 0, /* one smoother pass of the pressure solve */
     STABLE_FLUIDS_EXT_STAGE_VELOCITY_PASS  = 1, /* one smoother pass of one velocity component's diffusion */

@@ -20,6 +20,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -74,6 +75,14 @@ struct VectorField {
 };
 
 constexpr int kNormBlocks = 1024;
+// partial sums of the norm reduction: 4 per workgroup, then the 4 totals
+constexpr int kNormScratchDoubles = 4 * kNormBlocks + 4;
+
+// Scratch of stable_fluids_ext_field_norms_noalloc_cuda (32,800 bytes): the same layout
+// as a context's norm_partials. Every GPU the library runs on holds its own copy until
+// the process ends. In host code this name is only a stand-in for the symbol: the device
+// address comes from hipGetSymbolAddress (norm_scratch below), once per device.
+static __device__ __align__(256) double scratch[kNormScratchDoubles];
 
 struct Context {
     StableFluidsSimulationConfig config{};
@@ -632,6 +641,35 @@ __global__ __launch_bounds__(256) void k_norm_final(double* __restrict__ partial
     block_reduce<4>(acc, true, partials + 4 * kNormBlocks);
 }
 
+// totals {sum div^2, sum |v|^2, sum p^2, max |div|} -> StableFluidsExtNorms in device memory
+__global__ __launch_bounds__(64) void k_norm_export(const double* __restrict__ totals, StableFluidsExtNorms* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    out->divergence_l2  = sqrt(totals[0]);
+    out->divergence_max = totals[3];
+    out->velocity_l2    = sqrt(totals[1]);
+    out->pressure_l2    = sqrt(totals[2]);
+}
+
+// Device address of the current GPU's norm scratch, looked up on the first call on that
+// device (with it current) and kept per device.
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+double* scratch_ptr[kMaxDevices] = {};
+
+hipError_t norm_scratch(const int device, double** out) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (scratch_ptr[device] == nullptr) {
+        void* p            = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (p == nullptr) return hipErrorInvalidSymbol;
+        scratch_ptr[device] = static_cast<double*>(p);
+    }
+    *out = scratch_ptr[device];
+    return hipSuccess;
+}
+
 
 // ---- batched simulations: one launch per stage for many contexts ----
 //
@@ -1143,6 +1181,34 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_field_norms_cuda(void* co
     out->pressure_l2    = sqrt(host[2]);
     out->divergence_max = host[3];
     return STABLE_FLUIDS_RESULT_OK;
+}
+
+// The reduction of stable_fluids_ext_field_norms_cuda (same kernels, same workgroup count,
+// so the same bits) over caller-given fields, with the partial sums in the static scratch
+// array instead of a context buffer, and the result written to device memory on the
+// caller's stream: no allocation, no synchronisation.
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_field_norms_noalloc_cuda(const float* divergence, const float* velocity_x, const float* velocity_y, const float* velocity_z, const float* pressure, uint64_t cells, void* stream, StableFluidsExtNorms* out_norms_device) {
+    sf::Range r("stable.ext.field_norms_noalloc");
+    if (divergence == nullptr || velocity_x == nullptr || velocity_y == nullptr || velocity_z == nullptr || pressure == nullptr || out_norms_device == nullptr || cells == 0) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    int device = -1;
+    if (hipGetDevice(&device) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    const auto s = static_cast<hipStream_t>(stream);
+    if (s != nullptr) {
+        hipDevice_t stream_device = -1;
+        if (hipStreamGetDevice(s, &stream_device) != hipSuccess || stream_device != device) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    }
+    // the grid-stride reduction caps its workgroups at kNormBlocks, so every cell count
+    // fits the scratch; a partial count beyond it is refused all the same
+    const int nb = static_cast<int>(std::min<uint64_t>(sf::kNormBlocks, (cells + 255) / 256));
+    if (4 * nb + 4 > sf::kNormScratchDoubles) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    double* partials = nullptr;
+    if (sf::norm_scratch(device, &partials) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    // every partial slot the final kernel reads is written by this call first: nothing
+    // from an earlier call is used
+    sf::k_norm_partials<<<nb, 256, 0, s>>>(divergence, velocity_x, velocity_y, velocity_z, pressure, cells, partials);
+    sf::k_norm_final<<<1, 256, 0, s>>>(partials, nb);
+    sf::k_norm_export<<<1, 64, 0, s>>>(partials + 4 * sf::kNormBlocks, out_norms_device);
+    return hipGetLastError() == hipSuccess ? STABLE_FLUIDS_RESULT_OK : STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
 }
 
 STABLE_FLUIDS_API uint64_t stable_fluids_ext_stage_bytes_cuda(void* context, uint32_t stage) {

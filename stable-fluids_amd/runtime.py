@@ -31,7 +31,7 @@ EXT_ENTRY_POINTS = (
     "stable_fluids_ext_batch_create_cuda", "stable_fluids_ext_batch_destroy_cuda", "stable_fluids_ext_batch_step_cuda",
     "stable_fluids_ext_batch_get_stream_cuda", "stable_fluids_ext_batch_set_z_chunk_cuda", "stable_fluids_ext_batch_info_cuda",
     "stable_fluids_ext_batch_autotune_cuda", "stable_fluids_ext_set_force_animation_cuda",
-    "stable_fluids_ext_force_animation_step_cuda",
+    "stable_fluids_ext_force_animation_step_cuda", "stable_fluids_ext_field_norms_noalloc_cuda",
 )
 UNET_ENTRY_POINTS = (
     "stable_fluids_unet_param_count", "stable_fluids_unet_create_cuda", "stable_fluids_unet_destroy_cuda",
@@ -74,6 +74,7 @@ def load_library(path=LIB_PATH):
     lib.stable_fluids_ext_force_animation_step_cuda.argtypes = [vp, C.POINTER(u32)]
     lib.stable_fluids_ext_set_flow_pressure_cuda.argtypes = [vp, vp]
     lib.stable_fluids_ext_field_norms_cuda.argtypes = [vp, C.POINTER(ExtNorms)]
+    lib.stable_fluids_ext_field_norms_noalloc_cuda.argtypes = [vp, vp, vp, vp, vp, C.c_uint64, vp, vp]
     lib.stable_fluids_ext_run_stage_cuda.argtypes = [vp, u32, u32]
     lib.stable_fluids_ext_stage_bytes_cuda.argtypes = [vp, u32]
     lib.stable_fluids_ext_stage_bytes_cuda.restype = C.c_uint64
