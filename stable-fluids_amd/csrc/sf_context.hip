@@ -125,6 +125,7 @@ struct Context {
     struct {
         double *x = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *partials = nullptr, *scal = nullptr;
     } cg;
+    float* half[2] = {nullptr, nullptr}; // padded black-cell iterates of the RBGS passes (sf_rbgs_half.hip)
     std::vector<void*> allocations;
 
     ~Context() {
@@ -263,11 +264,40 @@ hipError_t enqueue_solve_mixed(Context& c, SolveKind solve, const float* b, cons
 // Whether this context's FP32 smoother passes run two per launch under these rules.
 bool use_pairs(const Context& c, const StencilBC& bc) { return c.options.pairs && !c.options.refine && smooth2_supported(c.options.smoother, c.dims, bc); }
 
+// Whether a plain FP32 RBGS solve under these rules runs on the black-cell passes.
+bool use_half(const Context& c, const StencilBC& bc) {
+    return c.half[0] != nullptr && !c.options.refine && c.options.smoother == kSmoothRBGS && !use_pairs(c, bc) && rbgs_half_supported(c.dims, bc);
+}
+
+// K RBGS passes from x0 (nullptr = zero guess) into `out` that carry only the black cells
+// (plus the ghosts of the red face cells) between passes: pass l writes half[l & 1], the
+// first reads the natural x0, the last writes `out` scaled by last_scale.
+hipError_t enqueue_solve_half(Context& c, SolveKind solve, const float* b, const float* x0, float* out, int passes, float coef, float last_scale, const StencilBC& bc) {
+    HalfPassArgs a{};
+    a.b    = b;
+    a.coef = coef;
+    a.d    = c.dims;
+    a.bc   = bc;
+    a.g    = half_plan(c.dims, 2, 0);
+    a.x_nat = x0;
+    a.x_out = out;
+    for (int l = 0; l < passes; ++l) {
+        const bool first = l == 0, last = l == passes - 1;
+        const int in     = first ? (x0 != nullptr ? kHalfInNatural : kHalfInZero) : kHalfInHalf;
+        a.h_in           = c.half[(l + 1) & 1];
+        a.h_out          = c.half[l & 1];
+        a.out_scale      = last ? last_scale : 1.0f;
+        SF_TRY(launch_rbgs_half(c.stream, solve, in, last ? kHalfOutNatural : kHalfOutHalf, a, 2));
+    }
+    return hipSuccess;
+}
+
 // K smoother passes from x0 (nullptr = zero guess) into `out`, ping-ponging with `aux`:
 // launch l of L writes out when L - 1 - l is even, so the last one lands in out. With
 // pass pairs, an odd K starts with one single pass.
 hipError_t enqueue_solve(Context& c, SolveKind solve, const float* b, const float* x0, float* out, float* aux, int passes, float coef, float last_scale, const StencilBC& bc, int slot_base) {
     if (c.options.refine) return enqueue_solve_mixed(c, solve, b, x0, out, aux, passes, coef, last_scale, bc, slot_base);
+    if (use_half(c, bc)) return enqueue_solve_half(c, solve, b, x0, out, passes, coef, last_scale, bc);
     const int pairs    = use_pairs(c, bc) ? passes / 2 : 0;
     const int singles  = passes - 2 * pairs;
     const int launches = singles + pairs;
@@ -539,6 +569,16 @@ StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const Stabl
             for (float** p : {&v.x, &v.y, &v.z})
                 if (!c->alloc(*p, c->bytes)) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
         if (!c->alloc(c->norm_partials, sizeof(double) * (kNormBlocks * 4 + 4))) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
+        // black-cell iterates, when some solve of a plain RBGS context can use them
+        bool half = !c->options.refine && c->options.smoother == kSmoothRBGS && !c->options.pairs;
+        if (half) {
+            bool any = rbgs_half_supported(c->dims, pressure_rules(cfg.flow_boundary)) || rbgs_half_supported(c->dims, velocity_rules(cfg.flow_boundary, 0));
+            for (const auto& sc : c->scalars) any = any || rbgs_half_supported(c->dims, scalar_rules(sc.desc.boundary));
+            half = any;
+        }
+        if (half)
+            for (float*& h : c->half)
+                if (!c->alloc(h, sizeof(float) * half_buffer_floats(c->dims))) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
         if (c->options.refine) {
             auto& m          = c->mixed;
             const size_t eb  = c->cells * (c->options.storage == kStoreF32 ? 4 : 2);
@@ -555,6 +595,8 @@ StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const Stabl
         auto& f = c->flow;
         for (float* p : {f.vx, f.vy, f.vz, f.tx, f.ty, f.tz, f.ax, f.ay, f.az, f.p, f.p_aux, f.div, f.mag, f.rhs})
             if (launch_fill(c->stream, p, 0.0f, c->cells) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        for (float* h : c->half)
+            if (h != nullptr && launch_fill(c->stream, h, 0.0f, half_buffer_floats(c->dims)) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         if (c->options.learned && launch_fill(c->stream, reinterpret_cast<float*>(c->warm.stats), 0.0f, 8) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         if (c->options.learned && launch_fill(c->stream, reinterpret_cast<float*>(c->warm.skip), 0.0f, 4) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         for (auto& s : c->scalars) {

@@ -80,6 +80,37 @@ bool smooth_batchable(SmootherKind smoother, Dims d, const StencilBC& bc);
 // Every item's descriptor must have even nx when `pairs`, odd nx otherwise.
 hipError_t launch_smooth_batched(hipStream_t s, SmootherKind smoother, SolveKind solve, bool zero, bool pairs, const SmoothDesc* descs, const BatchItem* items, uint32_t n_items);
 
+// ---- RBGS passes that carry only the black cells between passes (sf_rbgs_half.hip) ----
+// A solve of L passes: the first reads the natural start iterate (or zeros), the middle
+// ones read and write padded half buffers (the black cell of every x pair plus the ghost
+// values of the red face cells), the last writes the natural result scaled by out_scale.
+enum HalfIn : int { kHalfInZero = 0, kHalfInNatural = 1, kHalfInHalf = 2 };
+enum HalfOut : int { kHalfOutHalf = 0, kHalfOutNatural = 1 };
+struct HalfLayout {
+    int nxh; // pairs per row
+    int ps;  // floats per padded half plane
+};
+struct HalfPassArgs {
+    const float* x_nat; // natural start iterate (kHalfInNatural)
+    const float* b;
+    const float* h_in;  // padded half buffer (kHalfInHalf)
+    float* h_out;       // padded half buffer (kHalfOutHalf)
+    float* x_out;       // natural result (kHalfOutNatural)
+    float coef;
+    float out_scale;
+    Dims d;
+    StencilBC bc;
+    TileGeom g;         // half_plan
+    HalfLayout L;       // set by the launcher
+};
+// even nx >= 4, ny >= 2, nz >= 2, even extents on periodic axes, fields below 2 GiB
+bool rbgs_half_supported(Dims d, const StencilBC& bc);
+HalfLayout half_layout(Dims d);
+uint64_t half_buffer_floats(Dims d);
+// tiles of 64 pairs x 8*rows_per_wave rows (rows_per_wave 2 or 4)
+TileGeom half_plan(Dims d, int rows_per_wave, int zc_override);
+hipError_t launch_rbgs_half(hipStream_t s, SolveKind solve, int in_mode, int out_mode, const HalfPassArgs& args, int rows_per_wave);
+
 struct AdvectVelocityDesc {
     float* t[3];
     const float* v[3];

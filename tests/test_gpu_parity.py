@@ -144,6 +144,19 @@ def walls_desc(n=(12, 10, 8), d_iters=7, p_iters=9):
     return abi.Desc(nx, ny, nz, 1.0 / 16, 0.05, 2e-3, d_iters, p_iters, flow, scalars=scalars, vectors=vectors)
 
 
+def periodic_yz_desc(n=(12, 10, 8), d_iters=7, p_iters=9):
+    """Periodic y and z, a no-slip x- wall with a moving lid and an outflow x+ face."""
+    nx, ny, nz = n
+    wall_v = [(0.0, 0.15, -0.1), (0, 0, 0), (0, 0, 0), (0, 0, 0), (0, 0, 0), (0, 0, 0)]
+    flow = abi.flow_boundary([abi.FLOW_NO_SLIP, abi.FLOW_OUTFLOW, abi.FLOW_PERIODIC, abi.FLOW_PERIODIC,
+                              abi.FLOW_PERIODIC, abi.FLOW_PERIODIC], velocity=wall_v, pressure=[0, 0.3, 0, 0, 0, 0])
+    s1 = abi.scalar_boundary([abi.SCALAR_FIXED, abi.SCALAR_ZERO_FLUX, abi.SCALAR_PERIODIC, abi.SCALAR_PERIODIC,
+                              abi.SCALAR_PERIODIC, abi.SCALAR_PERIODIC], values=[0.75, 0, 0, 0, 0, 0])
+    scalars = [abi.scalar_desc(1e-3, 0.0, 0.25, s1)]
+    vectors = [abi.force_desc(0.1, -0.05, 0.2)]
+    return abi.Desc(nx, ny, nz, 1.0 / 16, 0.05, 2e-3, d_iters, p_iters, flow, scalars=scalars, vectors=vectors)
+
+
 # smoother tile shapes: 128 x 16 cells per tile, z chunks of the default plan
 TILE_CASES = {
     "periodic_one_full_tile": (mixed_desc, (128, 6, 3)),     # halo columns wrap
@@ -152,6 +165,11 @@ TILE_CASES = {
     "walls_partial_tile": (walls_desc, (130, 3, 4)),         # pair at nx carries the x+ ghost
     "walls_odd_nx": (walls_desc, (9, 7, 6)),
     "walls_slab_nz1": (walls_desc, (16, 12, 1)),             # one-plane chunks
+    # black-cell passes: a 2-pair x tile and a 2-row y tile past full ones, 10 z chunks
+    "walls_multi_tile_chunks": (walls_desc, (132, 34, 40)),
+    "periodic_yz_partial": (periodic_yz_desc, (136, 18, 20)),  # wrapped halo rows and planes
+    "mixed_two_cells": (mixed_desc, (4, 2, 2)),                # every cell on a face
+    "walls_two_cells": (walls_desc, (4, 2, 2)),
 }
 
 
