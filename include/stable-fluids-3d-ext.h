@@ -136,14 +136,19 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_field_norms_cuda(void* co
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_field_norms_noalloc_cuda(const float* divergence, const float* velocity_x, const float* velocity_y, const float* velocity_z, const float* pressure, uint64_t cells, void* stream, StableFluidsExtNorms* out_norms_device);
 
 typedef enum StableFluidsExtStage {
-    STABLE_FLUIDS_EXT_STAGE_PRESSURE_PASS  = 0, /* one smoother pass of the pressure solve */
-    STABLE_FLUIDS_EXT_STAGE_VELOCITY_PASS  = 1, /* one smoother pass of one velocity component's diffusion */
+    /* one smoother pass of the pressure solve as the step runs it: for a plain FP32 RBGS
+     * solve a middle pass that reads b and the black cells and writes the black cells
+     * (8 B/cell); otherwise the full-iterate pass */
+    STABLE_FLUIDS_EXT_STAGE_PRESSURE_PASS  = 0,
+    STABLE_FLUIDS_EXT_STAGE_VELOCITY_PASS  = 1, /* the same for one velocity component's diffusion */
     STABLE_FLUIDS_EXT_STAGE_ADVECT         = 2, /* velocity self-advection */
     STABLE_FLUIDS_EXT_STAGE_PROJECT        = 3, /* projection + divergence + |v| */
     STABLE_FLUIDS_EXT_STAGE_JACOBI_PASS    = 4, /* one Jacobi pass of the pressure solve, whatever the smoother option */
     STABLE_FLUIDS_EXT_STAGE_COPY           = 5, /* plain copy of one field (8 B/cell), to calibrate byte counters */
     STABLE_FLUIDS_EXT_STAGE_PRESSURE_PAIR  = 6, /* two RBGS passes of the pressure solve in one launch */
     STABLE_FLUIDS_EXT_STAGE_VELOCITY_PAIR  = 7, /* two RBGS passes of one velocity component's diffusion in one launch */
+    STABLE_FLUIDS_EXT_STAGE_FULL_PRESSURE_PASS = 8, /* one full-iterate pass (read x_old, b; write x_new: 12 B/cell) of the pressure solve */
+    STABLE_FLUIDS_EXT_STAGE_FULL_VELOCITY_PASS = 9, /* the same for one velocity component's diffusion */
 } StableFluidsExtStage;
 
 /* Enqueues `repeat` launches of one stage kernel on the context stream, outside the step
@@ -154,6 +159,22 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_run_stage_cuda(void* cont
 /* Algorithmic HBM bytes one launch of `stage` moves (each array read once and written
  * once), as used for roofline figures. */
 STABLE_FLUIDS_API uint64_t stable_fluids_ext_stage_bytes_cuda(void* context, uint32_t stage);
+
+/* Tile shapes of the RBGS passes that carry only black cells between passes (plain FP32
+ * RBGS solves on even nx): 64 x-pairs (128 cells) by 16 rows with 4 or 8 waves per
+ * workgroup, or by 32 rows with 8 waves. */
+typedef enum StableFluidsExtTile {
+    STABLE_FLUIDS_EXT_TILE_64X16_8W = 0,
+    STABLE_FLUIDS_EXT_TILE_64X16_4W = 1, /* default */
+    STABLE_FLUIDS_EXT_TILE_64X32_8W = 2,
+} StableFluidsExtTile;
+
+/* Sets the tile shape and z-chunk length (planes per workgroup, 0 = the default plan) of
+ * the context's RBGS passes and rebuilds the step graph (synchronous). Tiling never
+ * changes results. */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_set_smoother_tiling_cuda(void* context, uint32_t tile, uint32_t z_chunk);
+/* The tile shape and the z-chunk length in use (either pointer may be NULL). */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_get_smoother_tiling_cuda(void* context, uint32_t* out_tile, uint32_t* out_z_chunk);
 
 /* Returns the context's HIP stream (hipStream_t as void*). */
 STABLE_FLUIDS_API void* stable_fluids_ext_get_stream_cuda(void* context);

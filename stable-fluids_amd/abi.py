@@ -58,6 +58,11 @@ STAGE_JACOBI_PASS = 4
 STAGE_COPY = 5
 STAGE_PRESSURE_PAIR = 6
 STAGE_VELOCITY_PAIR = 7
+STAGE_FULL_PRESSURE_PASS = 8
+STAGE_FULL_VELOCITY_PASS = 9
+TILE_64X16_8W = 0
+TILE_64X16_4W = 1
+TILE_64X32_8W = 2
 
 
 class FlowFace(C.Structure):

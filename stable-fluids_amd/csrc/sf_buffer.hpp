@@ -34,6 +34,18 @@ __device__ __forceinline__ Rsrc plane_rsrc(const void* base, const uint32_t byte
     return __builtin_amdgcn_make_buffer_rsrc(q, 0, static_cast<int>(n), kBufferConfig);
 }
 
+// A wave-uniform pointer kept in scalar registers (see plane_rsrc), and a resource at it
+// of `bytes` bytes: build the pointer once outside a loop, the resource per use.
+__device__ __forceinline__ const void* uniform_ptr(const void* base) {
+    const uint64_t p  = reinterpret_cast<uint64_t>(base);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(p));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(p >> 32));
+    return reinterpret_cast<const void*>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+__device__ __forceinline__ Rsrc rsrc_at(const void* uniform, const uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(uniform), 0, static_cast<int>(__builtin_amdgcn_readfirstlane(bytes)), kBufferConfig);
+}
+
 // Aligned pair of elements at byte offset `off` (+ the wave-uniform `soff`).
 template <class St>
 __device__ __forceinline__ float2 buf_load2(const Rsrc r, const int off, const int soff = 0) {

@@ -126,6 +126,8 @@ struct Context {
         double *x = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *partials = nullptr, *scal = nullptr;
     } cg;
     float* half[2] = {nullptr, nullptr}; // padded black-cell iterates of the RBGS passes (sf_rbgs_half.hip)
+    int half_tile  = kHalfTile8x2;       // their tile shape and z-chunk length (0: default plan)
+    int half_zc    = 0;
     std::vector<void*> allocations;
 
     ~Context() {
@@ -278,7 +280,7 @@ hipError_t enqueue_solve_half(Context& c, SolveKind solve, const float* b, const
     a.coef = coef;
     a.d    = c.dims;
     a.bc   = bc;
-    a.g    = half_plan(c.dims, 2, 0);
+    a.g    = half_plan(c.dims, c.half_tile, c.half_zc);
     a.x_nat = x0;
     a.x_out = out;
     for (int l = 0; l < passes; ++l) {
@@ -287,7 +289,7 @@ hipError_t enqueue_solve_half(Context& c, SolveKind solve, const float* b, const
         a.h_in           = c.half[(l + 1) & 1];
         a.h_out          = c.half[l & 1];
         a.out_scale      = last ? last_scale : 1.0f;
-        SF_TRY(launch_rbgs_half(c.stream, solve, in, last ? kHalfOutNatural : kHalfOutHalf, a, 2));
+        SF_TRY(launch_rbgs_half(c.stream, solve, in, last ? kHalfOutNatural : kHalfOutHalf, a, c.half_tile));
     }
     return hipSuccess;
 }
@@ -474,6 +476,15 @@ StableFluidsResult build_graph(Context& c) {
     c.graph = graph;
     if (hipGraphInstantiate(&c.graph_exec, c.graph, nullptr, nullptr, 0) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     return STABLE_FLUIDS_RESULT_OK;
+}
+
+// Re-captures the step graph (after a change of what it launches).
+StableFluidsResult rebuild_graph(Context& c) {
+    if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
+    if (c.graph) (void)hipGraphDestroy(c.graph);
+    c.graph_exec = nullptr;
+    c.graph      = nullptr;
+    return build_graph(c);
 }
 
 StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const StableFluidsExtOptions* opt, void** out_context, StableFluidsScalarFieldHandle* out_s, uint32_t cap_s, StableFluidsVectorFieldHandle* out_v, uint32_t cap_v) {
@@ -1119,11 +1130,7 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_set_force_animation_cuda(
         v.animated = false;
     }
     // the step graph is captured with the animation kernels in it (or without them)
-    if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
-    if (c.graph) (void)hipGraphDestroy(c.graph);
-    c.graph_exec = nullptr;
-    c.graph      = nullptr;
-    return sf::build_graph(c);
+    return sf::rebuild_graph(c);
 }
 
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_force_animation_step_cuda(void* context, uint32_t* out_step) {
@@ -1258,7 +1265,14 @@ STABLE_FLUIDS_API uint64_t stable_fluids_ext_stage_bytes_cuda(void* context, uin
     const auto& c = *static_cast<Context*>(context);
     switch (stage) {
     case STABLE_FLUIDS_EXT_STAGE_PRESSURE_PASS:
-    case STABLE_FLUIDS_EXT_STAGE_VELOCITY_PASS:
+    case STABLE_FLUIDS_EXT_STAGE_VELOCITY_PASS: {
+        const auto& fb = c.config.flow_boundary;
+        const sf::StencilBC bc = stage == STABLE_FLUIDS_EXT_STAGE_PRESSURE_PASS ? sf::pressure_rules(fb) : sf::velocity_rules(fb, 0);
+        if (sf::use_half(c, bc)) return 8ull * c.cells; // read b and the black cells, write the black cells
+        return 12ull * c.cells;
+    }
+    case STABLE_FLUIDS_EXT_STAGE_FULL_PRESSURE_PASS:
+    case STABLE_FLUIDS_EXT_STAGE_FULL_VELOCITY_PASS:
     case STABLE_FLUIDS_EXT_STAGE_JACOBI_PASS:
     case STABLE_FLUIDS_EXT_STAGE_PRESSURE_PAIR:
     case STABLE_FLUIDS_EXT_STAGE_VELOCITY_PAIR: return 12ull * c.cells; // read x_old, b; write x_new (a pair: two passes)
@@ -1464,15 +1478,36 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_run_stage_cuda(void* cont
     for (uint32_t i = 0; i < repeat; ++i) {
         hipError_t e = hipSuccess;
         switch (stage) {
-        case STABLE_FLUIDS_EXT_STAGE_PRESSURE_PASS: {
-            const bool odd = i & 1;
-            e = sf::launch_smooth(c.stream, c.options.smoother, sf::kSolvePoisson, sf::SmoothPass{odd ? f.p : f.p_aux, f.div, odd ? f.p_aux : f.p, h * h, 1.0f}, c.dims, sf::pressure_rules(fb));
-            break;
-        }
+        case STABLE_FLUIDS_EXT_STAGE_PRESSURE_PASS:
         case STABLE_FLUIDS_EXT_STAGE_VELOCITY_PASS: {
-            const bool odd    = i & 1;
-            const float alpha = c.config.dt * c.config.viscosity / (h * h);
-            e = sf::launch_smooth(c.stream, c.options.smoother, sf::kSolveDiffusion, sf::SmoothPass{odd ? f.ax : f.vx, f.tx, odd ? f.vx : f.ax, alpha, 1.0f}, c.dims, sf::velocity_rules(fb, 0));
+            // the pass the step runs: a middle black-cell pass where the solve uses them
+            const bool pressure = stage == STABLE_FLUIDS_EXT_STAGE_PRESSURE_PASS;
+            const sf::StencilBC bc = pressure ? sf::pressure_rules(fb) : sf::velocity_rules(fb, 0);
+            const float coef       = pressure ? h * h : c.config.dt * c.config.viscosity / (h * h);
+            if (sf::use_half(c, bc)) {
+                sf::HalfPassArgs a{};
+                a.b     = pressure ? f.rhs : f.tx;
+                a.coef  = coef;
+                a.out_scale = 1.0f;
+                a.d     = c.dims;
+                a.bc    = bc;
+                a.g     = sf::half_plan(c.dims, c.half_tile, c.half_zc);
+                a.h_in  = c.half[i & 1];
+                a.h_out = c.half[(i + 1) & 1];
+                e       = sf::launch_rbgs_half(c.stream, pressure ? sf::kSolvePoisson : sf::kSolveDiffusion, sf::kHalfInHalf, sf::kHalfOutHalf, a, c.half_tile);
+                break;
+            }
+            [[fallthrough]];
+        }
+        case STABLE_FLUIDS_EXT_STAGE_FULL_PRESSURE_PASS:
+        case STABLE_FLUIDS_EXT_STAGE_FULL_VELOCITY_PASS: {
+            const bool odd = i & 1;
+            if (stage == STABLE_FLUIDS_EXT_STAGE_PRESSURE_PASS || stage == STABLE_FLUIDS_EXT_STAGE_FULL_PRESSURE_PASS) {
+                e = sf::launch_smooth(c.stream, c.options.smoother, sf::kSolvePoisson, sf::SmoothPass{odd ? f.p : f.p_aux, f.div, odd ? f.p_aux : f.p, h * h, 1.0f}, c.dims, sf::pressure_rules(fb));
+            } else {
+                const float alpha = c.config.dt * c.config.viscosity / (h * h);
+                e = sf::launch_smooth(c.stream, c.options.smoother, sf::kSolveDiffusion, sf::SmoothPass{odd ? f.ax : f.vx, f.tx, odd ? f.vx : f.ax, alpha, 1.0f}, c.dims, sf::velocity_rules(fb, 0));
+            }
             break;
         }
         case STABLE_FLUIDS_EXT_STAGE_JACOBI_PASS: {
@@ -1500,6 +1535,24 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_run_stage_cuda(void* cont
         }
         if (e != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     }
+    return STABLE_FLUIDS_RESULT_OK;
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_set_smoother_tiling_cuda(void* context, uint32_t tile, uint32_t z_chunk) {
+    sf::Range r("stable.ext.set_smoother_tiling");
+    if (context == nullptr || tile > STABLE_FLUIDS_EXT_TILE_64X32_8W || z_chunk > 4096) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& c = *static_cast<Context*>(context);
+    if (hipStreamSynchronize(c.stream) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    c.half_tile = static_cast<int>(tile);
+    c.half_zc   = static_cast<int>(z_chunk);
+    return sf::rebuild_graph(c);
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_get_smoother_tiling_cuda(void* context, uint32_t* out_tile, uint32_t* out_z_chunk) {
+    if (context == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    const auto& c = *static_cast<Context*>(context);
+    if (out_tile) *out_tile = static_cast<uint32_t>(c.half_tile);
+    if (out_z_chunk) *out_z_chunk = static_cast<uint32_t>(sf::half_plan(c.dims, c.half_tile, c.half_zc).zc);
     return STABLE_FLUIDS_RESULT_OK;
 }
 

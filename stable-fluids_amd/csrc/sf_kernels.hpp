@@ -107,9 +107,12 @@ struct HalfPassArgs {
 bool rbgs_half_supported(Dims d, const StencilBC& bc);
 HalfLayout half_layout(Dims d);
 uint64_t half_buffer_floats(Dims d);
-// tiles of 64 pairs x 8*rows_per_wave rows (rows_per_wave 2 or 4)
-TileGeom half_plan(Dims d, int rows_per_wave, int zc_override);
-hipError_t launch_rbgs_half(hipStream_t s, SolveKind solve, int in_mode, int out_mode, const HalfPassArgs& args, int rows_per_wave);
+// tile shapes: 64 pairs x 16 rows by 8 waves of 2 rows or 4 waves of 4 rows; 64 x 32 by 8 waves of 4
+enum HalfTileShape : int { kHalfTile8x2 = 0, kHalfTile4x4 = 1, kHalfTile8x4 = 2 };
+int half_tile_rows(int tile);
+int half_tile_threads(int tile);
+TileGeom half_plan(Dims d, int tile, int zc_override);
+hipError_t launch_rbgs_half(hipStream_t s, SolveKind solve, int in_mode, int out_mode, const HalfPassArgs& args, int tile);
 
 struct AdvectVelocityDesc {
     float* t[3];

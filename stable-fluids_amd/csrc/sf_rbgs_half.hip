@@ -42,6 +42,7 @@
 // instruction sits under a branch.
 #include <algorithm>
 #include <type_traits>
+#include <utility>
 
 #include "sf_buffer.hpp"
 #include "sf_kernels.hpp"
@@ -51,13 +52,16 @@ namespace sf {
 
 namespace {
 
-constexpr int kHThreads = 512;
-constexpr int kHWaves   = kHThreads / 64;
-constexpr int kHP       = 64; // pairs per tile row (128 cells)
+constexpr int kHP = 64; // pairs per tile row (128 cells)
+#ifndef SF_HALF_DEPTH
+#define SF_HALF_DEPTH 2
+#endif
+constexpr int kHalfDepth = SF_HALF_DEPTH; // planes of loads in flight
 
-template <int kRowsW>
+template <int kWaves, int kRowsW>
 struct HalfTile {
-    static constexpr int TY     = kHWaves * kRowsW; // tile rows
+    static constexpr int kThreads = 64 * kWaves;
+    static constexpr int TY     = kWaves * kRowsW;  // tile rows
     static constexpr int AW     = kHP + 4;          // A plane: pairs -2 .. 65
     static constexpr int AR     = TY + 4;           //          rows -2 .. TY + 1
     static constexpr int AF     = AW * AR;
@@ -67,7 +71,8 @@ struct HalfTile {
     static constexpr int kAHalo = 4 * AW + 4 * TY;  // A halo positions (one extra slot per lane)
     static constexpr int kRHalo = 2 * RW + 2 * TY;  // R halo positions
     static constexpr int kRHaloWaves = (kRHalo + 63) / 64;
-    static_assert(kAHalo <= kHThreads && kRHalo <= kHThreads && 128 + 2 * TY <= kHThreads, "one extra slot per lane");
+    static constexpr int kAPer  = (kAHalo + kThreads - 1) / kThreads; // A halo slots per lane
+    static_assert(kRHalo <= kThreads && 128 + 2 * TY <= kThreads, "one R halo and one ghost-store slot per lane");
 };
 
 // Position classes of a (pair, row) of the neighbourhood.
@@ -113,16 +118,17 @@ __device__ __forceinline__ HalfPos classify(const int i, const int y, const Dims
 template <class T>
 __device__ __forceinline__ T pick(const bool c, const T a, const T b) { return c ? a : b; }
 
-template <int kRowsW, int kIn>
+template <int kRowsW, int kAPer, int kIn>
 struct PfSet {
     using T = std::conditional_t<kIn == kHalfInNatural, float2, float>;
     T own[kRowsW];
-    T ex;
+    T ex[kAPer];
 };
 
-template <int kSolve, int kIn, int kOut, int kRowsW>
-__global__ __launch_bounds__(kHThreads) __attribute__((amdgpu_waves_per_eu(2, 4))) void k_rbgs_half(const HalfPassArgs args) {
-    using TT           = HalfTile<kRowsW>;
+template <int kSolve, int kIn, int kOut, int kWaves, int kRowsW, int kDepth>
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 4))) void k_rbgs_half(const HalfPassArgs args) {
+    using TT           = HalfTile<kWaves, kRowsW>;
+    constexpr int kAPer = TT::kAPer;
     constexpr int TY   = TT::TY;
     constexpr int AW   = TT::AW;
     constexpr int RW   = TT::RW;
@@ -163,7 +169,6 @@ __global__ __launch_bounds__(kHThreads) __attribute__((amdgpu_waves_per_eu(2, 4)
     const uint32_t nfield_b = nplane_b * static_cast<uint32_t>(nz);
     const uint32_t hplane_b = 4u * static_cast<uint32_t>(L.ps);
     const uint32_t hfield_b = hplane_b * static_cast<uint32_t>(nz + 2);
-    const int kb            = z0 - 2; // ring slots count planes from here
 
     // ---- own slots: rows wave + 8 j of the tile, pair i0 + lane ----
     int o_src[kRowsW], o_b[kRowsW], o_st[kRowsW], o_kind[kRowsW];
@@ -172,7 +177,7 @@ __global__ __launch_bounds__(kHThreads) __attribute__((amdgpu_waves_per_eu(2, 4)
     bool tile_ghost = false; // some own slot is not a real cell (partial tile)
 #pragma unroll
     for (int j = 0; j < kRowsW; ++j) {
-        const int y       = y0 + wave + kHWaves * j;
+        const int y       = y0 + wave + kWaves * j;
         const int i       = i0 + lane;
         const HalfPos P   = classify(i, y, d, L, px, py);
         o_kind[j]         = P.kind;
@@ -188,31 +193,43 @@ __global__ __launch_bounds__(kHThreads) __attribute__((amdgpu_waves_per_eu(2, 4)
     const int ypar = (y0 + wave) & 1; // row parity of every own slot (8 j is even)
 
     // ---- extra A slot: the halo of the A planes ----
-    int ex_src = kOutOfRange, ex_lds = 0, ex_y = 0;
-    bool ex_xflip = false, ex_on = false;
-    LinGhost ex_rule = lin_identity();
-    if (tid < TT::kAHalo) {
-        int r, l;
-        if (tid < 4 * AW) {
-            const int q = tid / AW;
-            r           = q < 2 ? q - 2 : TY + q - 2;
-            l           = tid % AW - 2;
-        } else {
-            const int e2 = tid - 4 * AW, c = e2 & 3;
-            r            = e2 >> 2;
-            l            = c < 2 ? c - 2 : kHP + c - 2;
+    int ex_src[kAPer], ex_lds[kAPer], ex_y[kAPer];
+    bool ex_xflip[kAPer], ex_on[kAPer];
+    LinGhost ex_rule[kAPer];
+#pragma unroll
+    for (int q = 0; q < kAPer; ++q) {
+        const int e = tid + q * TT::kThreads;
+        ex_src[q]   = kOutOfRange;
+        ex_lds[q] = ex_y[q] = 0;
+        ex_xflip[q] = ex_on[q] = false;
+        ex_rule[q]  = lin_identity();
+        if (e < TT::kAHalo) {
+            int r, l;
+            if (e < 4 * AW) {
+                const int rr = e / AW;
+                r            = rr < 2 ? rr - 2 : TY + rr - 2;
+                l            = e % AW - 2;
+            } else {
+                const int e2 = e - 4 * AW, c = e2 & 3;
+                r            = e2 >> 2;
+                l            = c < 2 ? c - 2 : kHP + c - 2;
+            }
+            const HalfPos P = classify(i0 + l, y0 + r, d, L, px, py);
+            ex_src[q]       = kIn == kHalfInNatural ? 4 * P.off_n : 4 * P.off_h;
+            ex_lds[q]       = (r + 2) * AW + l + 2;
+            ex_y[q]         = y0 + r;
+            ex_xflip[q]     = P.kind == kPosXm || P.kind == kPosXp;
+            ex_rule[q]      = kind_rule(P.kind);
+            ex_on[q]        = true;
         }
-        const HalfPos P = classify(i0 + l, y0 + r, d, L, px, py);
-        ex_src          = kIn == kHalfInNatural ? 4 * P.off_n : 4 * P.off_h;
-        ex_lds          = (r + 2) * AW + l + 2;
-        ex_y            = y0 + r;
-        ex_xflip        = P.kind == kPosXm || P.kind == kPosXp;
-        ex_rule         = kind_rule(P.kind);
-        ex_on           = true;
     }
 
     // ---- R halo slot (waves 0 .. kRHaloWaves - 1): the halo of the R planes ----
-    int h_b = kOutOfRange, h_alds = 0, h_rlds = 0, h_y = 0, h_kind = kPosNone;
+    // A ghost position takes the face rule of the adjacent cell's old value; the adjacent
+    // cell is black where it matters (x-: p = 1, x+: p = 0), so its A entry is at a fixed
+    // offset h_goff from the position (x- +1, x+ -1, y- +AW, y+ -AW).
+    int h_b = kOutOfRange, h_alds = 0, h_rlds = 0, h_y = 0, h_goff = 0;
+    bool h_real = false;
     LinGhost h_rule = lin_identity();
     if (tid < TT::kRHalo) {
         int r, l;
@@ -225,12 +242,13 @@ __global__ __launch_bounds__(kHThreads) __attribute__((amdgpu_waves_per_eu(2, 4)
             l            = (e2 & 1) ? kHP : -1;
         }
         const HalfPos P = classify(i0 + l, y0 + r, d, L, px, py);
-        h_kind          = P.kind;
-        h_b             = P.kind == kPosReal ? 4 * P.off_n : kOutOfRange;
+        h_real          = P.kind == kPosReal;
+        h_b             = h_real ? 4 * P.off_n : kOutOfRange;
         h_alds          = (r + 2) * AW + l + 2;
         h_rlds          = (r + 1) * RW + l + 1;
         h_y             = y0 + r;
         h_rule          = kind_rule(P.kind);
+        h_goff          = P.kind == kPosXm ? 1 : (P.kind == kPosXp ? -1 : (P.kind == kPosYm ? AW : (P.kind == kPosYp ? -AW : 0)));
     }
 
     // ---- ghost-store slot (half output): red face cells of the tile -> padded entries ----
@@ -257,55 +275,63 @@ __global__ __launch_bounds__(kHThreads) __attribute__((amdgpu_waves_per_eu(2, 4)
             }
         }
     }
+    const bool g_any = __builtin_amdgcn_readfirstlane(__any(g_off != kOutOfRange)) != 0; // wave-uniform
     const LinGhost zr_lo = rule(4), zr_hi = rule(5);
     const DiffusionDiv ddv = diffusion_div(coef); // loop-invariant reciprocal of 1 + 6 coef
 
-    auto slot_a = [&](const int k) { return ((k - kb) & 3) * AF; };
-    auto slot_r = [&](const int k) { return ((k - kb) & 1) * RF; };
     auto zghost = [&](const int k) { return !pz && (k < 0 || k >= nz); };
+    // periodic plane of k in [-2, nz + 1] (the planes a chunk reads; nz >= 2)
+    auto wrap_near_z = [&](const int k) { return k < 0 ? k + nz : (k >= nz ? k - nz : k); };
+    // buffer resources: uniform bases once, the size selected per plane (0 bytes: off)
+    const void* p_hin  = uniform_ptr(args.h_in);
+    const void* p_xnat = uniform_ptr(args.x_nat);
+    const void* p_b    = uniform_ptr(args.b);
+    const void* p_hout = uniform_ptr(args.h_out);
+    const void* p_xout = uniform_ptr(args.x_out);
 
-    // ---- loads ----
-    using Pf = PfSet<kRowsW, kIn>;
-    auto load_a = [&](const int k, Pf& pf) __attribute__((always_inline)) {
+    // ---- loads (kLive: the plane is known to be needed and real) ----
+    using Pf = PfSet<kRowsW, kAPer, kIn>;
+    auto load_a = [&]<bool kLive>(const int k, Pf& pf) __attribute__((always_inline)) {
         if constexpr (kIn == kHalfInHalf) {
-            const int kk    = pz ? wrap_index(k, nz) : k;
-            const bool live = k >= z0 - 2 && k <= z1 + 1 && kk >= -1 && kk <= nz;
-            const Rsrc r    = plane_rsrc(args.h_in, live ? hfield_b : 0u);
+            const int kk    = pz ? wrap_near_z(k) : k;
+            const bool live = kLive || (k >= z0 - 2 && k <= z1 + 1 && kk >= -1 && kk <= nz);
+            const Rsrc r    = rsrc_at(p_hin, live ? hfield_b : 0u);
             const int so    = live ? (kk + 1) * static_cast<int>(hplane_b) : 0;
 #pragma unroll
             for (int j = 0; j < kRowsW; ++j) pf.own[j] = buf_load1<StoreF32>(r, o_src[j], so);
-            pf.ex = buf_load1<StoreF32>(r, ex_src, so);
+#pragma unroll
+            for (int q = 0; q < kAPer; ++q) pf.ex[q] = buf_load1<StoreF32>(r, ex_src[q], so);
         } else if constexpr (kIn == kHalfInNatural) {
-            const int kc    = pz ? wrap_index(k, nz) : min(max(k, 0), nz - 1);
-            const bool live = k >= z0 - 2 && k <= z1 + 1 && (pz || (k >= -1 && k <= nz));
-            const Rsrc r    = plane_rsrc(args.x_nat, live ? nfield_b : 0u);
+            const int kc    = pz ? wrap_near_z(k) : min(max(k, 0), nz - 1);
+            const bool live = kLive || (k >= z0 - 2 && k <= z1 + 1 && (pz || (k >= -1 && k <= nz)));
+            const Rsrc r    = rsrc_at(p_xnat, live ? nfield_b : 0u);
             const int so    = live ? kc * static_cast<int>(nplane_b) : 0;
 #pragma unroll
             for (int j = 0; j < kRowsW; ++j) pf.own[j] = buf_load2<StoreF32>(r, o_src[j], so);
-            pf.ex = buf_load2<StoreF32>(r, ex_src, so);
+#pragma unroll
+            for (int q = 0; q < kAPer; ++q) pf.ex[q] = buf_load2<StoreF32>(r, ex_src[q], so);
         }
     };
     struct PfB {
         float2 own[kRowsW];
         float2 h;
     };
-    auto load_b = [&](const int k, PfB& pf) __attribute__((always_inline)) {
-        const int kc    = pz ? wrap_index(k, nz) : min(max(k, 0), nz - 1);
-        const bool live = k >= z0 - 1 && k <= z1 && (pz || (k >= 0 && k < nz));
-        const Rsrc r    = plane_rsrc(args.b, live ? nfield_b : 0u);
+    auto load_b = [&]<bool kLive>(const int k, PfB& pf) __attribute__((always_inline)) {
+        const int kc    = pz ? wrap_near_z(k) : (kLive ? k : min(max(k, 0), nz - 1));
+        const bool live = kLive || (k >= z0 - 1 && k <= z1 && (pz || (k >= 0 && k < nz)));
+        const Rsrc r    = rsrc_at(p_b, live ? nfield_b : 0u);
         const int so    = live ? kc * static_cast<int>(nplane_b) : 0;
 #pragma unroll
         for (int j = 0; j < kRowsW; ++j) pf.own[j] = buf_load2<StoreF32>(r, o_b[j], so);
         pf.h = buf_load2<StoreF32>(r, h_b, so);
     };
 
-    // value of an A position at plane k from its raw load
-    auto a_value = [&](const auto raw, const int k, const int y, const bool xflip, const LinGhost rule) __attribute__((always_inline)) -> float {
+    // value of an A position at plane k from its raw load (kReal: k is not a z ghost plane)
+    auto a_value = [&]<bool kReal>(const auto raw, const int k, const int y, const bool xflip, const LinGhost rl) __attribute__((always_inline)) -> float {
         if constexpr (kIn == kHalfInHalf) {
             return raw; // ghost entries hold the rule already applied
         } else {
-            const bool zg     = zghost(k);
-            const LinGhost gr = pick(zg, pick(k < 0, zr_lo, zr_hi), rule);
+            const LinGhost gr = kReal ? rl : pick(zghost(k), pick(k < 0, zr_lo, zr_hi), rl);
             float v           = 0.0f;
             if constexpr (kIn == kHalfInNatural) {
                 // black element of the pair by the position's parity; on y / z ghosts this is
@@ -328,44 +354,45 @@ __global__ __launch_bounds__(kHThreads) __attribute__((amdgpu_waves_per_eu(2, 4)
         bb[j] = bb_new[j] = 0.0f;
     }
 
-    // stage A(k) from `pf` into its ring slot; own values into `own`
-    auto stage_a = [&](const int k, const Pf& pf, float (&own)[kRowsW]) __attribute__((always_inline)) {
-        float* A = s_a + slot_a(k);
+    // stage A(k) from `pf` into the ring plane at `A`; own values into `own`
+    auto stage_a = [&]<bool kReal>(const int k, float* A, const Pf& pf, float (&own)[kRowsW]) __attribute__((always_inline)) {
 #pragma unroll
         for (int j = 0; j < kRowsW; ++j) {
             float v;
-            if constexpr (kIn == kHalfInZero) v = a_value(0.0f, k, 0, o_xflip[j], o_rule[j]);
-            else v = a_value(pf.own[j], k, y0 + wave + kHWaves * j, o_xflip[j], o_rule[j]);
+            if constexpr (kIn == kHalfInZero) v = a_value.template operator()<kReal>(0.0f, k, 0, o_xflip[j], o_rule[j]);
+            else v = a_value.template operator()<kReal>(pf.own[j], k, y0 + wave + kWaves * j, o_xflip[j], o_rule[j]);
             own[j] = v;
-            A[(wave + kHWaves * j + 2) * AW + lane + 2] = v;
+            A[(wave + kWaves * j + 2) * AW + lane + 2] = v;
         }
-        if (ex_on) {
-            float v;
-            if constexpr (kIn == kHalfInZero) v = a_value(0.0f, k, 0, ex_xflip, ex_rule);
-            else v = a_value(pf.ex, k, ex_y, ex_xflip, ex_rule);
-            A[ex_lds] = v;
+#pragma unroll
+        for (int q = 0; q < kAPer; ++q) {
+            if (ex_on[q]) {
+                float v;
+                if constexpr (kIn == kHalfInZero) v = a_value.template operator()<kReal>(0.0f, k, 0, ex_xflip[q], ex_rule[q]);
+                else v = a_value.template operator()<kReal>(pf.ex[q], k, ex_y[q], ex_xflip[q], ex_rule[q]);
+                A[ex_lds[q]] = v;
+            }
         }
     };
 
-    // red half-update of plane s (own slots and the R halo) into R(s)
-    auto red = [&](const int s, const PfB& pb) __attribute__((always_inline)) {
-        const float* A0 = s_a + slot_a(s);
-        const float* Am = s_a + slot_a(s - 1);
-        const float* Ap = s_a + slot_a(s + 1);
-        float* R0       = s_r + slot_r(s);
-        const bool zg   = zghost(s);
-        const int p     = (ypar + s) & 1;
-        float q[kRowsW];
+    // red half-update of plane s (own slots and the R halo) from the A planes s-1, s, s+1
+    // into R0 (kReal: s is not a z ghost plane)
+    auto red = [&]<bool kReal>(const int s, const float* Am, const float* A0, const float* Ap, float* R0, const PfB& pb) __attribute__((always_inline)) {
+        const bool zg = !kReal && zghost(s);
+        const int p   = (ypar + s) & 1;
+        float q[kRowsW], qh = 0.0f;
         if (zg) {
             // ghost plane: the z-face rule of the adjacent (black) old value
+            const LinGhost zr = pick(s < 0, zr_lo, zr_hi);
 #pragma unroll
-            for (int j = 0; j < kRowsW; ++j) q[j] = lin_apply(pick(s < 0, zr_lo, zr_hi), pick(s < 0, a_next[j], a_prev[j]));
+            for (int j = 0; j < kRowsW; ++j) q[j] = lin_apply(zr, pick(s < 0, a_next[j], a_prev[j]));
+            if (wave < TT::kRHaloWaves) qh = lin_apply(zr, pick(s < 0, Ap[h_alds], Am[h_alds]));
         } else {
             StageDiv<kSolve> dv(ddv);
-            float x[kRowsW], xm_[kRowsW], ym_[kRowsW];
+            float x[kRowsW], xm_[kRowsW], ym_[kRowsW], xh = 0.0f;
 #pragma unroll
             for (int j = 0; j < kRowsW; ++j) {
-                const int c       = (wave + kHWaves * j + 2) * AW + lane + 2;
+                const int c       = (wave + kWaves * j + 2) * AW + lane + 2;
                 const float other = A0[c - 1 + 2 * p];
                 const float xm    = p ? a_cur[j] : other;
                 const float xp    = p ? other : a_cur[j];
@@ -387,12 +414,29 @@ __global__ __launch_bounds__(kHThreads) __attribute__((amdgpu_waves_per_eu(2, 4)
                 for (int j = 0; j < kRowsW; ++j)
                     if (o_kind[j] != kPosReal) x[j] = 0.0f;
             }
+            float hg = 0.0f;
+            if (wave < TT::kRHaloWaves) {
+                const int c    = h_alds;
+                const int ph   = (h_y + s) & 1;
+                float sum      = A0[c - 1 + ph];
+                sum            = sum + A0[c + ph];
+                sum            = sum + A0[c - AW];
+                sum            = sum + A0[c + AW];
+                sum            = sum + Am[c];
+                sum            = sum + Ap[c];
+                const float xv = dv.num(ph ? pb.h.y : pb.h.x, sum, coef);
+                xh             = h_real ? xv : 0.0f;
+                hg             = lin_apply(h_rule, A0[c + h_goff]);
+            }
 #pragma unroll
             for (int j = 0; j < kRowsW; ++j) q[j] = dv.quot(x[j]);
-            if (dv.bad()) {
+            qh = dv.quot(xh);
+            if (__builtin_expect(dv.bad(), 0)) {
 #pragma unroll
                 for (int j = 0; j < kRowsW; ++j) q[j] = dv.exact(x[j]);
+                qh = dv.exact(xh);
             }
+            qh = h_real ? qh : hg;
             if (tile_ghost) {
                 // past the x+ face the ghost is the rule of the adjacent cell (black when
                 // it matters: p = 0), past the y+ face the rule of the cell below
@@ -406,55 +450,22 @@ __global__ __launch_bounds__(kHThreads) __attribute__((amdgpu_waves_per_eu(2, 4)
 #pragma unroll
         for (int j = 0; j < kRowsW; ++j) {
             r_0[j]                                       = q[j];
-            R0[(wave + kHWaves * j + 1) * RW + lane + 1] = q[j];
+            R0[(wave + kWaves * j + 1) * RW + lane + 1] = q[j];
         }
-        // the R halo
-        if (wave < TT::kRHaloWaves) {
-            if (tid < TT::kRHalo) {
-                const int c    = h_alds;
-                const int ph   = (h_y + s) & 1;
-                const float xm = A0[c - 1 + ph];
-                const float xp = A0[c + ph];
-                const float ym = A0[c - AW];
-                const float yp = A0[c + AW];
-                const float zm = Am[c];
-                const float zp = Ap[c];
-                float v;
-                if (zg) {
-                    v = lin_apply(pick(s < 0, zr_lo, zr_hi), pick(s < 0, zp, zm));
-                } else {
-                    float sum = xm;
-                    sum       = sum + xp;
-                    sum       = sum + ym;
-                    sum       = sum + yp;
-                    sum       = sum + zm;
-                    sum       = sum + zp;
-                    StageDiv<kSolve> dv(ddv);
-                    float x = dv.num(ph ? pb.h.y : pb.h.x, sum, coef);
-                    if (h_kind != kPosReal) x = 0.0f;
-                    v = dv.quot(x);
-                    if (dv.bad()) v = dv.exact(x);
-                    if (h_kind != kPosReal) {
-                        const float src = h_kind == kPosXm ? xp : (h_kind == kPosXp ? xm : (h_kind == kPosYm ? yp : ym));
-                        v               = h_kind == kPosNone ? 0.0f : lin_apply(h_rule, src);
-                    }
-                }
-                R0[h_rlds] = v;
-            }
-        }
+        if (wave < TT::kRHaloWaves && tid < TT::kRHalo) R0[h_rlds] = qh;
     };
 
-    // black half-update of plane m = s - 1 and the stores of plane m
-    auto black = [&](const int s) __attribute__((always_inline)) {
+    // black half-update of plane m = s - 1 from R1 = R(m) and the stores of plane m
+    // (kLive: m is a plane of the chunk and not a z face)
+    auto black = [&]<bool kLive>(const int s, const float* R1) __attribute__((always_inline)) {
         const int m     = s - 1;
-        const float* R1 = s_r + slot_r(m);
-        const bool live = m >= z0 && m < z1; // planes of the chunk (never a ghost plane)
+        const bool live = kLive || (m >= z0 && m < z1); // planes of the chunk (never a ghost plane)
         const int p     = (ypar + m) & 1;
         StageDiv<kSolve> dv(ddv);
         float x[kRowsW], q[kRowsW];
 #pragma unroll
         for (int j = 0; j < kRowsW; ++j) {
-            const int c       = (wave + kHWaves * j + 1) * RW + lane + 1;
+            const int c       = (wave + kWaves * j + 1) * RW + lane + 1;
             const float other = R1[c + 1 - 2 * p];
             const float xm    = p ? other : r_m[j];
             const float xp    = p ? r_m[j] : other;
@@ -473,27 +484,31 @@ __global__ __launch_bounds__(kHThreads) __attribute__((amdgpu_waves_per_eu(2, 4)
         }
 #pragma unroll
         for (int j = 0; j < kRowsW; ++j) q[j] = dv.quot(x[j]);
-        if (dv.bad()) {
+        if (__builtin_expect(dv.bad(), 0)) {
 #pragma unroll
             for (int j = 0; j < kRowsW; ++j) q[j] = dv.exact(x[j]);
         }
         if constexpr (kOut == kHalfOutHalf) {
-            const Rsrc r = plane_rsrc(args.h_out, live ? hfield_b : 0u);
+            const Rsrc r = rsrc_at(p_hout, live ? hfield_b : 0u);
             const int so = live ? (m + 1) * static_cast<int>(hplane_b) : 0;
 #pragma unroll
             for (int j = 0; j < kRowsW; ++j) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(q[j]), r, o_st[j], so, 0);
             // red face cells of plane m -> ghost entries: rows -1 / ny and the x columns
-            // (one slot per lane), the z planes -1 / nz (own slots)
-            const float g = lin_apply(g_rule, R1[g_rlds]);
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(g), r, g_off, so, 0);
-            const bool zlive = live && !pz && (m == 0 || m == nz - 1);
-            const Rsrc rz    = plane_rsrc(args.h_out, zlive ? hfield_b : 0u);
-            const int zso    = zlive ? (m == 0 ? 0 : (nz + 1) * static_cast<int>(hplane_b)) : 0;
-            const LinGhost zr = pick(m == 0, zr_lo, zr_hi);
+            // (one slot per lane, waves that hold face cells), the z planes -1 / nz (own
+            // slots). Only stores sit under these branches: a skipped store leaves every
+            // later wait count valid.
+            if (g_any && live) {
+                const float g = lin_apply(g_rule, R1[g_rlds]);
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(g), r, g_off, so, 0);
+            }
+            if (!kLive && live && !pz && (m == 0 || m == nz - 1)) {
+                const int zso     = m == 0 ? 0 : (nz + 1) * static_cast<int>(hplane_b);
+                const LinGhost zr = pick(m == 0, zr_lo, zr_hi);
 #pragma unroll
-            for (int j = 0; j < kRowsW; ++j) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(lin_apply(zr, r_m[j])), rz, o_st[j], zso, 0);
+                for (int j = 0; j < kRowsW; ++j) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(lin_apply(zr, r_m[j])), r, o_st[j], zso, 0);
+            }
         } else {
-            const Rsrc r  = plane_rsrc(args.x_out, live ? nfield_b : 0u);
+            const Rsrc r  = rsrc_at(p_xout, live ? nfield_b : 0u);
             const int so  = live ? m * static_cast<int>(nplane_b) : 0;
             const float f = args.out_scale;
 #pragma unroll
@@ -506,16 +521,26 @@ __global__ __launch_bounds__(kHThreads) __attribute__((amdgpu_waves_per_eu(2, 4)
         }
     };
 
-    // one plane step s: stage A(s+1) (fetched two steps ago into pa) and refetch pa with
-    // A(s+3); barrier; red of plane s with b(s) (fetched two steps ago into pb), refetch pb
-    // with b(s+2); black of plane s - 1
-    auto step = [&](const int s, Pf& pa, PfB& pb) __attribute__((always_inline)) {
-        stage_a(s + 1, pa, a_next);
-        load_a(s + 3, pa);
+    // One plane step s, t = s - (z0 - 1): stage A(s+1) (fetched kDepth steps ago into pa)
+    // and refetch pa with A(s+1+kDepth); barrier; red of plane s with b(s) (fetched kDepth
+    // steps ago into pb), refetch pb with b(s+kDepth); black of plane s - 1. Ring slots:
+    // A(s+1) (t+2)&3, A(s) (t+1)&3, A(s-1) t&3, R(s) (t+1)&1, R(s-1) t&1. kSteady: s in
+    // [z0 + 2, z1 - 3], where every load reads a plane the chunk needs (at most plane z1
+    // of A, z1 - 1 of b, so padded plane nz at most), no plane is a z ghost and the black
+    // plane s - 1 is neither z face; the slots are then compile-time (t mod 4 = kT).
+    auto step = [&]<bool kSteady, int kT>(const int s, const int t, Pf& pa, PfB& pb) __attribute__((always_inline)) {
+        const int tt  = kSteady ? kT : t;
+        float* A_next = s_a + ((tt + 2) & 3) * AF;
+        float* A_cur  = s_a + ((tt + 1) & 3) * AF;
+        float* A_prev = s_a + (tt & 3) * AF;
+        float* R_cur  = s_r + ((tt + 1) & 1) * RF;
+        float* R_prev = s_r + (tt & 1) * RF;
+        stage_a.template operator()<kSteady>(s + 1, A_next, pa, a_next);
+        load_a.template operator()<kSteady>(s + 1 + kDepth, pa);
         __syncthreads();
-        red(s, pb);
-        load_b(s + 2, pb);
-        black(s);
+        red.template operator()<kSteady>(s, A_prev, A_cur, A_next, R_cur, pb);
+        load_b.template operator()<kSteady>(s + kDepth, pb);
+        black.template operator()<kSteady>(s, R_prev);
 #pragma unroll
         for (int j = 0; j < kRowsW; ++j) {
             a_prev[j] = a_cur[j];
@@ -525,27 +550,42 @@ __global__ __launch_bounds__(kHThreads) __attribute__((amdgpu_waves_per_eu(2, 4)
             bb[j]     = bb_new[j];
         }
     };
+    static_assert(kDepth == 2, "register sets alternate with the step parity");
 
-    Pf pa0{}, pa1{};
-    PfB pb0{}, pb1{};
-    // prologue: A(z0-2), A(z0-1) staged; A(z0), A(z0+1) and b(z0-1), b(z0) in flight
-    load_a(z0 - 2, pa0);
-    load_a(z0 - 1, pa1);
-    stage_a(z0 - 2, pa0, a_cur);
-    stage_a(z0 - 1, pa1, a_next);
+    Pf pa[2];
+    PfB pb[2];
+    // prologue: A(z0-2), A(z0-1) staged into slots 0, 1; A(z0), A(z0+1) and b(z0-1), b(z0) in flight
+    load_a.template operator()<false>(z0 - 2, pa[0]);
+    load_a.template operator()<false>(z0 - 1, pa[1]);
+    stage_a.template operator()<false>(z0 - 2, s_a, pa[0], a_cur);
+    stage_a.template operator()<false>(z0 - 1, s_a + AF, pa[1], a_next);
 #pragma unroll
     for (int j = 0; j < kRowsW; ++j) {
         a_prev[j] = a_cur[j];
         a_cur[j]  = a_next[j];
     }
-    load_a(z0, pa0);
-    load_a(z0 + 1, pa1);
-    load_b(z0 - 1, pb0);
-    load_b(z0, pb1);
-    // steps z0 - 1 .. z1; an odd count ends with one idle step (loads and stores off)
-    for (int s = z0 - 1; s <= z1; s += 2) {
-        step(s, pa0, pb0);
-        step(s + 1, pa1, pb1);
+    load_a.template operator()<false>(z0, pa[0]);
+    load_a.template operator()<false>(z0 + 1, pa[1]);
+    load_b.template operator()<false>(z0 - 1, pb[0]);
+    load_b.template operator()<false>(z0, pb[1]);
+    // steps t = 0 .. n - 1 (s = z0 - 1 .. z1), register set t & 1: general steps t = 0, 1, 2;
+    // steady steps in groups of four, t = 3 .. 2 + 4 g, s = z0 + 2 .. z0 + 1 + 4 g <= z1 - 3;
+    // general steps for the rest (an odd count ends with one idle step, loads and stores off)
+    const int n      = z1 - z0 + 2;
+    const int groups = n >= 10 ? (n - 6) / 4 : 0;
+    step.template operator()<false, 0>(z0 - 1, 0, pa[0], pb[0]);
+    step.template operator()<false, 0>(z0, 1, pa[1], pb[1]);
+    if (n > 2) step.template operator()<false, 0>(z0 + 1, 2, pa[0], pb[0]);
+    for (int g = 0; g < groups; ++g) {
+        const int s = z0 + 2 + 4 * g; // t = 3 + 4 g
+        step.template operator()<true, 3>(s, 0, pa[1], pb[1]);
+        step.template operator()<true, 0>(s + 1, 0, pa[0], pb[0]);
+        step.template operator()<true, 1>(s + 2, 0, pa[1], pb[1]);
+        step.template operator()<true, 2>(s + 3, 0, pa[0], pb[0]);
+    }
+    for (int t = 3 + 4 * groups; t < n; t += 2) {
+        step.template operator()<false, 0>(z0 - 1 + t, t, pa[1], pb[1]);
+        step.template operator()<false, 0>(z0 + t, t + 1, pa[0], pb[0]);
     }
 }
 
@@ -575,15 +615,16 @@ uint64_t half_buffer_floats(const Dims d) {
     return static_cast<uint64_t>(L.ps) * static_cast<uint64_t>(d.nz + 2);
 }
 
-TileGeom half_plan(const Dims d, const int rows_per_wave, const int zc_override) {
+TileGeom half_plan(const Dims d, const int tile, const int zc_override) {
     TileGeom g{};
     const int nxh    = d.nx / 2;
-    const int ty     = kHWaves * rows_per_wave;
+    const int ty     = half_tile_rows(tile);
     g.tiles_x        = (nxh + kHP - 1) / kHP;
     g.tiles_y        = (d.ny + ty - 1) / ty;
     const long tiles = static_cast<long>(g.tiles_x) * g.tiles_y;
-    // about 512 workgroups (two per CU), even chunk lengths of at least 4 planes
-    long zc = (static_cast<long>(d.nz) * tiles + 511) / 512;
+    // about 512 workgroups of 512 threads (1024 of 256), even chunk lengths of at least 4 planes
+    const long target = half_tile_threads(tile) == 256 ? 1024 : 512;
+    long zc = (static_cast<long>(d.nz) * tiles + target - 1) / target;
     zc      = std::max<long>(4, std::min<long>(64, zc + (zc & 1)));
     if (zc_override > 0) zc = zc_override;
     if (zc > d.nz) zc = d.nz;
@@ -592,32 +633,36 @@ TileGeom half_plan(const Dims d, const int rows_per_wave, const int zc_override)
     return g;
 }
 
-hipError_t launch_rbgs_half(hipStream_t s, const SolveKind solve, const int in_mode, const int out_mode, const HalfPassArgs& a_in, const int rows_per_wave) {
+int half_tile_rows(const int tile) { return tile == kHalfTile8x4 ? 32 : 16; }
+int half_tile_threads(const int tile) { return tile == kHalfTile4x4 ? 256 : 512; }
+
+hipError_t launch_rbgs_half(hipStream_t s, const SolveKind solve, const int in_mode, const int out_mode, const HalfPassArgs& a_in, const int tile) {
     HalfPassArgs a        = a_in;
     a.L                   = half_layout(a.d);
     const unsigned blocks = smooth_blocks(a.g);
-#define SF_H(SO, IN, OUT, RW) k_rbgs_half<SO, IN, OUT, RW><<<blocks, kHThreads, 0, s>>>(a)
-#define SF_H_RW(SO, IN, OUT)                  \
-    do {                                      \
-        if (rows_per_wave == 4) SF_H(SO, IN, OUT, 4); \
-        else SF_H(SO, IN, OUT, 2);            \
+#define SF_H(SO, IN, OUT, W, RW) k_rbgs_half<SO, IN, OUT, W, RW, kHalfDepth><<<blocks, 64 * W, 0, s>>>(a)
+#define SF_H_T(SO, IN, OUT)                                    \
+    do {                                                       \
+        if (tile == kHalfTile4x4) SF_H(SO, IN, OUT, 4, 4);     \
+        else if (tile == kHalfTile8x4) SF_H(SO, IN, OUT, 8, 4); \
+        else SF_H(SO, IN, OUT, 8, 2);                          \
     } while (0)
-#define SF_H_OUT(SO, IN)                                        \
-    do {                                                        \
-        if (out_mode == kHalfOutHalf) SF_H_RW(SO, IN, kHalfOutHalf); \
-        else SF_H_RW(SO, IN, kHalfOutNatural);                  \
+#define SF_H_OUT(SO, IN)                                            \
+    do {                                                            \
+        if (out_mode == kHalfOutHalf) SF_H_T(SO, IN, kHalfOutHalf); \
+        else SF_H_T(SO, IN, kHalfOutNatural);                       \
     } while (0)
-#define SF_H_IN(SO)                                                          \
-    do {                                                                     \
-        if (in_mode == kHalfInZero) SF_H_OUT(SO, kHalfInZero);               \
-        else if (in_mode == kHalfInNatural) SF_H_OUT(SO, kHalfInNatural);    \
-        else SF_H_OUT(SO, kHalfInHalf);                                      \
+#define SF_H_IN(SO)                                                       \
+    do {                                                                  \
+        if (in_mode == kHalfInZero) SF_H_OUT(SO, kHalfInZero);            \
+        else if (in_mode == kHalfInNatural) SF_H_OUT(SO, kHalfInNatural); \
+        else SF_H_OUT(SO, kHalfInHalf);                                   \
     } while (0)
     if (solve == kSolvePoisson) SF_H_IN(kSolvePoisson);
     else SF_H_IN(kSolveDiffusion);
 #undef SF_H_IN
 #undef SF_H_OUT
-#undef SF_H_RW
+#undef SF_H_T
 #undef SF_H
     return hipGetLastError();
 }

@@ -32,6 +32,7 @@ EXT_ENTRY_POINTS = (
     "stable_fluids_ext_batch_get_stream_cuda", "stable_fluids_ext_batch_set_z_chunk_cuda", "stable_fluids_ext_batch_info_cuda",
     "stable_fluids_ext_batch_autotune_cuda", "stable_fluids_ext_set_force_animation_cuda",
     "stable_fluids_ext_force_animation_step_cuda", "stable_fluids_ext_field_norms_noalloc_cuda",
+    "stable_fluids_ext_set_smoother_tiling_cuda", "stable_fluids_ext_get_smoother_tiling_cuda",
 )
 UNET_ENTRY_POINTS = (
     "stable_fluids_unet_param_count", "stable_fluids_unet_create_cuda", "stable_fluids_unet_destroy_cuda",
@@ -75,6 +76,8 @@ def load_library(path=LIB_PATH):
     lib.stable_fluids_ext_set_flow_pressure_cuda.argtypes = [vp, vp]
     lib.stable_fluids_ext_field_norms_cuda.argtypes = [vp, C.POINTER(ExtNorms)]
     lib.stable_fluids_ext_field_norms_noalloc_cuda.argtypes = [vp, vp, vp, vp, vp, C.c_uint64, vp, vp]
+    lib.stable_fluids_ext_set_smoother_tiling_cuda.argtypes = [vp, C.c_uint32, C.c_uint32]
+    lib.stable_fluids_ext_get_smoother_tiling_cuda.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     lib.stable_fluids_ext_run_stage_cuda.argtypes = [vp, u32, u32]
     lib.stable_fluids_ext_stage_bytes_cuda.argtypes = [vp, u32]
     lib.stable_fluids_ext_stage_bytes_cuda.restype = C.c_uint64
@@ -216,6 +219,14 @@ class Context:
             raise StableFluidsError("stable_fluids_ext_field_norms_cuda", r)
         return {"divergence_l2": out.divergence_l2, "divergence_max": out.divergence_max,
                 "velocity_l2": out.velocity_l2, "pressure_l2": out.pressure_l2}
+
+    def set_smoother_tiling(self, tile, z_chunk=0):
+        return self._call("stable_fluids_ext_set_smoother_tiling_cuda", tile, z_chunk)
+
+    def smoother_tiling(self):
+        t, z = C.c_uint32(0), C.c_uint32(0)
+        self._call("stable_fluids_ext_get_smoother_tiling_cuda", C.byref(t), C.byref(z))
+        return t.value, z.value
 
     def run_stage(self, stage, repeat):
         return self._call("stable_fluids_ext_run_stage_cuda", stage, repeat)
