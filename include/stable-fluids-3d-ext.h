@@ -218,9 +218,10 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_set_pressure_model_cuda(v
 
 /* Warm-start outcome of the last step: out[0] = ||b - A p0|| (guess), out[1] = ||b||
  * (zero guess), out[2] = 1 when the step fell back to the plain passes. Synchronises the
- * context stream. Returns BACKEND_FAILURE when a fallback solve since creation could not
- * complete a grid barrier (its workgroups were not all resident on the GPU at once): the
- * pressure of that step is invalid. */
+ * context stream. The fallback passes skipped by an accepted guess run as one launch whose
+ * workgroups wait only for finished neighbour blocks, so it needs no exclusive use of the
+ * GPU; should one of its waits ever run past its bound (about 0.4 s), the step's pressure
+ * is filled with NaN and this call returns BACKEND_FAILURE. */
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_warm_start_stats_cuda(void* context, double* out3);
 
 /* Training pair of the warm-start network from the last step's pressure system: the

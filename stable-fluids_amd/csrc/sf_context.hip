@@ -119,8 +119,9 @@ struct Context {
         float* scal      = nullptr; // sigma, 1/sigma
         double* partials = nullptr;
         double* stats    = nullptr; // guess residual, zero residual, fallback
-        int* skip        = nullptr; // [0] 1: skip the fallback passes; [1] grid-barrier count; [2] barrier timeout
+        int* skip        = nullptr; // [0] 1: skip the fallback passes; [2] 1: the fallback solve of the last step failed
         uint32_t chain_grid = 0;    // workgroups of the one-launch fallback solve (0: not used)
+        unsigned* chain_ctr = nullptr; // its ticket and per-block pass counters
     } warm;
     struct {
         double *x = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *partials = nullptr, *scal = nullptr;
@@ -366,13 +367,21 @@ hipError_t enqueue_learned_pressure(Context& c, const StencilBC& p_bc) {
     SF_TRY(warm::launch_prolong(c.stream, unet::net_output(*w.net), w.scal, static_cast<float>(fac * fac) * h * h, guess, cd, fac));
     // (the chain zeroes the guess itself on fallback)
     SF_TRY(warm::launch_guess_test(c.stream, guess, f.rhs, h * h, 10.0, w.partials, w.stats, w.skip, c.dims, p_bc, !chain));
+    // the K passes after the skippable ones as black-cell passes from the guess buffer (the
+    // accepted guess, or the fallback result the chain leaves there), when nothing after
+    // the chain is skippable
+    const bool half_k = K > 0 && (chain || C == 0) && !c.options.pairs && use_half(c, p_bc);
     for (int l = 0; l < L; ++l) {
+        if (half_k && l >= skipped) {
+            SF_TRY(enqueue_solve_half(c, kSolvePoisson, f.rhs, guess, f.p, K, h * h, 1.0f, p_bc));
+            break;
+        }
         if (chain && l == 0) {
             // the chain's result lands in its first buffer for an even pass count, in its
             // second for an odd one: either way in the guess buffer
             float* other = guess == f.p ? f.p_aux : f.p;
-            if (C % 2 == 0) SF_TRY(launch_smooth_chain(c.stream, kSolvePoisson, guess, other, f.rhs, h * h, C, c.dims, p_bc, w.skip, w.chain_grid));
-            else SF_TRY(launch_smooth_chain(c.stream, kSolvePoisson, other, guess, f.rhs, h * h, C, c.dims, p_bc, w.skip, w.chain_grid));
+            if (C % 2 == 0) SF_TRY(launch_smooth_chain(c.stream, kSolvePoisson, guess, other, f.rhs, h * h, C, c.dims, p_bc, w.skip, w.chain_ctr, w.chain_grid));
+            else SF_TRY(launch_smooth_chain(c.stream, kSolvePoisson, other, guess, f.rhs, h * h, C, c.dims, p_bc, w.skip, w.chain_ctr, w.chain_grid));
             continue;
         }
         const float* src = l == 0 ? (C > 0 ? nullptr : guess) : dst(l - 1);
@@ -572,6 +581,7 @@ StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const Stabl
             if (!c->alloc(w.in2, c->bytes / 4) || !c->alloc(w.scal, 16) || !c->alloc(w.partials, sizeof(double) * 2 * warm::kPartialBlocks) || !c->alloc(w.stats, 4 * sizeof(double)) || !c->alloc(w.skip, 16)) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
             const bool chain = cfg.pressure_iterations > c->options.warm_iterations && smooth_chain_supported(c->options.smoother, c->dims, pressure_rules(cfg.flow_boundary));
             if (chain && smooth_chain_grid(c->dims, &w.chain_grid) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+            if (chain && !c->alloc(w.chain_ctr, sizeof(unsigned) * smooth_chain_counters(c->dims))) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
         }
         for (auto& s : c->scalars)
             for (float** p : {&s.data, &s.temp, &s.source, &s.aux})
@@ -610,6 +620,7 @@ StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const Stabl
             if (h != nullptr && launch_fill(c->stream, h, 0.0f, half_buffer_floats(c->dims)) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         if (c->options.learned && launch_fill(c->stream, reinterpret_cast<float*>(c->warm.stats), 0.0f, 8) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         if (c->options.learned && launch_fill(c->stream, reinterpret_cast<float*>(c->warm.skip), 0.0f, 4) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        if (c->warm.chain_ctr != nullptr && launch_fill(c->stream, reinterpret_cast<float*>(c->warm.chain_ctr), 0.0f, smooth_chain_counters(c->dims)) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         for (auto& s : c->scalars) {
             if (launch_fill(c->stream, s.data, s.desc.initial_value, c->cells) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
             if (launch_fill(c->stream, s.temp, s.desc.initial_value, c->cells) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
@@ -1300,8 +1311,8 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_warm_start_stats_cuda(voi
     auto& c = *static_cast<Context*>(context);
     if (c.warm.stats == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     if (hipStreamSynchronize(c.stream) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
-    // a grid barrier of the one-launch fallback solve that timed out (its workgroups were
-    // not all resident) left an invalid pressure: reported as a failure
+    // the one-launch fallback solve of the last step records a dependency wait that ran
+    // past its bound (its result is then NaN): reported as a failure
     int sync[3] = {0, 0, 0};
     if (hipMemcpy(sync, c.warm.skip, sizeof(sync), hipMemcpyDeviceToHost) != hipSuccess || sync[2] != 0) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     return sf::status(hipMemcpy(out3, c.warm.stats, 3 * sizeof(double), hipMemcpyDeviceToHost));

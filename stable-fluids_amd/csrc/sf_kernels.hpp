@@ -196,15 +196,18 @@ hipError_t launch_smooth(hipStream_t s, SmootherKind smoother, SolveKind solve, 
 bool smooth2_supported(SmootherKind smoother, Dims d, const StencilBC& bc);
 hipError_t launch_smooth2(hipStream_t s, SolveKind solve, const SmoothPass& pass, Dims d, const StencilBC& bc);
 // The first `passes` (> 0) passes of an RBGS solve from zero in one launch that runs only
-// when sync[0] == 0 (the learned warm start's fallback): zeros io, then ping-pongs
-// io -> other -> io ..., bit-identical to `passes` launch_smooth calls from zero, with the
-// result in io for an even count and in other for an odd one. Its `grid` workgroups (smooth_chain_grid, computed outside stream
-// capture) meet at grid barriers counting on sync[1], which an earlier kernel on the
-// stream must zero; a barrier that does not complete in bounded time sets sync[2] (the
-// result is then invalid) instead of hanging.
+// when skip[0] == 0 (the learned warm start's fallback), ping-ponging io -> other -> io
+// ..., bit-identical to `passes` launch_smooth calls from zero, with the result in io for
+// an even count and in other for an odd one. Its `grid` workgroups (smooth_chain_grid,
+// computed outside stream capture) take (pass, block) items in order and wait only for
+// the neighbouring blocks of the previous pass, so the launch needs no co-residency.
+// `counters` (smooth_chain_counters(d) words) must be zero before the first launch; each
+// launch leaves them zero. skip[2] is set (and the result filled with NaN) if a wait ever
+// exceeded its bound.
 bool smooth_chain_supported(SmootherKind smoother, Dims d, const StencilBC& bc);
 hipError_t smooth_chain_grid(Dims d, uint32_t* out_grid);
-hipError_t launch_smooth_chain(hipStream_t s, SolveKind solve, float* io, float* other, const float* b, float coef, int passes, Dims d, const StencilBC& bc, int* sync, uint32_t grid);
+uint32_t smooth_chain_counters(Dims d);
+hipError_t launch_smooth_chain(hipStream_t s, SolveKind solve, float* io, float* other, const float* b, float coef, int passes, Dims d, const StencilBC& bc, int* skip, unsigned* counters, uint32_t grid);
 hipError_t launch_correction(hipStream_t s, SmootherKind smoother, SolveKind solve, StorageKind storage, const CorrectionPass& pass, Dims d, const StencilBC& homogeneous_bc);
 // Defect of a refinement cycle: x' = x_in + s_e*e (x_in nullptr = zeros, e nullptr = no
 // correction yet, s_e from e_slot), r = rhs - A x' with the full boundary rules

@@ -136,6 +136,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
     constexpr int RF   = TT::RF;
     __shared__ float s_a[4 * AF];
     __shared__ float s_r[2 * RF];
+    __shared__ float s_sink[64]; // LDS target of halo-slot lanes with no position (one word per lane)
     // ghost rules (a, c, k) of faces 0..5 and the identity at 6, read with runtime indices
     // at set-up (indexing the kernel argument itself would copy it to scratch memory)
     __shared__ float s_rule_a[8], s_rule_c[8];
@@ -366,12 +367,10 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
         }
 #pragma unroll
         for (int q = 0; q < kAPer; ++q) {
-            if (ex_on[q]) {
-                float v;
-                if constexpr (kIn == kHalfInZero) v = a_value.template operator()<kReal>(0.0f, k, 0, ex_xflip[q], ex_rule[q]);
-                else v = a_value.template operator()<kReal>(pf.ex[q], k, ex_y[q], ex_xflip[q], ex_rule[q]);
-                A[ex_lds[q]] = v;
-            }
+            float v;
+            if constexpr (kIn == kHalfInZero) v = a_value.template operator()<kReal>(0.0f, k, 0, ex_xflip[q], ex_rule[q]);
+            else v = a_value.template operator()<kReal>(pf.ex[q], k, ex_y[q], ex_xflip[q], ex_rule[q]);
+            *(ex_on[q] ? A + ex_lds[q] : s_sink + lane) = v;
         }
     };
 
@@ -452,7 +451,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
             r_0[j]                                       = q[j];
             R0[(wave + kWaves * j + 1) * RW + lane + 1] = q[j];
         }
-        if (wave < TT::kRHaloWaves && tid < TT::kRHalo) R0[h_rlds] = qh;
+        if (wave < TT::kRHaloWaves) *(tid < TT::kRHalo ? R0 + h_rlds : s_sink + lane) = qh;
     };
 
     // black half-update of plane m = s - 1 from R1 = R(m) and the stores of plane m

@@ -1713,59 +1713,93 @@ __global__ __launch_bounds__(kThreads) void k_rbgs2_pass(const float* __restrict
     rbgs2_body<kSolve, kZero>(x_old, b, x_new, coef, out_scale, d, bc, g, xcd_block());
 }
 
-// Spins of one grid-barrier wait (s_sleep 4, about 0.1 us each): about 50 ms, far above a
-// pass, so only a grid that is not all resident runs into it.
-constexpr unsigned kBarrierSpins = 1u << 19;
+// Spins (s_sleep 4, about 0.1 us each) one waiting lane may spend over the whole launch:
+// about 0.4 s, far above a solve. The ticket order below guarantees every wait ends; the
+// budget only keeps the launch finite if that guarantee were ever broken.
+constexpr unsigned kDepSpins = 1u << 22;
 
-// Grid barrier: every wave publishes its stores at device scope (the XCDs' L2s are not
-// coherent with each other), one lane per workgroup counts in and waits until `target`
-// workgroups have; then every wave drops stale cached lines. A wait that runs past
-// kBarrierSpins sets *fault and goes on, so the kernel always ends.
-__device__ __forceinline__ void grid_barrier(unsigned* count, const unsigned target, int* fault) {
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned spins = 0;
-        while (__hip_atomic_load(count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (++spins > kBarrierSpins) {
-                __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(4);
-        }
-    }
-    __syncthreads();
-    __threadfence();
-}
-
-// `passes` RBGS passes from zero in one launch, run only when sync[0] == 0: zero `io`,
-// then io -> other -> io -> ... (the result lands in io for an even count, in other for
-// an odd one). Workgroup i runs
-// pass blocks i, i + G, ... (G = gridDim.x, each renumbered as the single pass does) and
-// the workgroups meet at a grid barrier between passes, counting on sync[1] (zeroed by an
-// earlier kernel on the stream); sync[2] flags a barrier that did not complete. Every relaxation is
-// the single pass's: a zero field gives the bits of its implicit zero iterate.
+// `passes` RBGS passes from zero in one launch, run only when skip[0] == 0 (the learned
+// warm start's fallback): io -> other -> io -> ... (the result lands in other for an odd
+// count, in io for an even one); the first pass reads the implicit zero iterate, as the
+// single pass from zero does. The work is the items (pass j, block t), handed out in
+// pass-major order by an atomic ticket (ctr[0]); block t is the single pass's block t
+// (tile_of). Before item (j, t) runs, the blocks whose cells its halo reads (its own and
+// the up to 26 around it, wrapped on periodic axes) must have finished pass j - 1
+// (done[u] = ctr[4 + u] counts the passes block u has finished): that covers the reads of
+// pass j - 1's output and the stores into the buffer pass j - 1 read. Every item waited
+// for has a smaller ticket, taken by a workgroup that is already running and waits only
+// for smaller tickets itself, so the launch completes whether or not all its workgroups
+// are resident at once (other kernels may hold part of the GPU). Stores are published at
+// device scope before a block's count moves (the XCDs' L2s are not coherent with each
+// other) and stale lines dropped after a wait; counts move by atomic read-modify-write,
+// which other XCDs observe (a plain store would sit in the writer's L2). The last
+// workgroup to leave resets the counters for the next launch. A waiting lane that used
+// up its spin budget (kDepSpins over the launch) waits no more and sets skip[2]; the last
+// workgroup then fills the result with NaN so the failure cannot pass unseen.
 template <int kSolve>
-__global__ __launch_bounds__(kThreads) void k_rbgs_chain(float* io, float* other, const float* __restrict__ b, const float coef, const int passes, const Dims d, const StencilBC bc, const TileGeom g, int* __restrict__ sync) {
-    if (sync[0] != 0) return;
-    unsigned* count = reinterpret_cast<unsigned*>(sync + 1);
-    int* fault      = sync + 2;
-    const unsigned G = gridDim.x;
-    const int n      = g.tiles_x * g.tiles_y * g.chunks;
-    const uint64_t pairs = static_cast<uint64_t>(d.nx / 2) * static_cast<uint64_t>(d.ny) * static_cast<uint64_t>(d.nz);
-    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x; i < pairs; i += static_cast<uint64_t>(G) * kThreads)
-        reinterpret_cast<float2*>(io)[i] = make_float2(0.0f, 0.0f);
-    grid_barrier(count, G, fault);
-    for (int j = 0; j < passes; ++j) {
+__global__ __launch_bounds__(kThreads) void k_rbgs_chain(float* io, float* other, const float* __restrict__ b, const float coef, const int passes, const Dims d, const StencilBC bc, const TileGeom g, int* __restrict__ skip, unsigned* __restrict__ ctr) {
+    if (skip[0] != 0) return;
+    __shared__ unsigned s_ticket;
+    __shared__ int s_late;
+    unsigned* ticket = ctr;
+    unsigned* exits  = ctr + 1;
+    unsigned* done   = ctr + 4;
+    const int n           = g.tiles_x * g.tiles_y * g.chunks;
+    const unsigned total  = static_cast<unsigned>(passes) * static_cast<unsigned>(n);
+    const bool px = bc.periodic & 1, py = (bc.periodic >> 1) & 1, pz = (bc.periodic >> 2) & 1;
+    unsigned budget = kDepSpins; // this lane's spins left over the launch
+    for (;;) {
+        if (threadIdx.x == 0) s_ticket = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const unsigned k = s_ticket;
+        if (k >= total) break;
+        const int j = static_cast<int>(k / static_cast<unsigned>(n));
+        const int t = static_cast<int>(k % static_cast<unsigned>(n));
+        if (j > 0) {
+            // lanes 0 .. 26 of the first wave each wait for one neighbour block
+            if (threadIdx.x < 27) {
+                int tx, ty, cz;
+                tile_of(g, t, tx, ty, cz);
+                const int q = static_cast<int>(threadIdx.x);
+                int ux = tx + q % 3 - 1, uy = ty + (q / 3) % 3 - 1, uz = cz + q / 9 - 1;
+                bool in = true;
+                if (ux < 0 || ux >= g.tiles_x) { ux = px ? wrap_index(ux, g.tiles_x) : 0; in = in && px; }
+                if (uy < 0 || uy >= g.tiles_y) { uy = py ? wrap_index(uy, g.tiles_y) : 0; in = in && py; }
+                if (uz < 0 || uz >= g.chunks) { uz = pz ? wrap_index(uz, g.chunks) : 0; in = in && pz; }
+                if (in) {
+                    const int u = (uy * g.tiles_x + ux) * g.chunks + uz;
+                    while (budget > 0 && __hip_atomic_load(done + u, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < static_cast<unsigned>(j)) {
+                        if (--budget == 0) __hip_atomic_fetch_or(skip + 2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        __builtin_amdgcn_s_sleep(4);
+                    }
+                }
+            }
+            __syncthreads();
+            __threadfence();
+        }
         const float* src = (j & 1) ? other : io;
         float* dst       = (j & 1) ? io : other;
-        for (int t = static_cast<int>(blockIdx.x); t < n; t += static_cast<int>(G)) {
-            const int w = (n & 7) == 0 ? (t & 7) * (n >> 3) + (t >> 3) : t;
-            rbgs_rows_body<kSolve, false, StoreF32, true>(src, b, dst, coef, 1.0f, d, bc, g, nullptr, w);
-            __syncthreads(); // the LDS rings are reused by the next block
-        }
-        if (j + 1 < passes) grid_barrier(count, G * static_cast<unsigned>(j + 2), fault);
+        if (j == 0) rbgs_rows_body<kSolve, true, StoreF32, true>(src, b, dst, coef, 1.0f, d, bc, g, nullptr, t);
+        else rbgs_rows_body<kSolve, false, StoreF32, true>(src, b, dst, coef, 1.0f, d, bc, g, nullptr, t);
+        __threadfence();
+        __syncthreads(); // every wave's stores published; the LDS rings free for the next item
+        if (threadIdx.x == 0) __hip_atomic_fetch_add(done + t, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT); // j -> j + 1
+    }
+    // leave; the last workgroup out resets the counters (and poisons a failed result)
+    __threadfence();
+    if (threadIdx.x == 0) s_late = __hip_atomic_fetch_add(exits, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1u;
+    __syncthreads();
+    if (!s_late) return;
+    __threadfence();
+    for (int i = static_cast<int>(threadIdx.x); i < n; i += kThreads) done[i] = 0u;
+    if (threadIdx.x == 0) {
+        ticket[0] = 0u;
+        exits[0]  = 0u;
+    }
+    if (__hip_atomic_fetch_or(skip + 2, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        float* result        = (passes & 1) ? other : io;
+        const uint64_t cells = static_cast<uint64_t>(d.nx) * static_cast<uint64_t>(d.ny) * static_cast<uint64_t>(d.nz);
+        for (uint64_t i = threadIdx.x; i < cells; i += kThreads) result[i] = __builtin_nanf("");
     }
 }
 
@@ -1920,19 +1954,19 @@ hipError_t smooth_chain_grid(const Dims d, uint32_t* out_grid) {
     const uint64_t resident = static_cast<uint64_t>(cus) * static_cast<uint64_t>(std::min(per_cu, per_cu_d));
     if (resident == 0) return hipErrorInvalidConfiguration;
     const uint64_t n = smooth_blocks(plan_tiles(d, 0));
-    // all workgroups of the grid must be resident together (they wait for each other); a
-    // multiple of 8 keeps every block on the XCD the single pass would give it
-    uint64_t grid = std::min(n, resident);
-    if (grid < n && grid >= 8) grid &= ~7ull;
+    // no more workgroups than can be resident at once (they need not all be)
+    const uint64_t grid = std::min(n, resident);
     *out_grid = static_cast<uint32_t>(grid);
     return hipSuccess;
 }
 
-hipError_t launch_smooth_chain(hipStream_t s, const SolveKind solve, float* io, float* other, const float* b, const float coef, const int passes, const Dims d, const StencilBC& bc, int* sync, const uint32_t grid) {
+uint32_t smooth_chain_counters(const Dims d) { return 4u + smooth_blocks(plan_tiles(d, 0)); }
+
+hipError_t launch_smooth_chain(hipStream_t s, const SolveKind solve, float* io, float* other, const float* b, const float coef, const int passes, const Dims d, const StencilBC& bc, int* skip, unsigned* counters, const uint32_t grid) {
     if (passes <= 0 || io == other || grid == 0) return hipErrorInvalidValue;
     const TileGeom g = plan_tiles(d, 0);
-    if (solve == kSolvePoisson) k_rbgs_chain<kSolvePoisson><<<grid, kThreads, 0, s>>>(io, other, b, coef, passes, d, bc, g, sync);
-    else k_rbgs_chain<kSolveDiffusion><<<grid, kThreads, 0, s>>>(io, other, b, coef, passes, d, bc, g, sync);
+    if (solve == kSolvePoisson) k_rbgs_chain<kSolvePoisson><<<grid, kThreads, 0, s>>>(io, other, b, coef, passes, d, bc, g, skip, counters);
+    else k_rbgs_chain<kSolveDiffusion><<<grid, kThreads, 0, s>>>(io, other, b, coef, passes, d, bc, g, skip, counters);
     return hipGetLastError();
 }
 

@@ -339,7 +339,7 @@ __global__ __launch_bounds__(kThreads) void k_guess_decide(const double* __restr
         stats[1]            = rz;
         stats[2]            = fallback ? 1.0 : 0.0;
         skip[0]             = fallback ? 0 : 1;
-        skip[1]             = 0; // grid-barrier count of a one-launch fallback solve
+        skip[2]             = 0; // failure flag of this step's one-launch fallback solve
     }
 }
 

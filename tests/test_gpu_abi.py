@@ -117,12 +117,16 @@ def test_norms_match_host_reduction():
 
 
 def test_stage_timing_hooks():
-    d = small_desc(32, 16, 8)
-    with sf.Context(d) as ctx:
-        for stage, per_cell in ((0, 12), (1, 12), (2, 24), (3, 36), (4, 12), (5, 8)):
-            assert ctx.stage_bytes(stage) == per_cell * d.cells
-            ctx.run_stage(stage, 3)
-        ctx.synchronize()
+    """Stage hooks and their algorithmic bytes. The smoother-pass stages time the pass the
+    step runs: with even nx, the black-cell pass (read b and the black cells, write the black
+    cells: 8 B/cell); with odd nx, and for the full-iterate stages, 12 B/cell."""
+    for n, pass_bytes in (((32, 16, 8), 8), ((31, 16, 8), 12)):
+        d = small_desc(*n)
+        with sf.Context(d) as ctx:
+            for stage, per_cell in ((0, pass_bytes), (1, pass_bytes), (2, 24), (3, 36), (4, 12), (5, 8), (8, 12), (9, 12)):
+                assert ctx.stage_bytes(stage) == per_cell * d.cells, (n, stage)
+                ctx.run_stage(stage, 3)
+            ctx.synchronize()
 
 
 def test_many_contexts_create_destroy():

@@ -136,8 +136,8 @@ def test_fallback_without_the_one_launch_solve(smoother, k):
 
 def test_fallback_in_rounds_is_the_plain_step_bitwise():
     """A 2048x1024x8 slab has 1024 smoother blocks, more than the workgroups that fit on the
-    GPU at once (3 per CU): the one-launch fallback solve runs its blocks in rounds between
-    grid barriers and still gives the plain step bit for bit."""
+    GPU at once (3 per CU): the one-launch fallback solve hands its blocks out by ticket,
+    each waiting only for its finished neighbours, and still gives the plain step bit for bit."""
     desc = flow_desc(n=(2048, 1024, 8), walls_only=True, p_iters=12, d_iters=2)
     inputs = Inputs(desc, seed=5, velocity_scale=1.0)
     wild = uo.join_params(uo.init_params(3, scale=300.0)).astype(np.float32)
@@ -208,3 +208,33 @@ def test_trained_network_gain():
           f"trained+{k} {d_trained[0]:.4e} {st_t[0]}, plain 40 passes {d_plain_40[0]:.4e}", flush=True)
     assert not st_t[0]["fallback"]
     assert d_trained[0] < d_plain_k[0] and d_trained[0] < d_untrained[0]
+
+
+def test_concurrent_fallbacks_share_the_gpu_bitwise():
+    """Two learned contexts on their own streams fall back in the same steps, so their
+    one-launch fallback solves (each sized to every resident workgroup slot) run on the GPU
+    at the same time and neither has all of its workgroups resident: both still give the
+    plain step bit for bit, and neither reports a failed solve."""
+    desc = flow_desc(n=(2048, 1024, 8), walls_only=True, p_iters=12, d_iters=2)
+    inputs = Inputs(desc, seed=7, velocity_scale=1.0)
+    wild = uo.join_params(uo.init_params(3, scale=300.0)).astype(np.float32)
+    ctxs = [sf.Context(desc, options=learned_opts(4)) for _ in range(2)]
+    try:
+        for ctx in ctxs:
+            ctx.set_pressure_model(wild)
+            inputs.seed_gpu(ctx)
+        sf.hip.synchronize()
+        for _ in range(2):
+            for ctx in ctxs:  # asynchronous launches on two streams
+                ctx.step(1)
+        results = []
+        for ctx in ctxs:
+            stats = ctx.warm_start_stats()  # BACKEND_FAILURE (raised) if a solve failed
+            assert stats["fallback"]
+            results.append(gpu_fields(ctx, desc))
+    finally:
+        for ctx in ctxs:
+            ctx.destroy()
+    plain = run_gpu(desc, inputs.seed_gpu, 2)
+    for i, got in enumerate(results):
+        assert_same(got, plain, f"concurrent fallback {i} vs plain P")
