@@ -160,13 +160,14 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_run_stage_cuda(void* cont
  * once), as used for roofline figures. */
 STABLE_FLUIDS_API uint64_t stable_fluids_ext_stage_bytes_cuda(void* context, uint32_t stage);
 
-/* Tile shapes of the RBGS passes that carry only black cells between passes (plain FP32
- * RBGS solves on even nx): 64 x-pairs (128 cells) by 16 rows with 4 or 8 waves per
- * workgroup, or by 32 rows with 8 waves. */
+/* Tile shapes (x cells by rows, workgroup threads) of the RBGS passes that carry only
+ * black cells between passes (plain FP32 RBGS solves on even nx). */
 typedef enum StableFluidsExtTile {
-    STABLE_FLUIDS_EXT_TILE_64X16_8W = 0,
-    STABLE_FLUIDS_EXT_TILE_64X16_4W = 1, /* default */
-    STABLE_FLUIDS_EXT_TILE_64X32_8W = 2,
+    STABLE_FLUIDS_EXT_TILE_128X16 = 0, /* 512 threads (default) */
+    STABLE_FLUIDS_EXT_TILE_64X32  = 1, /* 512 threads */
+    STABLE_FLUIDS_EXT_TILE_64X16  = 2, /* 256 threads */
+    STABLE_FLUIDS_EXT_TILE_32X32  = 3, /* 256 threads */
+    STABLE_FLUIDS_EXT_TILE_32X16  = 4, /* 128 threads */
 } StableFluidsExtTile;
 
 /* Sets the tile shape and z-chunk length (planes per workgroup, 0 = the default plan) of

@@ -60,9 +60,11 @@ STAGE_PRESSURE_PAIR = 6
 STAGE_VELOCITY_PAIR = 7
 STAGE_FULL_PRESSURE_PASS = 8
 STAGE_FULL_VELOCITY_PASS = 9
-TILE_64X16_8W = 0
-TILE_64X16_4W = 1
-TILE_64X32_8W = 2
+TILE_128X16 = 0
+TILE_64X32 = 1
+TILE_64X16 = 2
+TILE_32X32 = 3
+TILE_32X16 = 4
 
 
 class FlowFace(C.Structure):

@@ -127,7 +127,7 @@ struct Context {
         double *x = nullptr, *r = nullptr, *p = nullptr, *ap = nullptr, *partials = nullptr, *scal = nullptr;
     } cg;
     float* half[2] = {nullptr, nullptr}; // padded black-cell iterates of the RBGS passes (sf_rbgs_half.hip)
-    int half_tile  = kHalfTile8x2;       // their tile shape and z-chunk length (0: default plan)
+    int half_tile  = kHalfTile128x16;       // their tile shape and z-chunk length (0: default plan)
     int half_zc    = 0;
     std::vector<void*> allocations;
 
@@ -1551,7 +1551,7 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_run_stage_cuda(void* cont
 
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_set_smoother_tiling_cuda(void* context, uint32_t tile, uint32_t z_chunk) {
     sf::Range r("stable.ext.set_smoother_tiling");
-    if (context == nullptr || tile > STABLE_FLUIDS_EXT_TILE_64X32_8W || z_chunk > 4096) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    if (context == nullptr || tile >= sf::kHalfTileCount || z_chunk > 4096) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     auto& c = *static_cast<Context*>(context);
     if (hipStreamSynchronize(c.stream) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     c.half_tile = static_cast<int>(tile);

@@ -107,12 +107,17 @@ struct HalfPassArgs {
 bool rbgs_half_supported(Dims d, const StencilBC& bc);
 HalfLayout half_layout(Dims d);
 uint64_t half_buffer_floats(Dims d);
-// tile shapes: 64 pairs x 16 rows by 8 waves of 2 rows or 4 waves of 4 rows; 64 x 32 by 8 waves of 4
-enum HalfTileShape : int { kHalfTile8x2 = 0, kHalfTile4x4 = 1, kHalfTile8x4 = 2 };
+// tile shapes, x cells by rows (threads): 128 x 16 (512), 64 x 32 (512), 64 x 16 (256),
+// 32 x 32 (256), 32 x 16 (128)
+enum HalfTileShape : int { kHalfTile128x16 = 0, kHalfTile64x32 = 1, kHalfTile64x16 = 2, kHalfTile32x32 = 3, kHalfTile32x16 = 4, kHalfTileCount = 5 };
 int half_tile_rows(int tile);
 int half_tile_threads(int tile);
+int half_tile_cells(int tile);
 TileGeom half_plan(Dims d, int tile, int zc_override);
 hipError_t launch_rbgs_half(hipStream_t s, SolveKind solve, int in_mode, int out_mode, const HalfPassArgs& args, int tile);
+// one launch for many passes: workgroup i runs block items[i].block of descs[items[i].desc]
+// (every descriptor's L filled in, its g planned for `tile`)
+hipError_t launch_rbgs_half_batched(hipStream_t s, SolveKind solve, int in_mode, int out_mode, int tile, const HalfPassArgs* descs, const BatchItem* items, uint32_t n_items);
 
 struct AdvectVelocityDesc {
     float* t[3];

@@ -1,594 +1,9 @@
-// sf_rbgs_half.hip -- red-black Gauss-Seidel passes that carry only the black cells from one
-// pass to the next (gfx950).
-//
-// One pass is one reference RBGS iteration, a red half-sweep then a black one
-// (backend-cuda.cu:354-375, :398-407, loops :628-640, :646-649):
-//   diffusion: x = (b + c * sum6) / (1 + 6c)       Poisson: x = (sum6 - c * b) / 6
-// A red cell is relaxed from its six neighbours, which are all black, and a black cell
-// from six red ones. So the old value of a red cell is read only where the cell sits on a
-// non-periodic face, whose ghost is a rule of the cell itself (copy, 2 v_face - value or a
-// constant, backend-cuda.cu:62-189). Between passes the iterate is therefore kept as
-//   * the black cell of every x pair, in a "half" buffer of nx/2 floats per row, and
-//   * the ghost values of the red face cells (the rule applied to the red value) in the
-//     padding of that buffer: rows -1 and ny, planes -1 and nz, and two short x-face
-//     columns per plane.
-// A middle pass reads 4 B/cell of b and 2 B/cell of black cells and writes 2 B/cell
-// (8 B/cell against 12 for a full-iterate pass). The first pass of a solve reads the
-// natural-layout start iterate (or none: zeros) and the last one writes the natural-layout
-// result, both colours, scaled by out_scale.
-//
-// Padded half plane k (k = -1 .. nz; planes -1 and nz are the z-face ghosts), ps floats:
-//   entry (pair i, row y), y = -1 .. ny:  (y + 1) * nxh + i
-//   x-face ghosts (side s, row y):         (ny + 2) * nxh + s * ny + y
-// A pair (2i, 2i + 1) of row y, plane k holds its red cell at x = 2i + p, p = (y + k) & 1,
-// and its black cell at 2i + 1 - p. The six black neighbours of red cell i are the
-// black cells i - 1 + p and i + p of its row, and i of rows y -+ 1 and planes k -+ 1; the
-// red neighbours of black cell i are red cells i - p and i + 1 - p of its row and i of
-// the rows and planes above and below. Every sum keeps the reference order x-, x+, y-,
-// y+, z-, z+, and divisions are the correctly rounded reciprocal forms (sf_relax.hpp),
-// so a solve is bit-identical to the in-place reference iteration.
-//
-// Work decomposition. A 512-thread workgroup owns 64 pairs x 8*kRowsW rows of a z-chunk
-// and marches upward. Per plane step s, with one barrier:
-//   stage A(s+1) (black cells of the old iterate, tile plus two pairs and two rows of
-//   halo, ghosts filled) into a ring of four LDS planes -> barrier -> red half-update
-//   R(s) on the tile plus one pair and one row of halo (the halo is recomputed, so tiles
-//   exchange nothing) into a ring of two LDS planes -> black cells of plane s - 1 from
-//   R(s-2 .. s) -> stores.
-// A lane holds the same pair of its rows in every plane, so the z neighbours of its own
-// cells and one of the two x neighbours come from registers; LDS supplies the other x
-// neighbour and the y neighbours. Loads run two planes ahead through buffer resources
-// (idle lanes get an out-of-range offset, idle planes a 0-byte resource), so no memory
-// instruction sits under a branch.
-#include <algorithm>
-#include <type_traits>
-#include <utility>
-
-#include "sf_buffer.hpp"
-#include "sf_kernels.hpp"
-#include "sf_relax.hpp"
+// sf_rbgs_half.hip -- host side of the black-cell RBGS passes (sf_rbgs_half.hpp): layout,
+// tile plans and the single-context launches. The batched launches live in
+// sf_rbgs_half_batch.hip (built in parallel: each instantiates the kernel set once).
+#include "sf_rbgs_half.hpp"
 
 namespace sf {
-
-namespace {
-
-constexpr int kHP = 64; // pairs per tile row (128 cells)
-#ifndef SF_HALF_DEPTH
-#define SF_HALF_DEPTH 2
-#endif
-constexpr int kHalfDepth = SF_HALF_DEPTH; // planes of loads in flight
-
-template <int kWaves, int kRowsW>
-struct HalfTile {
-    static constexpr int kThreads = 64 * kWaves;
-    static constexpr int TY     = kWaves * kRowsW;  // tile rows
-    static constexpr int AW     = kHP + 4;          // A plane: pairs -2 .. 65
-    static constexpr int AR     = TY + 4;           //          rows -2 .. TY + 1
-    static constexpr int AF     = AW * AR;
-    static constexpr int RW     = kHP + 2;          // R plane: pairs -1 .. 64
-    static constexpr int RR     = TY + 2;           //          rows -1 .. TY
-    static constexpr int RF     = RW * RR;
-    static constexpr int kAHalo = 4 * AW + 4 * TY;  // A halo positions (one extra slot per lane)
-    static constexpr int kRHalo = 2 * RW + 2 * TY;  // R halo positions
-    static constexpr int kRHaloWaves = (kRHalo + 63) / 64;
-    static constexpr int kAPer  = (kAHalo + kThreads - 1) / kThreads; // A halo slots per lane
-    static_assert(kRHalo <= kThreads && 128 + 2 * TY <= kThreads, "one R halo and one ghost-store slot per lane");
-};
-
-// Position classes of a (pair, row) of the neighbourhood.
-enum HalfKind : int {
-    kPosReal = 0, // a grid cell (wrapped on periodic axes)
-    kPosXm   = 1, // x- ghost (pair -1), non-periodic x
-    kPosXp   = 2, // x+ ghost (pair nxh)
-    kPosYm   = 3, // y- ghost (row -1), non-periodic y
-    kPosYp   = 4, // y+ ghost (row ny)
-    kPosNone = 5, // further out, or outside on both axes: read by no real cell
-};
-
-struct HalfPos {
-    int kind;
-    int off_h; // float offset in a padded half plane (real: the black cell; ghost: its entry)
-    int off_n; // float offset of the aligned natural pair holding the cell (ghost: the adjacent pair)
-};
-
-__device__ __forceinline__ HalfPos classify(const int i, const int y, const Dims d, const HalfLayout& L, const bool px, const bool py) {
-    const int nxh = L.nxh;
-    const bool xo = !px && (i < 0 || i >= nxh);
-    const bool yo = !py && (y < 0 || y >= d.ny);
-    const int iw  = px ? wrap_index(i, nxh) : min(max(i, 0), nxh - 1);
-    const int yw  = py ? wrap_index(y, d.ny) : min(max(y, 0), d.ny - 1);
-    HalfPos p;
-    p.kind  = kPosReal;
-    p.off_n = yw * d.nx + 2 * iw;
-    p.off_h = (yw + 1) * nxh + iw;
-    if (xo && yo) {
-        p.kind = kPosNone;
-    } else if (xo) {
-        p.kind  = i == -1 ? kPosXm : (i == nxh ? kPosXp : kPosNone);
-        p.off_h = (d.ny + 2) * nxh + (i < 0 ? 0 : d.ny) + yw;
-    } else if (yo) {
-        p.kind  = y == -1 ? kPosYm : (y == d.ny ? kPosYp : kPosNone);
-        p.off_h = (y < 0 ? 0 : d.ny + 1) * nxh + iw;
-    }
-    return p;
-}
-
-// c ? a : b on values (a conditional on two lvalues selects their addresses, which keeps
-// the selected variables in memory)
-template <class T>
-__device__ __forceinline__ T pick(const bool c, const T a, const T b) { return c ? a : b; }
-
-template <int kRowsW, int kAPer, int kIn>
-struct PfSet {
-    using T = std::conditional_t<kIn == kHalfInNatural, float2, float>;
-    T own[kRowsW];
-    T ex[kAPer];
-};
-
-template <int kSolve, int kIn, int kOut, int kWaves, int kRowsW, int kDepth>
-__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 4))) void k_rbgs_half(const HalfPassArgs args) {
-    using TT           = HalfTile<kWaves, kRowsW>;
-    constexpr int kAPer = TT::kAPer;
-    constexpr int TY   = TT::TY;
-    constexpr int AW   = TT::AW;
-    constexpr int RW   = TT::RW;
-    constexpr int AF   = TT::AF;
-    constexpr int RF   = TT::RF;
-    __shared__ float s_a[4 * AF];
-    __shared__ float s_r[2 * RF];
-    __shared__ float s_sink[64]; // LDS target of halo-slot lanes with no position (one word per lane)
-    // ghost rules (a, c, k) of faces 0..5 and the identity at 6, read with runtime indices
-    // at set-up (indexing the kernel argument itself would copy it to scratch memory)
-    __shared__ float s_rule_a[8], s_rule_c[8];
-    __shared__ int s_rule_k[8];
-
-    const Dims d       = args.d;
-    const StencilBC& bc = args.bc;
-    if (threadIdx.x < 7) {
-        const int f       = static_cast<int>(threadIdx.x);
-        const LinGhost lg = f == 6 ? lin_identity() : lin_ghost(face_rule(bc, f));
-        s_rule_a[f]       = lg.a;
-        s_rule_c[f]       = lg.c;
-        s_rule_k[f]       = lg.k ? 1 : 0;
-    }
-    __syncthreads();
-    auto rule = [&](const int f) { return LinGhost{s_rule_a[f], s_rule_c[f], s_rule_k[f] != 0}; };
-    auto kind_rule = [&](const int kind) { return rule(kind >= kPosXm && kind <= kPosYp ? kind - 1 : 6); };
-    const HalfLayout L = args.L;
-    const float coef   = args.coef;
-    int tx, ty, cz;
-    tile_of(args.g, xcd_block(), tx, ty, cz);
-    const int tid  = static_cast<int>(threadIdx.x);
-    const int lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int i0 = tx * kHP, y0 = ty * TY;
-    const int z0 = cz * args.g.zc;
-    const int z1 = min(z0 + args.g.zc, d.nz);
-    const int nx = d.nx, ny = d.ny, nz = d.nz, nxh = L.nxh;
-    const bool px = bc.periodic & 1, py = (bc.periodic >> 1) & 1, pz = (bc.periodic >> 2) & 1;
-    const uint32_t nplane_b = 4u * static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny); // natural plane bytes
-    const uint32_t nfield_b = nplane_b * static_cast<uint32_t>(nz);
-    const uint32_t hplane_b = 4u * static_cast<uint32_t>(L.ps);
-    const uint32_t hfield_b = hplane_b * static_cast<uint32_t>(nz + 2);
-
-    // ---- own slots: rows wave + 8 j of the tile, pair i0 + lane ----
-    int o_src[kRowsW], o_b[kRowsW], o_st[kRowsW], o_kind[kRowsW];
-    bool o_xflip[kRowsW];
-    LinGhost o_rule[kRowsW];
-    bool tile_ghost = false; // some own slot is not a real cell (partial tile)
-#pragma unroll
-    for (int j = 0; j < kRowsW; ++j) {
-        const int y       = y0 + wave + kWaves * j;
-        const int i       = i0 + lane;
-        const HalfPos P   = classify(i, y, d, L, px, py);
-        o_kind[j]         = P.kind;
-        o_xflip[j]        = P.kind == kPosXm || P.kind == kPosXp;
-        o_rule[j]         = kind_rule(P.kind);
-        o_src[j]          = kIn == kHalfInNatural ? 4 * P.off_n : 4 * P.off_h;
-        o_b[j]            = P.kind == kPosReal ? 4 * P.off_n : kOutOfRange;
-        const bool owned  = i < nxh && y < ny;
-        o_st[j]           = !owned ? kOutOfRange : (kOut == kHalfOutHalf ? 4 * ((y + 1) * nxh + i) : 4 * (y * nx + 2 * i));
-        tile_ghost        = tile_ghost || P.kind != kPosReal;
-    }
-    tile_ghost = __builtin_amdgcn_readfirstlane(__any(tile_ghost)) != 0;
-    const int ypar = (y0 + wave) & 1; // row parity of every own slot (8 j is even)
-
-    // ---- extra A slot: the halo of the A planes ----
-    int ex_src[kAPer], ex_lds[kAPer], ex_y[kAPer];
-    bool ex_xflip[kAPer], ex_on[kAPer];
-    LinGhost ex_rule[kAPer];
-#pragma unroll
-    for (int q = 0; q < kAPer; ++q) {
-        const int e = tid + q * TT::kThreads;
-        ex_src[q]   = kOutOfRange;
-        ex_lds[q] = ex_y[q] = 0;
-        ex_xflip[q] = ex_on[q] = false;
-        ex_rule[q]  = lin_identity();
-        if (e < TT::kAHalo) {
-            int r, l;
-            if (e < 4 * AW) {
-                const int rr = e / AW;
-                r            = rr < 2 ? rr - 2 : TY + rr - 2;
-                l            = e % AW - 2;
-            } else {
-                const int e2 = e - 4 * AW, c = e2 & 3;
-                r            = e2 >> 2;
-                l            = c < 2 ? c - 2 : kHP + c - 2;
-            }
-            const HalfPos P = classify(i0 + l, y0 + r, d, L, px, py);
-            ex_src[q]       = kIn == kHalfInNatural ? 4 * P.off_n : 4 * P.off_h;
-            ex_lds[q]       = (r + 2) * AW + l + 2;
-            ex_y[q]         = y0 + r;
-            ex_xflip[q]     = P.kind == kPosXm || P.kind == kPosXp;
-            ex_rule[q]      = kind_rule(P.kind);
-            ex_on[q]        = true;
-        }
-    }
-
-    // ---- R halo slot (waves 0 .. kRHaloWaves - 1): the halo of the R planes ----
-    // A ghost position takes the face rule of the adjacent cell's old value; the adjacent
-    // cell is black where it matters (x-: p = 1, x+: p = 0), so its A entry is at a fixed
-    // offset h_goff from the position (x- +1, x+ -1, y- +AW, y+ -AW).
-    int h_b = kOutOfRange, h_alds = 0, h_rlds = 0, h_y = 0, h_goff = 0;
-    bool h_real = false;
-    LinGhost h_rule = lin_identity();
-    if (tid < TT::kRHalo) {
-        int r, l;
-        if (tid < 2 * RW) {
-            r = tid < RW ? -1 : TY;
-            l = tid % RW - 1;
-        } else {
-            const int e2 = tid - 2 * RW;
-            r            = e2 >> 1;
-            l            = (e2 & 1) ? kHP : -1;
-        }
-        const HalfPos P = classify(i0 + l, y0 + r, d, L, px, py);
-        h_real          = P.kind == kPosReal;
-        h_b             = h_real ? 4 * P.off_n : kOutOfRange;
-        h_alds          = (r + 2) * AW + l + 2;
-        h_rlds          = (r + 1) * RW + l + 1;
-        h_y             = y0 + r;
-        h_rule          = kind_rule(P.kind);
-        h_goff          = P.kind == kPosXm ? 1 : (P.kind == kPosXp ? -1 : (P.kind == kPosYm ? AW : (P.kind == kPosYp ? -AW : 0)));
-    }
-
-    // ---- ghost-store slot (half output): red face cells of the tile -> padded entries ----
-    //   lanes 0 .. 63: row 0 -> row -1; 64 .. 127: row ny - 1 -> row ny;
-    //   128 .. 128 + TY - 1: pair 0 -> x- column; the next TY: pair nxh - 1 -> x+ column
-    int g_off = kOutOfRange, g_rlds = 0;
-    LinGhost g_rule = lin_identity();
-    if constexpr (kOut == kHalfOutHalf) {
-        if (tid < 128) {
-            const int side = tid >> 6, l = tid & 63;
-            const int y    = side ? ny - 1 : 0;
-            if (!py && y >= y0 && y < y0 + TY && i0 + l < nxh) {
-                g_off  = 4 * ((side ? ny + 1 : 0) * nxh + i0 + l);
-                g_rlds = (y - y0 + 1) * RW + l + 1;
-                g_rule = rule(2 + side);
-            }
-        } else if (tid < 128 + 2 * TY) {
-            const int side = (tid - 128) / TY, r = (tid - 128) % TY;
-            const int i    = side ? nxh - 1 : 0;
-            if (!px && i >= i0 && i < i0 + kHP && y0 + r < ny) {
-                g_off  = 4 * ((ny + 2) * nxh + side * ny + y0 + r);
-                g_rlds = (r + 1) * RW + i - i0 + 1;
-                g_rule = rule(side);
-            }
-        }
-    }
-    const bool g_any = __builtin_amdgcn_readfirstlane(__any(g_off != kOutOfRange)) != 0; // wave-uniform
-    const LinGhost zr_lo = rule(4), zr_hi = rule(5);
-    const DiffusionDiv ddv = diffusion_div(coef); // loop-invariant reciprocal of 1 + 6 coef
-
-    auto zghost = [&](const int k) { return !pz && (k < 0 || k >= nz); };
-    // periodic plane of k in [-2, nz + 1] (the planes a chunk reads; nz >= 2)
-    auto wrap_near_z = [&](const int k) { return k < 0 ? k + nz : (k >= nz ? k - nz : k); };
-    // buffer resources: uniform bases once, the size selected per plane (0 bytes: off)
-    const void* p_hin  = uniform_ptr(args.h_in);
-    const void* p_xnat = uniform_ptr(args.x_nat);
-    const void* p_b    = uniform_ptr(args.b);
-    const void* p_hout = uniform_ptr(args.h_out);
-    const void* p_xout = uniform_ptr(args.x_out);
-
-    // ---- loads (kLive: the plane is known to be needed and real) ----
-    using Pf = PfSet<kRowsW, kAPer, kIn>;
-    auto load_a = [&]<bool kLive>(const int k, Pf& pf) __attribute__((always_inline)) {
-        if constexpr (kIn == kHalfInHalf) {
-            const int kk    = pz ? wrap_near_z(k) : k;
-            const bool live = kLive || (k >= z0 - 2 && k <= z1 + 1 && kk >= -1 && kk <= nz);
-            const Rsrc r    = rsrc_at(p_hin, live ? hfield_b : 0u);
-            const int so    = live ? (kk + 1) * static_cast<int>(hplane_b) : 0;
-#pragma unroll
-            for (int j = 0; j < kRowsW; ++j) pf.own[j] = buf_load1<StoreF32>(r, o_src[j], so);
-#pragma unroll
-            for (int q = 0; q < kAPer; ++q) pf.ex[q] = buf_load1<StoreF32>(r, ex_src[q], so);
-        } else if constexpr (kIn == kHalfInNatural) {
-            const int kc    = pz ? wrap_near_z(k) : min(max(k, 0), nz - 1);
-            const bool live = kLive || (k >= z0 - 2 && k <= z1 + 1 && (pz || (k >= -1 && k <= nz)));
-            const Rsrc r    = rsrc_at(p_xnat, live ? nfield_b : 0u);
-            const int so    = live ? kc * static_cast<int>(nplane_b) : 0;
-#pragma unroll
-            for (int j = 0; j < kRowsW; ++j) pf.own[j] = buf_load2<StoreF32>(r, o_src[j], so);
-#pragma unroll
-            for (int q = 0; q < kAPer; ++q) pf.ex[q] = buf_load2<StoreF32>(r, ex_src[q], so);
-        }
-    };
-    struct PfB {
-        float2 own[kRowsW];
-        float2 h;
-    };
-    auto load_b = [&]<bool kLive>(const int k, PfB& pf) __attribute__((always_inline)) {
-        const int kc    = pz ? wrap_near_z(k) : (kLive ? k : min(max(k, 0), nz - 1));
-        const bool live = kLive || (k >= z0 - 1 && k <= z1 && (pz || (k >= 0 && k < nz)));
-        const Rsrc r    = rsrc_at(p_b, live ? nfield_b : 0u);
-        const int so    = live ? kc * static_cast<int>(nplane_b) : 0;
-#pragma unroll
-        for (int j = 0; j < kRowsW; ++j) pf.own[j] = buf_load2<StoreF32>(r, o_b[j], so);
-        pf.h = buf_load2<StoreF32>(r, h_b, so);
-    };
-
-    // value of an A position at plane k from its raw load (kReal: k is not a z ghost plane)
-    auto a_value = [&]<bool kReal>(const auto raw, const int k, const int y, const bool xflip, const LinGhost rl) __attribute__((always_inline)) -> float {
-        if constexpr (kIn == kHalfInHalf) {
-            return raw; // ghost entries hold the rule already applied
-        } else {
-            const LinGhost gr = kReal ? rl : pick(zghost(k), pick(k < 0, zr_lo, zr_hi), rl);
-            float v           = 0.0f;
-            if constexpr (kIn == kHalfInNatural) {
-                // black element of the pair by the position's parity; on y / z ghosts this is
-                // the red cell of the adjacent pair, on x ghosts the red cell is the other one
-                const int e = ((y + k) & 1) ^ 1 ^ (xflip ? 1 : 0);
-                v           = e ? raw.y : raw.x;
-            }
-            return lin_apply(gr, v);
-        }
-    };
-
-    // ---- register state ----
-    float a_prev[kRowsW], a_cur[kRowsW], a_next[kRowsW]; // own A(s-1), A(s), A(s+1)
-    float r_mm[kRowsW], r_m[kRowsW], r_0[kRowsW];        // own R(s-2), R(s-1), R(s)
-    float bb[kRowsW], bb_new[kRowsW];                    // b of the own black cells, planes s-1 and s
-#pragma unroll
-    for (int j = 0; j < kRowsW; ++j) {
-        a_prev[j] = a_cur[j] = a_next[j] = 0.0f;
-        r_mm[j] = r_m[j] = r_0[j] = 0.0f;
-        bb[j] = bb_new[j] = 0.0f;
-    }
-
-    // stage A(k) from `pf` into the ring plane at `A`; own values into `own`
-    auto stage_a = [&]<bool kReal>(const int k, float* A, const Pf& pf, float (&own)[kRowsW]) __attribute__((always_inline)) {
-#pragma unroll
-        for (int j = 0; j < kRowsW; ++j) {
-            float v;
-            if constexpr (kIn == kHalfInZero) v = a_value.template operator()<kReal>(0.0f, k, 0, o_xflip[j], o_rule[j]);
-            else v = a_value.template operator()<kReal>(pf.own[j], k, y0 + wave + kWaves * j, o_xflip[j], o_rule[j]);
-            own[j] = v;
-            A[(wave + kWaves * j + 2) * AW + lane + 2] = v;
-        }
-#pragma unroll
-        for (int q = 0; q < kAPer; ++q) {
-            float v;
-            if constexpr (kIn == kHalfInZero) v = a_value.template operator()<kReal>(0.0f, k, 0, ex_xflip[q], ex_rule[q]);
-            else v = a_value.template operator()<kReal>(pf.ex[q], k, ex_y[q], ex_xflip[q], ex_rule[q]);
-            *(ex_on[q] ? A + ex_lds[q] : s_sink + lane) = v;
-        }
-    };
-
-    // red half-update of plane s (own slots and the R halo) from the A planes s-1, s, s+1
-    // into R0 (kReal: s is not a z ghost plane)
-    auto red = [&]<bool kReal>(const int s, const float* Am, const float* A0, const float* Ap, float* R0, const PfB& pb) __attribute__((always_inline)) {
-        const bool zg = !kReal && zghost(s);
-        const int p   = (ypar + s) & 1;
-        float q[kRowsW], qh = 0.0f;
-        if (zg) {
-            // ghost plane: the z-face rule of the adjacent (black) old value
-            const LinGhost zr = pick(s < 0, zr_lo, zr_hi);
-#pragma unroll
-            for (int j = 0; j < kRowsW; ++j) q[j] = lin_apply(zr, pick(s < 0, a_next[j], a_prev[j]));
-            if (wave < TT::kRHaloWaves) qh = lin_apply(zr, pick(s < 0, Ap[h_alds], Am[h_alds]));
-        } else {
-            StageDiv<kSolve> dv(ddv);
-            float x[kRowsW], xm_[kRowsW], ym_[kRowsW], xh = 0.0f;
-#pragma unroll
-            for (int j = 0; j < kRowsW; ++j) {
-                const int c       = (wave + kWaves * j + 2) * AW + lane + 2;
-                const float other = A0[c - 1 + 2 * p];
-                const float xm    = p ? a_cur[j] : other;
-                const float xp    = p ? other : a_cur[j];
-                const float ym    = A0[c - AW];
-                float sum         = xm;
-                sum               = sum + xp;
-                sum               = sum + ym;
-                sum               = sum + A0[c + AW];
-                sum               = sum + a_prev[j];
-                sum               = sum + a_next[j];
-                const float2 bp   = pb.own[j];
-                x[j]              = dv.num(p ? bp.y : bp.x, sum, coef);
-                bb_new[j]         = p ? bp.x : bp.y;
-                xm_[j]            = xm;
-                ym_[j]            = ym;
-            }
-            if (tile_ghost) {
-#pragma unroll
-                for (int j = 0; j < kRowsW; ++j)
-                    if (o_kind[j] != kPosReal) x[j] = 0.0f;
-            }
-            float hg = 0.0f;
-            if (wave < TT::kRHaloWaves) {
-                const int c    = h_alds;
-                const int ph   = (h_y + s) & 1;
-                float sum      = A0[c - 1 + ph];
-                sum            = sum + A0[c + ph];
-                sum            = sum + A0[c - AW];
-                sum            = sum + A0[c + AW];
-                sum            = sum + Am[c];
-                sum            = sum + Ap[c];
-                const float xv = dv.num(ph ? pb.h.y : pb.h.x, sum, coef);
-                xh             = h_real ? xv : 0.0f;
-                hg             = lin_apply(h_rule, A0[c + h_goff]);
-            }
-#pragma unroll
-            for (int j = 0; j < kRowsW; ++j) q[j] = dv.quot(x[j]);
-            qh = dv.quot(xh);
-            if (__builtin_expect(dv.bad(), 0)) {
-#pragma unroll
-                for (int j = 0; j < kRowsW; ++j) q[j] = dv.exact(x[j]);
-                qh = dv.exact(xh);
-            }
-            qh = h_real ? qh : hg;
-            if (tile_ghost) {
-                // past the x+ face the ghost is the rule of the adjacent cell (black when
-                // it matters: p = 0), past the y+ face the rule of the cell below
-#pragma unroll
-                for (int j = 0; j < kRowsW; ++j) {
-                    const int kd = o_kind[j];
-                    if (kd != kPosReal) q[j] = kd == kPosXp ? lin_apply(o_rule[j], xm_[j]) : (kd == kPosYp ? lin_apply(o_rule[j], ym_[j]) : 0.0f);
-                }
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < kRowsW; ++j) {
-            r_0[j]                                       = q[j];
-            R0[(wave + kWaves * j + 1) * RW + lane + 1] = q[j];
-        }
-        if (wave < TT::kRHaloWaves) *(tid < TT::kRHalo ? R0 + h_rlds : s_sink + lane) = qh;
-    };
-
-    // black half-update of plane m = s - 1 from R1 = R(m) and the stores of plane m
-    // (kLive: m is a plane of the chunk and not a z face)
-    auto black = [&]<bool kLive>(const int s, const float* R1) __attribute__((always_inline)) {
-        const int m     = s - 1;
-        const bool live = kLive || (m >= z0 && m < z1); // planes of the chunk (never a ghost plane)
-        const int p     = (ypar + m) & 1;
-        StageDiv<kSolve> dv(ddv);
-        float x[kRowsW], q[kRowsW];
-#pragma unroll
-        for (int j = 0; j < kRowsW; ++j) {
-            const int c       = (wave + kWaves * j + 1) * RW + lane + 1;
-            const float other = R1[c + 1 - 2 * p];
-            const float xm    = p ? other : r_m[j];
-            const float xp    = p ? r_m[j] : other;
-            float sum         = xm;
-            sum               = sum + xp;
-            sum               = sum + R1[c - RW];
-            sum               = sum + R1[c + RW];
-            sum               = sum + r_mm[j];
-            sum               = sum + r_0[j];
-            x[j]              = dv.num(bb[j], sum, coef);
-        }
-        if (!live || tile_ghost) {
-#pragma unroll
-            for (int j = 0; j < kRowsW; ++j)
-                if (!live || o_kind[j] != kPosReal) x[j] = 0.0f;
-        }
-#pragma unroll
-        for (int j = 0; j < kRowsW; ++j) q[j] = dv.quot(x[j]);
-        if (__builtin_expect(dv.bad(), 0)) {
-#pragma unroll
-            for (int j = 0; j < kRowsW; ++j) q[j] = dv.exact(x[j]);
-        }
-        if constexpr (kOut == kHalfOutHalf) {
-            const Rsrc r = rsrc_at(p_hout, live ? hfield_b : 0u);
-            const int so = live ? (m + 1) * static_cast<int>(hplane_b) : 0;
-#pragma unroll
-            for (int j = 0; j < kRowsW; ++j) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(q[j]), r, o_st[j], so, 0);
-            // red face cells of plane m -> ghost entries: rows -1 / ny and the x columns
-            // (one slot per lane, waves that hold face cells), the z planes -1 / nz (own
-            // slots). Only stores sit under these branches: a skipped store leaves every
-            // later wait count valid.
-            if (g_any && live) {
-                const float g = lin_apply(g_rule, R1[g_rlds]);
-                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(g), r, g_off, so, 0);
-            }
-            if (!kLive && live && !pz && (m == 0 || m == nz - 1)) {
-                const int zso     = m == 0 ? 0 : (nz + 1) * static_cast<int>(hplane_b);
-                const LinGhost zr = pick(m == 0, zr_lo, zr_hi);
-#pragma unroll
-                for (int j = 0; j < kRowsW; ++j) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(lin_apply(zr, r_m[j])), r, o_st[j], zso, 0);
-            }
-        } else {
-            const Rsrc r  = rsrc_at(p_xout, live ? nfield_b : 0u);
-            const int so  = live ? m * static_cast<int>(nplane_b) : 0;
-            const float f = args.out_scale;
-#pragma unroll
-            for (int j = 0; j < kRowsW; ++j) {
-                // p = 0: the red cell is the even one
-                const float ve = (p ? q[j] : r_m[j]) * f;
-                const float vo = (p ? r_m[j] : q[j]) * f;
-                buf_store2<StoreF32>(r, o_st[j], ve, vo, so);
-            }
-        }
-    };
-
-    // One plane step s, t = s - (z0 - 1): stage A(s+1) (fetched kDepth steps ago into pa)
-    // and refetch pa with A(s+1+kDepth); barrier; red of plane s with b(s) (fetched kDepth
-    // steps ago into pb), refetch pb with b(s+kDepth); black of plane s - 1. Ring slots:
-    // A(s+1) (t+2)&3, A(s) (t+1)&3, A(s-1) t&3, R(s) (t+1)&1, R(s-1) t&1. kSteady: s in
-    // [z0 + 2, z1 - 3], where every load reads a plane the chunk needs (at most plane z1
-    // of A, z1 - 1 of b, so padded plane nz at most), no plane is a z ghost and the black
-    // plane s - 1 is neither z face; the slots are then compile-time (t mod 4 = kT).
-    auto step = [&]<bool kSteady, int kT>(const int s, const int t, Pf& pa, PfB& pb) __attribute__((always_inline)) {
-        const int tt  = kSteady ? kT : t;
-        float* A_next = s_a + ((tt + 2) & 3) * AF;
-        float* A_cur  = s_a + ((tt + 1) & 3) * AF;
-        float* A_prev = s_a + (tt & 3) * AF;
-        float* R_cur  = s_r + ((tt + 1) & 1) * RF;
-        float* R_prev = s_r + (tt & 1) * RF;
-        stage_a.template operator()<kSteady>(s + 1, A_next, pa, a_next);
-        load_a.template operator()<kSteady>(s + 1 + kDepth, pa);
-        __syncthreads();
-        red.template operator()<kSteady>(s, A_prev, A_cur, A_next, R_cur, pb);
-        load_b.template operator()<kSteady>(s + kDepth, pb);
-        black.template operator()<kSteady>(s, R_prev);
-#pragma unroll
-        for (int j = 0; j < kRowsW; ++j) {
-            a_prev[j] = a_cur[j];
-            a_cur[j]  = a_next[j];
-            r_mm[j]   = r_m[j];
-            r_m[j]    = r_0[j];
-            bb[j]     = bb_new[j];
-        }
-    };
-    static_assert(kDepth == 2, "register sets alternate with the step parity");
-
-    Pf pa[2];
-    PfB pb[2];
-    // prologue: A(z0-2), A(z0-1) staged into slots 0, 1; A(z0), A(z0+1) and b(z0-1), b(z0) in flight
-    load_a.template operator()<false>(z0 - 2, pa[0]);
-    load_a.template operator()<false>(z0 - 1, pa[1]);
-    stage_a.template operator()<false>(z0 - 2, s_a, pa[0], a_cur);
-    stage_a.template operator()<false>(z0 - 1, s_a + AF, pa[1], a_next);
-#pragma unroll
-    for (int j = 0; j < kRowsW; ++j) {
-        a_prev[j] = a_cur[j];
-        a_cur[j]  = a_next[j];
-    }
-    load_a.template operator()<false>(z0, pa[0]);
-    load_a.template operator()<false>(z0 + 1, pa[1]);
-    load_b.template operator()<false>(z0 - 1, pb[0]);
-    load_b.template operator()<false>(z0, pb[1]);
-    // steps t = 0 .. n - 1 (s = z0 - 1 .. z1), register set t & 1: general steps t = 0, 1, 2;
-    // steady steps in groups of four, t = 3 .. 2 + 4 g, s = z0 + 2 .. z0 + 1 + 4 g <= z1 - 3;
-    // general steps for the rest (an odd count ends with one idle step, loads and stores off)
-    const int n      = z1 - z0 + 2;
-    const int groups = n >= 10 ? (n - 6) / 4 : 0;
-    step.template operator()<false, 0>(z0 - 1, 0, pa[0], pb[0]);
-    step.template operator()<false, 0>(z0, 1, pa[1], pb[1]);
-    if (n > 2) step.template operator()<false, 0>(z0 + 1, 2, pa[0], pb[0]);
-    for (int g = 0; g < groups; ++g) {
-        const int s = z0 + 2 + 4 * g; // t = 3 + 4 g
-        step.template operator()<true, 3>(s, 0, pa[1], pb[1]);
-        step.template operator()<true, 0>(s + 1, 0, pa[0], pb[0]);
-        step.template operator()<true, 1>(s + 2, 0, pa[1], pb[1]);
-        step.template operator()<true, 2>(s + 3, 0, pa[0], pb[0]);
-    }
-    for (int t = 3 + 4 * groups; t < n; t += 2) {
-        step.template operator()<false, 0>(z0 - 1 + t, t, pa[1], pb[1]);
-        step.template operator()<false, 0>(z0 + t, t + 1, pa[0], pb[0]);
-    }
-}
-
-} // namespace
 
 // ---- host side ----
 
@@ -614,15 +29,29 @@ uint64_t half_buffer_floats(const Dims d) {
     return static_cast<uint64_t>(L.ps) * static_cast<uint64_t>(d.nz + 2);
 }
 
+namespace {
+struct HalfShapeInfo {
+    int pairs, waves, rows;
+};
+// pairs per tile row, waves, tile rows (kTXP, kWaves, kWaves * 2 * 64 / kTXP)
+constexpr HalfShapeInfo kHalfShapes[kHalfTileCount] = {{64, 8, 16}, {32, 8, 32}, {32, 4, 16}, {16, 4, 32}, {16, 2, 16}};
+HalfShapeInfo shape_of(const int tile) { return kHalfShapes[tile >= 0 && tile < kHalfTileCount ? tile : 0]; }
+} // namespace
+
+int half_tile_rows(const int tile) { return shape_of(tile).rows; }
+int half_tile_threads(const int tile) { return 64 * shape_of(tile).waves; }
+int half_tile_cells(const int tile) { return 2 * shape_of(tile).pairs; }
+
 TileGeom half_plan(const Dims d, const int tile, const int zc_override) {
+    const HalfShapeInfo sh = shape_of(tile);
     TileGeom g{};
     const int nxh    = d.nx / 2;
-    const int ty     = half_tile_rows(tile);
-    g.tiles_x        = (nxh + kHP - 1) / kHP;
-    g.tiles_y        = (d.ny + ty - 1) / ty;
+    g.tiles_x        = (nxh + sh.pairs - 1) / sh.pairs;
+    g.tiles_y        = (d.ny + sh.rows - 1) / sh.rows;
     const long tiles = static_cast<long>(g.tiles_x) * g.tiles_y;
-    // about 512 workgroups of 512 threads (1024 of 256), even chunk lengths of at least 4 planes
-    const long target = half_tile_threads(tile) == 256 ? 1024 : 512;
+    // about 4096 waves (16 per CU: two 512-thread workgroups), even chunk lengths of at
+    // least 4 planes
+    const long target = 4096 / sh.waves;
     long zc = (static_cast<long>(d.nz) * tiles + target - 1) / target;
     zc      = std::max<long>(4, std::min<long>(64, zc + (zc & 1)));
     if (zc_override > 0) zc = zc_override;
@@ -632,37 +61,11 @@ TileGeom half_plan(const Dims d, const int tile, const int zc_override) {
     return g;
 }
 
-int half_tile_rows(const int tile) { return tile == kHalfTile8x4 ? 32 : 16; }
-int half_tile_threads(const int tile) { return tile == kHalfTile4x4 ? 256 : 512; }
-
 hipError_t launch_rbgs_half(hipStream_t s, const SolveKind solve, const int in_mode, const int out_mode, const HalfPassArgs& a_in, const int tile) {
     HalfPassArgs a        = a_in;
     a.L                   = half_layout(a.d);
     const unsigned blocks = smooth_blocks(a.g);
-#define SF_H(SO, IN, OUT, W, RW) k_rbgs_half<SO, IN, OUT, W, RW, kHalfDepth><<<blocks, 64 * W, 0, s>>>(a)
-#define SF_H_T(SO, IN, OUT)                                    \
-    do {                                                       \
-        if (tile == kHalfTile4x4) SF_H(SO, IN, OUT, 4, 4);     \
-        else if (tile == kHalfTile8x4) SF_H(SO, IN, OUT, 8, 4); \
-        else SF_H(SO, IN, OUT, 8, 2);                          \
-    } while (0)
-#define SF_H_OUT(SO, IN)                                            \
-    do {                                                            \
-        if (out_mode == kHalfOutHalf) SF_H_T(SO, IN, kHalfOutHalf); \
-        else SF_H_T(SO, IN, kHalfOutNatural);                       \
-    } while (0)
-#define SF_H_IN(SO)                                                       \
-    do {                                                                  \
-        if (in_mode == kHalfInZero) SF_H_OUT(SO, kHalfInZero);            \
-        else if (in_mode == kHalfInNatural) SF_H_OUT(SO, kHalfInNatural); \
-        else SF_H_OUT(SO, kHalfInHalf);                                   \
-    } while (0)
-    if (solve == kSolvePoisson) SF_H_IN(kSolvePoisson);
-    else SF_H_IN(kSolveDiffusion);
-#undef SF_H_IN
-#undef SF_H_OUT
-#undef SF_H_T
-#undef SF_H
+    half_dispatch(solve, in_mode, out_mode, tile, [&]<int SO, int IN, int OUT, int TXP, int W, int RW>() { k_rbgs_half<SO, IN, OUT, TXP, W, RW><<<blocks, 64 * W, 0, s>>>(a); });
     return hipGetLastError();
 }
 

@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from conftest import sf, sf_oracle
-from paritylib import Inputs, normwise, report, run_gpu, run_oracle
+from paritylib import Inputs, gpu_fields, normwise, report, run_gpu, run_oracle
 
 abi = sf.abi
 scenes = sf.scenes
@@ -183,6 +183,40 @@ def test_tile_shapes_bitexact(case, smoother):
     ref_kind = sf_oracle.JACOBI if smoother == "jacobi" else sf_oracle.RBGS
     gpu = run_gpu(desc, inputs.seed_gpu, 3, options=sf.ext_options(smoother=kind))
     assert_bitexact(gpu, run_oracle(desc, inputs.seed_oracle, 3, 32, smoother=ref_kind), f"{case} {smoother} 3 steps vs fp32")
+
+
+# (case, tile shape, z chunk override): each black-cell tile width and height, a chunk
+# override that splits nz unevenly, and tiles past the grid on both axes
+TILING_CASES = [
+    ("walls_multi_tile_chunks", abi.TILE_128X16, 0),
+    ("walls_multi_tile_chunks", abi.TILE_64X32, 6),
+    ("walls_multi_tile_chunks", abi.TILE_64X16, 4),
+    ("walls_multi_tile_chunks", abi.TILE_32X32, 10),
+    ("walls_multi_tile_chunks", abi.TILE_32X16, 0),
+    ("periodic_yz_partial", abi.TILE_64X32, 0),
+    ("periodic_yz_partial", abi.TILE_32X16, 6),
+    ("mixed_two_cells", abi.TILE_32X16, 0),
+    ("periodic_partial_tile", abi.TILE_32X32, 4),
+]
+
+
+@pytest.mark.parametrize("case,tile,zc", TILING_CASES)
+def test_smoother_tilings_bitexact(case, tile, zc):
+    """Every black-cell tile shape steps bit-exactly against the FP32 oracle."""
+    make, n = TILE_CASES[case]
+    desc = make(n=n)
+    inputs = Inputs(desc, seed=zlib.crc32(case.encode()) % 1000, velocity_scale=0.5)
+    ctx = sf.Context(desc)
+    try:
+        ctx.set_smoother_tiling(tile, zc)
+        got_tile, got_zc = ctx.smoother_tiling()
+        assert got_tile == tile and (got_zc == min(zc, n[2]) if zc else got_zc > 0)
+        inputs.seed_gpu(ctx)
+        ctx.step(3)
+        gpu = gpu_fields(ctx, desc)
+    finally:
+        ctx.destroy()
+    assert_bitexact(gpu, run_oracle(desc, inputs.seed_oracle, 3, 32), f"{case} tile {tile} zc {zc} 3 steps vs fp32")
 
 
 def test_plume256_noise_one_step_bitexact():
