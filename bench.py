@@ -290,31 +290,11 @@ def precision_study(pkg, args):
                       "runs": runs}), flush=True)
 
 
-def batch_scenes(pkg, count=64, seed=0, d_iters=40, p_iters=80):
-    """64 plume simulations: sizes drawn from {32^3, 48^3, 64^3, 96^3, 128^3, 128x64x64},
-    random centre, radius U[2, 8] cells, upward force U[1, 10]; free-slip box, h = 1/max(n),
-    dt = 0.1, nu = kappa = 1e-4."""
-    import numpy as np
-    scenes = pkg.scenes
-    rng = np.random.default_rng(seed)
-    sizes = [(32, 32, 32), (48, 48, 48), (64, 64, 64), (96, 96, 96), (128, 128, 128), (128, 64, 64)]
-    out = []
-    for i in range(count):
-        nx, ny, nz = sizes[int(rng.integers(0, len(sizes)))]
-        desc = scenes.plume_desc(max(nx, ny, nz), d_iters, p_iters, shape=(nx, ny, nz))
-        radius = float(rng.uniform(2.0, 8.0))
-        centre = (int(rng.integers(int(radius) + 1, nx - int(radius) - 1)), int(rng.integers(int(radius) + 1, max(int(radius) + 2, ny // 4))),
-                  int(rng.integers(int(radius) + 1, nz - int(radius) - 1)))
-        src, fx, fy, fz = scenes.plume_fields(desc.shape, centre, radius, 100.0, float(rng.uniform(1.0, 10.0)))
-        out.append(scenes.Scene(f"batch{i}", desc, src, (fx, fy, fz), None))
-    return out
-
-
 def batch_study(pkg, args):
     """Aggregate throughput of 64 uneven simulations: one-by-one vs one batched launch per stage."""
     hip = pkg.hip
     steps = args.steps if args.steps != 200 else 10
-    scenes_ = batch_scenes(pkg)
+    scenes_ = pkg.scenes.batch_plume_scenes()
     cells = sum(sc.desc.cells for sc in scenes_)
 
     def run(make_stream, batched, tune=False):

@@ -176,6 +176,15 @@ typedef enum StableFluidsExtTile {
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_set_smoother_tiling_cuda(void* context, uint32_t tile, uint32_t z_chunk);
 /* The tile shape and the z-chunk length in use (either pointer may be NULL). */
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_get_smoother_tiling_cuda(void* context, uint32_t* out_tile, uint32_t* out_z_chunk);
+/* Tiling autotuner for one context, in place of the reference's occupancy-derived block
+ * size (backend-cuda.cu:490-515): times `repeats` RBGS pressure passes (writing only
+ * scratch buffers) for every tile shape with z chunks {default, 4, 8, 16, 32} and keeps
+ * the fastest. A shape found in cache_path is taken from it, and the file is rewritten
+ * with every entry (cache_path may be NULL; the format is shared with the batch
+ * autotuner, single contexts under members 0). Rebuilds the step graph (synchronous);
+ * a context whose pressure solve does not run the black-cell passes is left as it is.
+ * out_tile / out_z_chunk (either may be NULL) = the tiling in use afterwards. */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_autotune_cuda(void* context, const char* cache_path, uint32_t repeats, uint32_t* out_tile, uint32_t* out_z_chunk);
 
 /* Returns the context's HIP stream (hipStream_t as void*). */
 STABLE_FLUIDS_API void* stable_fluids_ext_get_stream_cuda(void* context);
@@ -195,16 +204,29 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_destroy_cuda(void* 
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_step_cuda(void* batch);
 STABLE_FLUIDS_API void* stable_fluids_ext_batch_get_stream_cuda(void* batch);
 /* Smoother z-chunk length of one context in the batch (0 = default plan); rebuilds the
- * batch graph. Tiling never changes results. */
+ * batch graph. Tiling never changes results. A batch starts from each context's own
+ * smoother tiling. */
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_set_z_chunk_cuda(void* batch, uint32_t sim, uint32_t z_chunk);
-/* out_tiling4 = {tiles_x, tiles_y, z chunks, z chunk length} of context `sim`;
- * out_launches_per_step = kernel launches in one batch step. Either may be NULL. */
+/* Tile shape (StableFluidsExtTile; used by the black-cell passes of even-nx contexts) and
+ * z-chunk length of one context in the batch; rebuilds the batch graph. */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_set_tiling_cuda(void* batch, uint32_t sim, uint32_t tile, uint32_t z_chunk);
+/* out_tiling4 = {tiles_x, tiles_y, z chunks, z chunk length} of context `sim`'s pressure
+ * passes; out_launches_per_step = kernel launches in one batch step. Either may be NULL. */
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_info_cuda(void* batch, uint32_t sim, int32_t* out_tiling4, uint32_t* out_launches_per_step);
-/* Autotuner: for every grid shape in the batch, times `repeats` batched pressure passes
- * (writing only scratch buffers) for each z-chunk length in {default, 2, 4, ..., 64} and
- * keeps the fastest; shapes listed in cache_path (lines "nx ny nz smoother zc") are
- * taken from it, and the file is rewritten with all shapes (cache_path may be NULL).
- * Rebuilds the batch graph; *out_tuned_shapes = shapes timed (not from the cache). */
+/* Tile shape of context `sim` in the batch. */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_get_tile_cuda(void* batch, uint32_t sim, uint32_t* out_tile);
+/* Autotuner, in place of the reference's occupancy-derived block size
+ * (backend-cuda.cu:490-515). For every grid shape in the batch it times `repeats`
+ * batched pressure passes over that shape's contexts alone (writing only scratch
+ * buffers) for each candidate tiling -- every tile shape with z chunks {default, 4, 8,
+ * 16, 32} for black-cell passes, z chunks {default, 2, 4, ..., 64} otherwise -- taking
+ * the minimum of three interleaved rounds. It then times whole-batch passes under: each
+ * shape on its fastest tiling; all shapes on one tile shape (one launch per pass), each
+ * on its fastest chunk for it, for every tile shape; the starting tiling; and keeps the
+ * fastest. Shapes listed in cache_path (a line "stable-fluids-tiling 2", then lines "nx
+ * ny nz smoother members tile zc") are taken from it, and the file is rewritten with all
+ * entries (cache_path may be NULL). Rebuilds the batch graph; *out_tuned_shapes =
+ * shapes timed (not from the cache). */
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_autotune_cuda(void* batch, const char* cache_path, uint32_t repeats, uint32_t* out_tuned_shapes);
 
 /* Device array [nz][ny][nx] FP32 of the last step's pre-projection divergence (the

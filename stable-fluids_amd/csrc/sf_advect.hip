@@ -221,10 +221,9 @@ __device__ __forceinline__ Grid grid_of(const Dims d, const float h, const float
 // Blocks run on a 1D grid renumbered per XCD: workgroups are dealt round-robin over the 8
 // XCDs, so XCD k receives the contiguous block range [k n / 8, (k + 1) n / 8) in x, y, z
 // order, and the tap halos shared by neighbouring blocks meet in the same L2.
-__device__ __forceinline__ bool cell_of(const Dims d, int& x, int& y, int& z) {
-    const int n    = static_cast<int>(gridDim.x);
-    const int w    = static_cast<int>(blockIdx.x);
-    const int b    = (n & 7) == 0 ? (w & 7) * (n >> 3) + (w >> 3) : w;
+// Cell of this thread in 16 x 16 block b of d (blocks in x, y, z order; each wave a 16 x 4
+// patch); false past the x or y extent.
+__device__ __forceinline__ bool cell_in_block(const Dims d, const int b, int& x, int& y, int& z) {
     const int bx_n = (d.nx + kAX - 1) / kAX, by_n = (d.ny + kAY - 1) / kAY;
     const int bx = b % bx_n, by = (b / bx_n) % by_n;
     const int lane = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
@@ -234,15 +233,16 @@ __device__ __forceinline__ bool cell_of(const Dims d, int& x, int& y, int& z) {
     return x < d.nx && y < d.ny;
 }
 
+__device__ __forceinline__ bool cell_of(const Dims d, int& x, int& y, int& z) {
+    const int n = static_cast<int>(gridDim.x);
+    const int w = static_cast<int>(blockIdx.x);
+    return cell_in_block(d, (n & 7) == 0 ? (w & 7) * (n >> 3) + (w >> 3) : w, x, y, z);
+}
+
 // One midpoint trace per cell, three component samples (reference
 // advect_velocity_component_kernel backend-cuda.cu:330-340, launched per component).
 template <bool kPow2>
-__global__ __launch_bounds__(kAThreads) void k_advect_velocity_b(float* __restrict__ tx, float* __restrict__ ty, float* __restrict__ tz, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float dt, const Dims d, const float h, const float inv_h, const StableFluidsFlowBoundaryConfig bc) {
-    __shared__ float2 rules[3][8];
-    fill_velocity_rules(rules, bc, static_cast<int>(threadIdx.x));
-    __syncthreads();
-    int x, y, z;
-    if (!cell_of(d, x, y, z)) return;
+__device__ __forceinline__ void advect_velocity_cell(float* __restrict__ tx, float* __restrict__ ty, float* __restrict__ tz, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float dt, const Dims d, const float h, const float inv_h, const StableFluidsFlowBoundaryConfig& bc, const float2 (*rules)[8], const int x, const int y, const int z) {
     const uint32_t bytes = 4u * static_cast<uint32_t>(d.nx) * static_cast<uint32_t>(d.ny) * static_cast<uint32_t>(d.nz);
     const VelocityField v{{plane_rsrc(vx, bytes), plane_rsrc(vy, bytes), plane_rsrc(vz, bytes)}};
     const Grid g         = grid_of(d, h, inv_h);
@@ -255,6 +255,30 @@ __global__ __launch_bounds__(kAThreads) void k_advect_velocity_b(float* __restri
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(u[0]), plane_rsrc(tx, bytes), cell_off, 0, 0);
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(u[1]), plane_rsrc(ty, bytes), cell_off, 0, 0);
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(u[2]), plane_rsrc(tz, bytes), cell_off, 0, 0);
+}
+
+template <bool kPow2>
+__global__ __launch_bounds__(kAThreads) void k_advect_velocity_b(float* __restrict__ tx, float* __restrict__ ty, float* __restrict__ tz, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float dt, const Dims d, const float h, const float inv_h, const StableFluidsFlowBoundaryConfig bc) {
+    __shared__ float2 rules[3][8];
+    fill_velocity_rules(rules, bc, static_cast<int>(threadIdx.x));
+    __syncthreads();
+    int x, y, z;
+    if (!cell_of(d, x, y, z)) return;
+    advect_velocity_cell<kPow2>(tx, ty, tz, vx, vy, vz, dt, d, h, inv_h, bc, rules, x, y, z);
+}
+
+// batched: workgroup i runs 16 x 16 block items[i].block of descriptor items[i].desc
+__global__ __launch_bounds__(kAThreads) void k_advect_velocity_bb(const AdvectVelocityDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
+    __shared__ float2 rules[3][8];
+    const BatchItem it = items[blockIdx.x];
+    if (it.desc == kIdleItem) return;
+    const AdvectVelocityDesc& a = descs[it.desc];
+    fill_velocity_rules(rules, a.bc, static_cast<int>(threadIdx.x));
+    __syncthreads();
+    int x, y, z;
+    if (!cell_in_block(a.d, static_cast<int>(it.block), x, y, z)) return;
+    if (a.pow2) advect_velocity_cell<true>(a.t[0], a.t[1], a.t[2], a.v[0], a.v[1], a.v[2], a.dt, a.d, a.h, a.inv_h, a.bc, rules, x, y, z);
+    else advect_velocity_cell<false>(a.t[0], a.t[1], a.t[2], a.v[0], a.v[1], a.v[2], a.dt, a.d, a.h, a.inv_h, a.bc, rules, x, y, z);
 }
 
 // Scalar face data of one axis (reference tap_scalar, backend-cuda.cu:62-94): a
@@ -278,12 +302,7 @@ __device__ __forceinline__ ScalarAxis scalar_axis(const StableFluidsScalarBounda
 
 // reference advect_scalar_component_kernel backend-cuda.cu:342-352
 template <bool kPow2>
-__global__ __launch_bounds__(kAThreads) void k_advect_scalar_b(float* __restrict__ out, const float* __restrict__ src, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float dt, const Dims d, const float h, const float inv_h, const StableFluidsScalarBoundaryConfig sbc, const StableFluidsFlowBoundaryConfig fbc) {
-    __shared__ float2 rules[3][8];
-    fill_velocity_rules(rules, fbc, static_cast<int>(threadIdx.x));
-    __syncthreads();
-    int x, y, z;
-    if (!cell_of(d, x, y, z)) return;
+__device__ __forceinline__ void advect_scalar_cell(float* __restrict__ out, const float* __restrict__ src, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float dt, const Dims d, const float h, const float inv_h, const StableFluidsScalarBoundaryConfig& sbc, const StableFluidsFlowBoundaryConfig& fbc, const float2 (*rules)[8], const int x, const int y, const int z) {
     const uint32_t bytes = 4u * static_cast<uint32_t>(d.nx) * static_cast<uint32_t>(d.ny) * static_cast<uint32_t>(d.nz);
     const VelocityField v{{plane_rsrc(vx, bytes), plane_rsrc(vy, bytes), plane_rsrc(vz, bytes)}};
     const Rsrc s         = plane_rsrc(src, bytes);
@@ -327,6 +346,29 @@ __global__ __launch_bounds__(kAThreads) void k_advect_scalar_b(float* __restrict
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(lerp8(c, p)), plane_rsrc(out, bytes), cell_off, 0, 0);
 }
 
+template <bool kPow2>
+__global__ __launch_bounds__(kAThreads) void k_advect_scalar_b(float* __restrict__ out, const float* __restrict__ src, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float dt, const Dims d, const float h, const float inv_h, const StableFluidsScalarBoundaryConfig sbc, const StableFluidsFlowBoundaryConfig fbc) {
+    __shared__ float2 rules[3][8];
+    fill_velocity_rules(rules, fbc, static_cast<int>(threadIdx.x));
+    __syncthreads();
+    int x, y, z;
+    if (!cell_of(d, x, y, z)) return;
+    advect_scalar_cell<kPow2>(out, src, vx, vy, vz, dt, d, h, inv_h, sbc, fbc, rules, x, y, z);
+}
+
+__global__ __launch_bounds__(kAThreads) void k_advect_scalar_bb(const AdvectScalarDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
+    __shared__ float2 rules[3][8];
+    const BatchItem it = items[blockIdx.x];
+    if (it.desc == kIdleItem) return;
+    const AdvectScalarDesc& a = descs[it.desc];
+    fill_velocity_rules(rules, a.fbc, static_cast<int>(threadIdx.x));
+    __syncthreads();
+    int x, y, z;
+    if (!cell_in_block(a.d, static_cast<int>(it.block), x, y, z)) return;
+    if (a.pow2) advect_scalar_cell<true>(a.out, a.src, a.v[0], a.v[1], a.v[2], a.dt, a.d, a.h, a.inv_h, a.sbc, a.fbc, rules, x, y, z);
+    else advect_scalar_cell<false>(a.out, a.src, a.v[0], a.v[1], a.v[2], a.dt, a.d, a.h, a.inv_h, a.sbc, a.fbc, rules, x, y, z);
+}
+
 bool pow2_spacing(const float h) {
     int e;
     return h > 0.0f && std::isfinite(h) && std::frexp(h, &e) == 0.5f;
@@ -335,6 +377,20 @@ bool pow2_spacing(const float h) {
 inline unsigned adv_grid(const Dims d) { return static_cast<unsigned>(((d.nx + kAX - 1) / kAX) * ((d.ny + kAY - 1) / kAY) * d.nz); }
 
 } // namespace
+
+bool advect_pow2(const float h) { return pow2_spacing(h); }
+uint32_t advect_blocks(const Dims d) { return adv_grid(d); }
+
+hipError_t launch_advect_velocity_batched(hipStream_t s, const AdvectVelocityDesc* descs, const BatchItem* items, const uint32_t n_items) {
+    if (n_items == 0) return hipSuccess;
+    k_advect_velocity_bb<<<n_items, kAThreads, 0, s>>>(descs, items);
+    return hipGetLastError();
+}
+hipError_t launch_advect_scalar_batched(hipStream_t s, const AdvectScalarDesc* descs, const BatchItem* items, const uint32_t n_items) {
+    if (n_items == 0) return hipSuccess;
+    k_advect_scalar_bb<<<n_items, kAThreads, 0, s>>>(descs, items);
+    return hipGetLastError();
+}
 
 bool advect_buffers_fit(const Dims d) { return 4ull * static_cast<uint64_t>(d.nx) * static_cast<uint64_t>(d.ny) * static_cast<uint64_t>(d.nz) < (1ull << 31); }
 

@@ -16,6 +16,7 @@
 #include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -741,22 +742,33 @@ hipError_t norm_scratch(const int device, double** out) {
 // launched once for all simulations: workgroup i runs block items[i].block of the
 // stage descriptor items[i].desc (a simulation, or a simulation and velocity
 // component). The arithmetic of each block is the single-simulation kernel's, so
-// every simulation ends bit-identical to stepping it alone. The smoother's z-chunk
-// length per grid shape is the only tuned parameter (plan_tiles); it changes which
-// workgroup computes a cell, never the value.
+// every simulation ends bit-identical to stepping it alone. RBGS solves of simulations
+// with even nx run on the black-cell passes (one launch per tile shape in the batch),
+// the rest on the full-iterate passes (one launch per nx parity). The tile shape and
+// z-chunk length per simulation are the tuned parameters (half_plan, plan_tiles); they
+// change which workgroup computes a cell, never the value.
 
 struct Batch {
     std::vector<Context*> sims;
     hipStream_t stream = nullptr;
     bool owns_stream   = false;
-    std::vector<int> zc; // smoother z-chunk per simulation (0 = default)
+    std::vector<int> zc;   // smoother z-chunk per simulation (0 = default)
+    std::vector<int> tile; // black-cell tile shape per simulation
+    // black-cell iterates of velocity components 1 and 2 (component 0, the pressure and
+    // the scalars use the context's own pair): all three components advance together
+    std::vector<std::array<float*, 4>> hv;
+    std::vector<void*> owned; // hv, freed with the batch
     hipGraph_t graph         = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::vector<void*> allocations;
     std::vector<std::function<hipError_t()>> plan; // the step's launches, in order
     int launches = 0;
 
-    ~Batch() { release(); if (owns_stream && stream) (void)hipStreamDestroy(stream); }
+    ~Batch() {
+        release();
+        for (void* p : owned) (void)hipFree(p);
+        if (owns_stream && stream) (void)hipStreamDestroy(stream);
+    }
     void release() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -791,7 +803,7 @@ bool batch_supported(const Context& c) {
     if (c.options.refine || c.options.learned) return false;
     for (const auto& v : c.vectors)
         if (v.animated) return false;
-    if (cfg.diffuse_iterations <= 0 || cfg.pressure_iterations <= 0) return false;
+    if (cfg.diffuse_iterations <= 0 || cfg.pressure_iterations <= 0 || !advect_buffers_fit(c.dims)) return false;
     if (!(cfg.dt * cfg.viscosity / (cfg.cell_size * cfg.cell_size) > 0.0f)) return false;
     for (const auto& s : c.scalars)
         if (!(cfg.dt * s.desc.diffusion / (cfg.cell_size * cfg.cell_size) > 0.0f)) return false;
@@ -810,6 +822,20 @@ std::vector<BatchItem> batch_items(const size_t n_desc, F blocks) {
     std::vector<BatchItem> items;
     for (size_t d = 0; d < n_desc; ++d)
         for (uint32_t b = 0, n = blocks(d); b < n; ++b) items.push_back(BatchItem{static_cast<uint32_t>(d), b});
+    return items;
+}
+
+// A work list laid out for the round-robin dispatch of workgroups over the 8 XCDs:
+// position w holds item (w & 7) * n / 8 + w / 8 of the list padded to a multiple n of 8
+// with idle items, so each XCD walks a contiguous run of neighbouring blocks through its
+// own L2 (the kernels taking it return on kIdleItem).
+std::vector<BatchItem> xcd_order(const std::vector<BatchItem>& logical) {
+    const size_t n8 = (logical.size() + 7) / 8 * 8, per = n8 / 8;
+    std::vector<BatchItem> items(n8, BatchItem{kIdleItem, 0});
+    for (size_t w = 0; w < n8; ++w) {
+        const size_t j = (w & 7) * per + (w >> 3);
+        if (j < logical.size()) items[w] = logical[j];
+    }
     return items;
 }
 
@@ -846,6 +872,92 @@ hipError_t launch_smooth_lists(hipStream_t s, const SmootherKind sm, const Solve
     return hipSuccess;
 }
 
+// Whether simulation i runs its RBGS solves under rules bc (velocity component comp, or
+// -1 for a pressure or scalar solve) on the black-cell passes.
+bool batch_half(const Batch& B, const size_t i, const StencilBC& bc, const int comp) {
+    const Context& c = *B.sims[i];
+    if (c.half[0] == nullptr || c.options.smoother != kSmoothRBGS || !rbgs_half_supported(c.dims, bc)) return false;
+    return comp <= 0 || B.hv[i][0] != nullptr;
+}
+
+// Half-buffer pair of simulation i for velocity component comp (-1: pressure, scalars).
+std::array<float*, 2> batch_half_pair(const Batch& B, const size_t i, const int comp) {
+    if (comp <= 0) return {B.sims[i]->half[0], B.sims[i]->half[1]};
+    return {B.hv[i][2 * (comp - 1)], B.hv[i][2 * (comp - 1) + 1]};
+}
+
+TileGeom batch_half_plan(const Batch& B, const size_t i) { return half_plan(B.sims[i]->dims, B.tile[i], B.zc[i]); }
+
+// Work lists of one solve family: descriptor d is simulation sim_of(d)'s; it runs the
+// black-cell passes when half_of(d), one list per tile shape, otherwise the full-iterate
+// passes, one list per nx parity. A black-cell list orders its descriptors by z-chunk
+// length, longest first (they finish last), in XCD order (xcd_order).
+struct PassLists {
+    const BatchItem* half[kHalfTileCount] = {};
+    uint32_t n_half[kHalfTileCount]       = {};
+    SmoothLists full;
+    int launches() const {
+        int n = full.launches();
+        for (const uint32_t k : n_half) n += k > 0;
+        return n;
+    }
+};
+
+// only: when given, the simulations to include (the others get no items).
+template <class SimOf, class HalfOf>
+bool pass_lists(Batch& B, const size_t n_desc, SimOf sim_of, HalfOf half_of, PassLists* out, const std::vector<char>* only = nullptr) {
+    auto& S      = B.sims;
+    auto include = [&](size_t d) { return only == nullptr || (*only)[sim_of(d)] != 0; };
+    if (!smooth_lists(B, n_desc, [&](size_t d) { return S[sim_of(d)]->dims.nx; }, [&](size_t d) { return (!include(d) || half_of(d)) ? 0u : smooth_blocks(plan_tiles(S[sim_of(d)]->dims, B.zc[sim_of(d)])); }, &out->full)) return false;
+    for (int t = 0; t < kHalfTileCount; ++t) {
+        std::vector<size_t> members;
+        for (size_t d = 0; d < n_desc; ++d)
+            if (include(d) && half_of(d) && B.tile[sim_of(d)] == t) members.push_back(d);
+        std::stable_sort(members.begin(), members.end(), [&](size_t a, size_t b) { return batch_half_plan(B, sim_of(a)).zc > batch_half_plan(B, sim_of(b)).zc; });
+        std::vector<BatchItem> logical;
+        for (const size_t d : members)
+            for (uint32_t b = 0, n = smooth_blocks(batch_half_plan(B, sim_of(d))); b < n; ++b) logical.push_back(BatchItem{static_cast<uint32_t>(d), b});
+        if (logical.empty()) continue;
+        const auto items = xcd_order(logical);
+        out->half[t]     = B.upload(items);
+        out->n_half[t]   = static_cast<uint32_t>(items.size());
+        if (!out->half[t]) return false;
+    }
+    return true;
+}
+
+// Pass `pass` (1..passes) of a batched solve: the black-cell lists read descriptors hd,
+// the full-iterate lists od; the first pass starts from zero when zero_first.
+hipError_t launch_pass_lists(hipStream_t s, const SmootherKind sm, const SolveKind solve, const int pass, const int passes, const bool zero_first, const HalfPassArgs* hd, const SmoothDesc* od, const PassLists& L) {
+    const int in  = pass > 1 ? kHalfInHalf : (zero_first ? kHalfInZero : kHalfInNatural);
+    const int out = pass == passes ? kHalfOutNatural : kHalfOutHalf;
+    for (int t = 0; t < kHalfTileCount; ++t)
+        if (L.n_half[t] > 0) {
+            const hipError_t e = launch_rbgs_half_batched(s, solve, in, out, t, hd, L.half[t], L.n_half[t]);
+            if (e != hipSuccess) return e;
+        }
+    return launch_smooth_lists(s, sm, solve, zero_first && pass == 1, od, L.full);
+}
+
+// Descriptor of pass `pass` (1..passes) of simulation i's black-cell solve.
+HalfPassArgs batch_half_desc(const Batch& B, const size_t i, const int comp, const int pass, const int passes, const float* b, const float* x0, float* out, const float coef, const float last_scale, const StencilBC& bc) {
+    const Context& c = *B.sims[i];
+    const auto h     = batch_half_pair(B, i, comp);
+    HalfPassArgs a{};
+    a.x_nat     = x0;
+    a.b         = b;
+    a.h_in      = h[pass & 1]; // pass l = pass - 1 writes h[l & 1] and reads h[(l + 1) & 1]
+    a.h_out     = h[(pass - 1) & 1];
+    a.x_out     = out;
+    a.coef      = coef;
+    a.out_scale = pass == passes ? last_scale : 1.0f;
+    a.d         = c.dims;
+    a.bc        = bc;
+    a.g         = batch_half_plan(B, i);
+    a.L         = half_layout(c.dims);
+    return a;
+}
+
 #define SF_BTRY(expr)                                           \
     do {                                                        \
         if ((expr) != hipSuccess) return hipErrorUnknown;       \
@@ -865,14 +977,26 @@ hipError_t plan_batch_step(Batch& B) {
     std::vector<TileGeom> geo(N);
     for (size_t i = 0; i < N; ++i) geo[i] = plan_tiles(S[i]->dims, B.zc[i]);
 
-    const auto elem_items = B.upload(batch_items(N, [&](size_t d) { return elem_block_count(S[d]->cells); }));
-    const auto tile_items = B.upload(batch_items(N, [&](size_t d) { return tile_blocks(S[d]->dims); }));
-    const uint32_t n_elem = static_cast<uint32_t>(batch_items(N, [&](size_t d) { return elem_block_count(S[d]->cells); }).size());
-    const uint32_t n_tile = static_cast<uint32_t>(batch_items(N, [&](size_t d) { return tile_blocks(S[d]->dims); }).size());
-    SmoothLists sm1, sm3;
-    if (!elem_items || !tile_items) return hipErrorOutOfMemory;
-    if (!smooth_lists(B, N, [&](size_t d) { return S[d]->dims.nx; }, [&](size_t d) { return smooth_blocks(geo[d]); }, &sm1)) return hipErrorOutOfMemory;
-    if (!smooth_lists(B, 3 * N, [&](size_t d) { return S[d / 3]->dims.nx; }, [&](size_t d) { return smooth_blocks(geo[d / 3]); }, &sm3)) return hipErrorOutOfMemory;
+    const auto elem_list  = batch_items(N, [&](size_t d) { return elem_block_count(S[d]->cells); });
+    const auto tile_list  = xcd_order(batch_items(N, [&](size_t d) { return tile_blocks(S[d]->dims); }));
+    const auto adv_list   = xcd_order(batch_items(N, [&](size_t d) { return advect_blocks(S[d]->dims); }));
+    const auto elem_items = B.upload(elem_list);
+    const auto tile_items = B.upload(tile_list);
+    const auto adv_items  = B.upload(adv_list);
+    const auto n_elem     = static_cast<uint32_t>(elem_list.size());
+    const auto n_tile     = static_cast<uint32_t>(tile_list.size());
+    const auto n_adv      = static_cast<uint32_t>(adv_list.size());
+    if (!elem_items || !tile_items || !adv_items) return hipErrorOutOfMemory;
+    // solve families: velocity (descriptor 3 i + comp), pressure, and each scalar
+    auto vel_bc  = [&](size_t d) { return velocity_rules(S[d / 3]->config.flow_boundary, static_cast<int>(d % 3)); };
+    auto vel_half = [&](size_t d) { return batch_half(B, d / 3, vel_bc(d), static_cast<int>(d % 3)); };
+    auto p_half   = [&](size_t d) { return batch_half(B, d, pressure_rules(S[d]->config.flow_boundary), -1); };
+    PassLists lv, lp;
+    if (!pass_lists(B, 3 * N, [](size_t d) { return d / 3; }, vel_half, &lv)) return hipErrorOutOfMemory;
+    if (!pass_lists(B, N, [](size_t d) { return d; }, p_half, &lp)) return hipErrorOutOfMemory;
+    std::vector<PassLists> ls(c0.scalars.size());
+    for (size_t si = 0; si < ls.size(); ++si)
+        if (!pass_lists(B, N, [](size_t d) { return d; }, [&](size_t d) { return batch_half(B, d, scalar_rules(S[d]->scalars[si].desc.boundary), -1); }, &ls[si])) return hipErrorOutOfMemory;
     hipStream_t st = B.stream;
     auto add       = [&](std::function<hipError_t()> f, const int launches = 1) {
         B.plan.push_back(std::move(f));
@@ -896,15 +1020,17 @@ hipError_t plan_batch_step(Batch& B) {
         for (size_t i = 0; i < N; ++i) {
             auto& c = *S[i];
             auto& f = c.flow;
-            d[i]    = AdvectVelocityDesc{{f.tx, f.ty, f.tz}, {f.vx, f.vy, f.vz}, c.config.dt, c.dims, c.config.cell_size, c.config.flow_boundary};
+            const float h = c.config.cell_size;
+            d[i]          = AdvectVelocityDesc{{f.tx, f.ty, f.tz}, {f.vx, f.vy, f.vz}, c.config.dt, c.dims, h, c.config.flow_boundary, 1.0f / h, advect_pow2(h) ? 1 : 0};
         }
         const auto dd = B.upload(d);
         if (!dd) return hipErrorOutOfMemory;
-        SF_BTRY(add([=] { return launch_advect_velocity_batched(st, dd, tile_items, n_tile); }));
+        SF_BTRY(add([=] { return launch_advect_velocity_batched(st, dd, adv_items, n_adv); }));
     }
     // velocity diffusion: pass i of all three components of all simulations per launch
     for (int pass = 1; pass <= D; ++pass) {
         std::vector<SmoothDesc> d(3 * N);
+        std::vector<HalfPassArgs> h(3 * N);
         for (size_t i = 0; i < N; ++i) {
             auto& c           = *S[i];
             auto& f           = c.flow;
@@ -913,14 +1039,21 @@ hipError_t plan_batch_step(Batch& B) {
             float* a[3] = {f.ax, f.ay, f.az};
             float* v[3] = {f.vx, f.vy, f.vz};
             for (int comp = 0; comp < 3; ++comp) {
+                const size_t k      = 3 * i + comp;
+                const StencilBC bc  = velocity_rules(c.config.flow_boundary, comp);
+                if (vel_half(k)) {
+                    h[k] = batch_half_desc(B, i, comp, pass, D, t[comp], t[comp], a[comp], alpha, 1.0f, bc);
+                    continue;
+                }
                 float* dst       = ((D - pass) % 2 == 0) ? a[comp] : v[comp];
                 const float* src = pass == 1 ? t[comp] : (((D - pass + 1) % 2 == 0) ? a[comp] : v[comp]);
-                d[3 * i + comp]  = SmoothDesc{src, t[comp], dst, alpha, 1.0f, c.dims, velocity_rules(c.config.flow_boundary, comp), geo[i]};
+                d[k]             = SmoothDesc{src, t[comp], dst, alpha, 1.0f, c.dims, bc, geo[i]};
             }
         }
         const auto dd = B.upload(d);
-        if (!dd) return hipErrorOutOfMemory;
-        SF_BTRY(add([=] { return launch_smooth_lists(st, sm, kSolveDiffusion, false, dd, sm3); }, sm3.launches()));
+        const auto dh = B.upload(h);
+        if (!dd || !dh) return hipErrorOutOfMemory;
+        SF_BTRY(add([=] { return launch_pass_lists(st, sm, kSolveDiffusion, pass, D, false, dh, dd, lv); }, lv.launches()));
     }
     {
         std::vector<DivergenceDesc> d(N);
@@ -935,16 +1068,24 @@ hipError_t plan_batch_step(Batch& B) {
     }
     for (int pass = 1; pass <= P; ++pass) {
         std::vector<SmoothDesc> d(N);
+        std::vector<HalfPassArgs> h(N);
         for (size_t i = 0; i < N; ++i) {
-            auto& c          = *S[i];
-            auto& f          = c.flow;
+            auto& c            = *S[i];
+            auto& f            = c.flow;
+            const float coef   = c.config.cell_size * c.config.cell_size;
+            const StencilBC bc = pressure_rules(c.config.flow_boundary);
+            if (p_half(i)) {
+                h[i] = batch_half_desc(B, i, -1, pass, P, f.rhs, nullptr, f.p, coef, 1.0f, bc);
+                continue;
+            }
             float* dst       = ((P - pass) % 2 == 0) ? f.p : f.p_aux;
             const float* src = pass == 1 ? nullptr : (((P - pass + 1) % 2 == 0) ? f.p : f.p_aux);
-            d[i] = SmoothDesc{src, f.rhs, dst, c.config.cell_size * c.config.cell_size, 1.0f, c.dims, pressure_rules(c.config.flow_boundary), geo[i]};
+            d[i]             = SmoothDesc{src, f.rhs, dst, coef, 1.0f, c.dims, bc, geo[i]};
         }
         const auto dd = B.upload(d);
-        if (!dd) return hipErrorOutOfMemory;
-        SF_BTRY(add([=] { return launch_smooth_lists(st, sm, kSolvePoisson, pass == 1, dd, sm1); }, sm1.launches()));
+        const auto dh = B.upload(h);
+        if (!dd || !dh) return hipErrorOutOfMemory;
+        SF_BTRY(add([=] { return launch_pass_lists(st, sm, kSolvePoisson, pass, P, true, dh, dd, lp); }, lp.launches()));
     }
     {
         std::vector<ProjectDesc> d(N);
@@ -966,28 +1107,37 @@ hipError_t plan_batch_step(Batch& B) {
             auto& s = c.scalars[si];
             auto& f = c.flow;
             da[i]   = AddScaledDesc{s.data, s.source, c.config.dt, c.cells};
-            dv[i]   = AdvectScalarDesc{s.temp, s.data, {f.vx, f.vy, f.vz}, c.config.dt, c.dims, c.config.cell_size, s.desc.boundary, c.config.flow_boundary};
+            const float h = c.config.cell_size;
+            dv[i]         = AdvectScalarDesc{s.temp, s.data, {f.vx, f.vy, f.vz}, c.config.dt, c.dims, h, s.desc.boundary, c.config.flow_boundary, 1.0f / h, advect_pow2(h) ? 1 : 0};
         }
         const auto dda = B.upload(da);
         const auto ddv = B.upload(dv);
         if (!dda || !ddv) return hipErrorOutOfMemory;
         SF_BTRY(add([=] { return launch_add_scaled_batched(st, dda, elem_items, n_elem); }));
-        SF_BTRY(add([=] { return launch_advect_scalar_batched(st, ddv, tile_items, n_tile); }));
+        SF_BTRY(add([=] { return launch_advect_scalar_batched(st, ddv, adv_items, n_adv); }));
+        const PassLists& L = ls[si];
         for (int pass = 1; pass <= D; ++pass) {
             std::vector<SmoothDesc> d(N);
+            std::vector<HalfPassArgs> h(N);
             for (size_t i = 0; i < N; ++i) {
                 auto& c              = *S[i];
                 auto& s              = c.scalars[si];
                 const float alpha_s  = c.config.dt * s.desc.diffusion / (c.config.cell_size * c.config.cell_size);
                 const bool dissipate = s.desc.dissipation > 0.0f;
                 const float factor   = dissipate ? 1.0f / (1.0f + c.config.dt * s.desc.dissipation) : 1.0f;
-                float* dst           = ((D - pass) % 2 == 0) ? s.data : s.aux;
-                const float* src     = pass == 1 ? s.temp : (((D - pass + 1) % 2 == 0) ? s.data : s.aux);
-                d[i] = SmoothDesc{src, s.temp, dst, alpha_s, pass == D ? factor : 1.0f, c.dims, scalar_rules(s.desc.boundary), geo[i]};
+                const StencilBC bc   = scalar_rules(s.desc.boundary);
+                if (batch_half(B, i, bc, -1)) {
+                    h[i] = batch_half_desc(B, i, -1, pass, D, s.temp, s.temp, s.data, alpha_s, factor, bc);
+                    continue;
+                }
+                float* dst       = ((D - pass) % 2 == 0) ? s.data : s.aux;
+                const float* src = pass == 1 ? s.temp : (((D - pass + 1) % 2 == 0) ? s.data : s.aux);
+                d[i]             = SmoothDesc{src, s.temp, dst, alpha_s, pass == D ? factor : 1.0f, c.dims, bc, geo[i]};
             }
             const auto dd = B.upload(d);
-            if (!dd) return hipErrorOutOfMemory;
-            SF_BTRY(add([=] { return launch_smooth_lists(st, sm, kSolveDiffusion, false, dd, sm1); }, sm1.launches()));
+            const auto dh = B.upload(h);
+            if (!dd || !dh) return hipErrorOutOfMemory;
+            SF_BTRY(add([=] { return launch_pass_lists(st, sm, kSolveDiffusion, pass, D, false, dh, dd, L); }, L.launches()));
         }
     }
     return hipSuccess;
@@ -1015,55 +1165,144 @@ StableFluidsResult build_batch_graph(Batch& B) {
     return STABLE_FLUIDS_RESULT_OK;
 }
 
-// Mean time of `repeats` batched pressure passes (read rhs, write the scratch p_aux) under
-// the z-chunk plan B.zc; views are not touched.
-double time_batch_smoother(Batch& B, const int repeats) {
-    const size_t N = B.sims.size();
-    std::vector<TileGeom> geo(N);
-    for (size_t i = 0; i < N; ++i) geo[i] = plan_tiles(B.sims[i]->dims, B.zc[i]);
-    std::vector<SmoothDesc> d(N);
-    for (size_t i = 0; i < N; ++i) {
-        auto& c = *B.sims[i];
-        d[i]    = SmoothDesc{c.flow.rhs, c.flow.rhs, c.flow.p_aux, c.config.cell_size * c.config.cell_size, 1.0f, c.dims, pressure_rules(c.config.flow_boundary), geo[i]};
+// ---- tiling autotuner: per grid shape, the (tile shape, z chunk) of the fastest pass ----
+
+// A tuned tiling for nx x ny x nz under a smoother; members = the simulations of that
+// shape timed together (0 for a single context).
+struct TilingEntry {
+    int nx, ny, nz, smoother, members, tile, zc;
+};
+constexpr const char* kTilingCacheTag = "stable-fluids-tiling 2";
+
+// Entries of a cache file; a missing file or one in another format gives none.
+std::vector<TilingEntry> read_tiling_cache(const char* path) {
+    std::vector<TilingEntry> out;
+    if (path == nullptr) return out;
+    FILE* f = std::fopen(path, "r");
+    if (f == nullptr) return out;
+    char tag[64] = {};
+    if (std::fgets(tag, sizeof tag, f) != nullptr && std::strncmp(tag, kTilingCacheTag, std::strlen(kTilingCacheTag)) == 0) {
+        TilingEntry e{};
+        while (std::fscanf(f, "%d %d %d %d %d %d %d", &e.nx, &e.ny, &e.nz, &e.smoother, &e.members, &e.tile, &e.zc) == 7)
+            if (e.tile >= 0 && e.tile < kHalfTileCount && e.zc >= 0 && e.zc <= e.nz) out.push_back(e);
     }
-    // even-nx items first, then odd-nx ones: one launch per parity group
-    std::vector<BatchItem> items;
-    uint32_t n_even = 0;
-    for (int parity = 1; parity >= 0; --parity) {
-        std::vector<BatchItem> group;
-        for (size_t k = 0; k < N; ++k) {
-            if (((B.sims[k]->dims.nx & 1) == 0) != (parity == 1)) continue;
-            for (uint32_t b = 0, n = smooth_blocks(geo[k]); b < n; ++b) group.push_back(BatchItem{static_cast<uint32_t>(k), b});
-        }
-        if (parity == 1) n_even = static_cast<uint32_t>(group.size());
-        items.insert(items.end(), group.begin(), group.end());
+    std::fclose(f);
+    return out;
+}
+
+void write_tiling_cache(const char* path, const std::vector<TilingEntry>& entries) {
+    if (path == nullptr) return;
+    if (FILE* f = std::fopen(path, "w")) {
+        std::fprintf(f, "%s\n", kTilingCacheTag);
+        for (const auto& e : entries) std::fprintf(f, "%d %d %d %d %d %d %d\n", e.nx, e.ny, e.nz, e.smoother, e.members, e.tile, e.zc);
+        std::fclose(f);
     }
-    void *dd = nullptr, *di = nullptr;
+}
+
+const TilingEntry* find_tiling(const std::vector<TilingEntry>& cache, const Dims d, const int smoother, const int members) {
+    for (const auto& e : cache)
+        if (e.nx == d.nx && e.ny == d.ny && e.nz == d.nz && e.smoother == smoother && e.members == members) return &e;
+    return nullptr;
+}
+
+struct Tiling {
+    int tile, zc;
+};
+constexpr int kTuneRounds = 3;
+
+// Runs `timed` (mean ms of `passes` timed passes) until about 20 ms of GPU work have
+// passed, so that clocks have ramped before candidates are compared.
+template <class F>
+void tune_warmup(F timed, const int passes) {
+    double spent = 0.0;
+    for (int i = 0; i < 200 && spent < 20.0; ++i) {
+        const double t = timed();
+        if (!(t > 0.0)) return;
+        spent += t * passes;
+    }
+}
+// Candidates: every black-cell tile shape with the default and 4..32-plane chunks, or for
+// the full-iterate passes (whose tile is fixed) chunks of 2..64 planes.
+std::vector<Tiling> tiling_candidates(const Dims d, const bool half, const int tile) {
+    std::vector<Tiling> out;
+    if (half) {
+        for (int t = 0; t < kHalfTileCount; ++t)
+            for (const int zc : {0, 4, 8, 16, 32})
+                if (zc <= d.nz) out.push_back(Tiling{t, zc});
+    } else {
+        for (const int zc : {0, 2, 4, 8, 16, 32, 64})
+            if (zc <= d.nz) out.push_back(Tiling{tile, zc});
+    }
+    return out;
+}
+
+// Mean time of `repeats` middle RBGS pressure passes of a single context (half[0] ->
+// half[1], scratch between steps) under tile shape `tile` and z chunk zc.
+double time_half_pass(Context& c, const int tile, const int zc, const int repeats) {
+    HalfPassArgs a{};
+    a.b      = c.flow.rhs;
+    a.h_in   = c.half[0];
+    a.h_out  = c.half[1];
+    a.x_out  = c.flow.p_aux;
+    a.coef   = c.config.cell_size * c.config.cell_size;
+    a.out_scale = 1.0f;
+    a.d      = c.dims;
+    a.bc     = pressure_rules(c.config.flow_boundary);
+    a.g      = half_plan(c.dims, tile, zc);
+    auto pass = [&] { return launch_rbgs_half(c.stream, kSolvePoisson, kHalfInHalf, kHalfOutHalf, a, tile) == hipSuccess; };
     double ms = -1.0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (hipMalloc(&dd, d.size() * sizeof(SmoothDesc)) == hipSuccess && hipMalloc(&di, items.size() * sizeof(BatchItem)) == hipSuccess &&
-        hipMemcpy(dd, d.data(), d.size() * sizeof(SmoothDesc), hipMemcpyHostToDevice) == hipSuccess &&
-        hipMemcpy(di, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice) == hipSuccess &&
-        hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
-        const SmootherKind sm = B.sims[0]->options.smoother;
-        const auto* descs     = static_cast<const SmoothDesc*>(dd);
-        const auto* its       = static_cast<const BatchItem*>(di);
-        SmoothLists L;
-        L.items[1] = its;
-        L.n[1]     = n_even;
-        L.items[0] = its + n_even;
-        L.n[0]     = static_cast<uint32_t>(items.size()) - n_even;
-        bool ok    = launch_smooth_lists(B.stream, sm, kSolvePoisson, false, descs, L) == hipSuccess;
-        ok         = ok && hipEventRecord(e0, B.stream) == hipSuccess;
-        for (int r = 0; ok && r < repeats; ++r) ok = launch_smooth_lists(B.stream, sm, kSolvePoisson, false, descs, L) == hipSuccess;
-        ok       = ok && hipEventRecord(e1, B.stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
-        float t  = 0.0f;
+    if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+        bool ok = pass() && pass();
+        ok      = ok && hipEventRecord(e0, c.stream) == hipSuccess;
+        for (int r = 0; ok && r < repeats; ++r) ok = pass();
+        ok      = ok && hipEventRecord(e1, c.stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+        float t = 0.0f;
         if (ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms = t / repeats;
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    if (dd) (void)hipFree(dd);
-    if (di) (void)hipFree(di);
+    (void)hipGetLastError();
+    return ms;
+}
+
+// Mean time of `repeats` batched middle pressure passes over the simulations marked in
+// `only` under the batch's current tiling: black-cell passes read the context's half[0]
+// and write half[1], full-iterate passes read rhs and write p_aux (both scratch between
+// steps); views are not touched. The descriptor tables join the batch's allocations
+// (freed by the next graph build).
+double time_batch_smoother(Batch& B, const std::vector<char>& only, const int repeats) {
+    auto& S        = B.sims;
+    const size_t N = S.size();
+    auto p_half    = [&](size_t d) { return batch_half(B, d, pressure_rules(S[d]->config.flow_boundary), -1); };
+    PassLists L;
+    if (!pass_lists(B, N, [](size_t d) { return d; }, p_half, &L, &only)) return -1.0;
+    std::vector<SmoothDesc> d(N);
+    std::vector<HalfPassArgs> h(N);
+    for (size_t i = 0; i < N; ++i) {
+        auto& c            = *S[i];
+        const float coef   = c.config.cell_size * c.config.cell_size;
+        const StencilBC bc = pressure_rules(c.config.flow_boundary);
+        if (p_half(i)) h[i] = batch_half_desc(B, i, -1, 2, 3, c.flow.rhs, nullptr, c.flow.p_aux, coef, 1.0f, bc);
+        else d[i] = SmoothDesc{c.flow.rhs, c.flow.rhs, c.flow.p_aux, coef, 1.0f, c.dims, bc, plan_tiles(c.dims, B.zc[i])};
+    }
+    const auto dd = B.upload(d);
+    const auto dh = B.upload(h);
+    if (!dd || !dh) return -1.0;
+    const SmootherKind sm = S[0]->options.smoother;
+    auto pass             = [&] { return launch_pass_lists(B.stream, sm, kSolvePoisson, 2, 3, false, dh, dd, L) == hipSuccess; };
+    double ms             = -1.0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+        bool ok = pass() && pass();
+        ok      = ok && hipEventRecord(e0, B.stream) == hipSuccess;
+        for (int r = 0; ok && r < repeats; ++r) ok = pass();
+        ok      = ok && hipEventRecord(e1, B.stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+        float t = 0.0f;
+        if (ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms = t / repeats;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     (void)hipGetLastError();
     return ms;
 }
@@ -1360,12 +1599,33 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_create_cuda(void* c
         if (!sf::batch_supported(*c) || (i > 0 && !sf::batch_compatible(*b->sims[0], *c))) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         b->sims.push_back(c);
     }
-    b->zc.assign(count, 0);
     b->stream = static_cast<hipStream_t>(stream);
     if (b->stream == nullptr) {
         if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         b->owns_stream = true;
     }
+    // each simulation starts from its own smoother tiling (set_smoother_tiling)
+    b->zc.resize(count);
+    b->tile.resize(count);
+    b->hv.assign(count, {nullptr, nullptr, nullptr, nullptr});
+    for (uint32_t i = 0; i < count; ++i) {
+        auto& c     = *b->sims[i];
+        b->zc[i]    = c.half_zc;
+        b->tile[i]  = c.half_tile;
+        if (c.half[0] == nullptr || c.options.smoother != sf::kSmoothRBGS || !sf::rbgs_half_supported(c.dims, sf::velocity_rules(c.config.flow_boundary, 0))) continue;
+        const uint64_t n = sf::half_buffer_floats(c.dims);
+        for (float*& h : b->hv[i]) {
+            void* raw = nullptr;
+            if (hipMalloc(&raw, sizeof(float) * n) != hipSuccess) {
+                (void)hipGetLastError();
+                return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
+            }
+            b->owned.push_back(raw);
+            h = static_cast<float*>(raw);
+            if (sf::launch_fill(b->stream, h, 0.0f, n) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+        }
+    }
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     for (auto* c : b->sims)
         if (hipStreamSynchronize(c->stream) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     const StableFluidsResult g = sf::build_batch_graph(*b);
@@ -1400,11 +1660,30 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_set_z_chunk_cuda(vo
     return sf::build_batch_graph(b);
 }
 
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_set_tiling_cuda(void* batch, uint32_t sim, uint32_t tile, uint32_t z_chunk) {
+    if (batch == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& b = *static_cast<sf::Batch*>(batch);
+    if (sim >= b.sims.size() || tile >= sf::kHalfTileCount || z_chunk > static_cast<uint32_t>(b.sims[sim]->dims.nz)) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    if (hipStreamSynchronize(b.stream) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    b.tile[sim] = static_cast<int>(tile);
+    b.zc[sim]   = static_cast<int>(z_chunk);
+    return sf::build_batch_graph(b);
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_get_tile_cuda(void* batch, uint32_t sim, uint32_t* out_tile) {
+    if (batch == nullptr || out_tile == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& b = *static_cast<sf::Batch*>(batch);
+    if (sim >= b.sims.size()) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    *out_tile = static_cast<uint32_t>(b.tile[sim]);
+    return STABLE_FLUIDS_RESULT_OK;
+}
+
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_info_cuda(void* batch, uint32_t sim, int32_t* out_tiling4, uint32_t* out_launches_per_step) {
     if (batch == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     auto& b = *static_cast<sf::Batch*>(batch);
     if (sim >= b.sims.size()) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
-    const sf::TileGeom g = sf::plan_tiles(b.sims[sim]->dims, b.zc[sim]);
+    const bool half      = sf::batch_half(b, sim, sf::pressure_rules(b.sims[sim]->config.flow_boundary), -1);
+    const sf::TileGeom g = half ? sf::batch_half_plan(b, sim) : sf::plan_tiles(b.sims[sim]->dims, b.zc[sim]);
     if (out_tiling4) {
         out_tiling4[0] = g.tiles_x;
         out_tiling4[1] = g.tiles_y;
@@ -1420,61 +1699,110 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_autotune_cuda(void*
     if (batch == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
     auto& b = *static_cast<sf::Batch*>(batch);
     if (hipStreamSynchronize(b.stream) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
-    const int reps   = static_cast<int>(std::min<uint32_t>(std::max<uint32_t>(repeats, 1u), 1000u));
-    const int sm     = static_cast<int>(b.sims[0]->options.smoother);
-    struct Entry {
-        int nx, ny, nz, smoother, zc;
+    const int reps = static_cast<int>(std::min<uint32_t>(std::max<uint32_t>(repeats, 1u), 1000u));
+    const int sm   = static_cast<int>(b.sims[0]->options.smoother);
+    const size_t N = b.sims.size();
+    auto cache     = sf::read_tiling_cache(cache_path);
+    auto same      = [](const sf::Dims& a, const sf::Dims& d) { return a.nx == d.nx && a.ny == d.ny && a.nz == d.nz; };
+    // classes: the simulations of one grid shape, in first-appearance order
+    struct Class {
+        sf::Dims d;
+        std::vector<char> only;
+        int members = 0, first = -1;
+        bool half = false, cached = false;
+        std::vector<sf::Tiling> cands;
+        std::vector<double> t; // fastest pass time per candidate, this class alone
     };
-    std::vector<Entry> cache;
-    if (cache_path != nullptr) {
-        if (FILE* f = std::fopen(cache_path, "r")) {
-            Entry e{};
-            while (std::fscanf(f, "%d %d %d %d %d", &e.nx, &e.ny, &e.nz, &e.smoother, &e.zc) == 5) cache.push_back(e);
-            std::fclose(f);
+    std::vector<Class> classes;
+    for (size_t i = 0; i < N; ++i) {
+        const sf::Dims d = b.sims[i]->dims;
+        Class* k         = nullptr;
+        for (auto& c : classes)
+            if (same(c.d, d)) k = &c;
+        if (k == nullptr) {
+            classes.push_back(Class{});
+            k        = &classes.back();
+            k->d     = d;
+            k->only.assign(N, 0);
+            k->first = static_cast<int>(i);
+            k->half  = sf::batch_half(b, i, sf::pressure_rules(b.sims[i]->config.flow_boundary), -1);
         }
+        k->only[i] = 1;
+        ++k->members;
     }
-    // shapes in the batch, in first-appearance order
-    std::vector<sf::Dims> shapes;
-    for (auto* c : b.sims) {
-        bool seen = false;
-        for (auto& d : shapes) seen = seen || (d.nx == c->dims.nx && d.ny == c->dims.ny && d.nz == c->dims.nz);
-        if (!seen) shapes.push_back(c->dims);
-    }
-    auto assign = [&](const sf::Dims& d, const int zc) {
-        for (size_t i = 0; i < b.sims.size(); ++i)
-            if (b.sims[i]->dims.nx == d.nx && b.sims[i]->dims.ny == d.ny && b.sims[i]->dims.nz == d.nz) b.zc[i] = zc;
+    auto assign = [&](const Class& k, const sf::Tiling& t) {
+        for (size_t i = 0; i < N; ++i)
+            if (k.only[i]) {
+                b.tile[i] = t.tile;
+                b.zc[i]   = t.zc;
+            }
     };
+    auto effective = [&](const Class& k, const sf::Tiling& t) { return sf::Tiling{t.tile, k.half ? sf::half_plan(k.d, t.tile, t.zc).zc : sf::plan_tiles(k.d, t.zc).zc}; };
+    const std::vector<int> zc0 = b.zc, tile0 = b.tile;
+    const std::vector<char> all(N, 1);
     uint32_t tuned = 0;
-    for (auto& d : shapes) {
-        bool cached = false;
-        for (auto& e : cache)
-            if (e.nx == d.nx && e.ny == d.ny && e.nz == d.nz && e.smoother == sm) {
-                assign(d, e.zc);
-                cached = true;
-            }
-        if (cached) continue;
-        // coordinate descent: other shapes keep their current chunk lengths
-        int best_zc = 0;
-        double best = 1e300;
-        for (const int cand : {0, 2, 4, 8, 16, 32, 64}) {
-            if (cand > d.nz) continue;
-            assign(d, cand);
-            const double t = sf::time_batch_smoother(b, reps);
-            if (t > 0.0 && t < best) {
-                best    = t;
-                best_zc = sf::plan_tiles(d, cand).zc;
-            }
+    bool warm      = false;
+    for (auto& k : classes) {
+        if (const sf::TilingEntry* e = sf::find_tiling(cache, k.d, sm, k.members)) {
+            k.cached = true;
+            k.cands  = {sf::Tiling{e->tile, e->zc}};
+            k.t      = {0.0};
+            assign(k, k.cands[0]);
+            continue;
         }
-        assign(d, best_zc);
-        cache.push_back(Entry{d.nx, d.ny, d.nz, sm, best_zc});
+        if (!warm) sf::tune_warmup([&] { return sf::time_batch_smoother(b, all, reps); }, reps);
+        warm = true;
+        // every (tile shape, z chunk) candidate timed on this class alone: the minimum of
+        // kTuneRounds interleaved rounds, so clock ramps and stray work hit every candidate
+        k.cands = sf::tiling_candidates(k.d, k.half, b.tile[k.first]);
+        k.t.assign(k.cands.size(), 1e300);
+        for (int round = 0; round < sf::kTuneRounds; ++round)
+            for (size_t c = 0; c < k.cands.size(); ++c) {
+                assign(k, k.cands[c]);
+                const double t = sf::time_batch_smoother(b, k.only, reps);
+                if (t > 0.0) k.t[c] = std::min(k.t[c], t);
+            }
+        if (*std::min_element(k.t.begin(), k.t.end()) > 1e299) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         ++tuned;
     }
-    if (cache_path != nullptr) {
-        if (FILE* f = std::fopen(cache_path, "w")) {
-            for (auto& e : cache) std::fprintf(f, "%d %d %d %d %d\n", e.nx, e.ny, e.nz, e.smoother, e.zc);
-            std::fclose(f);
+    if (tuned > 0) {
+        // whole-batch plans (launches of different tile shapes run one after another):
+        // every class on its own fastest tiling; every black-cell class on one tile shape
+        // T, each with its fastest chunk under T; the starting tiling. The plan with the
+        // fastest whole-batch pass wins.
+        auto fastest = [](const Class& k, const int tile) {
+            size_t best = k.cands.size();
+            for (size_t c = 0; c < k.cands.size(); ++c)
+                if ((tile < 0 || k.cands[c].tile == tile) && (best == k.cands.size() || k.t[c] < k.t[best])) best = c;
+            if (best == k.cands.size())
+                for (size_t c = 0; c < k.cands.size(); ++c)
+                    if (best == k.cands.size() || k.t[c] < k.t[best]) best = c;
+            return k.cands[best];
+        };
+        std::vector<std::pair<std::vector<int>, std::vector<int>>> plans; // (tile, zc) per simulation
+        for (int T = -1; T < sf::kHalfTileCount; ++T) {
+            for (const auto& k : classes) assign(k, fastest(k, (k.half && !k.cached) ? T : -1));
+            plans.emplace_back(b.tile, b.zc);
+        }
+        plans.emplace_back(tile0, zc0);
+        std::vector<double> t_plan(plans.size(), 1e300);
+        for (int round = 0; round < sf::kTuneRounds; ++round)
+            for (size_t q = 0; q < plans.size(); ++q) {
+                b.tile         = plans[q].first;
+                b.zc           = plans[q].second;
+                const double t = sf::time_batch_smoother(b, all, reps);
+                if (t > 0.0) t_plan[q] = std::min(t_plan[q], t);
+            }
+        const size_t q = static_cast<size_t>(std::min_element(t_plan.begin(), t_plan.end()) - t_plan.begin());
+        b.tile         = plans[q].first;
+        b.zc           = plans[q].second;
+        for (const auto& k : classes) {
+            if (k.cached) continue;
+            const sf::Tiling t = effective(k, sf::Tiling{b.tile[k.first], b.zc[k.first]});
+            cache.push_back(sf::TilingEntry{k.d.nx, k.d.ny, k.d.nz, sm, k.members, t.tile, t.zc});
         }
     }
+    sf::write_tiling_cache(cache_path, cache);
     if (out_tuned_shapes) *out_tuned_shapes = tuned;
     return sf::build_batch_graph(b);
 }
@@ -1565,6 +1893,45 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_get_smoother_tiling_cuda(
     if (out_tile) *out_tile = static_cast<uint32_t>(c.half_tile);
     if (out_z_chunk) *out_z_chunk = static_cast<uint32_t>(sf::half_plan(c.dims, c.half_tile, c.half_zc).zc);
     return STABLE_FLUIDS_RESULT_OK;
+}
+
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_autotune_cuda(void* context, const char* cache_path, uint32_t repeats, uint32_t* out_tile, uint32_t* out_z_chunk) {
+    sf::Range r("stable.ext.autotune");
+    if (context == nullptr) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    auto& c = *static_cast<Context*>(context);
+    if (hipStreamSynchronize(c.stream) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+    const int reps     = static_cast<int>(std::min<uint32_t>(std::max<uint32_t>(repeats, 1u), 1000u));
+    const int sm       = static_cast<int>(c.options.smoother);
+    const sf::StencilBC bc = sf::pressure_rules(c.config.flow_boundary);
+    StableFluidsResult res = STABLE_FLUIDS_RESULT_OK;
+    if (sf::use_half(c, bc)) {
+        auto cache = sf::read_tiling_cache(cache_path);
+        if (const sf::TilingEntry* e = sf::find_tiling(cache, c.dims, sm, 0)) {
+            c.half_tile = e->tile;
+            c.half_zc   = e->zc;
+        } else {
+            const auto cands = sf::tiling_candidates(c.dims, true, c.half_tile);
+            std::vector<double> t_min(cands.size(), 1e300);
+            sf::tune_warmup([&] { return sf::time_half_pass(c, c.half_tile, c.half_zc, reps); }, reps);
+            for (int round = 0; round < sf::kTuneRounds; ++round)
+                for (size_t k = 0; k < cands.size(); ++k) {
+                    const double t = sf::time_half_pass(c, cands[k].tile, cands[k].zc, reps);
+                    if (t > 0.0) t_min[k] = std::min(t_min[k], t);
+                }
+            const size_t best = static_cast<size_t>(std::min_element(t_min.begin(), t_min.end()) - t_min.begin());
+            if (t_min[best] > 1e299) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
+            const int best_tile = cands[best].tile;
+            const int best_zc   = sf::half_plan(c.dims, best_tile, cands[best].zc).zc;
+            c.half_tile         = best_tile;
+            c.half_zc           = best_zc;
+            cache.push_back(sf::TilingEntry{c.dims.nx, c.dims.ny, c.dims.nz, sm, 0, best_tile, best_zc});
+            sf::write_tiling_cache(cache_path, cache);
+        }
+        res = sf::rebuild_graph(c);
+    }
+    if (out_tile) *out_tile = static_cast<uint32_t>(c.half_tile);
+    if (out_z_chunk) *out_z_chunk = static_cast<uint32_t>(sf::half_plan(c.dims, c.half_tile, c.half_zc).zc);
+    return res;
 }
 
 STABLE_FLUIDS_API void* stable_fluids_ext_get_stream_cuda(void* context) {

@@ -244,22 +244,8 @@ __device__ __forceinline__ void tile_block_of(const Dims d, const uint32_t b, in
 __global__ __launch_bounds__(kTileX* kTileY) void k_advect_velocity(float* __restrict__ tx, float* __restrict__ ty, float* __restrict__ tz, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float dt, const Dims d, const float h, const StableFluidsFlowBoundaryConfig bc) {
     advect_velocity_body(tx, ty, tz, vx, vy, vz, dt, d, h, bc, static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y), static_cast<int>(blockIdx.z));
 }
-__global__ __launch_bounds__(kTileX* kTileY) void k_advect_velocity_batched(const AdvectVelocityDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
-    const BatchItem it = items[blockIdx.x];
-    const AdvectVelocityDesc& sd   = descs[it.desc];
-    int bx, by, bz;
-    tile_block_of(sd.d, it.block, bx, by, bz);
-    advect_velocity_body(sd.t[0], sd.t[1], sd.t[2], sd.v[0], sd.v[1], sd.v[2], sd.dt, sd.d, sd.h, sd.bc, bx * kTileX + static_cast<int>(threadIdx.x), by * kTileY + static_cast<int>(threadIdx.y), bz);
-}
 __global__ __launch_bounds__(kTileX* kTileY) void k_advect_scalar(float* __restrict__ out, const float* __restrict__ src, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float dt, const Dims d, const float h, const StableFluidsScalarBoundaryConfig sbc, const StableFluidsFlowBoundaryConfig fbc) {
     advect_scalar_body(out, src, vx, vy, vz, dt, d, h, sbc, fbc, static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y), static_cast<int>(blockIdx.z));
-}
-__global__ __launch_bounds__(kTileX* kTileY) void k_advect_scalar_batched(const AdvectScalarDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
-    const BatchItem it = items[blockIdx.x];
-    const AdvectScalarDesc& sd   = descs[it.desc];
-    int bx, by, bz;
-    tile_block_of(sd.d, it.block, bx, by, bz);
-    advect_scalar_body(sd.out, sd.src, sd.v[0], sd.v[1], sd.v[2], sd.dt, sd.d, sd.h, sd.sbc, sd.fbc, bx * kTileX + static_cast<int>(threadIdx.x), by * kTileY + static_cast<int>(threadIdx.y), bz);
 }
 // Cell of this thread in a 64 x 4 tile: wave w covers x in [16 w, 16 w + 16) of all four
 // rows (four 64-byte row segments), so only the waves on the outermost 16 columns of a row
@@ -278,10 +264,12 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_divergence(float* __restrict
 }
 __global__ __launch_bounds__(kTileX* kTileY) void k_divergence_batched(const DivergenceDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
     const BatchItem it = items[blockIdx.x];
-    const DivergenceDesc& sd   = descs[it.desc];
-    int bx, by, bz;
+    if (it.desc == kIdleItem) return;
+    const DivergenceDesc& sd = descs[it.desc];
+    int bx, by, bz, x, y;
     tile_block_of(sd.d, it.block, bx, by, bz);
-    divergence_body(sd.div, sd.v[0], sd.v[1], sd.v[2], sd.d, sd.h, sd.dbc, bx * kTileX + static_cast<int>(threadIdx.x), by * kTileY + static_cast<int>(threadIdx.y), bz);
+    cell_of_thread(bx, by, x, y);
+    divergence_body(sd.div, sd.v[0], sd.v[1], sd.v[2], sd.d, sd.h, sd.dbc, x, y, bz);
 }
 __global__ __launch_bounds__(kTileX* kTileY) void k_project(float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz, float* __restrict__ div, float* __restrict__ mag, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float* __restrict__ p, const Dims d, const float h, const StencilBC pbc, const StencilBC dbc) {
     int x, y;
@@ -290,10 +278,12 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_project(float* __restrict__ 
 }
 __global__ __launch_bounds__(kTileX* kTileY) void k_project_batched(const ProjectDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
     const BatchItem it = items[blockIdx.x];
-    const ProjectDesc& sd   = descs[it.desc];
-    int bx, by, bz;
+    if (it.desc == kIdleItem) return;
+    const ProjectDesc& sd = descs[it.desc];
+    int bx, by, bz, x, y;
     tile_block_of(sd.d, it.block, bx, by, bz);
-    project_body(sd.o[0], sd.o[1], sd.o[2], sd.div, sd.mag, sd.v[0], sd.v[1], sd.v[2], sd.p, sd.d, sd.h, sd.pbc, sd.dbc, bx * kTileX + static_cast<int>(threadIdx.x), by * kTileY + static_cast<int>(threadIdx.y), bz);
+    cell_of_thread(bx, by, x, y);
+    project_body(sd.o[0], sd.o[1], sd.o[2], sd.div, sd.mag, sd.v[0], sd.v[1], sd.v[2], sd.p, sd.d, sd.h, sd.pbc, sd.dbc, x, y, bz);
 }
 
 // Batched elementwise stages: local block b covers elements [4 * 256 * b, + 1024).
@@ -409,16 +399,6 @@ hipError_t launch_add_force_batched(hipStream_t s, const AddForceDesc* descs, co
 hipError_t launch_add_scaled_batched(hipStream_t s, const AddScaledDesc* descs, const BatchItem* items, const uint32_t n_items) {
     if (n_items == 0) return hipSuccess;
     k_add_scaled_batched<<<n_items, kElemBlock, 0, s>>>(descs, items);
-    return hipGetLastError();
-}
-hipError_t launch_advect_velocity_batched(hipStream_t s, const AdvectVelocityDesc* descs, const BatchItem* items, const uint32_t n_items) {
-    if (n_items == 0) return hipSuccess;
-    k_advect_velocity_batched<<<n_items, tile_block(), 0, s>>>(descs, items);
-    return hipGetLastError();
-}
-hipError_t launch_advect_scalar_batched(hipStream_t s, const AdvectScalarDesc* descs, const BatchItem* items, const uint32_t n_items) {
-    if (n_items == 0) return hipSuccess;
-    k_advect_scalar_batched<<<n_items, tile_block(), 0, s>>>(descs, items);
     return hipGetLastError();
 }
 hipError_t launch_divergence_batched(hipStream_t s, const DivergenceDesc* descs, const BatchItem* items, const uint32_t n_items) {

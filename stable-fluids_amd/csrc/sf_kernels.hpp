@@ -56,6 +56,8 @@ struct BatchItem {
     uint32_t desc;
     uint32_t block;
 };
+// A work-list entry that does nothing (pads the black-cell lists to a multiple of 8).
+constexpr uint32_t kIdleItem = 0xffffffffu;
 
 // Smoother tiling: tiles_x * tiles_y tiles of 128 x 16 cells, z split in chunks of zc planes.
 struct TileGeom {
@@ -116,9 +118,11 @@ int half_tile_cells(int tile);
 TileGeom half_plan(Dims d, int tile, int zc_override);
 hipError_t launch_rbgs_half(hipStream_t s, SolveKind solve, int in_mode, int out_mode, const HalfPassArgs& args, int tile);
 // one launch for many passes: workgroup i runs block items[i].block of descs[items[i].desc]
-// (every descriptor's L filled in, its g planned for `tile`)
+// (every descriptor's L filled in, its g planned for `tile`); kIdleItem entries return
 hipError_t launch_rbgs_half_batched(hipStream_t s, SolveKind solve, int in_mode, int out_mode, int tile, const HalfPassArgs* descs, const BatchItem* items, uint32_t n_items);
 
+// Batched advection runs the buffer-resource kernels (advect_buffers_fit must hold) over
+// 16 x 16 blocks (advect_blocks); inv_h = 1 / h and pow2 = advect_pow2(h) set by the host.
 struct AdvectVelocityDesc {
     float* t[3];
     const float* v[3];
@@ -126,6 +130,8 @@ struct AdvectVelocityDesc {
     Dims d;
     float h;
     StableFluidsFlowBoundaryConfig bc;
+    float inv_h;
+    int pow2;
 };
 struct AdvectScalarDesc {
     float* out;
@@ -136,7 +142,11 @@ struct AdvectScalarDesc {
     float h;
     StableFluidsScalarBoundaryConfig sbc;
     StableFluidsFlowBoundaryConfig fbc;
+    float inv_h;
+    int pow2;
 };
+bool advect_pow2(float h);
+uint32_t advect_blocks(Dims d);
 struct DivergenceDesc {
     float* div;
     const float* v[3];

@@ -605,10 +605,11 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 
 }
 
 // batched: workgroup i runs block items[i].block of the pass descs[items[i].desc] (the
-// descriptor is read in place)
+// descriptor is read in place); an idle item does nothing
 template <int kSolve, int kIn, int kOut, int kTXP, int kWaves, int kRowsW>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 4))) void k_rbgs_half_batched(const HalfPassArgs* __restrict__ descs, const BatchItem* __restrict__ items) {
     const BatchItem it = items[blockIdx.x];
+    if (it.desc == kIdleItem) return; // the whole workgroup: no barrier is left waiting
     rbgs_half_body<kSolve, kIn, kOut, kTXP, kWaves, kRowsW, kHalfDepth>(descs[it.desc], static_cast<int>(it.block));
 }
 

@@ -33,6 +33,7 @@ EXT_ENTRY_POINTS = (
     "stable_fluids_ext_batch_autotune_cuda", "stable_fluids_ext_set_force_animation_cuda",
     "stable_fluids_ext_force_animation_step_cuda", "stable_fluids_ext_field_norms_noalloc_cuda",
     "stable_fluids_ext_set_smoother_tiling_cuda", "stable_fluids_ext_get_smoother_tiling_cuda",
+    "stable_fluids_ext_autotune_cuda", "stable_fluids_ext_batch_set_tiling_cuda", "stable_fluids_ext_batch_get_tile_cuda",
 )
 UNET_ENTRY_POINTS = (
     "stable_fluids_unet_param_count", "stable_fluids_unet_create_cuda", "stable_fluids_unet_destroy_cuda",
@@ -78,6 +79,7 @@ def load_library(path=LIB_PATH):
     lib.stable_fluids_ext_field_norms_noalloc_cuda.argtypes = [vp, vp, vp, vp, vp, C.c_uint64, vp, vp]
     lib.stable_fluids_ext_set_smoother_tiling_cuda.argtypes = [vp, C.c_uint32, C.c_uint32]
     lib.stable_fluids_ext_get_smoother_tiling_cuda.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    lib.stable_fluids_ext_autotune_cuda.argtypes = [vp, C.c_char_p, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     lib.stable_fluids_ext_run_stage_cuda.argtypes = [vp, u32, u32]
     lib.stable_fluids_ext_stage_bytes_cuda.argtypes = [vp, u32]
     lib.stable_fluids_ext_stage_bytes_cuda.restype = C.c_uint64
@@ -91,6 +93,8 @@ def load_library(path=LIB_PATH):
     lib.stable_fluids_ext_batch_get_stream_cuda.argtypes = [vp]
     lib.stable_fluids_ext_batch_get_stream_cuda.restype = vp
     lib.stable_fluids_ext_batch_set_z_chunk_cuda.argtypes = [vp, u32, u32]
+    lib.stable_fluids_ext_batch_set_tiling_cuda.argtypes = [vp, u32, u32, u32]
+    lib.stable_fluids_ext_batch_get_tile_cuda.argtypes = [vp, u32, C.POINTER(u32)]
     lib.stable_fluids_ext_batch_info_cuda.argtypes = [vp, u32, C.POINTER(C.c_int32), C.POINTER(u32)]
     lib.stable_fluids_ext_batch_autotune_cuda.argtypes = [vp, C.c_char_p, u32, C.POINTER(u32)]
     lib.stable_fluids_ext_get_pressure_rhs_cuda.argtypes = [vp]
@@ -228,6 +232,12 @@ class Context:
         self._call("stable_fluids_ext_get_smoother_tiling_cuda", C.byref(t), C.byref(z))
         return t.value, z.value
 
+    def autotune(self, cache_path=None, repeats=10):
+        """Times every smoother tiling and keeps the fastest; returns (tile, z_chunk)."""
+        t, z = C.c_uint32(0), C.c_uint32(0)
+        self._call("stable_fluids_ext_autotune_cuda", cache_path.encode() if cache_path else None, repeats, C.byref(t), C.byref(z))
+        return t.value, z.value
+
     def run_stage(self, stage, repeat):
         return self._call("stable_fluids_ext_run_stage_cuda", stage, repeat)
 
@@ -324,11 +334,16 @@ class Batch:
     def set_z_chunk(self, sim, z_chunk):
         self._call("stable_fluids_ext_batch_set_z_chunk_cuda", sim, z_chunk)
 
+    def set_tiling(self, sim, tile, z_chunk=0):
+        self._call("stable_fluids_ext_batch_set_tiling_cuda", sim, tile, z_chunk)
+
     def info(self, sim):
         t = (C.c_int32 * 4)()
-        n = C.c_uint32()
+        n, tile = C.c_uint32(), C.c_uint32()
         self._call("stable_fluids_ext_batch_info_cuda", sim, t, C.byref(n))
-        return {"tiles_x": t[0], "tiles_y": t[1], "chunks": t[2], "z_chunk": t[3], "launches_per_step": n.value}
+        self._call("stable_fluids_ext_batch_get_tile_cuda", sim, C.byref(tile))
+        return {"tiles_x": t[0], "tiles_y": t[1], "chunks": t[2], "z_chunk": t[3], "tile": tile.value,
+                "launches_per_step": n.value}
 
     def autotune(self, cache_path=None, repeats=10):
         n = C.c_uint32()

@@ -183,6 +183,26 @@ def viewer_plume_scene(nx=96, ny=128, nz=96, diffuse_iterations=24, pressure_ite
     return scene, (w_drift, swirl_x, swirl_z)
 
 
+BATCH_SIZES = [(32, 32, 32), (48, 48, 48), (64, 64, 64), (96, 96, 96), (128, 128, 128), (128, 64, 64)]
+
+
+def batch_plume_scenes(count=64, seed=0, diffuse_iterations=40, pressure_iterations=80):
+    """`count` plume simulations of uneven sizes: (nx, ny, nz) drawn from BATCH_SIZES, a
+    random centre, radius U[2, 8] cells, upward force U[1, 10]; free-slip box, h = 1/max(n),
+    dt = 0.1, nu = kappa = 1e-4."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        nx, ny, nz = BATCH_SIZES[int(rng.integers(0, len(BATCH_SIZES)))]
+        desc = plume_desc(max(nx, ny, nz), diffuse_iterations, pressure_iterations, shape=(nx, ny, nz))
+        radius = float(rng.uniform(2.0, 8.0))
+        centre = (int(rng.integers(int(radius) + 1, nx - int(radius) - 1)), int(rng.integers(int(radius) + 1, max(int(radius) + 2, ny // 4))),
+                  int(rng.integers(int(radius) + 1, nz - int(radius) - 1)))
+        src, fx, fy, fz = plume_fields(desc.shape, centre, radius, 100.0, float(rng.uniform(1.0, 10.0)))
+        out.append(Scene(f"batch{i}", desc, src, (fx, fy, fz), None))
+    return out
+
+
 def animated_force(masks, step, animation):
     """Host restatement of the per-step forcing (vulkan-app/scene_plume.cpp:327-334), FP32:
     force x and z of step `step` from the masks and an abi.ForceAnimation's constants."""

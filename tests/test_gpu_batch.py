@@ -2,16 +2,19 @@
 
 Pins: every context of a batch ends bit-identical to the same context stepped alone (on
 uneven grids, mixed boundary types, several scalars/forces, RBGS and Jacobi); the
-smoother tiling chosen per context (z-chunk length, autotuner) never changes a bit; the
-autotuner times every shape once and honours its cache file.
+smoother tiling chosen per context (tile shape, z-chunk length, autotuner) never changes
+a bit; the autotuner times every shape once and honours its cache file; the 64-plume
+workload of uneven sizes at D=40, P=80 steps batched and autotuned bit-identical to each
+simulation alone and, at 32^3, to the FP32 oracle.
 """
+import hashlib
 import os
 
 import numpy as np
 import pytest
 
-from conftest import sf
-from paritylib import Inputs, gpu_fields
+from conftest import sf, sf_oracle
+from paritylib import Inputs, gpu_fields, oracle_fields
 
 abi = sf.abi
 pytestmark = pytest.mark.gpu
@@ -93,12 +96,24 @@ def test_tiling_never_changes_results(tmp_path, smoother):
         b.step(4)
         ref = fields(base)
     destroy(base)
+    # fixed chunk lengths, odd ones and one-plane chunks included, and (RBGS) every
+    # black-cell tile shape, stepped as set
+    fixed = make(opts)
+    with sf.Batch([ctx for _, ctx in fixed]) as b:
+        for i, zc in enumerate([2, 3, 5, 1, 9, 20]):
+            if smoother == abi.SMOOTHER_RBGS:
+                b.set_tiling(i, i % 5, zc)
+                assert b.info(i)["tile"] == i % 5
+            else:
+                b.set_z_chunk(i, zc)
+            assert b.info(i)["z_chunk"] == min(zc, SIMS[i][0][2])
+        b.step(4)
+        got = fields(fixed)
+    destroy(fixed)
+    assert_same(got, ref, "fixed tilings vs default tiling")
     tuned = make(opts)
     cache = str(tmp_path / "tiles.txt")
     with sf.Batch([ctx for _, ctx in tuned]) as b:
-        for i, zc in enumerate([2, 3, 5, 1, 9, 20]):
-            b.set_z_chunk(i, zc)
-            assert b.info(i)["z_chunk"] == min(zc, SIMS[i][0][2])
         shapes = b.autotune(cache, repeats=3)
         assert shapes == len(SIMS)
         b.step(4)
@@ -106,15 +121,69 @@ def test_tiling_never_changes_results(tmp_path, smoother):
         plans = [b.info(i) for i in range(len(SIMS))]
     destroy(tuned)
     assert_same(got, ref, "tuned vs default tiling")
-    lines = open(cache).read().split("\n")
-    assert len([l for l in lines if l.strip()]) == len(SIMS)
-    print("tuned plans", [(p["z_chunk"], p["chunks"]) for p in plans], flush=True)
+    lines = [l for l in open(cache).read().split("\n") if l.strip()]
+    assert lines[0] == "stable-fluids-tiling 2" and len(lines) == len(SIMS) + 1
+    print("tuned plans", [(p["tile"], p["z_chunk"], p["chunks"]) for p in plans], flush=True)
     # a second batch takes every shape from the cache
     again = make(opts)
     with sf.Batch([ctx for _, ctx in again]) as b:
         assert b.autotune(cache, repeats=3) == 0
-        assert [b.info(i)["z_chunk"] for i in range(len(SIMS))] == [p["z_chunk"] for p in plans]
+        got_plans = [b.info(i) for i in range(len(SIMS))]
+        assert [(p["tile"], p["z_chunk"]) for p in got_plans] == [(p["tile"], p["z_chunk"]) for p in plans]
     destroy(again)
+
+
+def digests(ctx, desc):
+    return {k: hashlib.sha1(np.ascontiguousarray(v).tobytes()).hexdigest() for k, v in gpu_fields(ctx, desc).items()}
+
+
+def test_64_uneven_plumes_batched_autotuned(tmp_path):
+    """64 plumes with sizes drawn from {32^3, 48^3, 64^3, 96^3, 128^3, 128x64x64}, radius
+    U[2, 8], force U[1, 10], D=40, P=80: batched and autotuned, 3 steps."""
+    scs = sf.scenes.batch_plume_scenes(64, seed=0, diffuse_iterations=40, pressure_iterations=80)
+    assert len({sc.desc.shape for sc in scs}) == 6
+    alone = []
+    for sc in scs:
+        ctx = sf.Context(sc.desc)
+        try:
+            sf.scenes.seed_context(ctx, sc)
+            ctx.step(3)
+            alone.append(digests(ctx, sc.desc))
+        finally:
+            ctx.destroy()
+    ctxs = []
+    try:
+        for sc in scs:
+            ctxs.append(sf.Context(sc.desc))
+            sf.scenes.seed_context(ctxs[-1], sc)
+        with sf.Batch(ctxs) as b:
+            tuned = b.autotune(str(tmp_path / "tiles.txt"), repeats=3)
+            plans = {sc.desc.shape: (b.info(i)["tile"], b.info(i)["z_chunk"]) for i, sc in enumerate(scs)}
+            b.step(3)
+            sf.hip.synchronize()
+            small = [(i, gpu_fields(ctxs[i], sc.desc)) for i, sc in enumerate(scs) if sc.desc.shape == (32, 32, 32)]
+            batched = [digests(c, sc.desc) for c, sc in zip(ctxs, scs)]
+            launches = b.info(0)["launches_per_step"]
+    finally:
+        for c in ctxs:
+            c.destroy()
+    print(f"tuned {tuned} shapes: {plans}; {launches} launches per batch step", flush=True)
+    assert tuned == 6
+    for i, (g, a) in enumerate(zip(batched, alone)):
+        bad = [k for k in a if g[k] != a[k]]
+        assert not bad, f"simulation {i} {scs[i].desc.shape}: batched differs from alone in {bad}"
+    assert small, "no 32^3 simulation drawn"
+    for i, got in small:
+        sc = scs[i]
+        sim = sf_oracle.OracleSim(abi, sc.desc, precision=32)
+        try:
+            sf.scenes.seed_oracle(sim, sc)
+            sim.step(3)
+            ref = oracle_fields(sim, sc.desc)
+        finally:
+            sim.destroy()
+        bad = [k for k in ref if not np.array_equal(got[k], ref[k].astype(np.float64))]
+        assert not bad, f"simulation {i} (32^3) differs from the FP32 oracle in {bad}"
 
 
 def test_incompatible_contexts_are_rejected():

@@ -219,6 +219,33 @@ def test_smoother_tilings_bitexact(case, tile, zc):
     assert_bitexact(gpu, run_oracle(desc, inputs.seed_oracle, 3, 32), f"{case} tile {tile} zc {zc} 3 steps vs fp32")
 
 
+def test_single_context_autotune_bitexact(tmp_path):
+    """The tiling autotuner picks a plan for one context, caches it, and the tuned context
+    steps bit-exactly against the FP32 oracle; a second context takes the cached plan."""
+    make, n = TILE_CASES["walls_multi_tile_chunks"]
+    desc = make(n=n)
+    inputs = Inputs(desc, seed=7, velocity_scale=0.5)
+    cache = str(tmp_path / "tiles.txt")
+    ctx = sf.Context(desc)
+    try:
+        plan = ctx.autotune(cache, repeats=3)
+        assert ctx.smoother_tiling() == plan
+        inputs.seed_gpu(ctx)
+        ctx.step(3)
+        gpu = gpu_fields(ctx, desc)
+    finally:
+        ctx.destroy()
+    print("tuned tiling", plan, flush=True)
+    lines = [l for l in open(cache).read().split("\n") if l.strip()]
+    assert lines[0] == "stable-fluids-tiling 2" and lines[1].split()[:5] == [str(v) for v in (*n, abi.SMOOTHER_RBGS, 0)]
+    assert_bitexact(gpu, run_oracle(desc, inputs.seed_oracle, 3, 32), "autotuned 3 steps vs fp32")
+    again = sf.Context(desc)
+    try:
+        assert again.autotune(cache, repeats=3) == plan
+    finally:
+        again.destroy()
+
+
 def test_plume256_noise_one_step_bitexact():
     """The 256^3 throughput workload (seeded divergent initial velocity, D=40, P=80), one step."""
     scene = scenes.plume_scene(256, diffuse_iterations=40, pressure_iterations=80, noise_seed=0)
