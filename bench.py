@@ -5,16 +5,18 @@ Workload (default): 256^3 cells, h = 1/256, dt = 0.1, nu = kappa = 1e-4, 40 diff
 16 at (128, 24, 128) with source 100 and upward force 5, initial velocity = seeded N(0,1)
 noise smoothed by a 5^3 box filter. One step = one hipGraph launch of the whole step.
 
-Printed JSON line: steps/s (value), Gcell-updates/s, the dominant kernel's achieved HBM
-bandwidth against the 8 TB/s peak (timed live with HIP events on the context stream),
-the Jacobi pass bandwidth, the learned-vs-plain speedup at equal post-projection
+Printed JSON line: steps/s (value), Gcell-updates/s, the dominant kernel's algorithmic
+bandwidth against the 8 TB/s HBM peak (timed live with HIP events on the context stream,
+beside the average of the committed kernel trace of the same command, and again at 512^3
+where the pass's working set no longer fits the 256 MB last-level cache), the Jacobi
+pass bandwidth (256^3 and 512^3), the learned-vs-plain speedup at equal post-projection
 divergence L2 (warm-start network + K passes against 80 plain passes, from common
 states of the 256^3 plume; needs the trained parameters under stable-fluids_amd/weights)
-and the host-loop baseline (FP64 oracle on this host's cores, a bounded sample).
+and the host-loop baseline (the oracle on this host's cores, bounded samples).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid N] [--smoother rbgs|jacobi]
                   [--storage f32|f16|bf16|f32-refined] [--interval M]
-                  [--no-cpu-baseline] [--no-learned] [--stage-repeat R]
+                  [--no-cpu-baseline] [--no-learned] [--no-512] [--stage-repeat R]
   python bench.py --learned-step K [--steps S] [--warmup W]   (the learned step alone, for traces)
   python bench.py --batch [--tuning-cache FILE]                (64 uneven simulations, below)
 
@@ -37,9 +39,11 @@ sys.path.insert(0, ROOT)
 import __graft_entry__ as entry  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# the velocity diffusion RBGS pass (FP32 iterates, even rows loaded as aligned pairs,
-# whole-field buffer resources)
-VELOCITY_PASS_KERNEL = "k_rbgs_split_pass<0, false, sf::StoreF32, true, true>"
+# the dominant kernel: a middle velocity-diffusion RBGS pass on the black-cell buffers
+# (diffusion solve, half-buffer in, half-buffer out, 128 x 16 tiles)
+VELOCITY_PASS_KERNEL = "k_rbgs_half<0, 2, 0, 64, 8, 2>"
+TRACE_FILE = os.path.join(ROOT, "profiles", "step_plume256_kernel_trace.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_plume256_kernels.json")
 STORAGE = {"f32": 0, "f16": 1, "bf16": 2, "f32-refined": 3}
 METRIC = "steps/s at 256³; Jacobi HBM GB/s; learned-vs-plain speedup at equal div L2"
 
@@ -70,6 +74,7 @@ def main():
     ap.add_argument("--interval", type=int, default=10, help="passes per refinement cycle (16-bit storage)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain projection comparison")
+    ap.add_argument("--no-512", action="store_true", help="skip the 512^3 smoother-pass leg of the roofline")
     ap.add_argument("--precision", action="store_true", help="run the 512^3 storage precision study instead")
     ap.add_argument("--batch", action="store_true", help="run the 64-simulation batched study instead")
     ap.add_argument("--tuning-cache", default=None, help="autotuner cache file for --batch")
@@ -120,35 +125,28 @@ def main():
     cells = scene.desc.cells
     bpc = step_bytes_per_cell(40, 80)
 
-    # dominant kernel: one velocity-diffusion smoother pass (3 * 40 of them per step),
+    # dominant kernel: a middle velocity-diffusion smoother pass (3 * 38 of them per step),
     # timed live on the context stream with HIP events
-    def time_stage(stage, repeat):
-        ctx.run_stage(stage, 2)
-        hip.synchronize()
-        hip.check(hip.lib().hipEventRecord(ev0.handle, hs), "hipEventRecord")
-        ctx.run_stage(stage, repeat)
-        hip.check(hip.lib().hipEventRecord(ev1.handle, hs), "hipEventRecord")
-        hip.synchronize()
-        return ev1.elapsed_ms(ev0) / repeat
-
-    stages = {}
-    # (the *_pair stages run two RBGS passes per launch, the opt-in pass-pair mode; the step
-    # above runs one pass per launch)
-    stage_list = (("velocity_pass", 1), ("pressure_pass", 0), ("jacobi_pass", 4), ("advect", 2), ("project", 3), ("copy", 5),
-                  ("velocity_pair", 7), ("pressure_pair", 6))
-    for name, stage in stage_list:
-        ms = time_stage(stage, args.stage_repeat)
-        by = ctx.stage_bytes(stage)
-        stages[name] = {"us": round(ms * 1e3, 3), "bytes": by, "gbs": round(by / (ms * 1e-3) / 1e9, 1)}
+    stages = timed_stages(ctx, args.stage_repeat)
     dom = stages["velocity_pass"]
+    tiling = ctx.smoother_tiling()
     ctx.destroy()
+    default_run = n == 256 and args.smoother == "rbgs" and args.storage == "f32"
+    # the same passes at 512^3 (working set 4x the last-level cache)
+    big = None
+    if default_run and not args.no_512:
+        big_scene = scenes.plume_scene(512, diffuse_iterations=40, pressure_iterations=80)
+        with pkg.Context(big_scene.desc) as big_ctx:
+            scenes.seed_context(big_ctx, big_scene)
+            big = timed_stages(big_ctx, max(4, args.stage_repeat // 4), names=("velocity_pass", "jacobi_pass"))
 
-    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
-    # FETCH_SIZE / WRITE_SIZE counts of this workload (copy-kernel calibrated)
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_plume256_kernels.json")
-    if n == 256 and args.smoother == "rbgs" and os.path.exists(tfile):
-        rec = json.load(open(tfile))["kernels"].get(VELOCITY_PASS_KERNEL)
+    # the committed kernel trace (rocprofv3 --kernel-trace --stats of this command) and
+    # the committed FETCH_SIZE / WRITE_SIZE counts (copy-kernel calibrated) of the kernel
+    trace = traffic = None
+    if default_run and os.path.exists(TRACE_FILE):
+        trace = json.load(open(TRACE_FILE))["kernels"].get(VELOCITY_PASS_KERNEL)
+    if default_run and os.path.exists(TRAFFIC_FILE):
+        rec = json.load(open(TRAFFIC_FILE))["kernels"].get(VELOCITY_PASS_KERNEL)
         traffic = rec["traffic_bytes"] if rec else None
 
     # learned warm start vs plain pressure passes at equal post-projection divergence L2
@@ -165,6 +163,11 @@ def main():
                 "pressure_stage_speedup_vs_rbgs": ev.get("pressure_stage_speedup_vs_rbgs"),
                 "pressure_stage_speedup_vs_jacobi": ev.get("pressure_stage_speedup_vs_jacobi"),
                 "pressure_stage_ms": ev["pressure_stage_ms"],
+                "projection_stage_ms": ev["projection_stage_ms"],
+                "projection_stage_speedup_vs_rbgs": ev.get("projection_stage_speedup_vs_rbgs"),
+                "projection_stage_speedup_vs_jacobi": ev.get("projection_stage_speedup_vs_jacobi"),
+                "stage_timing": "each arm's pressure solve (and whole projection) enqueued alone through the stage hooks "
+                                "after one step from the step-30 state, HIP events on the context stream",
                 "plain_rbgs_div_l2": ev["plain_rbgs_div_l2"], "plain_jacobi_div_l2": ev["plain_jacobi_div_l2"],
                 "learned_div_l2_by_k": {str(c["k"]): c["div_l2"] for c in ev["learned_curve"]},
                 "fallbacks": sum(sum(c["fallback"]) for c in ev["learned_curve"]),
@@ -173,16 +176,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        oracle = entry.load_oracle()
-        sim = oracle.OracleSim(abi, scene.desc, precision=64)
-        scenes.seed_oracle(sim, scene)
-        cpu_steps = 2
-        t = time.perf_counter()
-        sim.step(cpu_steps)
-        cpu_s = time.perf_counter() - t
-        sim.destroy()
-        cpu = {"value": round(cpu_steps / cpu_s, 5), "unit": "steps/s", "cores": oracle.threads(abi), "kind": "port",
-               "sample": f"{cpu_steps} steps of the same {n}^3 workload (FP64 host loops, OpenMP), {cpu_s:.1f} s"}
+        cpu = cpu_baseline(pkg, scene, steps_per_s)
 
     out = {
         "metric": METRIC,
@@ -204,12 +198,10 @@ def main():
         "config": {"workload": f"plume {n}^3, seeded smoothed-noise initial velocity, D=40 P=80 {args.smoother}, hipGraph step",
                    "grid": [n, n, n], "diffuse_iterations": 40, "pressure_iterations": 80, "smoother": args.smoother,
                    "iterate_storage": args.storage, "refinement_interval": args.interval if args.storage != "f32" else None,
-                   "dt": 0.1, "viscosity": 1e-4, "diffusion": 1e-4},
-        "roofline": {"bound": "hbm", "kernel": "velocity diffusion smoother pass", "achieved": dom["gbs"],
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "bytes_per_launch": dom["bytes"], "avg_us": dom["us"],
-                     "traffic_source": "profiles/traffic_plume256_kernels.json" if traffic else None},
+                   "dt": 0.1, "viscosity": 1e-4, "diffusion": 1e-4, "smoother_tiling": list(tiling)},
+        "roofline": roofline(dom, trace, traffic, big),
         "jacobi_hbm_gbs": stages["jacobi_pass"]["gbs"],
+        "jacobi_hbm_gbs_512": big["jacobi_pass"]["gbs"] if big else None,
         "learned_vs_plain_speedup": learned["speedup_vs_rbgs"] if learned else None,
         "learned_vs_plain": learned,
         "stages": stages,
@@ -217,6 +209,78 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def timed_stages(ctx, repeat, names=None):
+    """Mean GPU time and algorithmic bandwidth of single stage launches (stage hooks)."""
+    stage_list = (("velocity_pass", 1), ("pressure_pass", 0), ("jacobi_pass", 4), ("advect", 2), ("project", 3), ("copy", 5))
+    out = {}
+    for name, stage in stage_list:
+        if names is not None and name not in names:
+            continue
+        ms = ctx.time_stage(stage, repeat)
+        by = ctx.stage_bytes(stage)
+        out[name] = {"us": round(ms * 1e3, 3), "bytes": by, "gbs": round(by / (ms * 1e-3) / 1e9, 1)}
+    return out
+
+
+def roofline(dom, trace, traffic, big):
+    """The dominant kernel against the HBM peak. achieved = its algorithmic bytes (8 B per
+    cell: b and the black cells read, the black cells written) over its live mean launch
+    time; trace_* are the same from the committed kernel trace of this command."""
+    out = {"bound": "hbm", "kernel": f"velocity diffusion RBGS middle pass ({VELOCITY_PASS_KERNEL})", "achieved": dom["gbs"],
+           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+           "bytes_per_launch": dom["bytes"], "avg_us": dom["us"],
+           "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if traffic else None,
+           "note": "at 256^3 the pass's 128 MB working set fits the 256 MB last-level cache, so part of its reads "
+                   "come from there; frac_512 is the same pass where it does not"}
+    if trace:
+        t_gbs = dom["bytes"] / (trace["avg_us"] * 1e-6) / 1e9
+        out.update({"trace_avg_us": trace["avg_us"], "trace_frac": round(t_gbs / HBM_PEAK_GBS, 4),
+                    "trace_source": os.path.relpath(TRACE_FILE, ROOT)})
+    if big:
+        v = big["velocity_pass"]
+        out.update({"avg_us_512": v["us"], "achieved_512": v["gbs"], "frac_512": round(v["gbs"] / HBM_PEAK_GBS, 4)})
+    return out
+
+
+def cpu_baseline(pkg, scene, gpu_steps_per_s):
+    """The oracle on this host's cores: the 256^3 workload (FP64 and FP32, one step each,
+    all threads) and the 64^3 plume of 20/40 passes (FP64, 100 steps, all threads; FP64
+    and FP32, 3 steps, one thread)."""
+    abi, scenes = pkg.abi, pkg.scenes
+    oracle = entry.load_oracle()
+    threads = oracle.threads(abi)
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count()
+    small = scenes.plume_scene(64)
+
+    def run(sc, precision, n_threads, steps):
+        oracle.set_threads(abi, n_threads)
+        sim = oracle.OracleSim(abi, sc.desc, precision=precision)
+        try:
+            scenes.seed_oracle(sim, sc)
+            t = time.perf_counter()
+            sim.step(steps)
+            dt = time.perf_counter() - t
+        finally:
+            sim.destroy()
+            oracle.set_threads(abi, threads)
+        return {"steps": steps, "seconds": round(dt, 3), "steps_per_s": round(steps / dt, 5),
+                "gcell_updates_per_s": round(sc.desc.cells * steps / dt / 1e9, 6), "threads": n_threads}
+
+    runs = {"plume256_fp64_all": run(scene, 64, threads, 1), "plume256_fp32_all": run(scene, 32, threads, 1),
+            "plume64_fp64_all_100_steps": run(small, 64, threads, 100), "plume64_fp64_1_thread": run(small, 64, 1, 3),
+            "plume64_fp32_1_thread": run(small, 32, 1, 3)}
+    main_run = runs["plume256_fp64_all"]
+    total = sum(r["seconds"] for r in runs.values())
+    return {"value": main_run["steps_per_s"], "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"1 step of the same 256^3 workload (FP64 host loops, {threads} OpenMP threads); beside it FP32 and "
+                      f"the 64^3 plume (D=20, P=40) with 1 and {threads} threads, {total:.1f} s in all",
+            "nproc": nproc, "omp_max_threads": threads, "runs": runs,
+            "gpu_over_cpu": round(gpu_steps_per_s / main_run["steps_per_s"], 1)}
 
 
 def learned_step_study(pkg, args):

@@ -46,13 +46,6 @@ typedef enum StableFluidsExtWarmStart {
     STABLE_FLUIDS_EXT_WARM_START_LEARNED = 1,
 } StableFluidsExtWarmStart;
 
-typedef enum StableFluidsExtFlags {
-    /* Run FP32 RBGS solves two passes per kernel launch (even nx, even extents on periodic
-     * axes, nz >= 12 on a periodic z, fields below 2 GiB; other grids keep one pass per
-     * launch): the old iterate is read once and the second iterate written once, half the
-     * HBM traffic of two launches. Results are bit-identical either way. */
-    STABLE_FLUIDS_EXT_FLAG_PASS_PAIRS = 1,
-} StableFluidsExtFlags;
 
 typedef struct StableFluidsExtOptions {
     uint32_t struct_size;                /* sizeof(StableFluidsExtOptions) */
@@ -63,7 +56,7 @@ typedef struct StableFluidsExtOptions {
     uint32_t warm_start_iterations;      /* passes after a learned guess (<= pressure_iterations) */
     uint32_t warm_start_coarsening;      /* the network runs on the grid coarsened 2^c times per
                                             axis (c = 1, 2 or 3; 0 = 1); extents multiples of 2^(c+2) */
-    uint32_t flags;                      /* StableFluidsExtFlags bits; other bits must be zero */
+    uint32_t flags;                      /* reserved, must be zero */
 } StableFluidsExtOptions;
 
 /* Same as stable_fluids_create_context_cuda, with options (NULL = reference behaviour).
@@ -145,15 +138,20 @@ typedef enum StableFluidsExtStage {
     STABLE_FLUIDS_EXT_STAGE_PROJECT        = 3, /* projection + divergence + |v| */
     STABLE_FLUIDS_EXT_STAGE_JACOBI_PASS    = 4, /* one Jacobi pass of the pressure solve, whatever the smoother option */
     STABLE_FLUIDS_EXT_STAGE_COPY           = 5, /* plain copy of one field (8 B/cell), to calibrate byte counters */
-    STABLE_FLUIDS_EXT_STAGE_PRESSURE_PAIR  = 6, /* two RBGS passes of the pressure solve in one launch */
-    STABLE_FLUIDS_EXT_STAGE_VELOCITY_PAIR  = 7, /* two RBGS passes of one velocity component's diffusion in one launch */
+    /* 6, 7: not used */
     STABLE_FLUIDS_EXT_STAGE_FULL_PRESSURE_PASS = 8, /* one full-iterate pass (read x_old, b; write x_new: 12 B/cell) of the pressure solve */
     STABLE_FLUIDS_EXT_STAGE_FULL_VELOCITY_PASS = 9, /* the same for one velocity component's diffusion */
+    /* whole sub-sequences of the step, each exactly as the step enqueues it (several
+     * kernels: stable_fluids_ext_stage_bytes_cuda gives 0 for them) */
+    STABLE_FLUIDS_EXT_STAGE_PRESSURE_SOLVE = 10, /* pressure solve from the last divergence (learned or plain) */
+    STABLE_FLUIDS_EXT_STAGE_PROJECTION     = 11, /* divergence + pressure solve + projection of the diffused velocity */
 } StableFluidsExtStage;
 
-/* Enqueues `repeat` launches of one stage kernel on the context stream, outside the step
- * graph and with the step's own buffers and shapes, for per-kernel timing. The
- * simulation state is left undefined afterwards (re-seed before stepping again). */
+/* Enqueues `repeat` launches of one stage on the context stream, outside the step graph
+ * and with the step's own buffers and shapes, for per-kernel (or per-sub-sequence)
+ * timing. The simulation state is left undefined afterwards (re-seed before stepping
+ * again), except after PRESSURE_SOLVE and PROJECTION, which recompute what the step's
+ * own sub-sequence computes from the same inputs. */
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_run_stage_cuda(void* context, uint32_t stage, uint32_t repeat);
 
 /* Algorithmic HBM bytes one launch of `stage` moves (each array read once and written

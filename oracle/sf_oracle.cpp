@@ -524,6 +524,14 @@ void sfo_step(void* sim, int steps) {
     for (int i = 0; i < steps; ++i) static_cast<SimBase*>(sim)->step();
 }
 int sfo_get_field(void* sim, uint32_t kind, uint32_t h, double* o0, double* o1, double* o2) { return static_cast<SimBase*>(sim)->get(kind, h, o0, o1, o2); }
+// Threads the next parallel loops use (n >= 1; the CPU baseline times 1 and all).
+void sfo_set_threads(int n) {
+#ifdef _OPENMP
+    if (n >= 1) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
 int sfo_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

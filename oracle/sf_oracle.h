@@ -36,6 +36,7 @@ void sfo_step(void* sim, int steps);
 /* kind = StableFluidsViewKind; out1/out2 used for the SOA3 kinds. */
 int sfo_get_field(void* sim, uint32_t kind, uint32_t handle, double* out0, double* out1, double* out2);
 int sfo_threads(void);
+void sfo_set_threads(int n); /* threads of the following parallel loops (n >= 1) */
 
 /* ---- per-stage entry points (FP64), used by the oracle's own pinning tests ---- */
 double sfo_load_scalar(const double* field, int x, int y, int z, int nx, int ny, int nz, const StableFluidsScalarBoundaryConfig* bc);

@@ -57,6 +57,7 @@ def load(abi):
     lib.sfo_divergence.argtypes = [dp, dp, dp, dp, i32, i32, i32, d, fb]
     lib.sfo_project.argtypes = [dp, dp, dp, dp, i32, i32, i32, d, fb]
     lib.sfo_threads.restype = i32
+    lib.sfo_set_threads.argtypes = [i32]
     _lib = lib
     return lib
 
@@ -133,3 +134,7 @@ class OracleSim:
 
 def threads(abi):
     return load(abi).sfo_threads()
+
+
+def set_threads(abi, n):
+    load(abi).sfo_set_threads(int(n))

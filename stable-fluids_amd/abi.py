@@ -49,17 +49,16 @@ STORAGE_BF16 = 2
 STORAGE_F32_REFINED = 3
 WARM_START_ZERO = 0
 WARM_START_LEARNED = 1
-FLAG_PASS_PAIRS = 1
 STAGE_PRESSURE_PASS = 0
 STAGE_VELOCITY_PASS = 1
 STAGE_ADVECT = 2
 STAGE_PROJECT = 3
 STAGE_JACOBI_PASS = 4
 STAGE_COPY = 5
-STAGE_PRESSURE_PAIR = 6
-STAGE_VELOCITY_PAIR = 7
 STAGE_FULL_PRESSURE_PASS = 8
 STAGE_FULL_VELOCITY_PASS = 9
+STAGE_PRESSURE_SOLVE = 10
+STAGE_PROJECTION = 11
 TILE_128X16 = 0
 TILE_64X32 = 1
 TILE_64X16 = 2

@@ -51,7 +51,6 @@ struct Options {
     bool learned          = false;     // learned pressure warm start (sf_warm.hip)
     int warm_iterations   = 0;         // passes after an accepted guess
     int warm_factor       = 2;         // restriction factor of the network input
-    bool pairs            = false;     // two RBGS passes per launch where supported
 };
 
 struct ScalarField {
@@ -266,11 +265,10 @@ hipError_t enqueue_solve_mixed(Context& c, SolveKind solve, const float* b, cons
 }
 
 // Whether this context's FP32 smoother passes run two per launch under these rules.
-bool use_pairs(const Context& c, const StencilBC& bc) { return c.options.pairs && !c.options.refine && smooth2_supported(c.options.smoother, c.dims, bc); }
 
 // Whether a plain FP32 RBGS solve under these rules runs on the black-cell passes.
 bool use_half(const Context& c, const StencilBC& bc) {
-    return c.half[0] != nullptr && !c.options.refine && c.options.smoother == kSmoothRBGS && !use_pairs(c, bc) && rbgs_half_supported(c.dims, bc);
+    return c.half[0] != nullptr && !c.options.refine && c.options.smoother == kSmoothRBGS && rbgs_half_supported(c.dims, bc);
 }
 
 // K RBGS passes from x0 (nullptr = zero guess) into `out` that carry only the black cells
@@ -297,20 +295,15 @@ hipError_t enqueue_solve_half(Context& c, SolveKind solve, const float* b, const
 }
 
 // K smoother passes from x0 (nullptr = zero guess) into `out`, ping-ponging with `aux`:
-// launch l of L writes out when L - 1 - l is even, so the last one lands in out. With
-// pass pairs, an odd K starts with one single pass.
+// pass l of K writes out when K - 1 - l is even, so the last one lands in out.
 hipError_t enqueue_solve(Context& c, SolveKind solve, const float* b, const float* x0, float* out, float* aux, int passes, float coef, float last_scale, const StencilBC& bc, int slot_base) {
     if (c.options.refine) return enqueue_solve_mixed(c, solve, b, x0, out, aux, passes, coef, last_scale, bc, slot_base);
     if (use_half(c, bc)) return enqueue_solve_half(c, solve, b, x0, out, passes, coef, last_scale, bc);
-    const int pairs    = use_pairs(c, bc) ? passes / 2 : 0;
-    const int singles  = passes - 2 * pairs;
-    const int launches = singles + pairs;
-    const float* src   = x0;
-    for (int l = 0; l < launches; ++l) {
-        float* dst = ((launches - 1 - l) % 2 == 0) ? out : aux;
-        const SmoothPass pass{src, b, dst, coef, l == launches - 1 ? last_scale : 1.0f};
-        if (l < singles) SF_TRY(launch_smooth(c.stream, c.options.smoother, solve, pass, c.dims, bc));
-        else SF_TRY(launch_smooth2(c.stream, solve, pass, c.dims, bc));
+    const float* src = x0;
+    for (int l = 0; l < passes; ++l) {
+        float* dst = ((passes - 1 - l) % 2 == 0) ? out : aux;
+        const SmoothPass pass{src, b, dst, coef, l == passes - 1 ? last_scale : 1.0f};
+        SF_TRY(launch_smooth(c.stream, c.options.smoother, solve, pass, c.dims, bc));
         src = dst;
     }
     return hipSuccess;
@@ -331,27 +324,18 @@ hipError_t enqueue_learned_pressure(Context& c, const StencilBC& p_bc) {
     const Dims cd  = unet::net_dims(*w.net);
     const uint64_t vc = static_cast<uint64_t>(cd.nx) * cd.ny * cd.nz;
     // The P passes run as a list of launches. The C fallback-only passes come first: one
-    // launch of the grid-barrier solve where the grid allows it (the accepted guess then
-    // costs one exiting dispatch for all C passes), else RBGS pass pairs where the grid
-    // allows them, else single passes; each skipped dispatch costs a few microseconds.
-    // The K passes after it run single, or in pairs with the pass-pair option. Launch l
+    // launch of the one-launch fallback solve where the grid allows it (the accepted guess
+    // then costs one exiting dispatch for all C passes), else single passes, each skipped
+    // dispatch costing a few microseconds; the K passes after them run single. Launch l
     // of L writes dst(l), the last one p. The launches covering the first C passes are
     // skipped on an accepted guess, which therefore goes where the next launch reads it,
     // and the first launch starts from zero when it can be skipped (it only runs on
-    // fallback, as the plain solve's first launch). Chain, pair and single launches give
+    // fallback, as the plain solve's first launch). Chain and single launches give
     // identical results.
-    const bool two   = smooth2_supported(c.options.smoother, c.dims, p_bc);
     const bool chain = C > 0 && w.chain_grid > 0;
     std::vector<int> per; // passes per launch
     if (chain) {
         per.push_back(C);
-        if (two && c.options.pairs && K % 2 == 0) per.insert(per.end(), K / 2, 2);
-        else per.insert(per.end(), K, 1);
-    } else if (two && c.options.pairs && C % 2 == 0 && K % 2 == 0) {
-        per.assign(P / 2, 2);
-    } else if (two) {
-        per.assign(C % 2, 1); // an odd C starts with one single pass
-        per.insert(per.end(), C / 2, 2);
         per.insert(per.end(), K, 1);
     } else {
         per.assign(P, 1);
@@ -371,7 +355,7 @@ hipError_t enqueue_learned_pressure(Context& c, const StencilBC& p_bc) {
     // the K passes after the skippable ones as black-cell passes from the guess buffer (the
     // accepted guess, or the fallback result the chain leaves there), when nothing after
     // the chain is skippable
-    const bool half_k = K > 0 && (chain || C == 0) && !c.options.pairs && use_half(c, p_bc);
+    const bool half_k = K > 0 && (chain || C == 0) && use_half(c, p_bc);
     for (int l = 0; l < L; ++l) {
         if (half_k && l >= skipped) {
             SF_TRY(enqueue_solve_half(c, kSolvePoisson, f.rhs, guess, f.p, K, h * h, 1.0f, p_bc));
@@ -388,10 +372,46 @@ hipError_t enqueue_learned_pressure(Context& c, const StencilBC& p_bc) {
         const float* src = l == 0 ? (C > 0 ? nullptr : guess) : dst(l - 1);
         SmoothPass pass{src, f.rhs, dst(l), h * h, 1.0f};
         if (l < skipped) pass.skip = w.skip;
-        if (per[l] == 2) SF_TRY(launch_smooth2(c.stream, kSolvePoisson, pass, c.dims, p_bc));
-        else SF_TRY(launch_smooth(c.stream, c.options.smoother, kSolvePoisson, pass, c.dims, p_bc));
+        SF_TRY(launch_smooth(c.stream, c.options.smoother, kSolvePoisson, pass, c.dims, p_bc));
     }
     return hipSuccess;
+}
+
+// The pressure solve of the step from f.rhs into f.p: learned warm start or P passes from
+// zero (reference :643-650: the pressure restarts from zero).
+hipError_t enqueue_pressure_solve(Context& c) {
+    const auto& cfg      = c.config;
+    auto& f              = c.flow;
+    const StencilBC p_bc = pressure_rules(cfg.flow_boundary);
+    if (cfg.pressure_iterations > 0 && c.options.learned) return enqueue_learned_pressure(c, p_bc);
+    if (cfg.pressure_iterations > 0) return enqueue_solve(c, kSolvePoisson, f.rhs, nullptr, f.p, f.p_aux, cfg.pressure_iterations, cfg.cell_size * cfg.cell_size, 1.0f, p_bc, 3 * c.mixed.cycles);
+    return launch_fill(c.stream, f.p, 0.0f, c.cells);
+}
+
+// Projection of the diffused velocity u (reference :643-650): divergence, pressure solve,
+// then the gradient subtraction fused with the post-projection divergence and |v|.
+hipError_t enqueue_projection(Context& c, const float* const (&u)[3]) {
+    const auto& fb         = c.config.flow_boundary;
+    const float h          = c.config.cell_size;
+    auto& f                = c.flow;
+    const StencilBC div_bc = divergence_rules(fb);
+    const StencilBC p_bc   = pressure_rules(fb);
+    SF_TRY(launch_divergence(c.stream, f.rhs, u[0], u[1], u[2], c.dims, h, div_bc));
+    SF_TRY(enqueue_pressure_solve(c));
+    SF_TRY(launch_project(c.stream, f.vx, f.vy, f.vz, f.div, f.mag, u[0], u[1], u[2], f.p, c.dims, h, p_bc, div_bc));
+    return hipSuccess;
+}
+
+// The velocity the projection reads: the diffused one, or the advected one when there are
+// no diffusion passes (positive viscosity); a copy otherwise (enqueue_step).
+void diffused_velocity(Context& c, const float* (&u)[3]) {
+    auto& f           = c.flow;
+    const auto& cfg   = c.config;
+    const float alpha = cfg.dt * cfg.viscosity / (cfg.cell_size * cfg.cell_size);
+    const bool adv    = alpha > 0.0f && cfg.diffuse_iterations <= 0;
+    u[0]              = adv ? f.tx : f.ax;
+    u[1]              = adv ? f.ty : f.ay;
+    u[2]              = adv ? f.tz : f.az;
 }
 
 // The step sequence of the reference graph (backend-cuda.cu:612-671).
@@ -438,17 +458,7 @@ hipError_t enqueue_step(Context& c) {
         u[2] = f.az;
     }
 
-    // projection; pressure restarts from zero (reference :643-650)
-    const StencilBC div_bc = divergence_rules(fb);
-    const StencilBC p_bc   = pressure_rules(fb);
-    SF_TRY(launch_divergence(c.stream, f.rhs, u[0], u[1], u[2], c.dims, h, div_bc));
-    if (cfg.pressure_iterations > 0 && c.options.learned)
-        SF_TRY(enqueue_learned_pressure(c, p_bc));
-    else if (cfg.pressure_iterations > 0)
-        SF_TRY(enqueue_solve(c, kSolvePoisson, f.rhs, nullptr, f.p, f.p_aux, cfg.pressure_iterations, cfg.cell_size * cfg.cell_size, 1.0f, p_bc, 3 * c.mixed.cycles));
-    else
-        SF_TRY(launch_fill(c.stream, f.p, 0.0f, c.cells));
-    SF_TRY(launch_project(c.stream, f.vx, f.vy, f.vz, f.div, f.mag, u[0], u[1], u[2], f.p, c.dims, h, p_bc, div_bc));
+    SF_TRY(enqueue_projection(c, u));
 
     // scalars (reference :652-668)
     for (size_t si = 0; si < c.scalars.size(); ++si) {
@@ -511,8 +521,7 @@ StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const Stabl
         if (opt->iterate_storage > STABLE_FLUIDS_EXT_STORAGE_F32_REFINED) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         if (opt->defect_correction_interval > 4096) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         if (opt->pressure_warm_start > STABLE_FLUIDS_EXT_WARM_START_LEARNED) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
-        if (opt->flags & ~static_cast<uint32_t>(STABLE_FLUIDS_EXT_FLAG_PASS_PAIRS)) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
-        options.pairs = (opt->flags & STABLE_FLUIDS_EXT_FLAG_PASS_PAIRS) != 0;
+        if (opt->flags != 0) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE; // no flag is defined
         options.smoother = opt->smoother == STABLE_FLUIDS_EXT_SMOOTHER_JACOBI ? kSmoothJacobi : kSmoothRBGS;
         options.refine   = opt->iterate_storage != STABLE_FLUIDS_EXT_STORAGE_F32;
         options.storage  = opt->iterate_storage == STABLE_FLUIDS_EXT_STORAGE_F16 ? kStoreF16 : (opt->iterate_storage == STABLE_FLUIDS_EXT_STORAGE_BF16 ? kStoreBF16 : kStoreF32);
@@ -592,7 +601,7 @@ StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const Stabl
                 if (!c->alloc(*p, c->bytes)) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
         if (!c->alloc(c->norm_partials, sizeof(double) * (kNormBlocks * 4 + 4))) return STABLE_FLUIDS_RESULT_OUT_OF_MEMORY;
         // black-cell iterates, when some solve of a plain RBGS context can use them
-        bool half = !c->options.refine && c->options.smoother == kSmoothRBGS && !c->options.pairs;
+        bool half = !c->options.refine && c->options.smoother == kSmoothRBGS;
         if (half) {
             bool any = rbgs_half_supported(c->dims, pressure_rules(cfg.flow_boundary)) || rbgs_half_supported(c->dims, velocity_rules(cfg.flow_boundary, 0));
             for (const auto& sc : c->scalars) any = any || rbgs_half_supported(c->dims, scalar_rules(sc.desc.boundary));
@@ -1523,9 +1532,7 @@ STABLE_FLUIDS_API uint64_t stable_fluids_ext_stage_bytes_cuda(void* context, uin
     }
     case STABLE_FLUIDS_EXT_STAGE_FULL_PRESSURE_PASS:
     case STABLE_FLUIDS_EXT_STAGE_FULL_VELOCITY_PASS:
-    case STABLE_FLUIDS_EXT_STAGE_JACOBI_PASS:
-    case STABLE_FLUIDS_EXT_STAGE_PRESSURE_PAIR:
-    case STABLE_FLUIDS_EXT_STAGE_VELOCITY_PAIR: return 12ull * c.cells; // read x_old, b; write x_new (a pair: two passes)
+    case STABLE_FLUIDS_EXT_STAGE_JACOBI_PASS: return 12ull * c.cells;   // read x_old, b; write x_new
     case STABLE_FLUIDS_EXT_STAGE_ADVECT: return 24ull * c.cells;        // read v (3); write t (3)
     case STABLE_FLUIDS_EXT_STAGE_PROJECT: return 36ull * c.cells;       // read a (3), p; write v (3), div, |v|
     case STABLE_FLUIDS_EXT_STAGE_COPY: return 8ull * c.cells;           // read one field, write one
@@ -1854,17 +1861,11 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_run_stage_cuda(void* cont
             e = sf::launch_smooth(c.stream, sf::kSmoothJacobi, sf::kSolvePoisson, sf::SmoothPass{odd ? f.p : f.p_aux, f.div, odd ? f.p_aux : f.p, h * h, 1.0f}, c.dims, sf::pressure_rules(fb));
             break;
         }
-        case STABLE_FLUIDS_EXT_STAGE_PRESSURE_PAIR: {
-            const bool odd = i & 1;
-            if (!sf::smooth2_supported(sf::kSmoothRBGS, c.dims, sf::pressure_rules(fb))) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
-            e = sf::launch_smooth2(c.stream, sf::kSolvePoisson, sf::SmoothPass{odd ? f.p : f.p_aux, f.div, odd ? f.p_aux : f.p, h * h, 1.0f}, c.dims, sf::pressure_rules(fb));
-            break;
-        }
-        case STABLE_FLUIDS_EXT_STAGE_VELOCITY_PAIR: {
-            const bool odd    = i & 1;
-            const float alpha = c.config.dt * c.config.viscosity / (h * h);
-            if (!sf::smooth2_supported(sf::kSmoothRBGS, c.dims, sf::velocity_rules(fb, 0))) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
-            e = sf::launch_smooth2(c.stream, sf::kSolveDiffusion, sf::SmoothPass{odd ? f.ax : f.vx, f.tx, odd ? f.vx : f.ax, alpha, 1.0f}, c.dims, sf::velocity_rules(fb, 0));
+        case STABLE_FLUIDS_EXT_STAGE_PRESSURE_SOLVE: e = sf::enqueue_pressure_solve(c); break;
+        case STABLE_FLUIDS_EXT_STAGE_PROJECTION: {
+            const float* u[3];
+            sf::diffused_velocity(c, u);
+            e = sf::enqueue_projection(c, u);
             break;
         }
         case STABLE_FLUIDS_EXT_STAGE_COPY: e = sf::launch_copy_scaled(c.stream, f.ax, f.vx, 1.0f, c.cells); break;

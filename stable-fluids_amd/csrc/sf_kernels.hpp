@@ -206,10 +206,6 @@ hipError_t launch_advect_velocity_buffers(hipStream_t s, float* tx, float* ty, f
 hipError_t launch_advect_scalar_buffers(hipStream_t s, float* out, const float* src, const float* vx, const float* vy, const float* vz, float dt, Dims d, float h, const StableFluidsScalarBoundaryConfig& sbc, const StableFluidsFlowBoundaryConfig& fbc);
 hipError_t launch_advect_scalar(hipStream_t s, float* out, const float* src, const float* vx, const float* vy, const float* vz, float dt, Dims d, float h, const StableFluidsScalarBoundaryConfig& sbc, const StableFluidsFlowBoundaryConfig& fbc);
 hipError_t launch_smooth(hipStream_t s, SmootherKind smoother, SolveKind solve, const SmoothPass& pass, Dims d, const StencilBC& bc);
-// Two RBGS passes in one launch (x_old -> x_new, the first pass unscaled, out_scale on the
-// second), bit-identical to two launch_smooth calls; x_old and x_new must differ.
-bool smooth2_supported(SmootherKind smoother, Dims d, const StencilBC& bc);
-hipError_t launch_smooth2(hipStream_t s, SolveKind solve, const SmoothPass& pass, Dims d, const StencilBC& bc);
 // The first `passes` (> 0) passes of an RBGS solve from zero in one launch that runs only
 // when skip[0] == 0 (the learned warm start's fallback), ping-ponging io -> other -> io
 // ..., bit-identical to `passes` launch_smooth calls from zero, with the result in io for

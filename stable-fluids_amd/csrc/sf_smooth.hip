@@ -1323,396 +1323,6 @@ __global__ __launch_bounds__(kThreads) void k_jacobi_batched(const SmoothDesc* _
         jacobi_body<kSolve, kZero, StoreF32>(sd.x_old, sd.b, sd.x_new, sd.coef, sd.out_scale, sd.d, sd.bc, sd.g, nullptr, static_cast<int>(it.block));
 }
 
-// ---- two RBGS iterations per pass ----
-//
-// One pass = two red-black iterations from x_old (or zeros) into x_new (even nx, FP32
-// storage): the old iterate is read and the second iterate written once, 12 B/cell for
-// two iterations. Five stages run as one z march:
-//   S0 black cells of the old iterate (loaded), S1 red cells of iteration 1, S2 its black
-//   cells, S3 red cells of iteration 2, S4 its black cells (stored with S3's reds).
-// At step k stage s handles plane k - s, relaxing it from planes k - s - 1 .. k - s + 1 of
-// stage s - 1, which that stage produced in this step or the two before (three-plane LDS
-// rings, four barriers per step). A ring plane stores one float per x pair, the cell of
-// the stage's colour: the pair p of row y, plane m holds x = 2p + t with t = (y + m) & 1
-// for red stages and its complement for black ones, and its x neighbours are the stored
-// cells of pairs p - 1 + t and p + t of the previous stage.
-// Regions, as the last stage needs them: S3 rows -1..16, S2 -2..17, S1 -3..18, S0 -4..19 of
-// the tile; pairs 0..63 on every row plus halo pairs S0 {-2,-1,64,65}, S1 {-1,64,65} (t=0)
-// or {-2,-1,64} (t=1), S2 {-1,64}, S3 {64} (t=0) or {-1} (t=1) on all but the outermost
-// row of the stage, which only feeds y neighbours of the next stage.
-// Ghosts. A ghost position read by stage s (outside a non-periodic face, one layer deep)
-// is adjacent to the cell being relaxed, and the reference's ghost is the face rule of that
-// cell's value before the stage (backend-cuda.cu:62-189). That value is the previous stage
-// of the cell's colour, so stage s - 1 stores at its ghost positions the rule applied to
-// its own input at the adjacent cell (S0: the old value of the adjacent red cell). Deeper
-// positions hold finite values that no real cell reads.
-// Work split. Waves relax whole 64-pair rows, where the colour offset, ghost rows and
-// ghost planes are wave-uniform; the halo pairs of S1, S2 and S3 go to one wave each
-// (waves 7, 6, 5), which takes one row of the stage, and those of S0 to waves 0 and 1.
-namespace two {
-constexpr int kH[5]      = {4, 3, 2, 1, 0};        // stage s covers rows -kH .. 15 + kH
-constexpr int kRows[5]   = {24, 22, 20, 18, 16};
-constexpr int kStride[5] = {8, 7, 7, 7, 8};        // row step between a wave's rows
-constexpr int kMiscW[5]  = {0, 7, 6, 5, -1};       // wave with the stage's halo pairs (S0: 0, 1)
-constexpr int kWp[4]     = {68, 68, 66, 66};       // LDS pairs per row
-constexpr int kPx[4]     = {2, 2, 1, 1};           // LDS column of pair 0
-constexpr int kFrame[4]  = {24 * 68, 22 * 68, 20 * 66, 18 * 66};
-constexpr int kOff[4]    = {0, 3 * kFrame[0], 3 * (kFrame[0] + kFrame[1]), 3 * (kFrame[0] + kFrame[1] + kFrame[2])};
-constexpr int kFloats    = 3 * (kFrame[0] + kFrame[1] + kFrame[2] + kFrame[3]);
-static_assert(kFloats * 4 <= 80 * 1024, "two workgroups per CU");
-
-__device__ __forceinline__ int frame_index(const int s, const int r, const int p) {
-    int off = kOff[0], h = kH[0], w = kWp[0], px = kPx[0];
-    if (s == 1) off = kOff[1], h = kH[1], w = kWp[1], px = kPx[1];
-    if (s == 2) off = kOff[2], h = kH[2], w = kWp[2], px = kPx[2];
-    if (s == 3) off = kOff[3], h = kH[3], w = kWp[3], px = kPx[3];
-    return off + (r + h) * w + p + px;
-}
-} // namespace two
-
-template <int kSolve, bool kZero>
-__device__ __forceinline__ void rbgs2_body(const float* __restrict__ x_old, const float* __restrict__ b, float* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC& bc, const TileGeom& g, const int w) {
-    using namespace two;
-    __shared__ float lds[kFloats];
-    __shared__ float s_rule_a[8], s_rule_c[8];
-    __shared__ int s_rule_k[8];
-
-    int tx, ty, cz;
-    tile_of(g, w, tx, ty, cz);
-    const int lane       = static_cast<int>(threadIdx.x) & 63;
-    const int wave       = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
-    const int x0         = tx * kTX, y0 = ty * kTY;
-    const int z0         = cz * g.zc;
-    const int z1         = min(z0 + g.zc, d.nz);
-    const int nz         = d.nz;
-    const bool px        = bc.periodic & 1, py = (bc.periodic >> 1) & 1, pz = (bc.periodic >> 2) & 1;
-    const int pbyte      = 4 * d.nx * d.ny;
-    const int row_b      = 4 * d.nx;
-    // whole-field resources (the field is below 2 GiB); planes are selected by the scalar
-    // offset, and a 0-byte resource drops the stores of inactive steps
-    const uint32_t fbyte = static_cast<uint32_t>(pbyte) * static_cast<uint32_t>(nz);
-    const Rsrc r_old     = plane_rsrc(x_old, kZero ? 0u : fbyte);
-    const Rsrc r_b       = plane_rsrc(b, fbyte);
-
-    // ghost rules in LDS (a, c, k of faces 0..5, identity at 6): read only where a tile
-    // touches a face, so they do not occupy scalar registers in the march
-    if (threadIdx.x < 7) {
-        const int f       = static_cast<int>(threadIdx.x);
-        const LinGhost lg = f == 6 ? lin_identity() : lin_ghost(face_rule(bc, f));
-        s_rule_a[f]       = lg.a;
-        s_rule_c[f]       = lg.c;
-        s_rule_k[f]       = lg.k ? 1 : 0;
-    }
-    auto rule = [&](const int f) { return LinGhost{s_rule_a[f], s_rule_c[f], s_rule_k[f] != 0}; };
-
-    // the lane's pair x0 + 2 lane; past a non-periodic x = nx it is a ghost (S0 loads cell
-    // nx - 1, later stages take pair lane - 1), on a periodic axis it wraps
-    const int xa        = x0 + 2 * lane;
-    const bool lane_xg  = !px && xa >= d.nx;
-    const bool x_ghosts = !px && x0 + kTX > d.nx; // block-uniform
-    const int xw        = px ? wrap_index(xa, d.nx) : min(xa, d.nx - 2);
-    const int ld0       = lane_xg ? 4 * (d.nx - 1) : 4 * xw;
-    const int ld1       = lane_xg ? 4 * (d.nx - 1) : 4 * xw + 4;
-    const int bx        = 4 * xw;
-
-    // rows of each stage: the wave's rows i0 + stride * j - kH (j < count). Packed per
-    // stage: bits 0-1 count, 2 colour offset t of the first row at even planes, 3 special
-    // (a ghost row or x ghosts), 4-6 ghost rows, 7-9 ghost rows below y = 0
-    int pk[5], wb[4], rb[5];
-    int o_off0[3], o_off1[3], b_off[5][3], st_off[2];
-#pragma unroll
-    for (int s = 0; s < 5; ++s) {
-        int i0, n;
-        if (s == 0 || s == 4) {
-            i0 = wave;
-            n  = s == 0 ? 3 : 2;
-        } else if (wave == kMiscW[s]) {
-            i0 = kRows[s] - 1;
-            n  = 1;
-        } else {
-            const int wp = wave - (wave > kMiscW[s] ? 1 : 0);
-            i0           = wp;
-            n            = wp + 14 < kRows[s] - 1 ? 3 : 2;
-        }
-        const int cs = (s & 1) ? 0 : 1; // black stages take the complement
-        int v        = n | ((((y0 + i0 - kH[s]) & 1) ^ cs) << 2);
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const int y  = y0 + i0 + kStride[s] * j - kH[s];
-            const int ro = fix_coord(y, d.ny, py) * row_b;
-            if (j < n && !py && (y < 0 || y >= d.ny)) v |= (1 << (4 + j)) | (1 << 3);
-            if (y < 0) v |= 1 << (7 + j);
-            if (s == 0) {
-                o_off0[j] = ld0 + ro;
-                o_off1[j] = ld1 + ro;
-            } else {
-                b_off[s][j] = bx + ro;
-            }
-            if (s == 4 && j < 2) st_off[j] = (xa < d.nx && y < d.ny) ? 4 * xa + y * row_b : kOutOfRange;
-        }
-        if (x_ghosts) v |= 1 << 3;
-        pk[s] = v;
-        if (s < 4) wb[s] = kOff[s] + i0 * kWp[s] + lane + kPx[s];
-        if (s > 0) rb[s] = kOff[s - 1] + (i0 + 1) * kWp[s - 1] + lane + kPx[s - 1];
-    }
-
-    // halo pairs: one per lane on the waves that hold them, for even and odd planes
-    const int ms = wave < 2 ? 0 : (wave == 7 ? 1 : (wave == 6 ? 2 : (wave == 5 ? 3 : -1)));
-    int m_wr[2], m_rd[2], m_rdx[2], m_adj[2], m_off[2], m_face[2];
-    {
-        int nq = 1, e = 0;
-        if (ms == 0) nq = 4, e = 44 * wave + lane % 44;
-        if (ms == 1) nq = 3, e = lane % 60;
-        if (ms == 2) nq = 2, e = lane % 36;
-        if (ms == 3) nq = 1, e = lane % 16;
-        const int s   = ms < 0 ? 1 : ms;
-        const int q   = e % nq;
-        const int r   = e / nq + 1 - kH[s];
-        const int y   = y0 + r;
-        const bool yg = !py && (y < 0 || y >= d.ny);
-        const int fy  = fix_coord(y, d.ny, py);
-        const int wp1 = s >= 3 ? kWp[2] : kWp[0]; // LDS row of stage s - 1
-#pragma unroll
-        for (int mp = 0; mp < 2; ++mp) {
-            const int t = ((y + mp) & 1) ^ ((s & 1) ? 0 : 1);
-            int p;
-            if (s == 0) p = q < 2 ? q - 2 : 62 + q;
-            else if (s == 1) p = t ? (q < 2 ? q - 2 : 64) : (q == 0 ? -1 : 63 + q);
-            else if (s == 2) p = q ? 64 : -1;
-            else p = t ? -1 : 64;
-            const int x   = x0 + 2 * p;
-            const bool xg = !px && (x < 0 || x >= d.nx);
-            const int fx  = px ? wrap_index(x, d.nx) : min(max(x, 0), d.nx - 2);
-            const int rd  = frame_index(s > 0 ? s - 1 : 0, r, p); // unused for s = 0
-            m_wr[mp]      = frame_index(s, r, p);
-            m_rd[mp]      = rd;
-            m_rdx[mp]     = rd + t - 1;
-            m_adj[mp]     = xg ? (x < 0 ? rd + 1 : rd - 1) : (yg ? (y < 0 ? rd + wp1 : rd - wp1) : rd);
-            if (s == 0) m_off[mp] = 4 * (fy * d.nx + (xg ? (x < 0 ? 0 : d.nx - 1) : fx + t));
-            else m_off[mp] = 4 * (fy * d.nx + fx + t);
-            m_face[mp] = xg ? (x < 0 ? 0 : 1) : (yg ? (y < 0 ? 2 : 3) : 6); // 6: real (identity)
-        }
-    }
-    const int mwr0 = m_wr[0], mwr1 = m_wr[1], mrd0 = m_rd[0], mrd1 = m_rd[1], mrdx0 = m_rdx[0], mrdx1 = m_rdx[1];
-    const int madj0 = m_adj[0], madj1 = m_adj[1], moff0 = m_off[0], moff1 = m_off[1], mface0 = m_face[0], mface1 = m_face[1];
-    auto pick = [](const int mp, const int v0, const int v1) { return mp ? v1 : v0; };
-    __syncthreads();
-
-    // plane offsets (bytes) and realness of planes k + 1 - i, i = 0..5, rolled per step
-    auto plane_off = [&](const int m) {
-        int q = m;
-        if (pz) q = q < 0 ? q + nz : (q >= nz ? q - nz : q); // nz >= 12 on a periodic z
-        else q = min(max(q, 0), nz - 1);
-        return q * pbyte;
-    };
-    auto plane_real = [&](const int m) { return (pz || static_cast<unsigned>(m) < static_cast<unsigned>(nz)) ? 1 : 0; };
-
-    float o_reg[3] = {0.0f, 0.0f, 0.0f}, o_m = 0.0f;
-    float b_reg[5][3], b_m[5];
-#pragma unroll
-    for (int s = 0; s < 5; ++s) {
-        b_m[s] = 0.0f;
-#pragma unroll
-        for (int j = 0; j < 3; ++j) b_reg[s][j] = 0.0f;
-    }
-    float f_out[3] = {0.0f, 0.0f, 0.0f}, f_red[2] = {0.0f, 0.0f};
-
-    // stage 0: S0 plane k (slot SL) from the registers; real = plane k is in the grid
-    auto stage0 = [&]<int SL>(const int k, const int real, const int p0) {
-        float* dst = lds + SL * kFrame[0];
-        float v[3];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) v[j] = kZero ? 0.0f : o_reg[j];
-        if (!real || (p0 & 8)) {
-            if (!real) {
-                const LinGhost gz = rule(k < 0 ? 4 : 5);
-#pragma unroll
-                for (int j = 0; j < 3; ++j) v[j] = lin_apply(gz, v[j]);
-            } else {
-#pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    if ((p0 >> (4 + j)) & 1) v[j] = lin_apply(rule(((p0 >> (7 + j)) & 1) ? 2 : 3), v[j]);
-            }
-            if (x_ghosts) {
-                const LinGhost gx = rule(1);
-#pragma unroll
-                for (int j = 0; j < 3; ++j) v[j] = lane_xg ? lin_apply(gx, v[j]) : v[j];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 3; ++j) dst[wb[0] + kStride[0] * j * kWp[0]] = v[j];
-        if (ms == 0) {
-            const int mp = k & 1;
-            float u      = kZero ? 0.0f : o_m;
-            u            = !real ? lin_apply(rule(k < 0 ? 4 : 5), u) : lin_apply(rule(pick(mp, mface0, mface1)), u);
-            dst[pick(mp, mwr0, mwr1)] = u;
-        }
-    };
-
-    // stage S (1..4) of plane m = k - S: S(S-1) planes m - 1, m, m + 1 in slots SM, S0_, SP,
-    // S(S) plane m into slot SW; S4 leaves its black values in f_out and reds in f_red
-    auto stage = [&]<int S, int SM, int S0_, int SP, int SW>(const int m, const int real, const int pS) {
-        constexpr int W1 = kWp[S - 1], F1 = kFrame[S - 1], kStr = kStride[S];
-        const float* P0 = lds + S0_ * F1;
-        const float* Pm = lds + SM * F1;
-        const float* Pp = lds + SP * F1;
-        const int n     = pS & 3;
-        const int t0    = ((pS >> 2) ^ m) & 1;
-        StageDiv<kSolve> dvs(coef);
-        float x[3], q[3];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            x[j] = 0.0f;
-            if (j == 0 || j < n) {
-                const int t = t0 ^ ((kStr * j) & 1);
-                const int c = rb[S] + kStr * j * W1;
-                float sum   = P0[c + t - 1];
-                sum         = sum + P0[c + t];
-                sum         = sum + P0[c - W1];
-                sum         = sum + P0[c + W1];
-                sum         = sum + Pm[c];
-                sum         = sum + Pp[c];
-                x[j]        = dvs.num(b_reg[S][j], sum, coef);
-            }
-            q[j] = dvs.quot(x[j]);
-        }
-        if (dvs.bad()) {
-#pragma unroll
-            for (int j = 0; j < 3; ++j) q[j] = dvs.exact(x[j]);
-        }
-        if (!real || (pS & 8)) {
-            if (!real) {
-                const LinGhost gz = rule(m < 0 ? 4 : 5);
-                const float* Pa   = m < 0 ? Pp : Pm;
-#pragma unroll
-                for (int j = 0; j < 3; ++j) q[j] = lin_apply(gz, Pa[rb[S] + kStr * j * W1]);
-            } else {
-#pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    if ((pS >> (4 + j)) & 1) {
-                        const bool below = (pS >> (7 + j)) & 1;
-                        q[j]             = lin_apply(rule(below ? 2 : 3), P0[rb[S] + kStr * j * W1 + (below ? W1 : -W1)]);
-                    }
-            }
-            if (x_ghosts) {
-                const LinGhost gx = rule(1);
-#pragma unroll
-                for (int j = 0; j < 3; ++j) q[j] = lane_xg ? lin_apply(gx, P0[rb[S] + kStr * j * W1 - 1]) : q[j];
-            }
-        }
-        if constexpr (S < 4) {
-            float* dst = lds + SW * kFrame[S];
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-                if (j == 0 || j < n) dst[wb[S] + kStr * j * kWp[S]] = q[j];
-            if (ms == S) {
-                const int mp = m & 1;
-                StageDiv<kSolve> dv2(coef);
-                const int c    = pick(mp, mrd0, mrd1);
-                const int cx   = pick(mp, mrdx0, mrdx1);
-                float sum      = P0[cx];
-                sum            = sum + P0[cx + 1];
-                sum            = sum + P0[c - W1];
-                sum            = sum + P0[c + W1];
-                sum            = sum + Pm[c];
-                sum            = sum + Pp[c];
-                const float xm = dv2.num(b_m[S], sum, coef);
-                float qm       = dv2.quot(xm);
-                if (dv2.bad()) qm = dv2.exact(xm);
-                const int face = pick(mp, mface0, mface1);
-                float u        = face == 6 ? qm : lin_apply(rule(face), P0[pick(mp, madj0, madj1)]);
-                if (!real) u = lin_apply(rule(m < 0 ? 4 : 5), m < 0 ? Pp[c] : Pm[c]);
-                dst[pick(mp, mwr0, mwr1)] = u;
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                f_out[j] = q[j];
-                f_red[j] = P0[rb[S] + kStr * j * W1];
-            }
-        }
-    };
-
-    // b of stage S's cells of plane m (byte offset off) into registers
-    auto fetch_b = [&]<int S>(const int m, const int off, const int pS) {
-        const int t0 = ((pS >> 2) ^ m) & 1;
-#pragma unroll
-        for (int j = 0; j < (S == 4 ? 2 : 3); ++j) {
-            const int t = t0 ^ ((kStride[S] * j) & 1);
-            b_reg[S][j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r_b, b_off[S][j], off + 4 * t, 0));
-        }
-        if constexpr (S < 4)
-            if (ms == S) b_m[S] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r_b, pick(m & 1, moff0, moff1), off, 0));
-    };
-    auto fetch_old = [&](const int m, const int off, const int p0) {
-        if constexpr (kZero) return;
-        const bool t = (((p0 >> 2) ^ m) & 1) != 0;
-#pragma unroll
-        for (int j = 0; j < 3; ++j) o_reg[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r_old, t ? o_off1[j] : o_off0[j], off, 0));
-        if (ms == 0) o_m = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r_old, pick(m & 1, moff0, moff1), off, 0));
-    };
-
-    // the march: step k relaxes plane k - s in stage s; steps come in threes so that every
-    // ring slot is a compile-time constant (plane m sits in slot (m - z0 + 8) % 3)
-    int k          = z0 - 4;
-    const int kend = z1 + 3;
-    int o0 = plane_off(k), o1 = 0, o2 = 0, o3 = 0, o4 = 0, o5 = 0; // planes k + 1 - i after the shift
-    unsigned rbits = static_cast<unsigned>(plane_real(k));
-    fetch_old(k, o0, pk[0]);
-    auto step = [&]<int U>(const int p0, const int p1, const int p2, const int p3, const int p4) {
-        // roll the plane window: o_i = offset of plane k + 1 - i, bit i of rbits its realness
-        o5    = o4;
-        o4    = o3;
-        o3    = o2;
-        o2    = o1;
-        o1    = o0;
-        o0    = plane_off(k + 1);
-        rbits = (rbits << 1) | static_cast<unsigned>(plane_real(k + 1));
-        constexpr int kSl = (U + 4) % 3; // slot of plane k
-        stage0.template operator()<kSl>(k, (rbits >> 1) & 1, p0);
-        fetch_old(k + 1, o0, p0);
-        __syncthreads();
-        if (k >= z0 - 2) stage.template operator()<1, (U + 2) % 3, (U + 3) % 3, (U + 4) % 3, (U + 3) % 3>(k - 1, (rbits >> 2) & 1, p1);
-        fetch_b.template operator()<1>(k, o1, p1);
-        __syncthreads();
-        if (k >= z0) stage.template operator()<2, (U + 1) % 3, (U + 2) % 3, (U + 3) % 3, (U + 2) % 3>(k - 2, (rbits >> 3) & 1, p2);
-        fetch_b.template operator()<2>(k - 1, o2, p2);
-        __syncthreads();
-        if (k >= z0 + 2) stage.template operator()<3, (U + 0) % 3, (U + 1) % 3, (U + 2) % 3, (U + 1) % 3>(k - 3, (rbits >> 4) & 1, p3);
-        fetch_b.template operator()<3>(k - 2, o3, p3);
-        __syncthreads();
-        const bool live = k >= z0 + 4;
-        if (live) stage.template operator()<4, (U + 2) % 3, (U + 0) % 3, (U + 1) % 3, 0>(k - 4, 1, p4);
-        const Rsrc dst = plane_rsrc(x_new, live ? fbyte : 0u); // a size select keeps it scalar
-        const int t    = (((p4 >> 2) ^ k) & 1) * 4; // black cell of the wave's rows (plane k - 4)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(f_out[j] * out_scale), dst, st_off[j], o5 + t, 0);
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(f_red[j] * out_scale), dst, st_off[j], o5 + 4 - t, 0);
-        }
-        fetch_b.template operator()<4>(k - 3, o4, p4);
-    };
-    for (;;) {
-        // the packed row facts pass through an empty asm once per three steps: values
-        // derived from them are then recomputed in the march instead of being hoisted
-        // into scalar registers that spill
-        int p0 = pk[0], p1 = pk[1], p2 = pk[2], p3 = pk[3], p4 = pk[4];
-        asm volatile("" : "+s"(p0), "+s"(p1), "+s"(p2), "+s"(p3), "+s"(p4));
-        step.template operator()<0>(p0, p1, p2, p3, p4);
-        if (++k > kend) break;
-        step.template operator()<1>(p0, p1, p2, p3, p4);
-        if (++k > kend) break;
-        step.template operator()<2>(p0, p1, p2, p3, p4);
-        if (++k > kend) break;
-    }
-}
-
-template <int kSolve, bool kZero>
-__global__ __launch_bounds__(kThreads) void k_rbgs2_pass(const float* __restrict__ x_old, const float* __restrict__ b, float* __restrict__ x_new, const float coef, const float out_scale, const Dims d, const StencilBC bc, const TileGeom g, const int* __restrict__ skip) {
-    if (skip != nullptr && *skip != 0) return;
-    rbgs2_body<kSolve, kZero>(x_old, b, x_new, coef, out_scale, d, bc, g, xcd_block());
-}
-
 // Spins (s_sleep 4, about 0.1 us each) one waiting lane may spend over the whole launch:
 // about 0.4 s, far above a solve. The ticket order below guarantees every wait ends; the
 // budget only keeps the launch finite if that guarantee were ever broken.
@@ -1912,29 +1522,6 @@ hipError_t launch_smooth(hipStream_t s, const SmootherKind smoother, const Solve
 #undef SF_SPASS
 #undef SF_JPASS
 #undef SF_PASS_GENERAL
-    return hipGetLastError();
-}
-
-// The pair pass addresses the whole field through one buffer resource (below 2 GiB) and
-// wraps periodic z planes once (nz >= 12).
-bool smooth2_supported(const SmootherKind smoother, const Dims d, const StencilBC& bc) {
-    const uint64_t bytes = 4ull * static_cast<uint64_t>(d.nx) * static_cast<uint64_t>(d.ny) * static_cast<uint64_t>(d.nz);
-    return smoother == kSmoothRBGS && (d.nx & 1) == 0 && !has_odd_periodic(d, bc) && bytes < (1ull << 31) && (!(bc.periodic & 4) || d.nz >= 12);
-}
-
-hipError_t launch_smooth2(hipStream_t s, const SolveKind solve, const SmoothPass& pass, const Dims d, const StencilBC& bc) {
-    const TileGeom g      = plan_tiles(d, 0);
-    const unsigned blocks = smooth_blocks(g);
-    const bool zero       = pass.x_old == nullptr;
-#define SF_P2(SO, Z) k_rbgs2_pass<SO, Z><<<blocks, kThreads, 0, s>>>(pass.x_old, pass.b, pass.x_new, pass.coef, pass.out_scale, d, bc, g, pass.skip)
-    if (solve == kSolvePoisson) {
-        if (zero) SF_P2(kSolvePoisson, true);
-        else SF_P2(kSolvePoisson, false);
-    } else {
-        if (zero) SF_P2(kSolveDiffusion, true);
-        else SF_P2(kSolveDiffusion, false);
-    }
-#undef SF_P2
     return hipGetLastError();
 }
 

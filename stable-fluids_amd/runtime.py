@@ -121,8 +121,7 @@ def load_library(path=LIB_PATH):
 
 
 def ext_options(smoother=abi.SMOOTHER_RBGS, iterate_storage=abi.STORAGE_F32, defect_correction_interval=0,
-                pressure_warm_start=abi.WARM_START_ZERO, warm_start_iterations=0, warm_start_coarsening=0,
-                pass_pairs=False):
+                pressure_warm_start=abi.WARM_START_ZERO, warm_start_iterations=0, warm_start_coarsening=0):
     o = abi.ExtOptions()
     o.struct_size = C.sizeof(abi.ExtOptions)
     o.smoother = smoother
@@ -131,7 +130,7 @@ def ext_options(smoother=abi.SMOOTHER_RBGS, iterate_storage=abi.STORAGE_F32, def
     o.pressure_warm_start = pressure_warm_start
     o.warm_start_iterations = warm_start_iterations
     o.warm_start_coarsening = warm_start_coarsening
-    o.flags = abi.FLAG_PASS_PAIRS if pass_pairs else 0
+    o.flags = 0
     return o
 
 
@@ -240,6 +239,23 @@ class Context:
 
     def run_stage(self, stage, repeat):
         return self._call("stable_fluids_ext_run_stage_cuda", stage, repeat)
+
+    def time_stage(self, stage, repeat, warmup=2):
+        """Mean GPU milliseconds of one run of `stage`: HIP events recorded on the context
+        stream around `repeat` runs, after `warmup` untimed ones."""
+        s = C.c_void_p(self.stream)
+        ev0, ev1 = hip.Event(), hip.Event()
+        try:
+            if warmup:
+                self.run_stage(stage, warmup)
+            hip.check(hip.lib().hipEventRecord(ev0.handle, s), "hipEventRecord")
+            self.run_stage(stage, repeat)
+            hip.check(hip.lib().hipEventRecord(ev1.handle, s), "hipEventRecord")
+            ev1.synchronize()
+            return ev1.elapsed_ms(ev0) / repeat
+        finally:
+            ev0.destroy()
+            ev1.destroy()
 
     def set_pressure_model(self, params):
         p = np.ascontiguousarray(params, dtype=np.float32)

@@ -19,7 +19,6 @@ import json
 # bytes per cell a launch must move at minimum (4-byte fields)
 ALGORITHMIC_BYTES_PER_CELL = {
     "k_rbgs_split_pass": 12,      # read x_old, b; write x_new
-    "k_rbgs2_pass": 12,           # two passes: read x_old, b; write x_new once
     "k_jacobi_pass": 12,          # read x_old, b; write x_new
     "k_copy_scaled": 8,           # read src; write dst
     "k_advect_velocity": 24,      # read 3 velocity components; write 3
@@ -33,6 +32,16 @@ ALGORITHMIC_BYTES_PER_CELL = {
 }
 # first pass from a zero guess: x_old is not read
 ZERO_GUESS_BYTES_PER_CELL = 8
+# black-cell RBGS passes k_rbgs_half<solve, in, out, ...>: b (4 B per cell) plus the
+# input iterate (in 0: zero guess, none; 1: the natural field, 4 B; 2: the black cells,
+# 2 B) plus the output (out 0: the black cells, 2 B; 1: the natural field, 4 B)
+HALF_IN_BYTES = {0: 0, 1: 4, 2: 2}
+HALF_OUT_BYTES = {0: 2, 1: 4}
+
+
+def half_pass_bytes_per_cell(name):
+    args = [a.strip() for a in name.split("<")[1].split(">")[0].split(",")]
+    return 4 + HALF_IN_BYTES[int(args[1])] + HALF_OUT_BYTES[int(args[2])]
 
 
 def short_name(kernel_name):
@@ -41,10 +50,12 @@ def short_name(kernel_name):
 
 def algorithmic_bytes(name, cells):
     base = name.split("<")[0]
+    if base == "k_rbgs_half":
+        return half_pass_bytes_per_cell(name) * cells
     per = ALGORITHMIC_BYTES_PER_CELL.get(base)
     if per is None:
         return None
-    if base in ("k_rbgs_split_pass", "k_rbgs2_pass", "k_jacobi_pass") and name.split("<")[1].split(",")[1].strip() == "true":
+    if base in ("k_rbgs_split_pass", "k_jacobi_pass") and name.split("<")[1].split(",")[1].strip() == "true":
         per = ZERO_GUESS_BYTES_PER_CELL
     return per * cells
 

@@ -4,7 +4,10 @@ From common simulation states (velocity and density checkpointed from a plain ru
 step is taken with the plain solver (pressure from zero, P passes) and with the learned
 warm start (network guess + K passes) for a range of K. The smallest K whose
 divergence L2 matches the plain P-pass value at every checkpoint is the equal-quality
-setting; steps/s of both settings are then timed on the GPU.
+setting. Both settings are then timed on the GPU: whole steps/s, and the step's own
+pressure solve and projection (divergence + solve + gradient subtraction), each
+enqueued alone through the stage hooks from the same state and timed with HIP events on
+the context stream -- the summed time of that arm's own kernels.
 """
 import os
 import time
@@ -70,6 +73,18 @@ def steps_per_second(scene, state, options, params=None, steps=20, warmup=3):
         return steps / (time.perf_counter() - t)
 
 
+def stage_ms(scene, state, options, params=None, repeat=10):
+    """GPU ms of the step's pressure solve and of its whole projection, after one step
+    from `state` (the hooks then see that step's divergence and diffused velocity)."""
+    with runtime.Context(scene.desc, options=options) as ctx:
+        if params is not None:
+            ctx.set_pressure_model(params)
+        _restore(ctx, scene, state)
+        ctx.step(1)
+        return {"pressure_solve": ctx.time_stage(abi.STAGE_PRESSURE_SOLVE, repeat),
+                "projection": ctx.time_stage(abi.STAGE_PROJECTION, repeat)}
+
+
 def evaluate(n=256, plain_passes=80, k_list=(2, 4, 6, 8, 10, 12, 13, 14, 15, 16, 17, 18, 19, 20, 24, 32, 40, 56, 80), at_steps=(10, 20, 30),
              coarsening=3, params=None, time_steps=60):
     """Equal-quality comparison on the plume at n^3 (centre (n/2, 3n/32, n/2), radius n/16,
@@ -103,25 +118,25 @@ def evaluate(n=256, plain_passes=80, k_list=(2, 4, 6, 8, 10, 12, 13, 14, 15, 16,
     base_state = states[-1]
     plain_sps = steps_per_second(scene, base_state, rbgs, steps=time_steps)
     jac_sps = steps_per_second(scene, base_state, jac, steps=time_steps)
-    # the same step without any pressure pass: what the pressure stage costs on top
-    no_p = scenes.plume_scene(n, diffuse_iterations=20, pressure_iterations=0)
-    t_none = 1.0 / steps_per_second(no_p, base_state, rbgs, steps=time_steps)
     out["plain_rbgs_steps_per_s"] = plain_sps
     out["plain_jacobi_steps_per_s"] = jac_sps
-    out["no_pressure_steps_per_s"] = 1.0 / t_none
-    out["pressure_stage_ms"] = {"plain_rbgs": 1e3 * (1.0 / plain_sps - t_none), "plain_jacobi": 1e3 * (1.0 / jac_sps - t_none)}
-    for tag, k, base in (("rbgs", k_rbgs, plain_sps), ("jacobi", k_jac, jac_sps)):
+    st_rbgs = stage_ms(scene, base_state, rbgs)
+    st_jac = stage_ms(scene, base_state, jac)
+    out["pressure_stage_ms"] = {"plain_rbgs": st_rbgs["pressure_solve"], "plain_jacobi": st_jac["pressure_solve"]}
+    out["projection_stage_ms"] = {"plain_rbgs": st_rbgs["projection"], "plain_jacobi": st_jac["projection"]}
+    for tag, k, base, st_base in (("rbgs", k_rbgs, plain_sps, st_rbgs), ("jacobi", k_jac, jac_sps, st_jac)):
         if k is None:
             out[f"speedup_vs_{tag}"] = None
             continue
-        # the learned context runs only K passes after an accepted guess: time it with P = K
-        # + the skipped fallback passes as configured (P = plain_passes)
+        # the learned context (P = plain_passes, of which the first P - K run only on fallback)
         opts = runtime.ext_options(smoother=abi.SMOOTHER_RBGS, pressure_warm_start=abi.WARM_START_LEARNED,
                                    warm_start_iterations=k, warm_start_coarsening=coarsening)
         sps = steps_per_second(scene, base_state, opts, params, steps=time_steps)
+        st = stage_ms(scene, base_state, opts, params)
         out[f"learned_k{k}_steps_per_s"] = sps
         out[f"speedup_vs_{tag}"] = sps / base
-        stage = 1.0 / sps - t_none
-        out["pressure_stage_ms"][f"learned_k{k}"] = 1e3 * stage
-        out[f"pressure_stage_speedup_vs_{tag}"] = (1.0 / base - t_none) / stage if stage > 0 else None
+        out["pressure_stage_ms"][f"learned_k{k}"] = st["pressure_solve"]
+        out["projection_stage_ms"][f"learned_k{k}"] = st["projection"]
+        out[f"pressure_stage_speedup_vs_{tag}"] = st_base["pressure_solve"] / st["pressure_solve"]
+        out[f"projection_stage_speedup_vs_{tag}"] = st_base["projection"] / st["projection"]
     return out

@@ -129,6 +129,28 @@ def test_stage_timing_hooks():
             ctx.synchronize()
 
 
+@pytest.mark.parametrize("smoother", [abi.SMOOTHER_RBGS, abi.SMOOTHER_JACOBI])
+def test_projection_stage_hooks_recompute_the_step(smoother):
+    """The PRESSURE_SOLVE and PROJECTION hooks enqueue exactly the step's sub-sequence: run
+    after a step (same diffused velocity, same divergence) they reproduce the step's
+    pressure, projected velocity and divergence bit for bit."""
+    d = small_desc(32, 24, 16)
+    rng = np.random.default_rng(4)
+    with sf.Context(d, options=sf.ext_options(smoother=smoother)) as ctx:
+        ctx.set_flow_velocity(*(ctx.upload(rng.standard_normal(d.shape)) for _ in range(3)))
+        ctx.step(2)
+        ref = {k: ctx.read(k) for k in (abi.VIEW_PRESSURE, abi.VIEW_DIVERGENCE)}
+        ref_v = ctx.read(abi.VIEW_VELOCITY)
+        for stage in (abi.STAGE_PRESSURE_SOLVE, abi.STAGE_PROJECTION):
+            assert ctx.stage_bytes(stage) == 0
+            ctx.run_stage(stage, 2)
+            ctx.synchronize()
+            for k, v in ref.items():
+                assert np.array_equal(ctx.read(k), v), (stage, k)
+            for a, b in zip(ctx.read(abi.VIEW_VELOCITY), ref_v):
+                assert np.array_equal(a, b), stage
+
+
 def test_many_contexts_create_destroy():
     for i in range(5):
         d = small_desc(8 + i, 8, 8)

@@ -55,3 +55,29 @@ def test_committed_record_has_the_bench_kernel():
     key = src.split('VELOCITY_PASS_KERNEL = "')[1].split('"')[0]
     assert key in rec["kernels"]
     assert rec["kernels"][key]["traffic_bytes"] >= rec["kernels"][key]["algorithmic_bytes"]
+
+
+def test_black_cell_pass_bytes():
+    """b (4 B) + the iterate read (zero guess: none; natural: 4 B; black cells: 2 B) + the
+    iterate written (black cells: 2 B; natural: 4 B)."""
+    for args, per in (("1, 0, 0", 6), ("0, 2, 0", 8), ("0, 1, 0", 10), ("1, 2, 1", 10), ("0, 1, 1", 12)):
+        assert tr.algorithmic_bytes(f"k_rbgs_half<{args}, 64, 8, 2>", CELLS) == per * CELLS, args
+
+
+def test_trace_record(tmp_path):
+    spec2 = importlib.util.spec_from_file_location("trace_record", os.path.join(ROOT, "stable-fluids_amd", "trace_record.py"))
+    trr = importlib.util.module_from_spec(spec2)
+    spec2.loader.exec_module(trr)
+    path = tmp_path / "stats.csv"
+    with open(path, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Name", "Calls", "TotalDurationNs", "AverageNs", "Percentage", "MinNs", "MaxNs", "StdDev"])
+        w.writeheader()
+        w.writerow({"Name": "void sf::(anonymous namespace)::k_rbgs_half<0, 2, 0, 64, 8, 2>(sf::HalfPassArgs)", "Calls": 10,
+                    "TotalDurationNs": 10 * 1048.576, "AverageNs": 1048.576, "Percentage": 90.0, "MinNs": 1000, "MaxNs": 1100, "StdDev": 1})
+        w.writerow({"Name": "k_other(int)", "Calls": 1, "TotalDurationNs": 5, "AverageNs": 5, "Percentage": 10.0, "MinNs": 5, "MaxNs": 5, "StdDev": 0})
+    rec = trr.build(str(path), CELLS, "test")
+    k = rec["kernels"]["k_rbgs_half<0, 2, 0, 64, 8, 2>"]
+    # 8 B per cell of 2^20 cells in 1.048576 us: 8 TB/s
+    assert k["calls"] == 10 and k["algorithmic_bytes"] == 8 * CELLS and k["frac"] == pytest.approx(1.0)
+    assert "gbs" not in rec["kernels"]["k_other"]
+    assert "| `k_rbgs_half<0, 2, 0, 64, 8, 2>` | 10 |" in trr.markdown(rec, ["cmd"], 5)
