@@ -187,3 +187,19 @@ def init_params(seed, scale=1.0):
         w = rng.standard_normal((27, co, ci)) * np.sqrt(2.0 / (27 * ci)) * scale
         layers.append((w, np.zeros(co)))
     return layers
+
+
+def adam_step(params, grads, m, v, t, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+    """One Adam update in float64 (Kingma and Ba, "Adam: A Method for Stochastic
+    Optimization", 2015, Algorithm 1), the update stable_fluids_unet_adam_step_cuda applies:
+    m <- b1 m + (1 - b1) g, v <- b2 v + (1 - b2) g^2, then
+    p <- p - lr (m / (1 - b1^t)) / (sqrt(v / (1 - b2^t)) + eps), t counting from 1.
+    Returns (params, m, v) as new arrays."""
+    g = np.asarray(grads, dtype=np.float64)
+    m = beta1 * np.asarray(m, dtype=np.float64) + (1.0 - beta1) * g
+    v = beta2 * np.asarray(v, dtype=np.float64) + (1.0 - beta2) * g * g
+    m_hat = m / (1.0 - beta1 ** t)
+    v_hat = v / (1.0 - beta2 ** t)
+    p = np.asarray(params, dtype=np.float64) - lr * m_hat / (np.sqrt(v_hat) + eps)
+    return p, m, v
+

@@ -25,6 +25,7 @@
 // grid planes below/above a non-periodic z range hold the z-face rule of the edge plane.
 // Periodic axes load the wrapped cell. Red/black parity uses wrapped coordinates, and an
 // odd periodic axis, where a pair can hold two cells of one colour, takes a second relax.
+#include <atomic>
 #include <algorithm>
 #include <type_traits>
 
@@ -1471,11 +1472,26 @@ hipError_t launch_smooth_batched(hipStream_t s, const SmootherKind smoother, con
 // Whether a field of `elem` byte elements is below 2 GiB (one buffer resource spans it).
 static bool field_below_2g(const Dims d, const int elem) { return static_cast<uint64_t>(elem) * static_cast<uint64_t>(d.nx) * static_cast<uint64_t>(d.ny) * static_cast<uint64_t>(d.nz) < (1ull << 31); }
 
+// Compute units of the current device, read once per device.
+static int device_cus() {
+    static std::atomic<int> cached[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int n = cached[dev].load(std::memory_order_relaxed);
+    if (n <= 0) {
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cached[dev].store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
+
 hipError_t launch_smooth(hipStream_t s, const SmootherKind smoother, const SolveKind solve, const SmoothPass& pass, const Dims d, const StencilBC& bc) {
-    // Jacobi passes run about 256 workgroups (one 512-thread group per CU, z-chunks twice as
-    // long): 33.2 us at 256^3 against 35.1 us with two per CU and 37.4 us with four
+    // Jacobi passes run about one 512-thread workgroup per CU (z-chunks twice as long as
+    // plan_tiles', at most 64 planes): 33.2 us at 256^3 against 35.1 us with two per CU
+    // and 37.4 us with four
     const long tiles_j    = static_cast<long>((d.nx + kTX - 1) / kTX) * ((d.ny + kTY - 1) / kTY);
-    const int zc_j        = static_cast<int>(std::max<long>(4, (static_cast<long>(d.nz) * tiles_j + 255) / 256));
+    const long cus        = device_cus();
+    const int zc_j        = static_cast<int>(std::max<long>(4, std::min<long>(64, (static_cast<long>(d.nz) * tiles_j + cus - 1) / cus)));
     const TileGeom g      = plan_tiles(d, smoother == kSmoothJacobi ? zc_j : 0);
     const unsigned blocks = static_cast<unsigned>(g.tiles_x * g.tiles_y * g.chunks);
     const bool zero       = pass.x_old == nullptr;

@@ -170,6 +170,9 @@ TILE_CASES = {
     "periodic_yz_partial": (periodic_yz_desc, (136, 18, 20)),  # wrapped halo rows and planes
     "mixed_two_cells": (mixed_desc, (4, 2, 2)),                # every cell on a face
     "walls_two_cells": (walls_desc, (4, 2, 2)),
+    # 32 tiles of 128 x 16: Jacobi passes run 8-plane chunks (one workgroup per CU),
+    # RBGS ones their own plan
+    "walls_jacobi_long_chunks": (walls_desc, (512, 128, 64)),
 }
 
 

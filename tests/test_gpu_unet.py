@@ -114,6 +114,39 @@ def test_network_gradient_matches_oracle():
         assert ew <= 1e-2 and eb <= 1e-2, l
 
 
+def test_adam_steps_match_the_oracle():
+    """Three Adam steps on the GPU (FP32 moments and parameters) against the float64
+    update of oracle/unet_oracle.adam_step, each fed the gradient the GPU computed: every
+    parameter within 2 ulp of its value plus 1e-5 of the step size. eps = 1e-3 (about the
+    gradients' size) pins its placement outside the square root; three steps pin both
+    bias corrections."""
+    shape = (8, 8, 8)
+    rng = np.random.default_rng(11)
+    inp = rng.standard_normal(shape + (2,)).astype(np.float32)
+    target = rng.standard_normal(shape).astype(np.float32)
+    nz, ny, nx = shape
+    lr, b1, b2, eps = 2e-3, 0.9, 0.99, 1e-3
+    with U.UNet(nx, ny, nz) as net:
+        p = uo.join_params(uo.init_params(2)).astype(np.float32)
+        net.set_params(p)
+        ref, m, v = p.astype(np.float64), 0.0, 0.0
+        inp_d, tgt_d = dev(inp), dev(target)
+        for t in range(1, 4):
+            net.backward(inp_d, tgt_d)
+            hip.synchronize()
+            g = net.grads().astype(np.float64)
+            net.adam_step(lr, b1, b2, eps)
+            hip.synchronize()
+            got = net.params().astype(np.float64)
+            ref, m, v = uo.adam_step(ref, g, m, v, t, lr, b1, b2, eps)
+            err = np.abs(got - ref)
+            bound = 2.0 ** -22 * np.abs(ref) + 1e-5 * lr
+            print(f"adam step {t}: max err {err.max():.3e}, max step {np.max(np.abs(ref - p)):.3e}", flush=True)
+            assert np.all(err <= bound), (t, float(np.max(err - bound)))
+            # keep the reference on the GPU's rounded parameters (the next gradient is taken there)
+            ref = got
+
+
 def test_adam_training_reduces_loss():
     """A smooth target (box-filtered input channel 0) is learnable: 60 Adam steps on the
     network's own kernels cut the loss at least in half."""

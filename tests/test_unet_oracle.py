@@ -118,3 +118,37 @@ def test_bf16_round():
     assert r[2] == 1.0 + 2 ** -7               # above the tie rounds up
     assert r[3] == -2.5 and r[4] == 0.0
     assert np.isfinite(r[5])
+
+
+def test_adam_step_matches_torch_adam():
+    """The oracle's Adam update equals torch.optim.Adam (float64, no weight decay, no
+    amsgrad) over 6 steps with changing gradients, eps large enough to pin its placement."""
+    rng = np.random.default_rng(3)
+    p0 = rng.standard_normal(257)
+    grads = [rng.standard_normal(257) * 10.0 ** rng.uniform(-4, 0) for _ in range(6)]
+    lr, b1, b2, eps = 3e-3, 0.85, 0.99, 1e-3
+    tp = torch.tensor(p0.copy(), dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.Adam([tp], lr=lr, betas=(b1, b2), eps=eps)
+    p, m, v = p0.copy(), np.zeros_like(p0), np.zeros_like(p0)
+    for t, g in enumerate(grads, start=1):
+        opt.zero_grad()
+        tp.grad = torch.tensor(g, dtype=torch.float64)
+        opt.step()
+        p, m, v = uo.adam_step(p, g, m, v, t, lr, b1, b2, eps)
+        np.testing.assert_allclose(p, tp.detach().numpy(), rtol=0, atol=1e-15)
+
+
+def test_adam_step_closed_forms():
+    """t = 1 from zero moments: the step is lr g / (|g| + eps) (bias corrections cancel the
+    (1 - b) factors); a zero gradient leaves the parameter and decays the moments."""
+    g = np.array([2.0, -0.5, 0.0, 1e-3])
+    p, m, v = uo.adam_step(np.ones(4), g, np.zeros(4), np.zeros(4), 1, 0.1, 0.9, 0.999, 1e-3)
+    np.testing.assert_allclose(p, 1.0 - 0.1 * g / (np.abs(g) + 1e-3), rtol=1e-14, atol=0)
+    np.testing.assert_allclose(m, 0.1 * g, rtol=1e-15)
+    np.testing.assert_allclose(v, 1e-3 * g * g, rtol=1e-12)
+    p2, m2, v2 = uo.adam_step(p, np.zeros(4), m, v, 2, 0.1, 0.9, 0.999, 1e-3)
+    np.testing.assert_allclose(m2, 0.9 * m, rtol=1e-15)
+    np.testing.assert_allclose(v2, 0.999 * v, rtol=1e-15)
+    # second step: m_hat = 0.09 g / 0.19, v_hat = 0.000999 g^2 / 0.001999
+    expect = p - 0.1 * (0.09 * g / 0.19) / (np.sqrt(0.000999 * g * g / 0.001999) + 1e-3)
+    np.testing.assert_allclose(p2, expect, rtol=1e-14, atol=0)
