@@ -227,6 +227,26 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_get_tile_cuda(void*
  * shapes timed (not from the cache). */
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_autotune_cuda(void* batch, const char* cache_path, uint32_t repeats, uint32_t* out_tuned_shapes);
 
+/* ---- hand-off to a Vulkan renderer (Linux opaque file descriptors) ----
+ * The compute side of the reference viewer's capture slots (vulkan-app/app.cpp:816-871,
+ * the fd branch; snapshot export vulkan-app/app.ixx:240-273 and scene_plume.cpp:343-360).
+ * import: takes a VkDeviceMemory exported as VK_EXTERNAL_MEMORY_HANDLE_TYPE_OPAQUE_FD_BIT
+ * (memory_bytes = its allocation size) and maps buffer_bytes of it from buffer_offset;
+ * semaphore_fd (or -1 for none) is a timeline VkSemaphore exported as
+ * VK_EXTERNAL_SEMAPHORE_HANDLE_TYPE_OPAQUE_FD_BIT. On success the runtime owns both
+ * descriptors (the caller must not close them). On failure the slot is released; a
+ * descriptor imported before the failing step (memory, then semaphore) is consumed.
+ * export_view: copies the view `request` names into the slot's buffer (SOA3 views as x,
+ * then y, then z) on request->consumer_stream (NULL: the context stream), then signals
+ * the semaphore to signal_value on that stream. Asynchronous: the renderer waits on the
+ * semaphore value (the reference also synchronizes the stream after signalling).
+ * Fails if the view does not fit the buffer. */
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_interop_import_cuda(int memory_fd, uint64_t memory_bytes, uint64_t buffer_offset, uint64_t buffer_bytes, int semaphore_fd, void** out_slot);
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_interop_destroy_cuda(void* slot);
+/* the slot's mapped device buffer (NULL for a NULL slot) */
+STABLE_FLUIDS_API void* stable_fluids_ext_interop_buffer_cuda(void* slot);
+STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_interop_export_view_cuda(void* context, void* slot, const StableFluidsViewRequest* request, uint64_t signal_value);
+
 /* Device array [nz][ny][nx] FP32 of the last step's pre-projection divergence (the
  * pressure right-hand side; the DIVERGENCE view holds the post-projection field). Valid
  * until the context is destroyed; NULL for a NULL context. */

@@ -34,6 +34,8 @@ EXT_ENTRY_POINTS = (
     "stable_fluids_ext_force_animation_step_cuda", "stable_fluids_ext_field_norms_noalloc_cuda",
     "stable_fluids_ext_set_smoother_tiling_cuda", "stable_fluids_ext_get_smoother_tiling_cuda",
     "stable_fluids_ext_autotune_cuda", "stable_fluids_ext_batch_set_tiling_cuda", "stable_fluids_ext_batch_get_tile_cuda",
+    "stable_fluids_ext_interop_import_cuda", "stable_fluids_ext_interop_destroy_cuda", "stable_fluids_ext_interop_buffer_cuda",
+    "stable_fluids_ext_interop_export_view_cuda",
 )
 UNET_ENTRY_POINTS = (
     "stable_fluids_unet_param_count", "stable_fluids_unet_create_cuda", "stable_fluids_unet_destroy_cuda",
@@ -80,6 +82,11 @@ def load_library(path=LIB_PATH):
     lib.stable_fluids_ext_set_smoother_tiling_cuda.argtypes = [vp, C.c_uint32, C.c_uint32]
     lib.stable_fluids_ext_get_smoother_tiling_cuda.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     lib.stable_fluids_ext_autotune_cuda.argtypes = [vp, C.c_char_p, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    lib.stable_fluids_ext_interop_import_cuda.argtypes = [C.c_int, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, C.POINTER(vp)]
+    lib.stable_fluids_ext_interop_destroy_cuda.argtypes = [vp]
+    lib.stable_fluids_ext_interop_buffer_cuda.argtypes = [vp]
+    lib.stable_fluids_ext_interop_buffer_cuda.restype = vp
+    lib.stable_fluids_ext_interop_export_view_cuda.argtypes = [vp, vp, C.POINTER(abi.ViewRequest), C.c_uint64]
     lib.stable_fluids_ext_run_stage_cuda.argtypes = [vp, u32, u32]
     lib.stable_fluids_ext_stage_bytes_cuda.argtypes = [vp, u32]
     lib.stable_fluids_ext_stage_bytes_cuda.restype = C.c_uint64
@@ -114,7 +121,8 @@ def load_library(path=LIB_PATH):
     lib.stable_fluids_unet_conv3_wgrad_cuda.argtypes = [vp, i32, i32, vp, vp, vp, vp, i32, i32, i32]
     for name in ENTRY_POINTS + EXT_ENTRY_POINTS + UNET_ENTRY_POINTS:
         if name not in ("stable_fluids_ext_stage_bytes_cuda", "stable_fluids_ext_get_stream_cuda", "stable_fluids_unet_param_count",
-                        "stable_fluids_ext_get_pressure_rhs_cuda", "stable_fluids_ext_batch_get_stream_cuda"):
+                        "stable_fluids_ext_get_pressure_rhs_cuda", "stable_fluids_ext_batch_get_stream_cuda",
+                        "stable_fluids_ext_interop_buffer_cuda"):
             getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib

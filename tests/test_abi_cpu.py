@@ -163,3 +163,19 @@ def test_null_context_calls_fail_cleanly(abi):
     assert lib.stable_fluids_ext_run_stage_cuda(None, 0, 1) == abi.RESULT_BACKEND_FAILURE
     assert lib.stable_fluids_ext_stage_bytes_cuda(None, 0) == 0
     assert lib.stable_fluids_create_context_cuda(None, None, None, 0, None, 0) == abi.RESULT_BACKEND_FAILURE
+
+
+def test_interop_rejects_bad_arguments_before_device_use(abi):
+    """The Vulkan hand-off entry points validate descriptors, sizes and handles on the
+    host before any HIP call (no GPU here; no Vulkan loader on the GPU hosts either)."""
+    lib = sf.load_library()
+    slot = C.c_void_p(1)
+    for args in ((-1, 4096, 0, 4096, -1), (3, 0, 0, 16, -1), (3, 4096, 0, 0, -1), (3, 4096, 4097, 1, -1),
+                 (3, 4096, 4000, 200, 5), (3, 4096, 2 ** 64 - 1, 2, -1)):
+        assert lib.stable_fluids_ext_interop_import_cuda(*args, C.byref(slot)) == abi.RESULT_BACKEND_FAILURE, args
+        assert slot.value is None
+    assert lib.stable_fluids_ext_interop_import_cuda(3, 4096, 0, 4096, -1, None) == abi.RESULT_BACKEND_FAILURE
+    assert lib.stable_fluids_ext_interop_destroy_cuda(None) == abi.RESULT_BACKEND_FAILURE
+    assert lib.stable_fluids_ext_interop_buffer_cuda(None) is None
+    req = abi.ViewRequest(abi.VIEW_FLOW_PRESSURE, 0, 0, None)
+    assert lib.stable_fluids_ext_interop_export_view_cuda(None, None, C.byref(req), 1) == abi.RESULT_BACKEND_FAILURE
