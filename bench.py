@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain projection comparison")
     ap.add_argument("--no-512", action="store_true", help="skip the 512^3 smoother-pass leg of the roofline")
     ap.add_argument("--precision", action="store_true", help="run the 512^3 storage precision study instead")
+    ap.add_argument("--fp64", action="store_true",
+                    help="with --precision: the well-conditioned variant (force 0.5, 5 steps) against the FP64 oracle")
     ap.add_argument("--batch", action="store_true", help="run the 64-simulation batched study instead")
     ap.add_argument("--tuning-cache", default=None, help="autotuner cache file for --batch")
     ap.add_argument("--learned-step", type=int, default=None, metavar="K",
@@ -301,12 +303,14 @@ def learned_step_study(pkg, args):
 
 
 def precision_study(pkg, args):
-    """Per-field error of 16-bit iterate storage against FP32 storage after K steps."""
+    """Per-field error of 16-bit iterate storage against FP32 storage after K steps (and,
+    with --fp64, of every storage against the FP64 oracle on a well-conditioned scene)."""
     import numpy as np
     abi, hip, scenes = pkg.abi, pkg.hip, pkg.scenes
     n = args.grid if args.grid != 256 else 512
-    steps = args.steps if args.steps != 200 else 50
-    scene = scenes.plume_scene(n, diffuse_iterations=20, pressure_iterations=40)
+    steps = args.steps if args.steps != 200 else (5 if args.fp64 else 50)
+    force = 0.5 if args.fp64 else 5.0
+    scene = scenes.plume_scene(n, diffuse_iterations=20, pressure_iterations=40, force=force)
     smoother = abi.SMOOTHER_JACOBI if args.smoother == "jacobi" else abi.SMOOTHER_RBGS
     kinds = {"pressure": abi.VIEW_PRESSURE, "divergence": abi.VIEW_DIVERGENCE, "speed": abi.VIEW_VELOCITY_MAGNITUDE}
 
@@ -332,25 +336,51 @@ def precision_study(pkg, args):
         finally:
             ctx.destroy()
 
-    ref, ref_norms, ref_ms = run("f32")
-    runs = {"f32": {"ms_per_step": round(ref_ms, 3), "final_divergence_l2": ref_norms["divergence_l2"]}}
-    # f32-refined runs the same cycles with FP32 storage: its difference from the plain
-    # run is rounding order alone, amplified by the flow, which bounds what storage can show
-    for storage in ("f32-refined", "f16", "bf16"):
-        got, norms, ms = run(storage)
-        errs = {}
+    def errors(got, ref):
+        out = {}
         for k, r in ref.items():
             r64, g64 = r.astype(np.float64), got[k].astype(np.float64)
             scale, l2 = float(np.max(np.abs(r64))), float(np.sqrt(np.sum(r64 * r64)))
             d = g64 - r64
-            errs[k] = {"max_rel": float(np.max(np.abs(d))) / max(scale, 1e-300),
-                       "l2_rel": float(np.sqrt(np.sum(d * d))) / max(l2, 1e-300)}
+            out[k] = {"max_rel": float(np.max(np.abs(d))) / max(scale, 1e-300),
+                      "l2_rel": float(np.sqrt(np.sum(d * d))) / max(l2, 1e-300)}
             del r64, g64, d
-        runs[storage] = {"ms_per_step": round(ms, 3), "final_divergence_l2": norms["divergence_l2"], "error_vs_f32_storage": errs}
+        return out
+
+    o64 = None
+    if args.fp64:
+        # the FP64 oracle on this host's cores, one step at a time (progress on stdout)
+        oracle = entry.load_oracle()
+        sim = oracle.OracleSim(abi, scene.desc, precision=64, smoother=oracle.JACOBI if args.smoother == "jacobi" else oracle.RBGS)
+        try:
+            scenes.seed_oracle(sim, scene)
+            t = time.perf_counter()
+            for i in range(steps):
+                sim.step(1)
+                print(f"fp64 oracle step {i + 1}/{steps}: {time.perf_counter() - t:.1f} s", flush=True)
+            o64 = {k: sim.read(v) for k, v in kinds.items()}
+            o64.update(zip(("vx", "vy", "vz"), sim.read(abi.VIEW_VELOCITY)))
+            o64["density"] = sim.read(abi.VIEW_SCALAR_DATA, 1)
+        finally:
+            sim.destroy()
+
+    ref, ref_norms, ref_ms = run("f32")
+    runs = {"f32": {"ms_per_step": round(ref_ms, 3), "final_divergence_l2": ref_norms["divergence_l2"]}}
+    if o64 is not None:
+        runs["f32"]["error_vs_fp64_oracle"] = errors(ref, o64)
+    # f32-refined runs the same cycles with FP32 storage: its difference from the plain
+    # run is rounding order alone, amplified by the flow, which bounds what storage can show
+    for storage in ("f32-refined", "f16", "bf16"):
+        got, norms, ms = run(storage)
+        runs[storage] = {"ms_per_step": round(ms, 3), "speedup_vs_f32": round(ref_ms / ms, 3),
+                         "final_divergence_l2": norms["divergence_l2"], "error_vs_f32_storage": errors(got, ref)}
+        if o64 is not None:
+            runs[storage]["error_vs_fp64_oracle"] = errors(got, o64)
         del got
     print(json.dumps({"study": "iterate storage precision", "grid": [n, n, n], "steps": steps, "smoother": args.smoother,
                       "refinement_interval": args.interval, "diffuse_iterations": 20, "pressure_iterations": 40,
-                      "workload": f"plume {n}^3, centre ({n // 2},{3 * n // 32},{n // 2}) radius {n // 16}, zero initial velocity",
+                      "workload": f"plume {n}^3, centre ({n // 2},{3 * n // 32},{n // 2}) radius {n // 16}, force {force}, "
+                                  f"zero initial velocity", "fp64_oracle": o64 is not None,
                       "runs": runs}), flush=True)
 
 
@@ -372,7 +402,11 @@ def batch_study(pkg, args):
         info = None
         if batch is not None and tune:
             tuned = batch.autotune(args.tuning_cache, repeats=5)
-            info = {"tuned_shapes": tuned}
+            plans = {}
+            for i, sc in enumerate(scenes_):
+                p = batch.info(i)
+                plans["x".join(map(str, reversed(sc.desc.shape)))] = {"tile": p["tile"], "z_chunk": p["z_chunk"]}
+            info = {"tuned_shapes": tuned, "plans": plans}
         step = batch.step if batch else (lambda: [c.step(1) for c in ctxs])
         step()
         hip.synchronize()

@@ -215,13 +215,15 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_info_cuda(void* bat
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_get_tile_cuda(void* batch, uint32_t sim, uint32_t* out_tile);
 /* Autotuner, in place of the reference's occupancy-derived block size
  * (backend-cuda.cu:490-515). For every grid shape in the batch it times `repeats`
- * batched pressure passes over that shape's contexts alone (writing only scratch
- * buffers) for each candidate tiling -- every tile shape with z chunks {default, 4, 8,
+ * rounds of batched smoother passes in the step's mix (one three-component velocity
+ * pass, three single-field passes; writing only scratch buffers) over that shape's
+ * contexts alone for each candidate tiling -- every tile shape with z chunks {default, 4, 8,
  * 16, 32} for black-cell passes, z chunks {default, 2, 4, ..., 64} otherwise -- taking
  * the minimum of three interleaved rounds. It then times whole-batch passes under: each
  * shape on its fastest tiling; all shapes on one tile shape (one launch per pass), each
- * on its fastest chunk for it, for every tile shape; the starting tiling; and keeps the
- * fastest. Shapes listed in cache_path (a line "stable-fluids-tiling 2", then lines "nx
+ * on its fastest chunk for it, for every tile shape; the starting tiling; keeps the
+ * fastest, then re-chooses each shape's z chunk (tile kept) by whole-batch passes.
+ * Shapes listed in cache_path (a line "stable-fluids-tiling 2", then lines "nx
  * ny nz smoother members tile zc") are taken from it, and the file is rewritten with all
  * entries (cache_path may be NULL). Rebuilds the batch graph; *out_tuned_shapes =
  * shapes timed (not from the cache). */

@@ -1275,37 +1275,57 @@ double time_half_pass(Context& c, const int tile, const int zc, const int repeat
     return ms;
 }
 
-// Mean time of `repeats` batched middle pressure passes over the simulations marked in
-// `only` under the batch's current tiling: black-cell passes read the context's half[0]
-// and write half[1], full-iterate passes read rhs and write p_aux (both scratch between
-// steps); views are not touched. The descriptor tables join the batch's allocations
-// (freed by the next graph build).
+// Mean time of `repeats` rounds of batched middle smoother passes over the simulations
+// marked in `only` under the batch's current tiling, in the step's mix: one velocity
+// diffusion pass (three components per simulation) and three single-field passes (the
+// step runs 38 of the first and 116 of the second per 40 diffusion and 80 pressure
+// passes). Black-cell passes write half buffers, full-iterate passes p_aux and the
+// diffused velocity a (all scratch between steps); views are not touched. The
+// descriptor tables join the batch's allocations (freed by the next graph build).
 double time_batch_smoother(Batch& B, const std::vector<char>& only, const int repeats) {
     auto& S        = B.sims;
     const size_t N = S.size();
     auto p_half    = [&](size_t d) { return batch_half(B, d, pressure_rules(S[d]->config.flow_boundary), -1); };
-    PassLists L;
-    if (!pass_lists(B, N, [](size_t d) { return d; }, p_half, &L, &only)) return -1.0;
-    std::vector<SmoothDesc> d(N);
-    std::vector<HalfPassArgs> h(N);
+    auto vel_bc    = [&](size_t d) { return velocity_rules(S[d / 3]->config.flow_boundary, static_cast<int>(d % 3)); };
+    auto vel_half  = [&](size_t d) { return batch_half(B, d / 3, vel_bc(d), static_cast<int>(d % 3)); };
+    PassLists lp, lv;
+    if (!pass_lists(B, N, [](size_t d) { return d; }, p_half, &lp, &only)) return -1.0;
+    if (!pass_lists(B, 3 * N, [](size_t d) { return d / 3; }, vel_half, &lv, &only)) return -1.0;
+    std::vector<SmoothDesc> dp(N), dv(3 * N);
+    std::vector<HalfPassArgs> hp(N), hv(3 * N);
     for (size_t i = 0; i < N; ++i) {
         auto& c            = *S[i];
+        auto& f            = c.flow;
         const float coef   = c.config.cell_size * c.config.cell_size;
         const StencilBC bc = pressure_rules(c.config.flow_boundary);
-        if (p_half(i)) h[i] = batch_half_desc(B, i, -1, 2, 3, c.flow.rhs, nullptr, c.flow.p_aux, coef, 1.0f, bc);
-        else d[i] = SmoothDesc{c.flow.rhs, c.flow.rhs, c.flow.p_aux, coef, 1.0f, c.dims, bc, plan_tiles(c.dims, B.zc[i])};
+        if (p_half(i)) hp[i] = batch_half_desc(B, i, -1, 2, 3, f.rhs, nullptr, f.p_aux, coef, 1.0f, bc);
+        else dp[i] = SmoothDesc{f.rhs, f.rhs, f.p_aux, coef, 1.0f, c.dims, bc, plan_tiles(c.dims, B.zc[i])};
+        const float alpha = c.config.dt * c.config.viscosity / (c.config.cell_size * c.config.cell_size);
+        float* t[3]       = {f.tx, f.ty, f.tz};
+        float* a[3]       = {f.ax, f.ay, f.az};
+        for (int comp = 0; comp < 3; ++comp) {
+            const size_t k = 3 * i + comp;
+            if (vel_half(k)) hv[k] = batch_half_desc(B, i, comp, 2, 3, t[comp], nullptr, a[comp], alpha, 1.0f, vel_bc(k));
+            else dv[k] = SmoothDesc{t[comp], t[comp], a[comp], alpha, 1.0f, c.dims, vel_bc(k), plan_tiles(c.dims, B.zc[i])};
+        }
     }
-    const auto dd = B.upload(d);
-    const auto dh = B.upload(h);
-    if (!dd || !dh) return -1.0;
+    const auto ddp = B.upload(dp);
+    const auto dhp = B.upload(hp);
+    const auto ddv = B.upload(dv);
+    const auto dhv = B.upload(hv);
+    if (!ddp || !dhp || !ddv || !dhv) return -1.0;
     const SmootherKind sm = S[0]->options.smoother;
-    auto pass             = [&] { return launch_pass_lists(B.stream, sm, kSolvePoisson, 2, 3, false, dh, dd, L) == hipSuccess; };
-    double ms             = -1.0;
+    auto round            = [&] {
+        bool ok = launch_pass_lists(B.stream, sm, kSolveDiffusion, 2, 3, false, dhv, ddv, lv) == hipSuccess;
+        for (int r = 0; ok && r < 3; ++r) ok = launch_pass_lists(B.stream, sm, kSolvePoisson, 2, 3, false, dhp, ddp, lp) == hipSuccess;
+        return ok;
+    };
+    double ms = -1.0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
-        bool ok = pass() && pass();
+        bool ok = round();
         ok      = ok && hipEventRecord(e0, B.stream) == hipSuccess;
-        for (int r = 0; ok && r < repeats; ++r) ok = pass();
+        for (int r = 0; ok && r < repeats; ++r) ok = round();
         ok      = ok && hipEventRecord(e1, B.stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
         float t = 0.0f;
         if (ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms = t / repeats;
@@ -1803,6 +1823,29 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_autotune_cuda(void*
         const size_t q = static_cast<size_t>(std::min_element(t_plan.begin(), t_plan.end()) - t_plan.begin());
         b.tile         = plans[q].first;
         b.zc           = plans[q].second;
+        // then each timed class's z chunk on the whole batch, its tile shape kept: a chunk
+        // length that ties when the class runs alone can leave the batch waiting on its
+        // longest workgroups
+        double t_best = t_plan[q];
+        for (const auto& k : classes) {
+            if (k.cached) continue;
+            const int tile = b.tile[k.first], zc_now = b.zc[k.first];
+            int zc_best    = zc_now;
+            for (const auto& cand : k.cands) {
+                if (cand.tile != tile || cand.zc == zc_now) continue;
+                assign(k, cand);
+                double t = 1e300;
+                for (int round = 0; round < sf::kTuneRounds; ++round) {
+                    const double r = sf::time_batch_smoother(b, all, reps);
+                    if (r > 0.0) t = std::min(t, r);
+                }
+                if (t < t_best) {
+                    t_best  = t;
+                    zc_best = cand.zc;
+                }
+            }
+            assign(k, sf::Tiling{tile, zc_best});
+        }
         for (const auto& k : classes) {
             if (k.cached) continue;
             const sf::Tiling t = effective(k, sf::Tiling{b.tile[k.first], b.zc[k.first]});

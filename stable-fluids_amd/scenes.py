@@ -51,12 +51,16 @@ def box_filter5(a):
 
 
 def noise_velocity(shape, seed=0, sigma=1.0):
+    """Three components of box-filtered N(0, sigma^2) noise (streams 0, 1, 2 of `seed`),
+    generated concurrently (numpy releases the interpreter lock in these loops)."""
+    from concurrent.futures import ThreadPoolExecutor
     nz, ny, nx = shape
-    out = []
-    for comp in range(3):
-        a = normal_noise(nx * ny * nz, seed, comp).reshape(shape) * sigma
-        out.append(box_filter5(a).astype(np.float32))
-    return out
+
+    def comp(c):
+        return box_filter5(normal_noise(nx * ny * nz, seed, c).reshape(shape) * sigma).astype(np.float32)
+
+    with ThreadPoolExecutor(max_workers=3) as pool:
+        return list(pool.map(comp, range(3)))
 
 
 def sphere_mask(shape, centre, radius):
@@ -100,8 +104,9 @@ class Scene:
         return self.desc.shape
 
 
-def plume_scene(n=64, steps_scale=1, diffuse_iterations=20, pressure_iterations=40, noise_seed=None, **kw):
-    """The plume at grid n^3: centre (n/2, 3n/32, n/2), radius n/16, rate 100, force 5.
+def plume_scene(n=64, steps_scale=1, diffuse_iterations=20, pressure_iterations=40, noise_seed=None, force=5.0, **kw):
+    """The plume at grid n^3: centre (n/2, 3n/32, n/2), radius n/16, rate 100, upward
+    force `force` (5 by default).
 
     n=64 gives the 64^3 oracle workload (centre (32,6,32), radius 4); n=256 the 256^3
     throughput workload (centre (128,24,128), radius 16) when noise_seed=0 and 40/80
@@ -109,7 +114,7 @@ def plume_scene(n=64, steps_scale=1, diffuse_iterations=20, pressure_iterations=
     """
     desc = plume_desc(n, diffuse_iterations, pressure_iterations, **kw)
     centre = (n // 2, (3 * n) // 32, n // 2)
-    src, fx, fy, fz = plume_fields(desc.shape, centre, n // 16)
+    src, fx, fy, fz = plume_fields(desc.shape, centre, n // 16, 100.0, force)
     vel = noise_velocity(desc.shape, noise_seed) if noise_seed is not None else None
     return Scene(f"plume{n}", desc, src, (fx, fy, fz), vel)
 

@@ -70,6 +70,36 @@ def assert_same(a, b, tag):
             assert np.array_equal(fa[k], fb[k]), f"{tag}: sim {i} field {k}"
 
 
+def test_jacobi_long_chunks_match_a_batch_of_one():
+    """A stepped context runs its Jacobi passes on one workgroup per CU, here 128-plane z
+    chunks; a batch of one runs the same passes on the default 64-plane plan: the fields
+    must agree bit for bit."""
+    n = (512, 512, 256)
+    desc = desc_for(n, "walls", d_iters=6, p_iters=10)
+    opts = sf.ext_options(smoother=abi.SMOOTHER_JACOBI)
+    rng = np.random.default_rng(12)
+    vel = [rng.standard_normal(desc.shape, dtype=np.float32) for _ in range(3)]
+
+    def run(batched):
+        ctx = sf.Context(desc, options=opts)
+        try:
+            ctx.set_flow_velocity(*(ctx.upload(v) for v in vel))
+            if batched:
+                with sf.Batch([ctx]) as b:
+                    assert b.info(0)["z_chunk"] == 64
+                    b.step(2)
+                    sf.hip.synchronize()
+            else:
+                ctx.step(2)
+            return gpu_fields(ctx, desc)
+        finally:
+            ctx.destroy()
+
+    alone, batch = run(False), run(True)
+    for k in alone:
+        assert np.array_equal(alone[k], batch[k]), k
+
+
 @pytest.mark.parametrize("smoother", [abi.SMOOTHER_RBGS, abi.SMOOTHER_JACOBI])
 def test_batch_equals_each_context_alone(smoother):
     opts = sf.ext_options(smoother=smoother)
