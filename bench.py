@@ -49,16 +49,24 @@ METRIC = "steps/s at 256³; Jacobi HBM GB/s; learned-vs-plain speedup at equal d
 
 
 def step_bytes_per_cell(d_iters, p_iters, n_scalars=1, n_forces=1, alpha_pos=True, alpha_s_pos=True, dissipation=False):
-    """Algorithmic bytes per cell of one step as the library's kernel set runs it
-    (each array read once and written once per kernel)."""
-    def solve(k):  # first pass reads b only (x0 = b or zero), later passes x_old + b
-        return 0 if k <= 0 else 8 + 12 * (k - 1)
-    b = 36 * n_forces + 24                               # add force, fused 3-component advection
-    b += 3 * solve(d_iters) if alpha_pos else 24         # velocity diffusion (or copy)
-    b += 16 + (solve(p_iters) if p_iters > 0 else 4) + 36  # divergence, pressure, project+div+|v|
+    """Algorithmic bytes per cell of one step as the library's kernel set runs it on an
+    even-nx grid (each array read once and written once per kernel). RBGS solves run on
+    the black-cell passes: the first reads b and the start iterate (natural, 4 B, or the
+    zero guess, none) and writes the black cells (2 B), the middle ones read b and the
+    black cells and write the black cells (8 B), the last writes the natural field (10 B)."""
+    def solve(k, zero):
+        if k <= 0:
+            return 0
+        first = 4 + (0 if zero else 4)
+        if k == 1:
+            return first + 4
+        return first + 2 + 8 * (k - 2) + 10
+    b = 36 * n_forces + 24                                      # add force, fused 3-component advection
+    b += 3 * solve(d_iters, False) if alpha_pos else 24         # velocity diffusion (or copy)
+    b += 16 + (solve(p_iters, True) if p_iters > 0 else 4) + 36  # divergence, pressure, project+div+|v|
     for _ in range(n_scalars):
-        b += 12 + 20                                     # source add, advection
-        b += solve(d_iters) if alpha_s_pos else (8 if dissipation else 0)
+        b += 12 + 20                                            # source add, advection
+        b += solve(d_iters, False) if alpha_s_pos else (8 if dissipation else 0)
     return b
 
 
