@@ -58,6 +58,16 @@ namespace {
 #define SF_HALF_DEPTH 2
 #endif
 constexpr int kHalfDepth = SF_HALF_DEPTH; // planes of loads in flight
+// Cache policy of the black-cell stores: sc1 (device scope), so they do not sit dirty in the
+// writer XCD's L2 when the pass ends (the next pass, on any XCD, reads them): 28.8 -> 28.0 us
+// per middle pass at 256^3 against the default policy; non-temporal stores measured slower.
+constexpr int kHalfStoreAux = 16;
+// Minimum waves per SIMD the register allocation must leave room for. Passes that read the
+// natural-layout start iterate hold float2 loads in flight and went to 133 VGPRs (three
+// waves per SIMD, one 512-thread workgroup per CU); capped at 128 they keep two workgroups
+// per CU without spilling (about 9 us less per first pass at 256^3).
+template <int kIn>
+constexpr int kHalfMinWaves = kIn == kHalfInNatural ? 4 : 2;
 
 // Tile shape: kTXP x-pairs (2 kTXP cells) by TY rows for kWaves waves of kRowsW row
 // slots; a slot of a wave covers kR = 64 / kTXP rows of kTXP lanes each.
@@ -72,6 +82,10 @@ struct HalfTile {
     static constexpr int RW     = kTXP + 2;         // R plane: pairs -1 .. kTXP
     static constexpr int RR     = TY + 2;           //          rows -1 .. TY
     static constexpr int RF     = RW * RR;
+    // ring plane strides: each plane ends in 64 words that take the writes of halo-slot
+    // lanes with no position (one word per lane), so the write offset is loop-invariant
+    static constexpr int AS     = AF + 64;
+    static constexpr int RS     = RF + 64;
     static constexpr int kAHalo = 4 * AW + 4 * TY;  // A halo positions (one extra slot per lane)
     static constexpr int kRHalo = 2 * RW + 2 * TY;  // R halo positions
     static constexpr int kRHaloWaves = (kRHalo + 63) / 64;
@@ -140,9 +154,10 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
     constexpr int RW   = TT::RW;
     constexpr int AF   = TT::AF;
     constexpr int RF   = TT::RF;
-    __shared__ float s_a[4 * AF];
-    __shared__ float s_r[2 * RF];
-    __shared__ float s_sink[64]; // LDS target of halo-slot lanes with no position (one word per lane)
+    constexpr int AS   = TT::AS;
+    constexpr int RS   = TT::RS;
+    __shared__ float s_a[4 * AS];
+    __shared__ float s_r[2 * RS];
     // ghost rules (a, c, k) of faces 0..5 and the identity at 6, read with runtime indices
     // at set-up (indexing the kernel argument itself would copy it to scratch memory)
     __shared__ float s_rule_a[8], s_rule_c[8];
@@ -150,14 +165,8 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
 
     const Dims d       = args.d;
     const StencilBC& bc = args.bc;
-    if (threadIdx.x < 7) {
-        const int f       = static_cast<int>(threadIdx.x);
-        const LinGhost lg = f == 6 ? lin_identity() : lin_ghost(face_rule(bc, f));
-        s_rule_a[f]       = lg.a;
-        s_rule_c[f]       = lg.c;
-        s_rule_k[f]       = lg.k ? 1 : 0;
-    }
-    __syncthreads();
+    // (the rule table is filled after the first plane loads are issued, so their latency
+    // overlaps the set-up; rules are looked up after its barrier)
     auto rule = [&](const int f) { return LinGhost{s_rule_a[f], s_rule_c[f], s_rule_k[f] != 0}; };
     auto kind_rule = [&](const int kind) { return rule(kind >= kPosXm && kind <= kPosYp ? kind - 1 : 6); };
     const HalfLayout L = args.L;
@@ -193,7 +202,6 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
         const HalfPos P   = classify(i, y, d, L, px, py);
         o_kind[j]         = P.kind;
         o_xflip[j]        = P.kind == kPosXm || P.kind == kPosXp;
-        o_rule[j]         = kind_rule(P.kind);
         o_src[j]          = kIn == kHalfInNatural ? 4 * P.off_n : 4 * P.off_h;
         o_b[j]            = P.kind == kPosReal ? 4 * P.off_n : kOutOfRange;
         const bool owned  = i < nxh && y < ny;
@@ -205,15 +213,17 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
 
     // ---- extra A slot: the halo of the A planes ----
     int ex_src[kAPer], ex_lds[kAPer], ex_y[kAPer];
-    bool ex_xflip[kAPer], ex_on[kAPer];
+    bool ex_xflip[kAPer];
+    int ex_kind[kAPer];
     LinGhost ex_rule[kAPer];
 #pragma unroll
     for (int q = 0; q < kAPer; ++q) {
         const int e = tid + q * TT::kThreads;
         ex_src[q]   = kOutOfRange;
-        ex_lds[q] = ex_y[q] = 0;
-        ex_xflip[q] = ex_on[q] = false;
-        ex_rule[q]  = lin_identity();
+        ex_lds[q]   = AF + lane; // no position: the plane's sink word
+        ex_y[q]     = 0;
+        ex_xflip[q] = false;
+        ex_kind[q]  = kPosReal; // identity rule
         if (e < TT::kAHalo) {
             int r, hl;
             if (e < 4 * AW) {
@@ -230,8 +240,7 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
             ex_lds[q]       = (r + 2) * AW + hl + 2;
             ex_y[q]         = y0 + r;
             ex_xflip[q]     = P.kind == kPosXm || P.kind == kPosXp;
-            ex_rule[q]      = kind_rule(P.kind);
-            ex_on[q]        = true;
+            ex_kind[q]      = P.kind;
         }
     }
 
@@ -239,9 +248,10 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
     // A ghost position takes the face rule of the adjacent cell's old value; the adjacent
     // cell is black where it matters (x-: p = 1, x+: p = 0), so its A entry is at a fixed
     // offset h_goff from the position (x- +1, x+ -1, y- +AW, y+ -AW).
-    int h_b = kOutOfRange, h_alds = 0, h_rlds = 0, h_y = 0, h_goff = 0;
+    int h_b = kOutOfRange, h_alds = 0, h_rlds = RF + lane, h_y = 0, h_goff = 0; // no position: the sink word
     bool h_real = false;
-    LinGhost h_rule = lin_identity();
+    int h_kind  = kPosReal; // identity rule
+    LinGhost h_rule;
     if (tid < TT::kRHalo) {
         int r, hl;
         if (tid < 2 * RW) {
@@ -258,7 +268,7 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
         h_alds          = (r + 2) * AW + hl + 2;
         h_rlds          = (r + 1) * RW + hl + 1;
         h_y             = y0 + r;
-        h_rule          = kind_rule(P.kind);
+        h_kind          = P.kind;
         h_goff          = P.kind == kPosXm ? 1 : (P.kind == kPosXp ? -1 : (P.kind == kPosYm ? AW : (P.kind == kPosYp ? -AW : 0)));
     }
 
@@ -266,7 +276,8 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
     //   threads 0 .. kTXP - 1: row 0 -> row -1; the next kTXP: row ny - 1 -> row ny;
     //   the next TY: pair 0 -> x- column; the next TY: pair nxh - 1 -> x+ column
     int g_off = kOutOfRange, g_rlds = 0;
-    LinGhost g_rule = lin_identity();
+    int g_face = 6; // identity
+    LinGhost g_rule;
     if constexpr (kOut == kHalfOutHalf) {
         if (tid < 2 * kTXP) {
             const int side = tid / kTXP, gl = tid % kTXP;
@@ -274,7 +285,7 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
             if (!py && y >= y0 && y < y0 + TY && i0 + gl < nxh) {
                 g_off  = 4 * ((side ? ny + 1 : 0) * nxh + i0 + gl);
                 g_rlds = (y - y0 + 1) * RW + gl + 1;
-                g_rule = rule(2 + side);
+                g_face = 2 + side;
             }
         } else if (tid < 2 * kTXP + 2 * TY) {
             const int side = (tid - 2 * kTXP) / TY, r = (tid - 2 * kTXP) % TY;
@@ -282,12 +293,12 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
             if (!px && i >= i0 && i < i0 + kTXP && y0 + r < ny) {
                 g_off  = 4 * ((ny + 2) * nxh + side * ny + y0 + r);
                 g_rlds = (r + 1) * RW + i - i0 + 1;
-                g_rule = rule(side);
+                g_face = side;
             }
         }
     }
     const bool g_any = __builtin_amdgcn_readfirstlane(__any(g_off != kOutOfRange)) != 0; // wave-uniform
-    const LinGhost zr_lo = rule(4), zr_hi = rule(5);
+    LinGhost zr_lo, zr_hi; // z-face rules (set after the rule table's barrier)
     const DiffusionDiv ddv = diffusion_div(coef); // loop-invariant reciprocal of 1 + 6 coef
 
     auto zghost = [&](const int k) { return !pz && (k < 0 || k >= nz); };
@@ -380,7 +391,7 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
             float v;
             if constexpr (kIn == kHalfInZero) v = a_value.template operator()<kReal>(0.0f, k, 0, ex_xflip[q], ex_rule[q]);
             else v = a_value.template operator()<kReal>(pf.ex[q], k, ex_y[q], ex_xflip[q], ex_rule[q]);
-            *(ex_on[q] ? A + ex_lds[q] : s_sink + lane) = v;
+            A[ex_lds[q]] = v;
         }
     };
 
@@ -461,12 +472,36 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
             r_0[j]                                       = q[j];
             R0[(orow(j) + 1) * RW + l + 1] = q[j];
         }
-        if (wave < TT::kRHaloWaves) *(tid < TT::kRHalo ? R0 + h_rlds : s_sink + lane) = qh;
+        if (wave < TT::kRHaloWaves) R0[h_rlds] = qh;
     };
 
-    // black half-update of plane m = s - 1 from R1 = R(m) and the stores of plane m
-    // (kLive: m is a plane of the chunk and not a z face)
-    auto black = [&]<bool kLive>(const int s, const float* R1) __attribute__((always_inline)) {
+    // LDS neighbours of the black cells of plane m = s - 1 in R1 = R(m): the other x
+    // neighbour and the rows above and below; g: the red face cell of the lane's
+    // ghost-store slot (waves that hold one)
+    struct BlackIn {
+        float other[kRowsW], ym[kRowsW], yp[kRowsW];
+        float g;
+    };
+    auto black_fetch = [&](const int s, const float* R1) __attribute__((always_inline)) {
+        const int p = (ypar + s - 1) & 1;
+        BlackIn bi;
+#pragma unroll
+        for (int j = 0; j < kRowsW; ++j) {
+            const int c    = (orow(j) + 1) * RW + l + 1;
+            bi.other[j]    = R1[c + 1 - 2 * p];
+            bi.ym[j]       = R1[c - RW];
+            bi.yp[j]       = R1[c + RW];
+        }
+        bi.g = 0.0f;
+        if constexpr (kOut == kHalfOutHalf)
+            if (g_any) bi.g = R1[g_rlds];
+        return bi;
+    };
+
+    // black half-update of plane m = s - 1 from the LDS neighbours `bi`, the own red
+    // values of planes m - 1 .. m + 1, and the stores of plane m (kLive: m is a plane of
+    // the chunk and not a z face)
+    auto black = [&]<bool kLive>(const int s, const BlackIn& bi) __attribute__((always_inline)) {
         const int m     = s - 1;
         const bool live = kLive || (m >= z0 && m < z1); // planes of the chunk (never a ghost plane)
         const int p     = (ypar + m) & 1;
@@ -474,17 +509,15 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
         float x[kRowsW], q[kRowsW];
 #pragma unroll
         for (int j = 0; j < kRowsW; ++j) {
-            const int c       = (orow(j) + 1) * RW + l + 1;
-            const float other = R1[c + 1 - 2 * p];
-            const float xm    = p ? other : r_m[j];
-            const float xp    = p ? r_m[j] : other;
-            float sum         = xm;
-            sum               = sum + xp;
-            sum               = sum + R1[c - RW];
-            sum               = sum + R1[c + RW];
-            sum               = sum + r_mm[j];
-            sum               = sum + r_0[j];
-            x[j]              = dv.num(bb[j], sum, coef);
+            const float xm = p ? bi.other[j] : r_m[j];
+            const float xp = p ? r_m[j] : bi.other[j];
+            float sum      = xm;
+            sum            = sum + xp;
+            sum            = sum + bi.ym[j];
+            sum            = sum + bi.yp[j];
+            sum            = sum + r_mm[j];
+            sum            = sum + r_0[j];
+            x[j]           = dv.num(bb[j], sum, coef);
         }
         if (!live || tile_ghost) {
 #pragma unroll
@@ -501,20 +534,20 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
             const Rsrc r = rsrc_at(p_hout, live ? hfield_b : 0u);
             const int so = live ? (m + 1) * static_cast<int>(hplane_b) : 0;
 #pragma unroll
-            for (int j = 0; j < kRowsW; ++j) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(q[j]), r, o_st[j], so, 0);
+            for (int j = 0; j < kRowsW; ++j) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(q[j]), r, o_st[j], so, kHalfStoreAux);
             // red face cells of plane m -> ghost entries: rows -1 / ny and the x columns
             // (one slot per lane, waves that hold face cells), the z planes -1 / nz (own
             // slots). Only stores sit under these branches: a skipped store leaves every
             // later wait count valid.
             if (g_any && live) {
-                const float g = lin_apply(g_rule, R1[g_rlds]);
-                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(g), r, g_off, so, 0);
+                const float g = lin_apply(g_rule, bi.g);
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(g), r, g_off, so, kHalfStoreAux);
             }
             if (!kLive && live && !pz && (m == 0 || m == nz - 1)) {
                 const int zso     = m == 0 ? 0 : (nz + 1) * static_cast<int>(hplane_b);
                 const LinGhost zr = pick(m == 0, zr_lo, zr_hi);
 #pragma unroll
-                for (int j = 0; j < kRowsW; ++j) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(lin_apply(zr, r_m[j])), r, o_st[j], zso, 0);
+                for (int j = 0; j < kRowsW; ++j) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(lin_apply(zr, r_m[j])), r, o_st[j], zso, kHalfStoreAux);
             }
         } else {
             const Rsrc r  = rsrc_at(p_xout, live ? nfield_b : 0u);
@@ -539,17 +572,20 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
     // plane s - 1 is neither z face; the slots are then compile-time (t mod 4 = kT).
     auto step = [&]<bool kSteady, int kT>(const int s, const int t, Pf& pa, PfB& pb) __attribute__((always_inline)) {
         const int tt  = kSteady ? kT : t;
-        float* A_next = s_a + ((tt + 2) & 3) * AF;
-        float* A_cur  = s_a + ((tt + 1) & 3) * AF;
-        float* A_prev = s_a + (tt & 3) * AF;
-        float* R_cur  = s_r + ((tt + 1) & 1) * RF;
-        float* R_prev = s_r + (tt & 1) * RF;
+        float* A_next = s_a + ((tt + 2) & 3) * AS;
+        float* A_cur  = s_a + ((tt + 1) & 3) * AS;
+        float* A_prev = s_a + (tt & 3) * AS;
+        float* R_cur  = s_r + ((tt + 1) & 1) * RS;
+        float* R_prev = s_r + (tt & 1) * RS;
         stage_a.template operator()<kSteady>(s + 1, A_next, pa, a_next);
         load_a.template operator()<kSteady>(s + 1 + kDepth, pa);
         __syncthreads();
+        // the black half-update's LDS reads (R(s-1), written before this barrier) go out
+        // first, so their latency overlaps the red half-update
+        const BlackIn bi = black_fetch(s, R_prev);
         red.template operator()<kSteady>(s, A_prev, A_cur, A_next, R_cur, pb);
         load_b.template operator()<kSteady>(s + kDepth, pb);
-        black.template operator()<kSteady>(s, R_prev);
+        black.template operator()<kSteady>(s, bi);
 #pragma unroll
         for (int j = 0; j < kRowsW; ++j) {
             a_prev[j] = a_cur[j];
@@ -563,20 +599,41 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
 
     Pf pa[2];
     PfB pb[2];
-    // prologue: A(z0-2), A(z0-1) staged into slots 0, 1; A(z0), A(z0+1) and b(z0-1), b(z0) in flight
-    load_a.template operator()<false>(z0 - 2, pa[0]);
-    load_a.template operator()<false>(z0 - 1, pa[1]);
-    stage_a.template operator()<false>(z0 - 2, s_a, pa[0], a_cur);
-    stage_a.template operator()<false>(z0 - 1, s_a + AF, pa[1], a_next);
+    // prologue: every load of A(z0-2) .. A(z0+1), b(z0-1), b(z0) goes out first; then the
+    // rule table (one barrier) and the rules of every slot; A(z0-2), A(z0-1) are staged
+    // into slots 0, 1 while A(z0), A(z0+1) and b(z0-1), b(z0) stay in flight
+    {
+        Pf p0, p1;
+        load_a.template operator()<false>(z0 - 2, p0);
+        load_a.template operator()<false>(z0 - 1, p1);
+        load_a.template operator()<false>(z0, pa[0]);
+        load_a.template operator()<false>(z0 + 1, pa[1]);
+        load_b.template operator()<false>(z0 - 1, pb[0]);
+        load_b.template operator()<false>(z0, pb[1]);
+        if (threadIdx.x < 7) {
+            const int f       = static_cast<int>(threadIdx.x);
+            const LinGhost lg = f == 6 ? lin_identity() : lin_ghost(face_rule(bc, f));
+            s_rule_a[f]       = lg.a;
+            s_rule_c[f]       = lg.c;
+            s_rule_k[f]       = lg.k ? 1 : 0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kRowsW; ++j) o_rule[j] = kind_rule(o_kind[j]);
+#pragma unroll
+        for (int q = 0; q < kAPer; ++q) ex_rule[q] = kind_rule(ex_kind[q]);
+        h_rule = kind_rule(h_kind);
+        g_rule = rule(g_face);
+        zr_lo  = rule(4);
+        zr_hi  = rule(5);
+        stage_a.template operator()<false>(z0 - 2, s_a, p0, a_cur);
+        stage_a.template operator()<false>(z0 - 1, s_a + AS, p1, a_next);
+    }
 #pragma unroll
     for (int j = 0; j < kRowsW; ++j) {
         a_prev[j] = a_cur[j];
         a_cur[j]  = a_next[j];
     }
-    load_a.template operator()<false>(z0, pa[0]);
-    load_a.template operator()<false>(z0 + 1, pa[1]);
-    load_b.template operator()<false>(z0 - 1, pb[0]);
-    load_b.template operator()<false>(z0, pb[1]);
     // steps t = 0 .. n - 1 (s = z0 - 1 .. z1), register set t & 1: general steps t = 0, 1, 2;
     // steady steps in groups of four, t = 3 .. 2 + 4 g, s = z0 + 2 .. z0 + 1 + 4 g <= z1 - 3;
     // general steps for the rest (an odd count ends with one idle step, loads and stores off)
@@ -600,14 +657,14 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
 
 // one pass over the grid args.g
 template <int kSolve, int kIn, int kOut, int kTXP, int kWaves, int kRowsW>
-__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 4))) void k_rbgs_half(const HalfPassArgs args) {
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(kHalfMinWaves<kIn>, 4))) void k_rbgs_half(const HalfPassArgs args) {
     rbgs_half_body<kSolve, kIn, kOut, kTXP, kWaves, kRowsW, kHalfDepth>(args, xcd_block());
 }
 
 // batched: workgroup i runs block items[i].block of the pass descs[items[i].desc] (the
 // descriptor is read in place); an idle item does nothing
 template <int kSolve, int kIn, int kOut, int kTXP, int kWaves, int kRowsW>
-__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(2, 4))) void k_rbgs_half_batched(const HalfPassArgs* __restrict__ descs, const BatchItem* __restrict__ items) {
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(kHalfMinWaves<kIn>, 4))) void k_rbgs_half_batched(const HalfPassArgs* __restrict__ descs, const BatchItem* __restrict__ items) {
     const BatchItem it = items[blockIdx.x];
     if (it.desc == kIdleItem) return; // the whole workgroup: no barrier is left waiting
     rbgs_half_body<kSolve, kIn, kOut, kTXP, kWaves, kRowsW, kHalfDepth>(descs[it.desc], static_cast<int>(it.block));
