@@ -137,6 +137,18 @@ __device__ __forceinline__ HalfPos classify(const int i, const int y, const Dims
 template <class T>
 __device__ __forceinline__ T pick(const bool c, const T a, const T b) { return c ? a : b; }
 
+// LDS of one workgroup: the A and R rings and the ghost-rule table (declared by the kernel,
+// so both march directions share one allocation)
+template <class TT>
+struct HalfLds {
+    float a[4 * TT::AS];
+    float r[2 * TT::RS];
+    // ghost rules (a, c, k) of faces 0..5 and the identity at 6, read with runtime indices
+    // at set-up (indexing the kernel argument itself would copy it to scratch memory)
+    float rule_a[8], rule_c[8];
+    int rule_k[8];
+};
+
 template <int kRowsW, int kAPer, int kIn>
 struct PfSet {
     using T = std::conditional_t<kIn == kHalfInNatural, float2, float>;
@@ -144,8 +156,14 @@ struct PfSet {
     T ex[kAPer];
 };
 
-template <int kSolve, int kIn, int kOut, int kTXP, int kWaves, int kRowsW, int kDepth>
-__device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const int w_block) {
+// kDown: the chunk is marched from its top plane down (odd chunks), so two neighbouring
+// chunks reach the planes around their shared boundary at the same time and the second
+// read of each halo plane hits the XCD's L2. A red-black half-sweep has no order between
+// planes, so either direction gives the same bits; sums keep the order z-, z+ of the
+// grid, whatever the direction. Ring positions follow the march coordinate u; memory,
+// ghosts and parities the grid plane pl(u).
+template <int kSolve, int kIn, int kOut, int kTXP, int kWaves, int kRowsW, int kDepth, bool kDown>
+__device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const int w_block, HalfLds<HalfTile<kTXP, kWaves, kRowsW>>& lds) {
     using TT           = HalfTile<kTXP, kWaves, kRowsW>;
     constexpr int kR   = TT::kR;
     constexpr int kAPer = TT::kAPer;
@@ -156,12 +174,11 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
     constexpr int RF   = TT::RF;
     constexpr int AS   = TT::AS;
     constexpr int RS   = TT::RS;
-    __shared__ float s_a[4 * AS];
-    __shared__ float s_r[2 * RS];
-    // ghost rules (a, c, k) of faces 0..5 and the identity at 6, read with runtime indices
-    // at set-up (indexing the kernel argument itself would copy it to scratch memory)
-    __shared__ float s_rule_a[8], s_rule_c[8];
-    __shared__ int s_rule_k[8];
+    float* const s_a      = lds.a;
+    float* const s_r      = lds.r;
+    float* const s_rule_a = lds.rule_a;
+    float* const s_rule_c = lds.rule_c;
+    int* const s_rule_k   = lds.rule_k;
 
     const Dims d       = args.d;
     const StencilBC& bc = args.bc;
@@ -302,6 +319,8 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
     const DiffusionDiv ddv = diffusion_div(coef); // loop-invariant reciprocal of 1 + 6 coef
 
     auto zghost = [&](const int k) { return !pz && (k < 0 || k >= nz); };
+    const int zmir = z0 + z1 - 1;
+    auto pl        = [&](const int u) { return kDown ? zmir - u : u; }; // grid plane of march plane u
     // periodic plane of k in [-2, nz + 1] (the planes a chunk reads; nz >= 2)
     auto wrap_near_z = [&](const int k) { return k < 0 ? k + nz : (k >= nz ? k - nz : k); };
     // buffer resources: uniform bases once, the size selected per plane (0 bytes: off)
@@ -313,7 +332,8 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
 
     // ---- loads (kLive: the plane is known to be needed and real) ----
     using Pf = PfSet<kRowsW, kAPer, kIn>;
-    auto load_a = [&]<bool kLive>(const int k, Pf& pf) __attribute__((always_inline)) {
+    auto load_a = [&]<bool kLive>(const int u, Pf& pf) __attribute__((always_inline)) {
+        const int k = pl(u);
         if constexpr (kIn == kHalfInHalf) {
             const int kk    = pz ? wrap_near_z(k) : k;
             const bool live = kLive || (k >= z0 - 2 && k <= z1 + 1 && kk >= -1 && kk <= nz);
@@ -338,7 +358,8 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
         float2 own[kRowsW];
         float2 h;
     };
-    auto load_b = [&]<bool kLive>(const int k, PfB& pf) __attribute__((always_inline)) {
+    auto load_b = [&]<bool kLive>(const int u, PfB& pf) __attribute__((always_inline)) {
+        const int k     = pl(u);
         const int kc    = pz ? wrap_near_z(k) : (kLive ? k : min(max(k, 0), nz - 1));
         const bool live = kLive || (k >= z0 - 1 && k <= z1 && (pz || (k >= 0 && k < nz)));
         const Rsrc r    = rsrc_at(p_b, live ? nfield_b : 0u);
@@ -365,7 +386,7 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
         }
     };
 
-    // ---- register state ----
+    // ---- register state (march planes) ----
     float a_prev[kRowsW], a_cur[kRowsW], a_next[kRowsW]; // own A(s-1), A(s), A(s+1)
     float r_mm[kRowsW], r_m[kRowsW], r_0[kRowsW];        // own R(s-2), R(s-1), R(s)
     float bb[kRowsW], bb_new[kRowsW];                    // b of the own black cells, planes s-1 and s
@@ -376,8 +397,9 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
         bb[j] = bb_new[j] = 0.0f;
     }
 
-    // stage A(k) from `pf` into the ring plane at `A`; own values into `own`
-    auto stage_a = [&]<bool kReal>(const int k, float* A, const Pf& pf, float (&own)[kRowsW]) __attribute__((always_inline)) {
+    // stage A(u) from `pf` into the ring plane at `A`; own values into `own`
+    auto stage_a = [&]<bool kReal>(const int u, float* A, const Pf& pf, float (&own)[kRowsW]) __attribute__((always_inline)) {
+        const int k = pl(u);
 #pragma unroll
         for (int j = 0; j < kRowsW; ++j) {
             float v;
@@ -397,16 +419,22 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
 
     // red half-update of plane s (own slots and the R halo) from the A planes s-1, s, s+1
     // into R0 (kReal: s is not a z ghost plane)
-    auto red = [&]<bool kReal>(const int s, const float* Am, const float* A0, const float* Ap, float* R0, const PfB& pb) __attribute__((always_inline)) {
+    auto red = [&]<bool kReal>(const int u, const float* Am, const float* A0, const float* Ap, float* R0, const PfB& pb) __attribute__((always_inline)) {
+        const int s   = pl(u);
         const bool zg = !kReal && zghost(s);
         const int p   = (ypar + s) & 1;
+        // A planes below and above in the grid (Am / Ap are the march neighbours)
+        const float* Ab   = kDown ? Ap : Am;
+        const float* Aa   = kDown ? Am : Ap;
+        auto a_bl = [&](const int j) { if constexpr (kDown) return a_next[j]; else return a_prev[j]; };
+        auto a_ab = [&](const int j) { if constexpr (kDown) return a_prev[j]; else return a_next[j]; };
         float q[kRowsW], qh = 0.0f;
         if (zg) {
             // ghost plane: the z-face rule of the adjacent (black) old value
             const LinGhost zr = pick(s < 0, zr_lo, zr_hi);
 #pragma unroll
-            for (int j = 0; j < kRowsW; ++j) q[j] = lin_apply(zr, pick(s < 0, a_next[j], a_prev[j]));
-            if (wave < TT::kRHaloWaves) qh = lin_apply(zr, pick(s < 0, Ap[h_alds], Am[h_alds]));
+            for (int j = 0; j < kRowsW; ++j) q[j] = lin_apply(zr, pick(s < 0, a_ab(j), a_bl(j)));
+            if (wave < TT::kRHaloWaves) qh = lin_apply(zr, pick(s < 0, Aa[h_alds], Ab[h_alds]));
         } else {
             StageDiv<kSolve> dv(ddv);
             float x[kRowsW], xm_[kRowsW], ym_[kRowsW], xh = 0.0f;
@@ -421,8 +449,8 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
                 sum               = sum + xp;
                 sum               = sum + ym;
                 sum               = sum + A0[c + AW];
-                sum               = sum + a_prev[j];
-                sum               = sum + a_next[j];
+                sum               = sum + a_bl(j);
+                sum               = sum + a_ab(j);
                 const float2 bp   = pb.own[j];
                 x[j]              = dv.num(p ? bp.y : bp.x, sum, coef);
                 bb_new[j]         = p ? bp.x : bp.y;
@@ -442,8 +470,8 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
                 sum            = sum + A0[c + ph];
                 sum            = sum + A0[c - AW];
                 sum            = sum + A0[c + AW];
-                sum            = sum + Am[c];
-                sum            = sum + Ap[c];
+                sum            = sum + Ab[c];
+                sum            = sum + Aa[c];
                 const float xv = dv.num(ph ? pb.h.y : pb.h.x, sum, coef);
                 xh             = h_real ? xv : 0.0f;
                 hg             = lin_apply(h_rule, A0[c + h_goff]);
@@ -482,8 +510,8 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
         float other[kRowsW], ym[kRowsW], yp[kRowsW];
         float g;
     };
-    auto black_fetch = [&](const int s, const float* R1) __attribute__((always_inline)) {
-        const int p = (ypar + s - 1) & 1;
+    auto black_fetch = [&](const int u, const float* R1) __attribute__((always_inline)) {
+        const int p = (ypar + pl(u - 1)) & 1;
         BlackIn bi;
 #pragma unroll
         for (int j = 0; j < kRowsW; ++j) {
@@ -501,10 +529,13 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
     // black half-update of plane m = s - 1 from the LDS neighbours `bi`, the own red
     // values of planes m - 1 .. m + 1, and the stores of plane m (kLive: m is a plane of
     // the chunk and not a z face)
-    auto black = [&]<bool kLive>(const int s, const BlackIn& bi) __attribute__((always_inline)) {
-        const int m     = s - 1;
+    auto black = [&]<bool kLive>(const int u, const BlackIn& bi) __attribute__((always_inline)) {
+        const int m     = pl(u - 1);
         const bool live = kLive || (m >= z0 && m < z1); // planes of the chunk (never a ghost plane)
         const int p     = (ypar + m) & 1;
+        // own red values of the grid planes m - 1 and m + 1
+        auto r_bl = [&](const int j) { if constexpr (kDown) return r_0[j]; else return r_mm[j]; };
+        auto r_ab = [&](const int j) { if constexpr (kDown) return r_mm[j]; else return r_0[j]; };
         StageDiv<kSolve> dv(ddv);
         float x[kRowsW], q[kRowsW];
 #pragma unroll
@@ -515,8 +546,8 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
             sum            = sum + xp;
             sum            = sum + bi.ym[j];
             sum            = sum + bi.yp[j];
-            sum            = sum + r_mm[j];
-            sum            = sum + r_0[j];
+            sum            = sum + r_bl(j);
+            sum            = sum + r_ab(j);
             x[j]           = dv.num(bb[j], sum, coef);
         }
         if (!live || tile_ghost) {
@@ -658,7 +689,10 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
 // one pass over the grid args.g
 template <int kSolve, int kIn, int kOut, int kTXP, int kWaves, int kRowsW>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(kHalfMinWaves<kIn>, 4))) void k_rbgs_half(const HalfPassArgs args) {
-    rbgs_half_body<kSolve, kIn, kOut, kTXP, kWaves, kRowsW, kHalfDepth>(args, xcd_block());
+    __shared__ HalfLds<HalfTile<kTXP, kWaves, kRowsW>> lds;
+    const int w = xcd_block();
+    if ((w % args.g.chunks) & 1) rbgs_half_body<kSolve, kIn, kOut, kTXP, kWaves, kRowsW, kHalfDepth, true>(args, w, lds);
+    else rbgs_half_body<kSolve, kIn, kOut, kTXP, kWaves, kRowsW, kHalfDepth, false>(args, w, lds);
 }
 
 // batched: workgroup i runs block items[i].block of the pass descs[items[i].desc] (the
@@ -667,7 +701,11 @@ template <int kSolve, int kIn, int kOut, int kTXP, int kWaves, int kRowsW>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(kHalfMinWaves<kIn>, 4))) void k_rbgs_half_batched(const HalfPassArgs* __restrict__ descs, const BatchItem* __restrict__ items) {
     const BatchItem it = items[blockIdx.x];
     if (it.desc == kIdleItem) return; // the whole workgroup: no barrier is left waiting
-    rbgs_half_body<kSolve, kIn, kOut, kTXP, kWaves, kRowsW, kHalfDepth>(descs[it.desc], static_cast<int>(it.block));
+    __shared__ HalfLds<HalfTile<kTXP, kWaves, kRowsW>> lds;
+    const HalfPassArgs& a = descs[it.desc];
+    const int w           = static_cast<int>(it.block);
+    if ((w % a.g.chunks) & 1) rbgs_half_body<kSolve, kIn, kOut, kTXP, kWaves, kRowsW, kHalfDepth, true>(a, w, lds);
+    else rbgs_half_body<kSolve, kIn, kOut, kTXP, kWaves, kRowsW, kHalfDepth, false>(a, w, lds);
 }
 
 // Calls f.template operator()<solve, in, out, pairs, waves, row slots>() for the runtime choice.
