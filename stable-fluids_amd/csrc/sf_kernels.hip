@@ -8,6 +8,7 @@
 //  * projection, divergence of the projected field and |v| run in one pass.
 #include <algorithm>
 
+#include "sf_buffer.hpp"
 #include "sf_kernels.hpp"
 
 namespace sf {
@@ -286,6 +287,284 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_project_batched(const Projec
     project_body(sd.o[0], sd.o[1], sd.o[2], sd.div, sd.mag, sd.v[0], sd.v[1], sd.v[2], sd.p, sd.d, sd.h, sd.pbc, sd.dbc, x, y, bz);
 }
 
+// ---------------- divergence / projection, z-marching ----------------
+//
+// The same cell arithmetic, reorganised so every input is read once: a 512-thread
+// workgroup owns a 64 x 16 tile of a z chunk and marches upward. Per plane it stages the
+// tile plus its halo of the fields whose x and y neighbours are needed (pressure: two
+// cells, the x / y velocity components: one cell along their own axis) in LDS,
+// double-buffered so one barrier per plane orders the ring; the z neighbours come from
+// per-lane register windows (pressure z - 2 .. z + 2, the z component z - 1 .. z + 1).
+// Loads run one plane ahead through buffer resources; halo positions on periodic axes
+// are loaded from the wrapped cell, others outside the grid read 0 and are never used.
+// Every cell evaluates the general rule of project_body / divergence_body on the staged
+// values: a neighbour step that stays in the grid (or wraps on a periodic axis) takes the
+// neighbour, any other the face's ghost rule of the centre (apply_ghost), in the same
+// operand order, so the results are bit-identical. Fields below 2 GiB.
+constexpr int kZX = 64, kZWaves = 8, kZRows = 2, kZY = kZWaves * kZRows; // lane: column x; wave w: rows w, w + 8
+constexpr int kZThreads = 64 * kZWaves;
+constexpr int kZC       = 32; // planes per chunk
+
+struct ZTile {
+    int x0, y0, z0, z1;
+};
+
+// workgroup -> tile and chunk, XCD-contiguous (each XCD receives a run of chunks of a few
+// tile columns, whose register-window planes meet in its L2)
+__device__ __forceinline__ ZTile z_tile(const Dims d) {
+    const int n  = static_cast<int>(gridDim.x), w0 = static_cast<int>(blockIdx.x);
+    const int w  = (n & 7) == 0 ? (w0 & 7) * (n >> 3) + (w0 >> 3) : w0;
+    const int tx = (d.nx + kZX - 1) / kZX, ch = (d.nz + kZC - 1) / kZC;
+    ZTile t;
+    const int cz = w % ch, tt = w / ch;
+    t.x0 = (tt % tx) * kZX;
+    t.y0 = (tt / tx) * kZY;
+    t.z0 = cz * kZC;
+    t.z1 = min(t.z0 + kZC, d.nz);
+    return t;
+}
+
+// A staged plane of W x H positions (the tile plus halos HX, HY) spread over the
+// workgroup's threads, kSlots per thread: in-plane byte offsets (wrapped on periodic
+// axes, kOutOfRange outside the grid otherwise) and the values of the plane in flight.
+template <int W, int H, int HX, int HY>
+struct ZStage {
+    static constexpr int kN     = W * H;
+    static constexpr int kSlots = (kN + kZThreads - 1) / kZThreads;
+    int off[kSlots];
+    float v[kSlots];
+    __device__ __forceinline__ void init(const ZTile& t, const Dims d, const int periodic) {
+#pragma unroll
+        for (int k = 0; k < kSlots; ++k) {
+            const int e = static_cast<int>(threadIdx.x) + k * kZThreads;
+            int gx = t.x0 + e % W - HX, gy = t.y0 + e / W - HY;
+            bool in = e < kN;
+            if (gx < 0 || gx >= d.nx) {
+                in = in && (periodic & 1);
+                gx = wrap_index(gx, d.nx);
+            }
+            if (gy < 0 || gy >= d.ny) {
+                in = in && (periodic & 2);
+                gy = wrap_index(gy, d.ny);
+            }
+            off[k] = in ? 4 * (gy * d.nx + gx) : kOutOfRange;
+        }
+    }
+    __device__ __forceinline__ void load(const Rsrc r, const int soff) {
+#pragma unroll
+        for (int k = 0; k < kSlots; ++k) v[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off[k], soff, 0));
+    }
+    __device__ __forceinline__ void store(float* lds) const {
+#pragma unroll
+        for (int k = 0; k < kSlots; ++k) {
+            const int e = static_cast<int>(threadIdx.x) + k * kZThreads;
+            if (k + 1 < kSlots || e < kN) lds[e] = v[k];
+        }
+    }
+};
+
+// Resource over the whole field and the byte offset of plane z, wrapped on a periodic z
+// axis; a 0-byte resource (reads 0) for planes outside a non-periodic one.
+__device__ __forceinline__ Rsrc z_plane(const void* base, int z, const Dims d, const bool periodic, int& soff) {
+    const uint32_t plane = 4u * static_cast<uint32_t>(d.nx) * static_cast<uint32_t>(d.ny);
+    if (periodic) z = wrap_index(z, d.nz);
+    const bool in = z >= 0 && z < d.nz;
+    soff          = in ? z * static_cast<int>(plane) : 0;
+    return plane_rsrc(base, in ? plane * static_cast<uint32_t>(d.nz) : 0u);
+}
+
+// One axis of the divergence of the projected field at a cell (project_body's general
+// rule): c0 centre of the velocity component along the axis, v1m / v1p its neighbours,
+// p2m .. p2p the pressure at -2 .. +2 along the axis; a1m, a1p, a2m, a2p: the pressure
+// steps -1, +1, -2, +2 stay in the grid (pressure rules); d1m, d1p: the velocity steps do
+// (divergence rules); fp = 2 axis. Returns the projected component, and the term in dd.
+__device__ __forceinline__ float project_axis(const float c0, const float v1m, const float v1p, const float p2m, const float p1m, const float pc, const float p1p, const float p2p, const bool a2m, const bool a1m, const bool a1p, const bool a2p, const bool d1m, const bool d1p, const StencilBC& pbc, const StencilBC& dbc, const int fp, const float inv_2h, float& dd) {
+    const float gplus  = a1p ? p1p : apply_ghost(pbc.face[fp + 1], pc);
+    const float gminus = a1m ? p1m : apply_ghost(pbc.face[fp], pc);
+    const float proj   = c0 - (gplus - gminus) * inv_2h;
+    // the neighbours' projected values: their gradient has this cell on the inner side
+    const float npp   = a2p ? p2p : apply_ghost(pbc.face[fp + 1], p1p);
+    const float nmm   = a2m ? p2m : apply_ghost(pbc.face[fp], p1m);
+    const float plus  = d1p ? v1p - (npp - pc) * inv_2h : apply_ghost(dbc.face[fp + 1], proj);
+    const float minus = d1m ? v1m - (pc - nmm) * inv_2h : apply_ghost(dbc.face[fp], proj);
+    dd                = (plus - minus) * inv_2h;
+    return proj;
+}
+
+__global__ __launch_bounds__(kZThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_project_z(float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz, float* __restrict__ div, float* __restrict__ mag, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float* __restrict__ p, const Dims d, const float h, const StencilBC pbc, const StencilBC dbc) {
+    constexpr int PW = kZX + 4, PH = kZY + 4; // pressure: two-cell halo
+    constexpr int XW = kZX + 2;               // x component: one-cell x halo
+    constexpr int YH = kZY + 2;               // y component: one-cell y halo
+    using SP = ZStage<PW, PH, 2, 2>;
+    using SX = ZStage<XW, kZY, 1, 0>;
+    using SY = ZStage<kZX, YH, 0, 1>;
+    __shared__ float s_p[2][PW * PH], s_x[2][XW * kZY], s_y[2][kZX * YH];
+    const ZTile t      = z_tile(d);
+    const int lane     = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
+    const int x        = t.x0 + lane;
+    const float inv_2h = 0.5f / h;
+    const int pp = pbc.periodic, dp = dbc.periodic;
+    SP sp;
+    SX sx;
+    SY sy;
+    sp.init(t, d, pp);
+    sx.init(t, d, dp);
+    sy.init(t, d, dp);
+    // own columns (rows y0 + wave + 8 r): in-plane offsets and register windows
+    int o_off[kZRows];
+    float pw[5][kZRows], zw[3][kZRows], pn[kZRows], zn[kZRows];
+#pragma unroll
+    for (int r = 0; r < kZRows; ++r) {
+        const int y = t.y0 + wave + kZWaves * r;
+        o_off[r]    = (x < d.nx && y < d.ny) ? 4 * (y * d.nx + x) : kOutOfRange;
+    }
+    auto own = [&](const float* f, const int z, const bool per, float (&out)[kZRows]) {
+        int so;
+        const Rsrc rs = z_plane(f, z, d, per, so);
+#pragma unroll
+        for (int r = 0; r < kZRows; ++r) out[r] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, o_off[r], so, 0));
+    };
+    auto stage_loads = [&](const int z) {
+        int so;
+        Rsrc rs = z_plane(p, z, d, false, so);
+        sp.load(rs, so);
+        rs = z_plane(vx, z, d, false, so);
+        sx.load(rs, so);
+        rs = z_plane(vy, z, d, false, so);
+        sy.load(rs, so);
+    };
+    // prologue: windows for plane z0, the staged planes of z0 in flight
+    for (int k = 0; k < 5; ++k) own(p, t.z0 - 2 + k, pp & 4, pw[k]);
+    for (int k = 0; k < 3; ++k) own(vz, t.z0 - 1 + k, dp & 4, zw[k]);
+    stage_loads(t.z0);
+    // x step flags of the lane's column (y and z per row and plane)
+    const bool ax2m = (pp & 1) || x >= 2, ax1m = (pp & 1) || x >= 1, ax1p = (pp & 1) || x + 1 < d.nx, ax2p = (pp & 1) || x + 2 < d.nx;
+    const bool dx1m = (dp & 1) || x >= 1, dx1p = (dp & 1) || x + 1 < d.nx;
+    for (int z = t.z0; z < t.z1; ++z) {
+        const int b = z & 1;
+        sp.store(s_p[b]);
+        sx.store(s_x[b]);
+        sy.store(s_y[b]);
+        if (z + 1 < t.z1) stage_loads(z + 1);
+        own(p, z + 3, pp & 4, pn);
+        own(vz, z + 2, dp & 4, zn);
+        __syncthreads();
+        const bool az2m = (pp & 4) || z >= 2, az1m = (pp & 4) || z >= 1, az1p = (pp & 4) || z + 1 < d.nz, az2p = (pp & 4) || z + 2 < d.nz;
+        const bool dz1m = (dp & 4) || z >= 1, dz1p = (dp & 4) || z + 1 < d.nz;
+#pragma unroll
+        for (int r = 0; r < kZRows; ++r) {
+            const int ty = wave + kZWaves * r, y = t.y0 + ty;
+            if (x >= d.nx || y >= d.ny) continue;
+            const bool ay2m = (pp & 2) || y >= 2, ay1m = (pp & 2) || y >= 1, ay1p = (pp & 2) || y + 1 < d.ny, ay2p = (pp & 2) || y + 2 < d.ny;
+            const bool dy1m = (dp & 2) || y >= 1, dy1p = (dp & 2) || y + 1 < d.ny;
+            const float* P = s_p[b] + (ty + 2) * PW + lane + 2;
+            const float* X = s_x[b] + ty * XW + lane + 1;
+            const float* Y = s_y[b] + (ty + 1) * kZX + lane;
+            const float pc = pw[2][r];
+            float proj[3], acc, dd;
+            proj[0] = project_axis(X[0], X[-1], X[1], P[-2], P[-1], pc, P[1], P[2], ax2m, ax1m, ax1p, ax2p, dx1m, dx1p, pbc, dbc, 0, inv_2h, dd);
+            acc     = dd;
+            proj[1] = project_axis(Y[0], Y[-kZX], Y[kZX], P[-2 * PW], P[-PW], pc, P[PW], P[2 * PW], ay2m, ay1m, ay1p, ay2p, dy1m, dy1p, pbc, dbc, 2, inv_2h, dd);
+            acc     = acc + dd;
+            proj[2] = project_axis(zw[1][r], zw[0][r], zw[2][r], pw[0][r], pw[1][r], pc, pw[3][r], pw[4][r], az2m, az1m, az1p, az2p, dz1m, dz1p, pbc, dbc, 4, inv_2h, dd);
+            acc     = acc + dd;
+            const uint64_t i = lin(x, y, z, d);
+            ox[i]  = proj[0];
+            oy[i]  = proj[1];
+            oz[i]  = proj[2];
+            div[i] = acc;
+            mag[i] = sqrtf(proj[0] * proj[0] + proj[1] * proj[1] + proj[2] * proj[2]);
+        }
+#pragma unroll
+        for (int r = 0; r < kZRows; ++r) {
+            pw[0][r] = pw[1][r];
+            pw[1][r] = pw[2][r];
+            pw[2][r] = pw[3][r];
+            pw[3][r] = pw[4][r];
+            pw[4][r] = pn[r];
+            zw[0][r] = zw[1][r];
+            zw[1][r] = zw[2][r];
+            zw[2][r] = zn[r];
+        }
+    }
+}
+
+// One axis of divergence_body's general rule: (plus - minus) / 2h with ghost rules of the
+// centre at the faces.
+__device__ __forceinline__ float divergence_axis(const float c0, const float v1m, const float v1p, const bool d1m, const bool d1p, const StencilBC& dbc, const int fp, const float inv_2h) {
+    const float plus  = d1p ? v1p : apply_ghost(dbc.face[fp + 1], c0);
+    const float minus = d1m ? v1m : apply_ghost(dbc.face[fp], c0);
+    return (plus - minus) * inv_2h;
+}
+
+__global__ __launch_bounds__(kZThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_divergence_z(float* __restrict__ div, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const Dims d, const float h, const StencilBC dbc) {
+    constexpr int XW = kZX + 2, YH = kZY + 2;
+    using SX = ZStage<XW, kZY, 1, 0>;
+    using SY = ZStage<kZX, YH, 0, 1>;
+    __shared__ float s_x[2][XW * kZY], s_y[2][kZX * YH];
+    const ZTile t      = z_tile(d);
+    const int lane     = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
+    const int x        = t.x0 + lane;
+    const float inv_2h = 0.5f / h;
+    const int dp       = dbc.periodic;
+    SX sx;
+    SY sy;
+    sx.init(t, d, dp);
+    sy.init(t, d, dp);
+    int o_off[kZRows];
+    float zw[3][kZRows], zn[kZRows];
+#pragma unroll
+    for (int r = 0; r < kZRows; ++r) {
+        const int y = t.y0 + wave + kZWaves * r;
+        o_off[r]    = (x < d.nx && y < d.ny) ? 4 * (y * d.nx + x) : kOutOfRange;
+    }
+    auto own = [&](const int z, float (&out)[kZRows]) {
+        int so;
+        const Rsrc rs = z_plane(vz, z, d, dp & 4, so);
+#pragma unroll
+        for (int r = 0; r < kZRows; ++r) out[r] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, o_off[r], so, 0));
+    };
+    auto stage_loads = [&](const int z) {
+        int so;
+        Rsrc rs = z_plane(vx, z, d, false, so);
+        sx.load(rs, so);
+        rs = z_plane(vy, z, d, false, so);
+        sy.load(rs, so);
+    };
+    for (int k = 0; k < 3; ++k) own(t.z0 - 1 + k, zw[k]);
+    stage_loads(t.z0);
+    const bool dx1m = (dp & 1) || x >= 1, dx1p = (dp & 1) || x + 1 < d.nx;
+    for (int z = t.z0; z < t.z1; ++z) {
+        const int b = z & 1;
+        sx.store(s_x[b]);
+        sy.store(s_y[b]);
+        if (z + 1 < t.z1) stage_loads(z + 1);
+        own(z + 2, zn);
+        __syncthreads();
+        const bool dz1m = (dp & 4) || z >= 1, dz1p = (dp & 4) || z + 1 < d.nz;
+#pragma unroll
+        for (int r = 0; r < kZRows; ++r) {
+            const int ty = wave + kZWaves * r, y = t.y0 + ty;
+            if (x >= d.nx || y >= d.ny) continue;
+            const bool dy1m = (dp & 2) || y >= 1, dy1p = (dp & 2) || y + 1 < d.ny;
+            const float* X = s_x[b] + ty * XW + lane + 1;
+            const float* Y = s_y[b] + (ty + 1) * kZX + lane;
+            float acc      = divergence_axis(X[0], X[-1], X[1], dx1m, dx1p, dbc, 0, inv_2h);
+            acc            = acc + divergence_axis(Y[0], Y[-kZX], Y[kZX], dy1m, dy1p, dbc, 2, inv_2h);
+            acc            = acc + divergence_axis(zw[1][r], zw[0][r], zw[2][r], dz1m, dz1p, dbc, 4, inv_2h);
+            div[lin(x, y, z, d)] = acc;
+        }
+#pragma unroll
+        for (int r = 0; r < kZRows; ++r) {
+            zw[0][r] = zw[1][r];
+            zw[1][r] = zw[2][r];
+            zw[2][r] = zn[r];
+        }
+    }
+}
+
+uint32_t z_blocks(const Dims d) { return static_cast<uint32_t>(((d.nx + kZX - 1) / kZX) * ((d.ny + kZY - 1) / kZY) * ((d.nz + kZC - 1) / kZC)); }
+
 // Batched elementwise stages: local block b covers elements [4 * 256 * b, + 1024).
 __global__ __launch_bounds__(kElemBlock) void k_add_force_batched(const AddForceDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
     const BatchItem it = items[blockIdx.x];
@@ -379,12 +658,14 @@ hipError_t launch_advect_scalar(hipStream_t s, float* out, const float* src, con
 }
 
 hipError_t launch_divergence(hipStream_t s, float* div, const float* vx, const float* vy, const float* vz, const Dims d, const float h, const StencilBC& div_bc) {
-    k_divergence<<<tile_grid(d), tile_block(), 0, s>>>(div, vx, vy, vz, d, h, div_bc);
+    if (advect_buffers_fit(d)) k_divergence_z<<<z_blocks(d), kZThreads, 0, s>>>(div, vx, vy, vz, d, h, div_bc);
+    else k_divergence<<<tile_grid(d), tile_block(), 0, s>>>(div, vx, vy, vz, d, h, div_bc);
     return hipGetLastError();
 }
 
 hipError_t launch_project(hipStream_t s, float* ox, float* oy, float* oz, float* div, float* mag, const float* vx, const float* vy, const float* vz, const float* p, const Dims d, const float h, const StencilBC& p_bc, const StencilBC& div_bc) {
-    k_project<<<tile_grid(d), tile_block(), 0, s>>>(ox, oy, oz, div, mag, vx, vy, vz, p, d, h, p_bc, div_bc);
+    if (advect_buffers_fit(d)) k_project_z<<<z_blocks(d), kZThreads, 0, s>>>(ox, oy, oz, div, mag, vx, vy, vz, p, d, h, p_bc, div_bc);
+    else k_project<<<tile_grid(d), tile_block(), 0, s>>>(ox, oy, oz, div, mag, vx, vy, vz, p, d, h, p_bc, div_bc);
     return hipGetLastError();
 }
 
