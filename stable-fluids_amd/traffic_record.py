@@ -27,6 +27,8 @@ ALGORITHMIC_BYTES_PER_CELL = {
     "k_advect_scalar_b": 20,
     "k_project": 36,              # read a (3 components), p; write v (3), div, |v|
     "k_divergence": 16,           # read 3 velocity components; write div
+    "k_project_z": 36,            # z-marching variants of the two above
+    "k_divergence_z": 16,
     "k_add_force": 36,            # read 3 velocity + 3 force components; write 3
     "k_add_scaled": 12,           # read the scalar and its source; write 1
 }
