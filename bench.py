@@ -9,14 +9,16 @@ Printed JSON line: steps/s (value), Gcell-updates/s, the dominant kernel's algor
 bandwidth against the 8 TB/s HBM peak (timed live with HIP events on the context stream,
 beside the average of the committed kernel trace of the same command, and again at 512^3
 where the pass's working set no longer fits the 256 MB last-level cache), the Jacobi
-pass bandwidth (256^3 and 512^3), the learned-vs-plain speedup at equal post-projection
+pass bandwidth (256^3 and 512^3), the in-graph cost of one more smoother pass (RBGS and
+Jacobi, pressure and diffusion, from step times at two pass counts), the learned-vs-plain
+speedup at equal post-projection
 divergence L2 (warm-start network + K passes against 80 plain passes, from common
 states of the 256^3 plume; needs the trained parameters under stable-fluids_amd/weights)
 and the host-loop baseline (the oracle on this host's cores, bounded samples).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid N] [--smoother rbgs|jacobi]
                   [--storage f32|f16|bf16|f32-refined] [--interval M]
-                  [--no-cpu-baseline] [--no-learned] [--no-512] [--stage-repeat R]
+                  [--no-cpu-baseline] [--no-learned] [--no-512] [--no-in-graph] [--stage-repeat R]
   python bench.py --learned-step K [--steps S] [--warmup W]   (the learned step alone, for traces)
   python bench.py --batch [--tuning-cache FILE]                (64 uneven simulations, below)
 
@@ -83,6 +85,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain projection comparison")
     ap.add_argument("--no-512", action="store_true", help="skip the 512^3 smoother-pass leg of the roofline")
+    ap.add_argument("--no-in-graph", action="store_true", help="skip the in-graph marginal pass costs")
     ap.add_argument("--precision", action="store_true", help="run the 512^3 storage precision study instead")
     ap.add_argument("--fp64", action="store_true",
                     help="with --precision: the well-conditioned variant (force 0.5, 5 steps) against the FP64 oracle")
@@ -150,6 +153,11 @@ def main():
             scenes.seed_context(big_ctx, big_scene)
             big = timed_stages(big_ctx, max(4, args.stage_repeat // 4), names=("velocity_pass", "jacobi_pass"))
 
+    # in-graph cost of one more smoother pass, per smoother and solve (no profiler)
+    in_graph = None
+    if default_run and not args.no_in_graph:
+        in_graph = in_graph_pass_costs(pkg, n)
+
     # the committed kernel trace (rocprofv3 --kernel-trace --stats of this command) and
     # the committed FETCH_SIZE / WRITE_SIZE counts (copy-kernel calibrated) of the kernel
     trace = traffic = None
@@ -212,6 +220,7 @@ def main():
         "roofline": roofline(dom, trace, traffic, big),
         "jacobi_hbm_gbs": stages["jacobi_pass"]["gbs"],
         "jacobi_hbm_gbs_512": big["jacobi_pass"]["gbs"] if big else None,
+        "in_graph_pass": in_graph,
         "learned_vs_plain_speedup": learned["speedup_vs_rbgs"] if learned else None,
         "learned_vs_plain": learned,
         "stages": stages,
@@ -231,6 +240,40 @@ def timed_stages(ctx, repeat, names=None):
         ms = ctx.time_stage(stage, repeat)
         by = ctx.stage_bytes(stage)
         out[name] = {"us": round(ms * 1e3, 3), "bytes": by, "gbs": round(by / (ms * 1e-3) / 1e9, 1)}
+    return out
+
+
+def in_graph_pass_costs(pkg, n, reps=3, steps=10):
+    """Marginal cost of one smoother pass inside the captured step, without a profiler: the
+    step's GPU time (best of `reps` runs of `steps` graph launches) at P = 160 minus P = 80
+    over the 80 extra pressure passes, and at D = 80 minus D = 40 over the 160 extra
+    diffusion passes (three velocity components and one scalar). It includes each pass's
+    launch as a graph node."""
+    abi, hip, scenes = pkg.abi, pkg.hip, pkg.scenes
+    alg = {"rbgs": 8 * n ** 3, "jacobi": 12 * n ** 3}  # algorithmic bytes of a middle pass
+    out = {"method": "step time at P=160 minus P=80 (per pressure pass) and at D=80 minus D=40 (per diffusion pass), "
+                     f"best of {reps} x {steps} graph launches; includes the per-node launch"}
+    for name, sm in (("rbgs", abi.SMOOTHER_RBGS), ("jacobi", abi.SMOOTHER_JACOBI)):
+        ms = {}
+        for d, p in ((40, 80), (40, 160), (80, 80)):
+            scene = scenes.plume_scene(n, diffuse_iterations=d, pressure_iterations=p, noise_seed=0)
+            with pkg.Context(scene.desc, options=pkg.ext_options(smoother=sm)) as ctx:
+                scenes.seed_context(ctx, scene)
+                ctx.step(3)
+                hip.synchronize()
+                best = 1e9
+                for _ in range(reps):
+                    t0 = time.perf_counter()
+                    ctx.step(steps)
+                    hip.synchronize()
+                    best = min(best, (time.perf_counter() - t0) / steps)
+                ms[(d, p)] = best * 1e3
+        pres = (ms[(40, 160)] - ms[(40, 80)]) / 80 * 1e3
+        diff = (ms[(80, 80)] - ms[(40, 80)]) / 160 * 1e3
+        out[name] = {"step_ms": round(ms[(40, 80)], 4), "pressure_pass_us": round(pres, 3), "diffusion_pass_us": round(diff, 3),
+                     "bytes_per_pass": alg[name],
+                     "pressure_frac": round(alg[name] / (pres * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                     "diffusion_frac": round(alg[name] / (diff * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
     return out
 
 

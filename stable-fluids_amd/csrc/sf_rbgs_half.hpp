@@ -58,10 +58,9 @@ namespace {
 #define SF_HALF_DEPTH 2
 #endif
 constexpr int kHalfDepth = SF_HALF_DEPTH; // planes of loads in flight
-// Cache policy of the black-cell stores: sc1 (device scope), so they do not sit dirty in the
-// writer XCD's L2 when the pass ends (the next pass, on any XCD, reads them): 28.8 -> 28.0 us
-// per middle pass at 256^3 against the default policy; non-temporal stores measured slower.
-constexpr int kHalfStoreAux = 16;
+// Cache policy of the black-cell stores: device scope (kDeviceScopeStore): 28.8 -> 28.0 us per
+// middle pass at 256^3 against the default policy; non-temporal stores measured slower.
+constexpr int kHalfStoreAux = kDeviceScopeStore;
 // Minimum waves per SIMD the register allocation must leave room for. Passes that read the
 // natural-layout start iterate hold float2 loads in flight and went to 133 VGPRs (three
 // waves per SIMD, one 512-thread workgroup per CU); capped at 128 they keep two workgroups
@@ -589,7 +588,7 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
                 // p = 0: the red cell is the even one
                 const float ve = (p ? q[j] : r_m[j]) * f;
                 const float vo = (p ? r_m[j] : q[j]) * f;
-                buf_store2<StoreF32>(r, o_st[j], ve, vo, so);
+                buf_store2<StoreF32, kHalfStoreAux>(r, o_st[j], ve, vo, so);
             }
         }
     };

@@ -479,7 +479,7 @@ __device__ __forceinline__ void jacobi_rows_body(const typename St::T* __restric
             const float2 L = so[c - 1], C = so[c], R = so[c + 1], YM = so[c - kHPX], YP = so[c + kHPX], ZM = sm[c], ZP = sp[c];
             const float v0 = relax_elem<kSolve>(0, L, C, R, YM, YP, ZM, ZP, bv[i].x * inv_s, coef) * out_scale;
             const float v1 = relax_elem<kSolve>(1, L, C, R, YM, YP, ZM, ZP, bv[i].y * inv_s, coef) * out_scale;
-            buf_store2<St>(dst, x_off[i], v0, v1);
+            buf_store2<St, kDeviceScopeStore>(dst, x_off[i], v0, v1);
         }
     };
     // plane k: stage k + 1 (fetched two steps ago into r) and refill r with k + 3; after the
