@@ -452,7 +452,7 @@ def batch_study(pkg, args):
         batch = pkg.Batch(ctxs) if batched else None
         info = None
         if batch is not None and tune:
-            tuned = batch.autotune(args.tuning_cache, repeats=5)
+            tuned = batch.autotune(args.tuning_cache, repeats=20)
             plans = {}
             for i, sc in enumerate(scenes_):
                 p = batch.info(i)

@@ -987,7 +987,7 @@ hipError_t plan_batch_step(Batch& B) {
     for (size_t i = 0; i < N; ++i) geo[i] = plan_tiles(S[i]->dims, B.zc[i]);
 
     const auto elem_list  = batch_items(N, [&](size_t d) { return elem_block_count(S[d]->cells); });
-    const auto tile_list  = xcd_order(batch_items(N, [&](size_t d) { return tile_blocks(S[d]->dims); }));
+    const auto tile_list  = xcd_order(batch_items(N, [&](size_t d) { return z_march_blocks(S[d]->dims); })); // divergence, projection
     const auto adv_list   = xcd_order(batch_items(N, [&](size_t d) { return advect_blocks(S[d]->dims); }));
     const auto elem_items = B.upload(elem_list);
     const auto tile_items = B.upload(tile_list);
@@ -1073,7 +1073,7 @@ hipError_t plan_batch_step(Batch& B) {
         }
         const auto dd = B.upload(d);
         if (!dd) return hipErrorOutOfMemory;
-        SF_BTRY(add([=] { return launch_divergence_batched(st, dd, tile_items, n_tile); }));
+        SF_BTRY(add([=] { return launch_divergence_z_batched(st, dd, tile_items, n_tile); }));
     }
     for (int pass = 1; pass <= P; ++pass) {
         std::vector<SmoothDesc> d(N);
@@ -1105,7 +1105,7 @@ hipError_t plan_batch_step(Batch& B) {
         }
         const auto dd = B.upload(d);
         if (!dd) return hipErrorOutOfMemory;
-        SF_BTRY(add([=] { return launch_project_batched(st, dd, tile_items, n_tile); }));
+        SF_BTRY(add([=] { return launch_project_z_batched(st, dd, tile_items, n_tile); }));
     }
     // scalars (reference :652-668)
     for (size_t si = 0; si < c0.scalars.size(); ++si) {
@@ -1329,6 +1329,72 @@ double time_batch_smoother(Batch& B, const std::vector<char>& only, const int re
         ok      = ok && hipEventRecord(e1, B.stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
         float t = 0.0f;
         if (ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms = t / repeats;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipGetLastError();
+    return ms;
+}
+
+// State a batched simulation carries from one step to the next or shows as a view (the
+// velocity, pressure, divergence and |v| views and the scalar fields; batched simulations
+// have constant forces and sources, and every other array is rewritten each step before
+// it is read), copied aside while the tuner times real steps.
+struct BatchState {
+    void* buf = nullptr;
+    std::vector<std::pair<float*, size_t>> arrays; // (array, floats)
+    ~BatchState() {
+        if (buf) (void)hipFree(buf);
+    }
+};
+
+std::vector<std::pair<float*, size_t>> batch_state_arrays(const Batch& B) {
+    std::vector<std::pair<float*, size_t>> out;
+    for (const Context* c : B.sims) {
+        for (float* p : {c->flow.vx, c->flow.vy, c->flow.vz, c->flow.p, c->flow.div, c->flow.mag}) out.emplace_back(p, c->cells);
+        for (const auto& s : c->scalars) out.emplace_back(s.data, c->cells);
+    }
+    return out;
+}
+
+bool copy_batch_state(Batch& B, BatchState& st, const bool save) {
+    size_t off = 0;
+    for (const auto& [p, n] : st.arrays) {
+        float* mine = static_cast<float*>(st.buf) + off;
+        if (hipMemcpyAsync(save ? mine : p, save ? p : mine, n * sizeof(float), hipMemcpyDeviceToDevice, B.stream) != hipSuccess) return false;
+        off += n;
+    }
+    return hipStreamSynchronize(B.stream) == hipSuccess;
+}
+
+bool save_batch_state(Batch& B, BatchState& st) {
+    for (const Context* c : B.sims)
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return false;
+    st.arrays    = batch_state_arrays(B);
+    size_t total = 0;
+    for (const auto& a : st.arrays) total += a.second;
+    if (hipMalloc(&st.buf, total * sizeof(float)) != hipSuccess) {
+        (void)hipGetLastError();
+        st.buf = nullptr;
+        return false;
+    }
+    return copy_batch_state(B, st, true);
+}
+
+bool restore_batch_state(Batch& B, BatchState& st) { return copy_batch_state(B, st, false); }
+
+// Mean GPU milliseconds of one batch step under the batch's current tiling: the graph
+// rebuilt, one untimed step, then `steps` timed ones.
+double time_batch_steps(Batch& B, const int steps) {
+    if (build_batch_graph(B) != STABLE_FLUIDS_RESULT_OK) return -1.0;
+    double ms     = -1.0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+        bool ok = hipGraphLaunch(B.graph_exec, B.stream) == hipSuccess && hipEventRecord(e0, B.stream) == hipSuccess;
+        for (int r = 0; ok && r < steps; ++r) ok = hipGraphLaunch(B.graph_exec, B.stream) == hipSuccess;
+        ok      = ok && hipEventRecord(e1, B.stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+        float t = 0.0f;
+        if (ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms = t / steps;
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -1795,8 +1861,8 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_autotune_cuda(void*
     if (tuned > 0) {
         // whole-batch plans (launches of different tile shapes run one after another):
         // every class on its own fastest tiling; every black-cell class on one tile shape
-        // T, each with its fastest chunk under T; the starting tiling. The plan with the
-        // fastest whole-batch pass wins.
+        // T, each with its fastest chunk under T, its default chunk or 16 planes; the
+        // starting tiling.
         auto fastest = [](const Class& k, const int tile) {
             size_t best = k.cands.size();
             for (size_t c = 0; c < k.cands.size(); ++c)
@@ -1810,23 +1876,55 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_autotune_cuda(void*
         for (int T = -1; T < sf::kHalfTileCount; ++T) {
             for (const auto& k : classes) assign(k, fastest(k, (k.half && !k.cached) ? T : -1));
             plans.emplace_back(b.tile, b.zc);
+            if (T < 0) continue;
+            // the same tile shape with every black-cell class on its default chunk, and on
+            // 16-plane chunks: chunks picked on a class alone can leave the whole batch
+            // waiting on long workgroups
+            for (const int zc : {0, 16}) {
+                for (const auto& k : classes)
+                    if (k.half && !k.cached) assign(k, sf::Tiling{T, zc});
+                plans.emplace_back(b.tile, b.zc);
+            }
         }
         plans.emplace_back(tile0, zc0);
-        std::vector<double> t_plan(plans.size(), 1e300);
-        for (int round = 0; round < sf::kTuneRounds; ++round)
-            for (size_t q = 0; q < plans.size(); ++q) {
-                b.tile         = plans[q].first;
-                b.zc           = plans[q].second;
-                const double t = sf::time_batch_smoother(b, all, reps);
-                if (t > 0.0) t_plan[q] = std::min(t_plan[q], t);
+        {
+            std::vector<std::pair<std::vector<int>, std::vector<int>>> distinct;
+            for (const auto& q : plans)
+                if (std::find(distinct.begin(), distinct.end(), q) == distinct.end()) distinct.push_back(q);
+            plans.swap(distinct);
+        }
+        // The plans and then each timed class's z chunk are decided on the batch's real
+        // step (its graph rebuilt per plan, two rounds of steps, minimum): the smoother
+        // proxy above ranks single classes well but misranked whole plans in about one
+        // tuning in three (the batch then ran 10% slower). The simulations' state is
+        // saved first and restored after, so tuning changes no result. Without memory for
+        // the copy, the proxy decides.
+        sf::BatchState saved;
+        const bool real = sf::save_batch_state(b, saved);
+        auto timed = [&]() {
+            double t = 1e300;
+            for (int round = 0; round < 2; ++round) {
+                const double r = real ? sf::time_batch_steps(b, 2) : sf::time_batch_smoother(b, all, reps);
+                if (r > 0.0) t = std::min(t, r);
             }
-        const size_t q = static_cast<size_t>(std::min_element(t_plan.begin(), t_plan.end()) - t_plan.begin());
-        b.tile         = plans[q].first;
-        b.zc           = plans[q].second;
-        // then each timed class's z chunk on the whole batch, its tile shape kept: a chunk
-        // length that ties when the class runs alone can leave the batch waiting on its
-        // longest workgroups
-        double t_best = t_plan[q];
+            return t;
+        };
+        double t_best = 1e300;
+        size_t q_best = 0;
+        for (size_t q = 0; q < plans.size(); ++q) {
+            b.tile         = plans[q].first;
+            b.zc           = plans[q].second;
+            const double t = timed();
+            if (t < t_best) {
+                t_best = t;
+                q_best = q;
+            }
+        }
+        b.tile = plans[q_best].first;
+        b.zc   = plans[q_best].second;
+        // each timed class's z chunk on the whole batch, its tile shape kept: a chunk length
+        // that ties when the class runs alone can leave the batch waiting on its longest
+        // workgroups
         for (const auto& k : classes) {
             if (k.cached) continue;
             const int tile = b.tile[k.first], zc_now = b.zc[k.first];
@@ -1834,11 +1932,7 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_autotune_cuda(void*
             for (const auto& cand : k.cands) {
                 if (cand.tile != tile || cand.zc == zc_now) continue;
                 assign(k, cand);
-                double t = 1e300;
-                for (int round = 0; round < sf::kTuneRounds; ++round) {
-                    const double r = sf::time_batch_smoother(b, all, reps);
-                    if (r > 0.0) t = std::min(t, r);
-                }
+                const double t = timed();
                 if (t < t_best) {
                     t_best  = t;
                     zc_best = cand.zc;
@@ -1846,6 +1940,7 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_autotune_cuda(void*
             }
             assign(k, sf::Tiling{tile, zc_best});
         }
+        if (real && !sf::restore_batch_state(b, saved)) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         for (const auto& k : classes) {
             if (k.cached) continue;
             const sf::Tiling t = effective(k, sf::Tiling{b.tile[k.first], b.zc[k.first]});

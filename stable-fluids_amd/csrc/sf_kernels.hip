@@ -234,13 +234,6 @@ __device__ __forceinline__ void project_body(float* __restrict__ ox, float* __re
     mag[i] = sqrtf(proj[0] * proj[0] + proj[1] * proj[1] + proj[2] * proj[2]);
 }
 
-// Local block b of a tile-kernel launch over d -> (block x, block y, plane).
-__device__ __forceinline__ void tile_block_of(const Dims d, const uint32_t b, int& bx, int& by, int& bz) {
-    const uint32_t gx = static_cast<uint32_t>((d.nx + kTileX - 1) / kTileX), gy = static_cast<uint32_t>((d.ny + kTileY - 1) / kTileY);
-    bx = static_cast<int>(b % gx);
-    by = static_cast<int>((b / gx) % gy);
-    bz = static_cast<int>(b / (gx * gy));
-}
 
 __global__ __launch_bounds__(kTileX* kTileY) void k_advect_velocity(float* __restrict__ tx, float* __restrict__ ty, float* __restrict__ tz, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float dt, const Dims d, const float h, const StableFluidsFlowBoundaryConfig bc) {
     advect_velocity_body(tx, ty, tz, vx, vy, vz, dt, d, h, bc, static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), static_cast<int>(blockIdx.y * blockDim.y + threadIdx.y), static_cast<int>(blockIdx.z));
@@ -263,28 +256,10 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_divergence(float* __restrict
     cell_of_thread(static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), x, y);
     divergence_body(div, vx, vy, vz, d, h, dbc, x, y, static_cast<int>(blockIdx.z));
 }
-__global__ __launch_bounds__(kTileX* kTileY) void k_divergence_batched(const DivergenceDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
-    const BatchItem it = items[blockIdx.x];
-    if (it.desc == kIdleItem) return;
-    const DivergenceDesc& sd = descs[it.desc];
-    int bx, by, bz, x, y;
-    tile_block_of(sd.d, it.block, bx, by, bz);
-    cell_of_thread(bx, by, x, y);
-    divergence_body(sd.div, sd.v[0], sd.v[1], sd.v[2], sd.d, sd.h, sd.dbc, x, y, bz);
-}
 __global__ __launch_bounds__(kTileX* kTileY) void k_project(float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz, float* __restrict__ div, float* __restrict__ mag, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float* __restrict__ p, const Dims d, const float h, const StencilBC pbc, const StencilBC dbc) {
     int x, y;
     cell_of_thread(static_cast<int>(blockIdx.x), static_cast<int>(blockIdx.y), x, y);
     project_body(ox, oy, oz, div, mag, vx, vy, vz, p, d, h, pbc, dbc, x, y, static_cast<int>(blockIdx.z));
-}
-__global__ __launch_bounds__(kTileX* kTileY) void k_project_batched(const ProjectDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
-    const BatchItem it = items[blockIdx.x];
-    if (it.desc == kIdleItem) return;
-    const ProjectDesc& sd = descs[it.desc];
-    int bx, by, bz, x, y;
-    tile_block_of(sd.d, it.block, bx, by, bz);
-    cell_of_thread(bx, by, x, y);
-    project_body(sd.o[0], sd.o[1], sd.o[2], sd.div, sd.mag, sd.v[0], sd.v[1], sd.v[2], sd.p, sd.d, sd.h, sd.pbc, sd.dbc, x, y, bz);
 }
 
 // ---------------- divergence / projection, z-marching ----------------
@@ -309,11 +284,10 @@ struct ZTile {
     int x0, y0, z0, z1;
 };
 
-// workgroup -> tile and chunk, XCD-contiguous (each XCD receives a run of chunks of a few
+// local block w of a z-marching launch over d -> tile and chunk (chunks of a tile column
+// are consecutive, so an XCD-contiguous block order gives each XCD runs of chunks of a few
 // tile columns, whose register-window planes meet in its L2)
-__device__ __forceinline__ ZTile z_tile(const Dims d) {
-    const int n  = static_cast<int>(gridDim.x), w0 = static_cast<int>(blockIdx.x);
-    const int w  = (n & 7) == 0 ? (w0 & 7) * (n >> 3) + (w0 >> 3) : w0;
+__device__ __forceinline__ ZTile z_tile_of(const Dims d, const int w) {
     const int tx = (d.nx + kZX - 1) / kZX, ch = (d.nz + kZC - 1) / kZC;
     ZTile t;
     const int cz = w % ch, tt = w / ch;
@@ -322,6 +296,11 @@ __device__ __forceinline__ ZTile z_tile(const Dims d) {
     t.z0 = cz * kZC;
     t.z1 = min(t.z0 + kZC, d.nz);
     return t;
+}
+// the workgroup's tile in a single-context launch (blocks renumbered per XCD)
+__device__ __forceinline__ ZTile z_tile(const Dims d) {
+    const int n = static_cast<int>(gridDim.x), w0 = static_cast<int>(blockIdx.x);
+    return z_tile_of(d, (n & 7) == 0 ? (w0 & 7) * (n >> 3) + (w0 >> 3) : w0);
 }
 
 // A staged plane of W x H positions (the tile plus halos HX, HY) spread over the
@@ -391,7 +370,7 @@ __device__ __forceinline__ float project_axis(const float c0, const float v1m, c
     return proj;
 }
 
-__global__ __launch_bounds__(kZThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_project_z(float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz, float* __restrict__ div, float* __restrict__ mag, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float* __restrict__ p, const Dims d, const float h, const StencilBC pbc, const StencilBC dbc) {
+__device__ __forceinline__ void project_z_body(float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz, float* __restrict__ div, float* __restrict__ mag, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float* __restrict__ p, const Dims d, const float h, const StencilBC& pbc, const StencilBC& dbc, const ZTile t) {
     constexpr int PW = kZX + 4, PH = kZY + 4; // pressure: two-cell halo
     constexpr int XW = kZX + 2;               // x component: one-cell x halo
     constexpr int YH = kZY + 2;               // y component: one-cell y halo
@@ -399,7 +378,6 @@ __global__ __launch_bounds__(kZThreads) __attribute__((amdgpu_waves_per_eu(4))) 
     using SX = ZStage<XW, kZY, 1, 0>;
     using SY = ZStage<kZX, YH, 0, 1>;
     __shared__ float s_p[2][PW * PH], s_x[2][XW * kZY], s_y[2][kZX * YH];
-    const ZTile t      = z_tile(d);
     const int lane     = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
     const int x        = t.x0 + lane;
     const float inv_2h = 0.5f / h;
@@ -497,12 +475,11 @@ __device__ __forceinline__ float divergence_axis(const float c0, const float v1m
     return (plus - minus) * inv_2h;
 }
 
-__global__ __launch_bounds__(kZThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_divergence_z(float* __restrict__ div, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const Dims d, const float h, const StencilBC dbc) {
+__device__ __forceinline__ void divergence_z_body(float* __restrict__ div, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const Dims d, const float h, const StencilBC& dbc, const ZTile t) {
     constexpr int XW = kZX + 2, YH = kZY + 2;
     using SX = ZStage<XW, kZY, 1, 0>;
     using SY = ZStage<kZX, YH, 0, 1>;
     __shared__ float s_x[2][XW * kZY], s_y[2][kZX * YH];
-    const ZTile t      = z_tile(d);
     const int lane     = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
     const int x        = t.x0 + lane;
     const float inv_2h = 0.5f / h;
@@ -561,6 +538,26 @@ __global__ __launch_bounds__(kZThreads) __attribute__((amdgpu_waves_per_eu(4))) 
             zw[2][r] = zn[r];
         }
     }
+}
+
+__global__ __launch_bounds__(kZThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_project_z(float* __restrict__ ox, float* __restrict__ oy, float* __restrict__ oz, float* __restrict__ div, float* __restrict__ mag, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const float* __restrict__ p, const Dims d, const float h, const StencilBC pbc, const StencilBC dbc) {
+    project_z_body(ox, oy, oz, div, mag, vx, vy, vz, p, d, h, pbc, dbc, z_tile(d));
+}
+__global__ __launch_bounds__(kZThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_divergence_z(float* __restrict__ div, const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz, const Dims d, const float h, const StencilBC dbc) {
+    divergence_z_body(div, vx, vy, vz, d, h, dbc, z_tile(d));
+}
+// batched: workgroup i runs z-marching block items[i].block of descriptor items[i].desc
+__global__ __launch_bounds__(kZThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_project_z_batched(const ProjectDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
+    const BatchItem it = items[blockIdx.x];
+    if (it.desc == kIdleItem) return;
+    const ProjectDesc& sd = descs[it.desc];
+    project_z_body(sd.o[0], sd.o[1], sd.o[2], sd.div, sd.mag, sd.v[0], sd.v[1], sd.v[2], sd.p, sd.d, sd.h, sd.pbc, sd.dbc, z_tile_of(sd.d, static_cast<int>(it.block)));
+}
+__global__ __launch_bounds__(kZThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_divergence_z_batched(const DivergenceDesc* __restrict__ descs, const BatchItem* __restrict__ items) {
+    const BatchItem it = items[blockIdx.x];
+    if (it.desc == kIdleItem) return;
+    const DivergenceDesc& sd = descs[it.desc];
+    divergence_z_body(sd.div, sd.v[0], sd.v[1], sd.v[2], sd.d, sd.h, sd.dbc, z_tile_of(sd.d, static_cast<int>(it.block)));
 }
 
 uint32_t z_blocks(const Dims d) { return static_cast<uint32_t>(((d.nx + kZX - 1) / kZX) * ((d.ny + kZY - 1) / kZY) * ((d.nz + kZC - 1) / kZC)); }
@@ -669,7 +666,6 @@ hipError_t launch_project(hipStream_t s, float* ox, float* oy, float* oz, float*
     return hipGetLastError();
 }
 
-uint32_t tile_blocks(const Dims d) { return static_cast<uint32_t>(((d.nx + kTileX - 1) / kTileX) * ((d.ny + kTileY - 1) / kTileY) * d.nz); }
 uint32_t elem_block_count(const uint64_t n) { return static_cast<uint32_t>((n + 4 * kElemBlock - 1) / (4 * kElemBlock)); }
 
 hipError_t launch_add_force_batched(hipStream_t s, const AddForceDesc* descs, const BatchItem* items, const uint32_t n_items) {
@@ -682,15 +678,16 @@ hipError_t launch_add_scaled_batched(hipStream_t s, const AddScaledDesc* descs, 
     k_add_scaled_batched<<<n_items, kElemBlock, 0, s>>>(descs, items);
     return hipGetLastError();
 }
-hipError_t launch_divergence_batched(hipStream_t s, const DivergenceDesc* descs, const BatchItem* items, const uint32_t n_items) {
+hipError_t launch_divergence_z_batched(hipStream_t s, const DivergenceDesc* descs, const BatchItem* items, const uint32_t n_items) {
     if (n_items == 0) return hipSuccess;
-    k_divergence_batched<<<n_items, tile_block(), 0, s>>>(descs, items);
+    k_divergence_z_batched<<<n_items, kZThreads, 0, s>>>(descs, items);
     return hipGetLastError();
 }
-hipError_t launch_project_batched(hipStream_t s, const ProjectDesc* descs, const BatchItem* items, const uint32_t n_items) {
+hipError_t launch_project_z_batched(hipStream_t s, const ProjectDesc* descs, const BatchItem* items, const uint32_t n_items) {
     if (n_items == 0) return hipSuccess;
-    k_project_batched<<<n_items, tile_block(), 0, s>>>(descs, items);
+    k_project_z_batched<<<n_items, kZThreads, 0, s>>>(descs, items);
     return hipGetLastError();
 }
+uint32_t z_march_blocks(const Dims d) { return z_blocks(d); }
 
 } // namespace sf

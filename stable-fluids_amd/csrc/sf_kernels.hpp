@@ -177,16 +177,17 @@ struct AddScaledDesc {
     float dt;
     uint64_t n;
 };
-// blocks per simulation of the tile kernels (64 x 4 cells per plane) and of the
-// elementwise kernels (1024 elements)
-uint32_t tile_blocks(Dims d);
+// blocks per simulation of the elementwise kernels (1024 elements)
 uint32_t elem_block_count(uint64_t n);
 hipError_t launch_add_force_batched(hipStream_t s, const AddForceDesc* descs, const BatchItem* items, uint32_t n_items);
 hipError_t launch_add_scaled_batched(hipStream_t s, const AddScaledDesc* descs, const BatchItem* items, uint32_t n_items);
 hipError_t launch_advect_velocity_batched(hipStream_t s, const AdvectVelocityDesc* descs, const BatchItem* items, uint32_t n_items);
 hipError_t launch_advect_scalar_batched(hipStream_t s, const AdvectScalarDesc* descs, const BatchItem* items, uint32_t n_items);
-hipError_t launch_divergence_batched(hipStream_t s, const DivergenceDesc* descs, const BatchItem* items, uint32_t n_items);
-hipError_t launch_project_batched(hipStream_t s, const ProjectDesc* descs, const BatchItem* items, uint32_t n_items);
+// z-marching projection and divergence over a batch (fields below 2 GiB): one item per
+// (descriptor, block) with block < z_march_blocks(d)
+hipError_t launch_project_z_batched(hipStream_t s, const ProjectDesc* descs, const BatchItem* items, uint32_t n_items);
+hipError_t launch_divergence_z_batched(hipStream_t s, const DivergenceDesc* descs, const BatchItem* items, uint32_t n_items);
+uint32_t z_march_blocks(Dims d);
 
 hipError_t launch_fill(hipStream_t s, float* p, float value, uint64_t n);
 hipError_t launch_copy_scaled(hipStream_t s, float* dst, const float* src, float factor, uint64_t n);
