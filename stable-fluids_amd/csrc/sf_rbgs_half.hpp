@@ -67,6 +67,10 @@ constexpr int kHalfStoreAux = kDeviceScopeStore;
 // per CU without spilling (about 9 us less per first pass at 256^3).
 template <int kIn>
 constexpr int kHalfMinWaves = kIn == kHalfInNatural ? 4 : 2;
+// progress-based wave priority in the plane loop (0 turns it off)
+#ifndef SF_HALF_PRIO
+#define SF_HALF_PRIO 1
+#endif
 
 // Tile shape: kTXP x-pairs (2 kTXP cells) by TY rows for kWaves waves of kRowsW row
 // slots; a slot of a wave covers kR = 64 / kTXP rows of kTXP lanes each.
@@ -669,16 +673,37 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
     // general steps for the rest (an odd count ends with one idle step, loads and stores off)
     const int n      = z1 - z0 + 2;
     const int groups = n >= 10 ? (n - 6) / 4 : 0;
+#if SF_HALF_PRIO
+    // Progress-based wave priority: a CU runs two workgroups, and oldest-first issue lets
+    // the one dispatched first run ahead and finish while the other still has a quarter of
+    // its planes left, which it then runs alone. Priority falls as a wave advances, so the
+    // workgroup that is ahead yields issue slots to the one behind (27.4 -> 27.15 us per
+    // middle pass at 256^3).
+    auto set_prio = [](const int q) __attribute__((always_inline)) {
+        if (q <= 0) __builtin_amdgcn_s_setprio(3);
+        else if (q == 1) __builtin_amdgcn_s_setprio(2);
+        else if (q == 2) __builtin_amdgcn_s_setprio(1);
+        else __builtin_amdgcn_s_setprio(0);
+    };
+    set_prio(0);
+#endif
     step.template operator()<false, 0>(z0 - 1, 0, pa[0], pb[0]);
     step.template operator()<false, 0>(z0, 1, pa[1], pb[1]);
     if (n > 2) step.template operator()<false, 0>(z0 + 1, 2, pa[0], pb[0]);
     for (int g = 0; g < groups; ++g) {
+#if SF_HALF_PRIO
+        const int gq = 4 * g + 3; // priority level floor(gq / groups), without a division
+        set_prio((gq >= groups) + (gq >= 2 * groups) + (gq >= 3 * groups));
+#endif
         const int s = z0 + 2 + 4 * g; // t = 3 + 4 g
         step.template operator()<true, 3>(s, 0, pa[1], pb[1]);
         step.template operator()<true, 0>(s + 1, 0, pa[0], pb[0]);
         step.template operator()<true, 1>(s + 2, 0, pa[1], pb[1]);
         step.template operator()<true, 2>(s + 3, 0, pa[0], pb[0]);
     }
+#if SF_HALF_PRIO
+    set_prio(3);
+#endif
     for (int t = 3 + 4 * groups; t < n; t += 2) {
         step.template operator()<false, 0>(z0 - 1 + t, t, pa[1], pb[1]);
         step.template operator()<false, 0>(z0 + t, t + 1, pa[0], pb[0]);
