@@ -67,7 +67,7 @@ constexpr int kHalfStoreAux = kDeviceScopeStore;
 // per CU without spilling (about 9 us less per first pass at 256^3).
 template <int kIn>
 constexpr int kHalfMinWaves = kIn == kHalfInNatural ? 4 : 2;
-// progress-based wave priority in the plane loop (0 turns it off)
+// progress-based wave priority in the plane loop of single-grid passes (0 turns it off)
 #ifndef SF_HALF_PRIO
 #define SF_HALF_PRIO 1
 #endif
@@ -164,8 +164,8 @@ struct PfSet {
 // read of each halo plane hits the XCD's L2. A red-black half-sweep has no order between
 // planes, so either direction gives the same bits; sums keep the order z-, z+ of the
 // grid, whatever the direction. Ring positions follow the march coordinate u; memory,
-// ghosts and parities the grid plane pl(u).
-template <int kSolve, int kIn, int kOut, int kTXP, int kWaves, int kRowsW, int kDepth, bool kDown>
+// ghosts and parities the grid plane pl(u). kPrio: progress-based wave priority (below).
+template <int kSolve, int kIn, int kOut, int kTXP, int kWaves, int kRowsW, int kDepth, bool kDown, bool kPrio>
 __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const int w_block, HalfLds<HalfTile<kTXP, kWaves, kRowsW>>& lds) {
     using TT           = HalfTile<kTXP, kWaves, kRowsW>;
     constexpr int kR   = TT::kR;
@@ -673,37 +673,34 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
     // general steps for the rest (an odd count ends with one idle step, loads and stores off)
     const int n      = z1 - z0 + 2;
     const int groups = n >= 10 ? (n - 6) / 4 : 0;
-#if SF_HALF_PRIO
     // Progress-based wave priority: a CU runs two workgroups, and oldest-first issue lets
     // the one dispatched first run ahead and finish while the other still has a quarter of
     // its planes left, which it then runs alone. Priority falls as a wave advances, so the
     // workgroup that is ahead yields issue slots to the one behind (27.4 -> 27.15 us per
-    // middle pass at 256^3).
+    // middle pass at 256^3). Batched passes run without it: there it cost 3% of the
+    // 64-plume batch's throughput.
     auto set_prio = [](const int q) __attribute__((always_inline)) {
-        if (q <= 0) __builtin_amdgcn_s_setprio(3);
-        else if (q == 1) __builtin_amdgcn_s_setprio(2);
-        else if (q == 2) __builtin_amdgcn_s_setprio(1);
-        else __builtin_amdgcn_s_setprio(0);
+        if constexpr (kPrio) {
+            if (q <= 0) __builtin_amdgcn_s_setprio(3);
+            else if (q == 1) __builtin_amdgcn_s_setprio(2);
+            else if (q == 2) __builtin_amdgcn_s_setprio(1);
+            else __builtin_amdgcn_s_setprio(0);
+        }
     };
     set_prio(0);
-#endif
     step.template operator()<false, 0>(z0 - 1, 0, pa[0], pb[0]);
     step.template operator()<false, 0>(z0, 1, pa[1], pb[1]);
     if (n > 2) step.template operator()<false, 0>(z0 + 1, 2, pa[0], pb[0]);
     for (int g = 0; g < groups; ++g) {
-#if SF_HALF_PRIO
         const int gq = 4 * g + 3; // priority level floor(gq / groups), without a division
         set_prio((gq >= groups) + (gq >= 2 * groups) + (gq >= 3 * groups));
-#endif
         const int s = z0 + 2 + 4 * g; // t = 3 + 4 g
         step.template operator()<true, 3>(s, 0, pa[1], pb[1]);
         step.template operator()<true, 0>(s + 1, 0, pa[0], pb[0]);
         step.template operator()<true, 1>(s + 2, 0, pa[1], pb[1]);
         step.template operator()<true, 2>(s + 3, 0, pa[0], pb[0]);
     }
-#if SF_HALF_PRIO
     set_prio(3);
-#endif
     for (int t = 3 + 4 * groups; t < n; t += 2) {
         step.template operator()<false, 0>(z0 - 1 + t, t, pa[1], pb[1]);
         step.template operator()<false, 0>(z0 + t, t + 1, pa[0], pb[0]);
@@ -715,8 +712,9 @@ template <int kSolve, int kIn, int kOut, int kTXP, int kWaves, int kRowsW>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(kHalfMinWaves<kIn>, 4))) void k_rbgs_half(const HalfPassArgs args) {
     __shared__ HalfLds<HalfTile<kTXP, kWaves, kRowsW>> lds;
     const int w = xcd_block();
-    if ((w % args.g.chunks) & 1) rbgs_half_body<kSolve, kIn, kOut, kTXP, kWaves, kRowsW, kHalfDepth, true>(args, w, lds);
-    else rbgs_half_body<kSolve, kIn, kOut, kTXP, kWaves, kRowsW, kHalfDepth, false>(args, w, lds);
+    constexpr bool kPrio = SF_HALF_PRIO && kWaves == 8; // measured on the 512-thread tiles
+    if ((w % args.g.chunks) & 1) rbgs_half_body<kSolve, kIn, kOut, kTXP, kWaves, kRowsW, kHalfDepth, true, kPrio>(args, w, lds);
+    else rbgs_half_body<kSolve, kIn, kOut, kTXP, kWaves, kRowsW, kHalfDepth, false, kPrio>(args, w, lds);
 }
 
 // batched: workgroup i runs block items[i].block of the pass descs[items[i].desc] (the
@@ -728,8 +726,8 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(kHa
     __shared__ HalfLds<HalfTile<kTXP, kWaves, kRowsW>> lds;
     const HalfPassArgs& a = descs[it.desc];
     const int w           = static_cast<int>(it.block);
-    if ((w % a.g.chunks) & 1) rbgs_half_body<kSolve, kIn, kOut, kTXP, kWaves, kRowsW, kHalfDepth, true>(a, w, lds);
-    else rbgs_half_body<kSolve, kIn, kOut, kTXP, kWaves, kRowsW, kHalfDepth, false>(a, w, lds);
+    if ((w % a.g.chunks) & 1) rbgs_half_body<kSolve, kIn, kOut, kTXP, kWaves, kRowsW, kHalfDepth, true, false>(a, w, lds);
+    else rbgs_half_body<kSolve, kIn, kOut, kTXP, kWaves, kRowsW, kHalfDepth, false, false>(a, w, lds);
 }
 
 // Calls f.template operator()<solve, in, out, pairs, waves, row slots>() for the runtime choice.
