@@ -688,12 +688,14 @@ __device__ __forceinline__ void rbgs_half_body(const HalfPassArgs& args, const i
         }
     };
     set_prio(0);
+    const int pg1 = groups >> 2, pg2 = (2 * groups) >> 2, pg3 = (3 * groups) >> 2;
     step.template operator()<false, 0>(z0 - 1, 0, pa[0], pb[0]);
     step.template operator()<false, 0>(z0, 1, pa[1], pb[1]);
     if (n > 2) step.template operator()<false, 0>(z0 + 1, 2, pa[0], pb[0]);
     for (int g = 0; g < groups; ++g) {
-        const int gq = 4 * g + 3; // priority level floor(gq / groups), without a division
-        set_prio((gq >= groups) + (gq >= 2 * groups) + (gq >= 3 * groups));
+        // level floor((4 g + 3) / groups) rises at g = floor(k groups / 4), k = 1, 2, 3
+        if constexpr (kPrio)
+            if (g == pg1 || g == pg2 || g == pg3) set_prio((g >= pg1) + (g >= pg2) + (g >= pg3));
         const int s = z0 + 2 + 4 * g; // t = 3 + 4 g
         step.template operator()<true, 3>(s, 0, pa[1], pb[1]);
         step.template operator()<true, 0>(s + 1, 0, pa[0], pb[0]);
