@@ -566,6 +566,7 @@ StableFluidsResult create(const StableFluidsContextCreateDesc* desc, const Stabl
     c->cells   = static_cast<uint64_t>(cfg.nx) * static_cast<uint64_t>(cfg.ny) * static_cast<uint64_t>(cfg.nz);
     c->bytes   = c->cells * sizeof(float);
     c->stream  = static_cast<hipStream_t>(desc->stream);
+    c->half_tile = default_half_tile(c->dims);
     if (c->stream == nullptr) {
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return STABLE_FLUIDS_RESULT_BACKEND_FAILURE;
         c->owns_stream = true;
@@ -1236,7 +1237,7 @@ std::vector<Tiling> tiling_candidates(const Dims d, const bool half, const int t
     std::vector<Tiling> out;
     if (half) {
         for (int t = 0; t < kHalfTileCount; ++t)
-            for (const int zc : {0, 4, 8, 16, 32})
+            for (const int zc : {0, 4, 6, 8, 12, 16, 32})
                 if (zc <= d.nz) out.push_back(Tiling{t, zc});
     } else {
         for (const int zc : {0, 2, 4, 8, 16, 32, 64})

@@ -116,6 +116,11 @@ int half_tile_rows(int tile);
 int half_tile_threads(int tile);
 int half_tile_cells(int tile);
 TileGeom half_plan(Dims d, int tile, int zc_override);
+// Tile shape a context starts with: 128x16 cells, or 64x16 where that leaves fewer x
+// positions of the last tile column idle.
+int default_half_tile(Dims d);
+// Compute units of the current device (read once per device; 256 if the query fails).
+int device_cus();
 hipError_t launch_rbgs_half(hipStream_t s, SolveKind solve, int in_mode, int out_mode, const HalfPassArgs& args, int tile);
 // one launch for many passes: workgroup i runs block items[i].block of descs[items[i].desc]
 // (every descriptor's L filled in, its g planned for `tile`); kIdleItem entries return

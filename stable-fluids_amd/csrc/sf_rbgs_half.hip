@@ -54,11 +54,32 @@ TileGeom half_plan(const Dims d, const int tile, const int zc_override) {
     const long target = 4096 / sh.waves;
     long zc = (static_cast<long>(d.nz) * tiles + target - 1) / target;
     zc      = std::max<long>(4, std::min<long>(64, zc + (zc & 1)));
-    if (zc_override > 0) zc = zc_override;
+    if (zc_override > 0) {
+        zc = zc_override;
+    } else if (sh.waves == 8) {
+        // More 512-thread workgroups than CUs but at most 1.5 per CU leaves a minority of
+        // CUs running two of them, one after the other in effect, and the pass lasts that
+        // long; longer chunks that put at most one on each CU finish sooner (the reference
+        // benchmark's 128x160x128 grid: 320 workgroups of 4 planes 2.68 ms per step, 220 of
+        // 6 planes 2.43 ms).
+        const long cus = device_cus();
+        auto blocks    = [&](const long z) { return tiles * ((d.nz + z - 1) / z); };
+        if (blocks(zc) > cus && 2 * blocks(zc) <= 3 * cus)
+            while (blocks(zc) > cus && zc < std::min<long>(64, d.nz)) zc += 2;
+    }
     if (zc > d.nz) zc = d.nz;
     g.zc     = static_cast<int>(zc);
     g.chunks = (d.nz + g.zc - 1) / g.zc;
     return g;
+}
+
+int default_half_tile(const Dims d) {
+    // idle x positions (pairs) of the last tile column under 64- and 32-pair tiles: 64
+    // and 192 cells wide grids leave half or a quarter of a 128-cell tile idle, which the
+    // 64x16 tiles avoid (64^3 plume 2.85 -> 2.14 ms per step, 192^3 6.21 -> 4.83 ms)
+    const int nxh  = d.nx / 2;
+    const int idle0 = (nxh + 63) / 64 * 64 - nxh, idle2 = (nxh + 31) / 32 * 32 - nxh;
+    return idle2 < idle0 ? kHalfTile64x16 : kHalfTile128x16;
 }
 
 hipError_t launch_rbgs_half(hipStream_t s, const SolveKind solve, const int in_mode, const int out_mode, const HalfPassArgs& a_in, const int tile) {

@@ -1473,7 +1473,7 @@ hipError_t launch_smooth_batched(hipStream_t s, const SmootherKind smoother, con
 static bool field_below_2g(const Dims d, const int elem) { return static_cast<uint64_t>(elem) * static_cast<uint64_t>(d.nx) * static_cast<uint64_t>(d.ny) * static_cast<uint64_t>(d.nz) < (1ull << 31); }
 
 // Compute units of the current device, read once per device.
-static int device_cus() {
+int device_cus() {
     static std::atomic<int> cached[64];
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
