@@ -176,7 +176,7 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_set_smoother_tiling_cuda(
 STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_get_smoother_tiling_cuda(void* context, uint32_t* out_tile, uint32_t* out_z_chunk);
 /* Tiling autotuner for one context, in place of the reference's occupancy-derived block
  * size (backend-cuda.cu:490-515): times `repeats` RBGS pressure passes (writing only
- * scratch buffers) for every tile shape with z chunks {default, 4, 8, 16, 32} and keeps
+ * scratch buffers) for every tile shape with z chunks {default, 4, 6, 8, 12, 16, 32} and keeps
  * the fastest. A shape found in cache_path is taken from it, and the file is rewritten
  * with every entry (cache_path may be NULL; the format is shared with the batch
  * autotuner, single contexts under members 0). Rebuilds the step graph (synchronous);
@@ -217,8 +217,8 @@ STABLE_FLUIDS_API StableFluidsResult stable_fluids_ext_batch_get_tile_cuda(void*
  * (backend-cuda.cu:490-515). For every grid shape in the batch it times `repeats`
  * rounds of batched smoother passes in the step's mix (one three-component velocity
  * pass, three single-field passes; writing only scratch buffers) over that shape's
- * contexts alone for each candidate tiling -- every tile shape with z chunks {default, 4, 8,
- * 16, 32} for black-cell passes, z chunks {default, 2, 4, ..., 64} otherwise -- taking
+ * contexts alone for each candidate tiling -- every tile shape with z chunks {default, 4, 6,
+ * 8, 12, 16, 32} for black-cell passes, z chunks {default, 2, 4, ..., 64} otherwise -- taking
  * the minimum of three interleaved rounds. It then times whole-batch passes under: each
  * shape on its fastest tiling; all shapes on one tile shape (one launch per pass), each
  * on its fastest chunk for it, for every tile shape; the starting tiling; keeps the
