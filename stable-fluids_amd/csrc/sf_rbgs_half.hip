@@ -50,23 +50,33 @@ TileGeom half_plan(const Dims d, const int tile, const int zc_override) {
     g.tiles_y        = (d.ny + sh.rows - 1) / sh.rows;
     const long tiles = static_cast<long>(g.tiles_x) * g.tiles_y;
     // about 4096 waves (16 per CU: two 512-thread workgroups), even chunk lengths of at
-    // least 4 planes
+    // least 4 planes (the CU count of MI355X; the adjustments below read the device's)
     const long target = 4096 / sh.waves;
     long zc = (static_cast<long>(d.nz) * tiles + target - 1) / target;
-    zc      = std::max<long>(4, std::min<long>(64, zc + (zc & 1)));
+    zc      = std::max<long>(4, zc + (zc & 1));
+    const long cus = device_cus();
+    // Chunks past 64 planes at two 512-thread workgroups per CU: one workgroup per CU with
+    // chunks twice as long is faster still (512^3 middle pass: 64-plane chunks, four per
+    // CU, 242 us; 128 planes, two per CU, 224 us; 256 planes, one per CU, 214 us). At
+    // 256^3 (16 planes) the two are within 1%. Otherwise chunks stop at 64 planes.
+    const bool one_per_cu = zc_override <= 0 && zc > 64 && sh.waves == 8 && tiles <= cus;
     if (zc_override > 0) {
         zc = zc_override;
-    } else if (sh.waves == 8) {
+    } else if (one_per_cu) {
+        const long chunks = cus / tiles;
+        zc                = (d.nz + chunks - 1) / chunks;
+        zc                = zc + (zc & 1);
+    } else if (sh.waves == 8 && zc <= 64) {
         // More 512-thread workgroups than CUs but at most 1.5 per CU leaves a minority of
         // CUs running two of them, one after the other in effect, and the pass lasts that
         // long; longer chunks that put at most one on each CU finish sooner (the reference
         // benchmark's 128x160x128 grid: 320 workgroups of 4 planes 2.68 ms per step, 220 of
         // 6 planes 2.43 ms).
-        const long cus = device_cus();
-        auto blocks    = [&](const long z) { return tiles * ((d.nz + z - 1) / z); };
+        auto blocks = [&](const long z) { return tiles * ((d.nz + z - 1) / z); };
         if (blocks(zc) > cus && 2 * blocks(zc) <= 3 * cus)
             while (blocks(zc) > cus && zc < std::min<long>(64, d.nz)) zc += 2;
     }
+    if (zc_override <= 0 && !one_per_cu) zc = std::min<long>(zc, 64);
     if (zc > d.nz) zc = d.nz;
     g.zc     = static_cast<int>(zc);
     g.chunks = (d.nz + g.zc - 1) / g.zc;
