@@ -278,7 +278,14 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_project(float* __restrict__ 
 // operand order, so the results are bit-identical. Fields below 2 GiB.
 constexpr int kZX = 64, kZWaves = 8, kZRows = 2, kZY = kZWaves * kZRows; // lane: column x; wave w: rows w, w + 8
 constexpr int kZThreads = 64 * kZWaves;
-constexpr int kZC       = 32; // planes per chunk
+// Planes per chunk: 32, fewer where the grid's tiles would leave CUs idle -- about 512
+// workgroups (two per CU of MI355X's 256), at least 4 planes. Host and device both take
+// it from here, and results do not depend on it.
+__host__ __device__ __forceinline__ int z_chunk(const Dims d) {
+    const int tiles = ((d.nx + kZX - 1) / kZX) * ((d.ny + kZY - 1) / kZY);
+    const int zc    = (d.nz * tiles + 511) / 512;
+    return zc < 4 ? 4 : (zc > 32 ? 32 : zc);
+}
 
 struct ZTile {
     int x0, y0, z0, z1;
@@ -288,13 +295,14 @@ struct ZTile {
 // are consecutive, so an XCD-contiguous block order gives each XCD runs of chunks of a few
 // tile columns, whose register-window planes meet in its L2)
 __device__ __forceinline__ ZTile z_tile_of(const Dims d, const int w) {
-    const int tx = (d.nx + kZX - 1) / kZX, ch = (d.nz + kZC - 1) / kZC;
+    const int zc = z_chunk(d);
+    const int tx = (d.nx + kZX - 1) / kZX, ch = (d.nz + zc - 1) / zc;
     ZTile t;
     const int cz = w % ch, tt = w / ch;
     t.x0 = (tt % tx) * kZX;
     t.y0 = (tt / tx) * kZY;
-    t.z0 = cz * kZC;
-    t.z1 = min(t.z0 + kZC, d.nz);
+    t.z0 = cz * zc;
+    t.z1 = min(t.z0 + zc, d.nz);
     return t;
 }
 // the workgroup's tile in a single-context launch (blocks renumbered per XCD)
@@ -560,7 +568,7 @@ __global__ __launch_bounds__(kZThreads) __attribute__((amdgpu_waves_per_eu(4))) 
     divergence_z_body(sd.div, sd.v[0], sd.v[1], sd.v[2], sd.d, sd.h, sd.dbc, z_tile_of(sd.d, static_cast<int>(it.block)));
 }
 
-uint32_t z_blocks(const Dims d) { return static_cast<uint32_t>(((d.nx + kZX - 1) / kZX) * ((d.ny + kZY - 1) / kZY) * ((d.nz + kZC - 1) / kZC)); }
+uint32_t z_blocks(const Dims d) { return static_cast<uint32_t>(((d.nx + kZX - 1) / kZX) * ((d.ny + kZY - 1) / kZY) * ((d.nz + z_chunk(d) - 1) / z_chunk(d))); }
 
 // Batched elementwise stages: local block b covers elements [4 * 256 * b, + 1024).
 __global__ __launch_bounds__(kElemBlock) void k_add_force_batched(const AddForceDesc* __restrict__ descs, const BatchItem* __restrict__ items) {

@@ -168,8 +168,9 @@ TILE_CASES = {
     # black-cell passes: a 2-pair x tile and a 2-row y tile past full ones, 10 z chunks
     "walls_multi_tile_chunks": (walls_desc, (132, 34, 40)),
     "periodic_yz_partial": (periodic_yz_desc, (136, 18, 20)),  # wrapped halo rows and planes
-    # periodic y and z across several 32-plane chunks of the z-marching projection and
-    # divergence (their z windows wrap at the first and last chunk), partial x and y tiles
+    # periodic y and z across many z chunks of the z-marching projection and divergence
+    # (4-plane chunks at this size, the last one 2 planes; their z windows wrap at the
+    # first and last chunk), partial x and y tiles
     "periodic_yz_z_chunks": (periodic_yz_desc, (70, 20, 70)),
     "mixed_two_cells": (mixed_desc, (4, 2, 2)),                # every cell on a face
     "walls_two_cells": (walls_desc, (4, 2, 2)),
