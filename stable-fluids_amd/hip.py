@@ -40,6 +40,16 @@ def device_count():
     return n.value
 
 
+_ATTR_MULTIPROCESSOR_COUNT = 63  # hipDeviceAttributeMultiprocessorCount (hip_runtime_api.h)
+
+
+def compute_units(device=0):
+    """Compute units of a device (hipDeviceGetAttribute)."""
+    n = C.c_int(0)
+    check(lib().hipDeviceGetAttribute(C.byref(n), _ATTR_MULTIPROCESSOR_COUNT, device), "hipDeviceGetAttribute")
+    return n.value
+
+
 class DeviceBuffer:
     """A hipMalloc'd device array of float32 (or raw bytes)."""
 
