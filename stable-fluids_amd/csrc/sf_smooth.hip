@@ -1486,12 +1486,15 @@ int device_cus() {
 }
 
 hipError_t launch_smooth(hipStream_t s, const SmootherKind smoother, const SolveKind solve, const SmoothPass& pass, const Dims d, const StencilBC& bc) {
-    // Jacobi passes run about one 512-thread workgroup per CU (long z-chunks, not capped at
-    // plan_tiles' 64 planes): 33.2 us at 256^3 against 35.1 us with two per CU and 37.4 us
-    // with four; 290 us at 512^3 (256-plane chunks) against 327 us with 64-plane ones
+    // Jacobi passes run at most one 512-thread workgroup per CU (long z-chunks, not capped
+    // at plan_tiles' 64 planes): 33.2 us at 256^3 against 35.1 us with two per CU and 37.4
+    // us with four; 290 us at 512^3 (256-plane chunks) against 327 us with 64-plane ones.
+    // The chunk count comes first, so the workgroups never outnumber the CUs by a few (the
+    // 128x160x128 grid: 22 chunks of 6 planes, 220 workgroups, not 26 of 5, 260).
     const long tiles_j    = static_cast<long>((d.nx + kTX - 1) / kTX) * ((d.ny + kTY - 1) / kTY);
     const long cus        = device_cus();
-    const int zc_j        = static_cast<int>(std::max<long>(4, (static_cast<long>(d.nz) * tiles_j + cus - 1) / cus));
+    const long chunks_j   = std::max<long>(1, cus / tiles_j);
+    const int zc_j        = static_cast<int>(std::max<long>(4, (d.nz + chunks_j - 1) / chunks_j));
     const TileGeom g      = plan_tiles(d, smoother == kSmoothJacobi ? zc_j : 0);
     const unsigned blocks = static_cast<unsigned>(g.tiles_x * g.tiles_y * g.chunks);
     const bool zero       = pass.x_old == nullptr;
