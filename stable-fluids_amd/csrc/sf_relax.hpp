@@ -45,11 +45,22 @@ __device__ __forceinline__ float div6(const float x) {
 struct DiffusionDiv {
     float den, inv;
     bool fast;
+    double rd;    // RN64(1 / den), for StageDiv
+    uint32_t tlo; // StageDiv's key of den * 2^-126 (numerators below give subnormal quotients)
 };
+
+// StageDiv's key of |x|: (bits << 1) - 2, the sign shifted out; zeros map to the largest keys
+__device__ __forceinline__ uint32_t stage_key(const float x) { return (__float_as_uint(x) << 1) - 2u; }
 
 __device__ __forceinline__ DiffusionDiv diffusion_div(const float coef) {
     const float den = 1.0f + 6.0f * coef;
-    return DiffusionDiv{den, 1.0f / den, den >= 1.0f && den <= 0x1p20f};
+    const bool fast = den >= 1.0f && den <= 0x1p20f;
+    // the reciprocal is wave-uniform: scalar registers hold it, not a vector register pair
+    const uint64_t r  = __builtin_bit_cast(uint64_t, 1.0 / static_cast<double>(den));
+    const uint32_t rl = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(r)));
+    const uint32_t rh = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(r >> 32)));
+    const double rd   = __builtin_bit_cast(double, (static_cast<uint64_t>(rh) << 32) | rl);
+    return DiffusionDiv{den, 1.0f / den, fast, rd, static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(stage_key(den * 0x1p-126f)))};
 }
 
 __device__ __forceinline__ float div_by(const DiffusionDiv& dv, const float x) {
@@ -89,18 +100,21 @@ __device__ __forceinline__ LinGhost lin_ghost(const FaceRule f) {
 
 __device__ __forceinline__ float lin_apply(const LinGhost g, const float v) { return g.k ? g.c : __builtin_fmaf(g.a, v, g.c); }
 
-// Relaxation quotients of a stage with the range guard of div6 / div_by deferred: each
-// quotient is taken by the corrected reciprocal (its sign copied from the numerator, which
-// makes zeros exact), the numerator's |x| bit pattern is folded into a running max and
-// min, and one test per stage sends the rare stage with an out-of-range numerator back
-// through the exact quotient. Results are those of relax_fast bit for bit.
+// Relaxation quotients of a stage, taken as q = RN32(RN64(x * RN64(1/d))). For floats x and
+// d, x/d is never a rounding boundary of the float grid (a midpoint between consecutive
+// floats) in the normal range: d times a midpoint's 25-bit odd significand has more
+// significant bits than a float. Its relative distance to such a boundary is at least
+// 2^-49, while the float64 product is within 2^-52 of x/d, so q is the IEEE quotient for
+// every result that is normal, zero or infinite. Subnormal results can sit on a midpoint of
+// the subnormal grid: a nonzero numerator below d * 2^-126 is folded into a running
+// minimum of keys, and one test per stage sends such a stage back through the exact
+// quotient. Results are those of relax_fast bit for bit.
 template <int kSolve>
 struct StageDiv {
-    // |x| in [2^-110, FLT_MAX] (x / 6) or [2^-100, 2^100] (x / den), or x = 0
-    static constexpr uint32_t kLo = kSolve == kSolvePoisson ? 0x08800000u : 0x0d800000u;
-    static constexpr uint32_t kHi = kSolve == kSolvePoisson ? 0x7f7fffffu : 0x71800000u;
+    // key of 6 * 2^-126 (x / 6 below 2^-126)
+    static constexpr uint32_t kTlo6 = (__builtin_bit_cast(uint32_t, 6.0f * 0x1p-126f) << 1) - 2u;
     DiffusionDiv dv;
-    uint32_t hi = 0u, lo = 0xffffffffu;
+    uint32_t lo = 0xffffffffu;
     __device__ __forceinline__ explicit StageDiv(const float coef) : dv(diffusion_div(coef)) {}
     __device__ __forceinline__ explicit StageDiv(const DiffusionDiv d) : dv(d) {}
     __device__ __forceinline__ float num(const float bv, const float sum, const float coef) const {
@@ -108,24 +122,14 @@ struct StageDiv {
         else return bv + coef * sum;
     }
     __device__ __forceinline__ float quot(const float x) {
-        const uint32_t bits = __float_as_uint(x) & 0x7fffffffu;
-        hi                  = max(hi, bits);
-        lo                  = min(lo, bits - 1u);
-        float q;
-        if constexpr (kSolve == kSolvePoisson) {
-            constexpr float r = 0x1.555556p-3f;
-            const float q1    = x * r;
-            q                 = __builtin_fmaf(__builtin_fmaf(-q1, 6.0f, x), r, q1);
-        } else {
-            const float q1 = x * dv.inv;
-            q              = __builtin_fmaf(__builtin_fmaf(-q1, dv.den, x), dv.inv, q1);
-        }
-        return __builtin_copysignf(q, x);
+        lo = min(lo, stage_key(x));
+        if constexpr (kSolve == kSolvePoisson) return static_cast<float>(static_cast<double>(x) * (1.0 / 6.0));
+        else return static_cast<float>(static_cast<double>(x) * dv.rd);
     }
     // true when some quotient of this lane needs the exact path
     __device__ __forceinline__ bool bad() const {
-        const bool fast = kSolve == kSolvePoisson || dv.fast;
-        return !fast || hi > kHi || lo < kLo - 1u;
+        if constexpr (kSolve == kSolvePoisson) return lo < kTlo6;
+        else return !dv.fast || lo < dv.tlo;
     }
     __device__ __forceinline__ float exact(const float x) const {
         if constexpr (kSolve == kSolvePoisson) return x / 6.0f;

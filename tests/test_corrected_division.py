@@ -2,7 +2,8 @@
 
 The HIP smoothers divide by 6 (Poisson) and by d = 1 + 6 alpha (diffusion) as
 q1 = x * RN(1/d), q = fma(fma(-q1, d, x), RN(1/d), q1), sign copied from x
-(stable-fluids_amd/csrc/sf_smooth.hip: div6, div_by, StageDiv), and fall back to the
+(stable-fluids_amd/csrc/sf_relax.hpp: div6, div_by; the stage quotients of the black-cell
+and full-iterate passes use the float64 form of test_float64_quotient.py), and fall back to the
 division instruction outside 2^-100 <= |x| <= 2^100 (2^-110 and FLT_MAX for d = 6) or
 for d outside [1, 2^20]. Markstein's theorem makes the corrected quotient the correctly
 rounded one when RN(1/d) is correctly rounded and nothing under- or overflows; this test
