@@ -94,6 +94,9 @@ def main():
     ap.add_argument("--learned-step", type=int, default=None, metavar="K",
                     help="time only the learned-warm-start step (network guess + K RBGS passes, P = 80) on the 256^3 "
                          "plume from its step-10 state, --steps steps after --warmup (for kernel traces)")
+    ap.add_argument("--learned-guess-stage", type=int, default=None, metavar="R",
+                    help="enqueue the learned projection stage (K = 0: the network guess, its residual test and the "
+                         "projection) R times through the stage hook at 256^3, with no graph launch (for kernel traces)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the stable-fluids step here runs one simulation per GPU; "
@@ -109,6 +112,9 @@ def main():
         return
     if args.learned_step is not None:
         learned_step_study(pkg, args)
+        return
+    if args.learned_guess_stage is not None:
+        learned_guess_stage(pkg, args)
         return
     abi, hip, scenes = pkg.abi, pkg.hip, pkg.scenes
     n = args.grid
@@ -351,6 +357,27 @@ def learned_step_study(pkg, args):
     print(json.dumps({"workload": "learned warm-start step, plume 256^3 from its step-10 state, D=20 P=80, "
                                   "network on the 32^3 restriction", "k": args.learned_step, "steps": args.steps,
                       "steps_per_s": round(sps, 3)}), flush=True)
+
+
+def learned_guess_stage(pkg, args):
+    """GPU ms of the learned projection stage with K = 0 at 256^3 (divergence, restriction,
+    network, prolongation, residual test, the exiting fallback launch, projection), enqueued
+    eagerly through the stage hook: no step and no graph launch precede it. The context
+    holds the seeded plume at rest, so the divergence is zero and the all-zero network's
+    guess is accepted; every kernel still runs its whole grid."""
+    import numpy as np
+    abi, scenes = pkg.abi, pkg.scenes
+    scene = scenes.plume_scene(256, diffuse_iterations=20, pressure_iterations=80)
+    opts = pkg.ext_options(smoother=abi.SMOOTHER_RBGS, pressure_warm_start=abi.WARM_START_LEARNED,
+                           warm_start_iterations=0, warm_start_coarsening=3)
+    with pkg.Context(scene.desc, options=opts) as ctx:
+        ctx.set_pressure_model(np.zeros(pkg.unet.param_count(), np.float32))
+        scenes.seed_context(ctx, scene)
+        ms = ctx.time_stage(abi.STAGE_PROJECTION, args.learned_guess_stage)
+        stats = ctx.warm_start_stats()
+    print(json.dumps({"workload": "learned projection stage, K = 0, plume 256^3 at rest, network on the 32^3 "
+                                  "restriction (all-zero parameters)", "repeat": args.learned_guess_stage,
+                      "stage_ms": round(ms, 4), "fallback": stats["fallback"]}), flush=True)
 
 
 def precision_study(pkg, args):

@@ -26,6 +26,7 @@ constexpr int kHalo    = kHX * kHY * kHZ;                  // 432
 constexpr int kTile    = kTX * kTY * kTZ;                  // 128
 constexpr int kThreads = 256;
 constexpr int kElem    = 256;
+constexpr int kRingFew = 8; // operand sets in flight on grids of few tiles
 
 __device__ __forceinline__ float bf2f(const bf16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
 __device__ __forceinline__ bf16_t f2bf(const float f) {
@@ -100,7 +101,8 @@ __device__ __forceinline__ void stage_tile(const bf16_t* __restrict__ g, bf16_t*
 // NB output-channel blocks of 16 per workgroup; with NB < CO / 16 the CO / 16 / NB
 // workgroups of one tile split the output channels (for grids of few tiles, where one
 // workgroup per tile leaves the GPU nearly idle); every output keeps the same K order.
-template <int CI, int CO, bool kF32, int NB>
+// R: operand sets in flight (the prefetch distance is R - 1 steps).
+template <int CI, int CO, bool kF32, int NB, int R>
 __global__ __launch_bounds__(kThreads) void k_conv3(const bf16_t* __restrict__ x, const bf16_t* __restrict__ w, const float* __restrict__ bias, void* __restrict__ y, const bf16_t* __restrict__ aux, const int epi, const Dims d) {
     static_assert(NB >= 1 && (CO / 16) % NB == 0, "channel split");
     constexpr int kSplit = CO / 16 / NB;
@@ -121,7 +123,7 @@ __global__ __launch_bounds__(kThreads) void k_conv3(const bf16_t* __restrict__ x
         for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
     // Operands of step ks: A from the staged tile in LDS, B (the weights) from L2. They are
-    // fetched two steps ahead into a ring of three register sets (compile-time slots):
+    // fetched R - 1 steps ahead into a ring of R register sets (compile-time slots):
     // fetched at their use, each step waited out the weights' L2 latency. The
     // accumulation order is that of the plain step loop.
     struct Ops {
@@ -152,18 +154,18 @@ __global__ __launch_bounds__(kThreads) void k_conv3(const bf16_t* __restrict__ x
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(op.a[mb], op.b[nb], acc[mb][nb], 0, 0, 0);
     };
-    Ops o0, o1, o2;
-    fetch(0, o0);
-    if (KSTEPS > 1) fetch(1, o1);
-    for (int ks = 0; ks < KSTEPS; ks += 3) {
-        if (ks + 2 < KSTEPS) fetch(ks + 2, o2);
-        mma(o0);
-        if (ks + 1 >= KSTEPS) break;
-        if (ks + 3 < KSTEPS) fetch(ks + 3, o0);
-        mma(o1);
-        if (ks + 2 >= KSTEPS) break;
-        if (ks + 4 < KSTEPS) fetch(ks + 4, o1);
-        mma(o2);
+    Ops ring[R];
+#pragma unroll
+    for (int i = 0; i < R - 1; ++i)
+        if (i < KSTEPS) fetch(i, ring[i]);
+    for (int ks = 0; ks < KSTEPS; ks += R) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            if (ks + j < KSTEPS) {
+                if (ks + j + R - 1 < KSTEPS) fetch(ks + j + R - 1, ring[(j + R - 1) % R]);
+                mma(ring[j]);
+            }
+        }
     }
 
 #pragma unroll
@@ -454,14 +456,15 @@ template <int CI, int CO>
 hipError_t conv3_t(hipStream_t s, const bf16_t* x, const bf16_t* w, const float* bias, bf16_t* yb, float* yf, const bf16_t* aux, const int epi, const Dims d) {
     const unsigned tiles = tile_count(d);
     constexpr int kNB    = CO / 16;
-    // fewer tiles than CUs: one workgroup per 16 output channels of a tile
-    if (kNB > 1 && tiles * kNB <= 1024) {
-        if (yf) k_conv3<CI, CO, true, 1><<<tiles * kNB, kThreads, 0, s>>>(x, w, bias, yf, aux, epi, d);
-        else k_conv3<CI, CO, false, 1><<<tiles * kNB, kThreads, 0, s>>>(x, w, bias, yb, aux, epi, d);
+    // few tiles: one workgroup per 16 output channels of a tile, and (with so few waves on
+    // the GPU) the operands of kRingFew - 1 steps in flight in each
+    if (tiles * kNB <= 1024) {
+        if (yf) k_conv3<CI, CO, true, 1, kRingFew><<<tiles * kNB, kThreads, 0, s>>>(x, w, bias, yf, aux, epi, d);
+        else k_conv3<CI, CO, false, 1, kRingFew><<<tiles * kNB, kThreads, 0, s>>>(x, w, bias, yb, aux, epi, d);
     } else if (yf) {
-        k_conv3<CI, CO, true, kNB><<<tiles, kThreads, 0, s>>>(x, w, bias, yf, aux, epi, d);
+        k_conv3<CI, CO, true, kNB, 3><<<tiles, kThreads, 0, s>>>(x, w, bias, yf, aux, epi, d);
     } else {
-        k_conv3<CI, CO, false, kNB><<<tiles, kThreads, 0, s>>>(x, w, bias, yb, aux, epi, d);
+        k_conv3<CI, CO, false, kNB, 3><<<tiles, kThreads, 0, s>>>(x, w, bias, yb, aux, epi, d);
     }
     return hipGetLastError();
 }

@@ -236,81 +236,96 @@ __global__ __launch_bounds__(kThreads) void k_prolong(const float* __restrict__ 
 }
 
 // Residual sums of the guess and of the zero guess. A block takes one row y and a chunk of
-// zc planes and marches up z with the lane's column in registers (z - 1, z, z + 1,
-// and z + 2 in flight), so each plane of p0 is fetched once from HBM; the y -+ 1 rows
-// come from the L2 the neighbouring blocks share (blocks renumbered so that consecutive
-// rows run on one XCD) and x -+ 1 from the lane's own cache lines. Interior cells take
-// the six direct neighbours (same order as neighbour_sum); cells on a face or a periodic
-// boundary take the ghost rules in a second phase.
+// zc planes and marches up z with the lane's column in registers, so each plane of p0 is
+// fetched once from HBM; the y -+ 1 rows come from the L2 the neighbouring blocks share
+// (blocks renumbered so that consecutive rows run on one XCD) and x -+ 1 from the lane's
+// own cache lines. Every cell, on a face or not, is summed in the march: a neighbour
+// across a periodic boundary is loaded from its wrapped position, a ghost takes the
+// homogeneous rule (copy: the centre value, else 0) and the outflow pressures around the
+// cell enter b. This is neighbour_sum and ghost_constant cell by cell, in their face order.
+constexpr int kAhead = 2; // planes of loads in flight (deeper measured slower)
+static_assert(kAhead >= 2, "the sum reads planes z - 1 .. z + 1 from the column ring");
+constexpr int kSlots = kAhead + 2; // planes z - 1 .. z + kAhead of the column are live
+
 __global__ __launch_bounds__(kThreads) void k_guess_partials(const float* __restrict__ p0, const float* __restrict__ div, const float h2, double* __restrict__ partials, const Dims d, const StencilBC bc, const int zc) {
     double acc_g = 0.0, acc_z = 0.0;
     const uint64_t sy = static_cast<uint64_t>(d.nx);
-    auto face_cell    = [&](const int x, const int y, const int z) {
-        const uint64_t i = (static_cast<uint64_t>(z) * static_cast<uint64_t>(d.ny) + static_cast<uint64_t>(y)) * sy + static_cast<uint64_t>(x);
-        const float c    = p0[i];
-        const double b   = -static_cast<double>(h2) * div[i] + ghost_constant(x, y, z, d, bc);
-        const double ap  = 6.0 * c - static_cast<double>(neighbour_sum<float>(p0, x, y, z, d, bc, c, true));
-        acc_g += (b - ap) * (b - ap);
-        acc_z += b * b;
-    };
     const int n = static_cast<int>(gridDim.x), bid = static_cast<int>(blockIdx.x);
     const int wid = (n & 7) == 0 ? (bid & 7) * (n >> 3) + (bid >> 3) : bid;
     const int y   = wid % d.ny;
     const int z0  = (wid / d.ny) * zc;
     const int z1  = min(z0 + zc, d.nz);
-    const bool y_in = y > 0 && y < d.ny - 1;
-    auto row = [&](const int z) { return (static_cast<uint64_t>(z) * static_cast<uint64_t>(d.ny) + static_cast<uint64_t>(y)) * sy; };
-    if (y_in && d.nz > 1) {
-        for (int x = static_cast<int>(threadIdx.x); x < d.nx; x += kThreads) {
-            const bool x_in = x > 0 && x < d.nx - 1;
-            const int xl = x_in ? x - 1 : x, xr = x_in ? x + 1 : x;
-            float zm = p0[row(max(z0 - 1, 0)) + x];
-            float c  = p0[row(z0) + x];
-            float zp = p0[row(min(z0 + 1, d.nz - 1)) + x];
-            // the taps of planes z + 1 and z + 2 are in flight while plane z is summed
-            struct Taps {
-                float ym, yp, xm, xp, dv;
-            };
-            auto taps = [&](const int z) {
-                const uint64_t r = row(min(z, d.nz - 1));
-                return Taps{p0[r - sy + x], p0[r + sy + x], p0[r + xl], p0[r + xr], div[r + x]};
-            };
-            Taps t0 = taps(z0), t1 = taps(z0 + 1);
-            for (int z = z0; z < z1; ++z) {
-                const float nxt = p0[row(min(z + 2, d.nz - 1)) + x];
-                const Taps t2   = taps(z + 2);
-                const float ym = t0.ym, yp = t0.yp, xm = t0.xm, xp = t0.xp, dv = t0.dv;
-                if (x_in && z > 0 && z < d.nz - 1) {
-                    float sum = xm;
-                    sum += xp;
-                    sum += ym;
-                    sum += yp;
-                    sum += zm;
-                    sum += zp;
-                    const double b  = -static_cast<double>(h2) * dv;
-                    const double ap = 6.0 * c - static_cast<double>(sum);
-                    acc_g += (b - ap) * (b - ap);
-                    acc_z += b * b;
-                }
-                zm = c;
-                c  = zp;
-                zp = nxt;
-                t0 = t1;
-                t1 = t2;
+    const bool per_x = bc.periodic & 1, per_y = (bc.periodic >> 1) & 1, per_z = (bc.periodic >> 2) & 1;
+    // homogeneous ghost of face f: the centre value for copy rules, else 0
+    bool copy[6];
+    // outflow pressure of face f, counted where the cell touches that (non-periodic) face
+    double g_val[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        copy[f]  = bc.face[f].mode == kGhostCopy;
+        g_val[f] = ((bc.periodic >> (f >> 1)) & 1) || copy[f] ? 0.0 : static_cast<double>(bc.face[f].value);
+    }
+    // neighbour rows: wrapped on a periodic axis, -1 for a ghost
+    int ym = y, yp = y;
+    const bool has_ym = step_axis(ym, -1, d.ny, per_y), has_yp = step_axis(yp, 1, d.ny, per_y);
+    const uint64_t o_y = static_cast<uint64_t>(y) * sy, o_ym = static_cast<uint64_t>(has_ym ? ym : y) * sy, o_yp = static_cast<uint64_t>(has_yp ? yp : y) * sy;
+    const uint64_t plane = static_cast<uint64_t>(d.ny) * sy;
+    // plane p of the march (p may run from z0 - 1 to z1 + kAhead - 1): wrapped on a periodic
+    // z axis, else clamped (a clamped plane is never summed)
+    auto plane_of = [&](const int p) { return per_z ? wrap_index(p, d.nz) : min(max(p, 0), d.nz - 1); };
+    for (int x = static_cast<int>(threadIdx.x); x < d.nx; x += kThreads) {
+        int xl = x, xr = x;
+        const bool has_xl = step_axis(xl, -1, d.nx, per_x), has_xr = step_axis(xr, 1, d.nx, per_x);
+        if (!has_xl) xl = x;
+        if (!has_xr) xr = x;
+        // ghost_constant's order: x faces, y faces, z faces (each term exact in double)
+        const double g_xy = (((0.0 + (x == 0 ? g_val[0] : 0.0)) + (x == d.nx - 1 ? g_val[1] : 0.0)) + (y == 0 ? g_val[2] : 0.0)) + (y == d.ny - 1 ? g_val[3] : 0.0);
+        // Register rings of kSlots slots, unrolled so that every slot is a compile-time
+        // register set: the centre column of plane p sits in slot (p - z0 + 1) % kSlots,
+        // the other taps of plane p in slot (p - z0) % kSlots. The loads of plane
+        // z + kAhead are issued while plane z is summed. (Shifting registers from slot
+        // to slot instead makes the compiler wait for every load in flight: the moves
+        // read the registers being loaded.)
+        struct Taps {
+            float ym, yp, xm, xp, dv;
+        };
+        auto col_of = [&](const int p) { return p0[static_cast<uint64_t>(plane_of(p)) * plane + o_y + x]; };
+        auto taps   = [&](const int p) {
+            const uint64_t r = static_cast<uint64_t>(plane_of(p)) * plane;
+            return Taps{p0[r + o_ym + x], p0[r + o_yp + x], p0[r + o_y + xl], p0[r + o_y + xr], div[r + o_y + x]};
+        };
+        float col[kSlots];
+        Taps t[kSlots];
+#pragma unroll
+        for (int j = 0; j <= kAhead; ++j) col[j] = col_of(z0 + j - 1);
+#pragma unroll
+        for (int i = 0; i < kAhead; ++i) t[i] = taps(z0 + i);
+        // plane z, whose ring slots are those of j = (z - z0) % kSlots
+        auto step = [&](const int z, const int j) __attribute__((always_inline)) {
+            col[(j + kAhead + 1) % kSlots] = col_of(z + kAhead);
+            t[(j + kAhead) % kSlots]       = taps(z + kAhead);
+            const float c  = col[(j + 1) % kSlots];
+            const Taps& tz = t[j];
+            const bool has_zm = z > 0 || per_z, has_zp = z < d.nz - 1 || per_z;
+            float sum = has_xl ? tz.xm : (copy[0] ? c : 0.0f);
+            sum += has_xr ? tz.xp : (copy[1] ? c : 0.0f);
+            sum += has_ym ? tz.ym : (copy[2] ? c : 0.0f);
+            sum += has_yp ? tz.yp : (copy[3] ? c : 0.0f);
+            sum += has_zm ? col[j] : (copy[4] ? c : 0.0f);
+            sum += has_zp ? col[(j + 2) % kSlots] : (copy[5] ? c : 0.0f);
+            const double g  = (g_xy + (z == 0 ? g_val[4] : 0.0)) + (z == d.nz - 1 ? g_val[5] : 0.0);
+            const double b  = -static_cast<double>(h2) * tz.dv + g;
+            const double ap = 6.0 * c - static_cast<double>(sum);
+            acc_g += (b - ap) * (b - ap);
+            acc_z += b * b;
+        };
+        for (int zb = z0; zb < z1; zb += kSlots) {
+#pragma unroll
+            for (int j = 0; j < kSlots; ++j) {
+                if (zb + j >= z1) break;
+                step(zb + j, j);
             }
         }
-    }
-    // face cells of the block: all of them on a y face row; else x = 0 and nx - 1 of every
-    // plane, then the rest of the rows on a z face
-    const int planes = z1 - z0;
-    if (!y_in || d.nz == 1) {
-        for (int e = static_cast<int>(threadIdx.x); e < planes * d.nx; e += kThreads) face_cell(e % d.nx, y, z0 + e / d.nx);
-    } else {
-        const int per = d.nx > 1 ? 2 : 1;
-        for (int e = static_cast<int>(threadIdx.x); e < per * planes; e += kThreads) face_cell((e % per) ? d.nx - 1 : 0, y, z0 + e / per);
-        for (int z = z0; z < z1; ++z)
-            if (z == 0 || z == d.nz - 1)
-                for (int x = 1 + static_cast<int>(threadIdx.x); x < d.nx - 1; x += kThreads) face_cell(x, y, z);
     }
     block_sum_to(acc_g, partials + 2 * blockIdx.x);
     block_sum_to(acc_z, partials + 2 * blockIdx.x + 1);

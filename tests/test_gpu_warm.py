@@ -185,6 +185,35 @@ def test_accepted_guess_matches_oracle(coarsening):
     assert abs(stats["guess_residual"] - g) <= 1e-5 * g and abs(stats["zero_residual"] - z) <= 1e-6 * z and not fb
 
 
+@pytest.mark.parametrize("periodic_z", [False, True])
+def test_residual_statistics_with_outflow_pressures(periodic_z):
+    """The accepted guess's residual statistics where outflow faces carry nonzero pressures
+    (their values enter b on those faces) beside wall and periodic faces, on a grid of
+    three z chunks: against the FP64 residual of the pressure view (the guess, K = 0)."""
+    zt = [abi.FLOW_PERIODIC, abi.FLOW_PERIODIC] if periodic_z else [abi.FLOW_OUTFLOW, abi.FLOW_NO_SLIP]
+    flow = abi.flow_boundary([abi.FLOW_OUTFLOW, abi.FLOW_FREE_SLIP, abi.FLOW_PERIODIC, abi.FLOW_PERIODIC] + zt,
+                             pressure=[0.3, 0.0, 0.0, 0.0, -0.2, 0.0])
+    desc = abi.Desc(32, 16, 48, 1.0 / 16, 0.05, 1e-3, 2, 6, flow, scalars=[abi.scalar_desc(1e-3)], vectors=[abi.force_desc()])
+    inputs = Inputs(desc, seed=11, velocity_scale=1.0, smooth=True)
+    layers = uo.init_params(5)
+    w, b = layers[-1]
+    layers[-1] = (w * 0.05, b)
+    params = uo.join_params(layers).astype(np.float32)
+    with sf.Context(desc, options=learned_opts(0)) as ctx:
+        ctx.set_pressure_model(params)
+        inputs.seed_gpu(ctx)
+        ctx.step(1)
+        stats = ctx.warm_start_stats()
+        pressure = gpu_fields(ctx, desc)["pressure"]
+        rhs = ctx.pressure_rhs().astype(np.float64)
+    h = float(np.float32(1.0 / 16))
+    rules = wo.pressure_rules(flow)
+    g, z, fb = wo.guess_test(pressure, rhs, h * h, rules)
+    print(f"periodic_z={periodic_z}: stats={stats} oracle g={g:.6e} z={z:.6e}", flush=True)
+    assert not stats["fallback"] and not fb
+    assert abs(stats["guess_residual"] - g) <= 1e-5 * g and abs(stats["zero_residual"] - z) <= 1e-6 * z
+
+
 def test_trained_network_gain():
     """The trained network (stable-fluids_amd/weights) against the method without it: one
     step of the 128^3 plume from a common state, post-projection divergence L2 with the
