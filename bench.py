@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 # the dominant kernel: a middle velocity-diffusion RBGS pass on the black-cell buffers
 # (diffusion solve, half-buffer in, half-buffer out, 128 x 16 tiles)
 VELOCITY_PASS_KERNEL = "k_rbgs_half<0, 2, 0, 64, 8, 2>"
-TRACE_FILE = os.path.join(ROOT, "profiles", "step_plume256_kernel_trace.json")
+TRACE_FILE = os.path.join(ROOT, "profiles", "stage_kernels_plume256_kernel_trace.json")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_plume256_kernels.json")
 STORAGE = {"f32": 0, "f16": 1, "bf16": 2, "f32-refined": 3}
 METRIC = "steps/s at 256³; Jacobi HBM GB/s; learned-vs-plain speedup at equal div L2"
@@ -94,6 +94,9 @@ def main():
     ap.add_argument("--learned-step", type=int, default=None, metavar="K",
                     help="time only the learned-warm-start step (network guess + K RBGS passes, P = 80) on the 256^3 "
                          "plume from its step-10 state, --steps steps after --warmup (for kernel traces)")
+    ap.add_argument("--stages-only", action="store_true",
+                    help="only the stage-hook timings of the default workload (no step, no graph launch: for kernel "
+                         "traces of the smoother passes)")
     ap.add_argument("--learned-guess-stage", type=int, default=None, metavar="R",
                     help="enqueue the learned projection stage (K = 0: the network guess, its residual test and the "
                          "projection) R times through the stage hook at 256^3, with no graph launch (for kernel traces)")
@@ -115,6 +118,9 @@ def main():
         return
     if args.learned_guess_stage is not None:
         learned_guess_stage(pkg, args)
+        return
+    if args.stages_only:
+        stages_only(pkg, args)
         return
     abi, hip, scenes = pkg.abi, pkg.hip, pkg.scenes
     n = args.grid
@@ -164,8 +170,9 @@ def main():
     if default_run and not args.no_in_graph:
         in_graph = in_graph_pass_costs(pkg, n)
 
-    # the committed kernel trace (rocprofv3 --kernel-trace --stats of this command) and
-    # the committed FETCH_SIZE / WRITE_SIZE counts (copy-kernel calibrated) of the kernel
+    # the committed kernel trace (rocprofv3 --kernel-trace --stats of the same stage launches,
+    # bench.py --stages-only) and the committed FETCH_SIZE / WRITE_SIZE counts of the kernel
+    # (copy-kernel calibrated)
     trace = traffic = None
     if default_run and os.path.exists(TRACE_FILE):
         trace = json.load(open(TRACE_FILE))["kernels"].get(VELOCITY_PASS_KERNEL)
@@ -286,7 +293,8 @@ def in_graph_pass_costs(pkg, n, reps=3, steps=10):
 def roofline(dom, trace, traffic, big):
     """The dominant kernel against the HBM peak. achieved = its algorithmic bytes (8 B per
     cell: b and the black cells read, the black cells written) over its live mean launch
-    time; trace_* are the same from the committed kernel trace of this command."""
+    time; trace_* are the same from the committed kernel trace of these stage launches
+    (bench.py --stages-only)."""
     out = {"bound": "hbm", "kernel": f"velocity diffusion RBGS middle pass ({VELOCITY_PASS_KERNEL})", "achieved": dom["gbs"],
            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
            "bytes_per_launch": dom["bytes"], "avg_us": dom["us"],
@@ -357,6 +365,24 @@ def learned_step_study(pkg, args):
     print(json.dumps({"workload": "learned warm-start step, plume 256^3 from its step-10 state, D=20 P=80, "
                                   "network on the 32^3 restriction", "k": args.learned_step, "steps": args.steps,
                       "steps_per_s": round(sps, 3)}), flush=True)
+
+
+def stages_only(pkg, args):
+    """The `stages` object of the default run alone: each stage kernel enqueued through the
+    stage hooks on the seeded plume, with no step and no graph launch before it. One
+    advection and one projection enqueued first give the smoothers nonzero right-hand sides
+    (the advected velocity, the divergence); the seeded velocity is white noise, so the
+    advection's own time is not the plume's."""
+    abi, scenes = pkg.abi, pkg.scenes
+    scene = scenes.plume_scene(args.grid, diffuse_iterations=40, pressure_iterations=80, noise_seed=0)
+    with pkg.Context(scene.desc) as ctx:
+        scenes.seed_context(ctx, scene)
+        ctx.run_stage(abi.STAGE_ADVECT, 1)
+        ctx.run_stage(abi.STAGE_PROJECTION, 1)
+        stages = timed_stages(ctx, args.stage_repeat)
+        tiling = ctx.smoother_tiling()
+    print(json.dumps({"workload": f"plume {args.grid}^3, seeded smoothed-noise velocity, D=40 P=80 RBGS, stage hooks only",
+                      "stage_repeat": args.stage_repeat, "stages": stages, "tiling": tiling}), flush=True)
 
 
 def learned_guess_stage(pkg, args):
